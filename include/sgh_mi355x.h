@@ -1,0 +1,164 @@
+/*
+ * C ABI of libsgh_mi355x.so: hand-written gfx950 HIP kernels for the spectral
+ * Galerkin ETD1 + Picard step of spectral_galerkin_heat.
+ *
+ * Conventions
+ *   - Every array is a caller-owned device pointer to a C-contiguous [z, y, x]
+ *     (x fastest) volume or [y, x] surface, in the precision named by `dtype`
+ *     (SGH_F64 = double, SGH_F32 = float). Reductions and norms are always double.
+ *   - `stream` is a hipStream_t passed as void* (NULL = the default stream). Every
+ *     call is asynchronous and stream-ordered; the caller keeps buffers alive until
+ *     the stream has been synchronised.
+ *   - Return 0 on success or a negative SGH_ERR_* code; sgh_last_error() gives a
+ *     per-thread message. No C++ exception crosses the ABI. No call allocates
+ *     device memory except sgh_batch_create (which owns its packed state).
+ *
+ * Each entry point cites the reference function it replaces (paths relative to
+ * the reference checkout; sgh/ = src/spectral_galerkin_heat/).
+ */
+#ifndef SGH_MI355X_H
+#define SGH_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { SGH_F64 = 0, SGH_F32 = 1 };
+enum {
+  SGH_OK = 0,
+  SGH_ERR_ARG = -1,   /* bad shape, null pointer, unsupported dtype           */
+  SGH_ERR_HIP = -2,   /* HIP runtime / launch failure                         */
+  SGH_ERR_STATE = -3, /* misuse of a plan (wrong order, destroyed handle)     */
+  SGH_ERR_NOMEM = -4  /* device allocation failed (sgh_batch_create only)     */
+};
+
+/* ---- library ------------------------------------------------------------- */
+int sgh_version(void);
+const char* sgh_last_error(void);
+
+/* ---- propagator tables ---------------------------------------------------
+ * E[p,n,m] = exp(-αλΔt), KK = (1 - E)/(αλ)/(ρc) with λ = (πm/Lx)²+(πn/Ly)²+(πp/Lz)²,
+ * evaluated in FP64 with expm1 and stored in `dtype`; KK[0,0,0] = Δt/(ρc).
+ * Replaces precompute_K_KK (sgh/physics/spectral_state.py:374-421). */
+int sgh_etd_tables(int dtype, int nx, int ny, int nz, double Lx, double Ly, double Lz,
+                   double alpha, double rho_cp, double dt, void* E, void* KK, void* stream);
+
+/* ---- pointwise modal updates ---------------------------------------------
+ * out = aK + KK·(wt[p]·St[n,m] + wb[p]·Sb[n,m] + Q[p,n,m]); (wb,Sb) and Q may be NULL.
+ * Replaces update_modes_etd1 (sgh/physics/spectral_cpu_kernels.py:83-91; GPU K1
+ * sgh/physics/spectral_gpu_kernels.py:59-62,129-131) fused with the accumulators below. */
+int sgh_etd1_update(int dtype, int nx, int ny, int nz, const void* aK, const void* KK,
+                    const void* wt, const void* St, const void* wb, const void* Sb,
+                    const void* Q, void* out, void* stream);
+
+/* a += KK·Q.  Replaces add_source_term_modes (spectral_cpu_kernels.py:93-103). */
+int sgh_add_volume_source(int dtype, long long n, void* a, const void* KK, const void* Q,
+                          void* stream);
+
+/* a += KK·w[p]·B[n,m].  Replaces add_bottom_surface_source (spectral_cpu_kernels.py:105-113). */
+int sgh_add_surface_source(int dtype, int nx, int ny, int nz, void* a, const void* KK,
+                           const void* w, const void* B, void* stream);
+
+/* a *= E.  Replaces xp.multiply(a, K, out=a) (sgh/solvers/spectral.py:283). */
+int sgh_mul_inplace(int dtype, long long n, void* a, const void* E, void* stream);
+
+/* parts[c][n,m] = Σ_{p in chunk c} w[p]·a[p,n,m]; the full contraction is the sum of the
+ * `nchunks` planes.  Replaces the einsum of reconstruct_surface_temperature /
+ * reconstruct_bottom_temperature (sgh/physics/spectral_ops.py:210-231). */
+int sgh_zcontract(int dtype, int nx, int ny, int nz, const void* a, const void* w,
+                  int nchunks, void* parts, void* stream);
+
+/* ---- fused Picard iteration ----------------------------------------------
+ * a_raw = aK + KK·(wt⊗St + wb⊗Sb + Q);  r = a_raw - a_old;  a_new = a_old + ω·r;
+ * partials[2b] = Σ r², partials[2b+1] = Σ a_old² (FP64, one pair per block);
+ * if wz != NULL also zc_parts[c] = Σ_{p in chunk c} wz[p]·a_new[p] (next iterate's
+ * surface contraction).  Replaces spectral.py:431-465 (update, accumulators,
+ * relaxation and both vdots).  `partials` must hold 2·sgh_picard_partials_count(). */
+int sgh_picard_partials_count(int nx, int ny, int nz, int nchunks);
+int sgh_picard_iter_fused(int dtype, int nx, int ny, int nz, const void* aK, const void* KK,
+                          const void* a_old, const void* wt, const void* St, const void* wb,
+                          const void* Sb, const void* Q, double omega, const void* wz,
+                          int nchunks, void* zc_parts, void* a_new, double* partials,
+                          void* stream);
+
+/* out[0] = √(Σr²/N), out[1] = √(Σa²/N), out[2] = out[0]/max(out[1], 1e-9), summed in a
+ * fixed order.  Replaces the norm of spectral.py:464-468. */
+int sgh_reduce_norms(const double* partials, int n_pairs, double n_elements, double* out,
+                     void* stream);
+
+/* out[0] = Σ x (op 0) or max x (op 1) over n elements, in FP64.  Replaces xp.sum /
+ * xp.max on the path (spectral.py:278,537). */
+int sgh_reduce(int dtype, long long n, const void* x, int op, double* out, void* stream);
+
+/* ---- separable transforms ------------------------------------------------
+ * out = scale · L · X · R for X = Σ_{c<nparts} Xparts[c] (ny×nx, plane stride ny·nx),
+ * L (ny×ny) and R (nx×nx) row-major.  With L, R the ortho DCT-II matrix or its
+ * transpose this is DCT_II / IDCT_II of a surface field (spectral_cpu_kernels.py:374-389;
+ * GPU K9/K10 spectral_gpu_kernels.py:371-451); with the cell-centred cosine basis it is
+ * the surface reconstruction / projection directly.  `batch` independent transforms
+ * share L and R (stride ny·nx·nparts in, ny·nx out). */
+int sgh_sep2d(int dtype, int batch, int ny, int nx, const void* L, const void* R,
+              const void* X, int nparts, double scale, void* out, void* stream);
+
+/* out[o,k,i] = scale · Σ_j M[k,j]·in[o,j,i] for an [outer, n, inner] view; M is n×n
+ * row-major.  Three calls (x, y, z) make the 3-D DCT of project_volume /
+ * reconstruct_volume (spectral_ops.py:83-102). in != out. */
+int sgh_axis_contract(int dtype, long long outer, int n, long long inner, const void* M,
+                      const void* in, double scale, void* out, void* stream);
+
+/* ---- surface forcing -----------------------------------------------------
+ * Laser flux on cell centres x_i = (i+½)dx, y_j = (j+½)dy: I0·exp(-2ρⁿ) (order n), or
+ * the cell-mean Gaussian when cell_integrated != 0.  Replaces LaserProfile.flux
+ * (sgh/core/laser.py:42-53,65-90,122-135). */
+int sgh_laser_flux(int dtype, int nx, int ny, double dx, double dy, double x0, double y0,
+                   double rx, double ry, double order, double I0, int cell_integrated,
+                   void* out, void* stream);
+
+/* out = base + alpha · shift(in), bilinear, out[o] = in(o - s), zero outside
+ * (scipy.ndimage.shift order=1 mode='constant'); base may be NULL.
+ * Replaces shift_flux / _ndshift (spectral_cpu_kernels.py:281-396). */
+int sgh_shift2d(int dtype, int ny, int nx, const void* in, double sy, double sx,
+                const void* base, double alpha, void* out, void* stream);
+
+/* q_evap = evaporation(T) (0 below T_liq);
+ * q_net = (q_las - q_evap - h_top·(T - T0)) · (prog ? kbar/k(T) : 1).  q_las may be NULL.
+ * Replaces compute_evaporation_flux (spectral_cpu_kernels.py:144-162; GPU K4) and the
+ * top-flux assembly of spectral.py:390-407. `prog` is a packed property program. */
+int sgh_surface_forcing(int dtype, long long S, const void* T, const void* q_las,
+                        double P0, double T_boil, double dH, double R_v, double T_liq,
+                        double h_top, double T0, const void* prog, double kbar,
+                        void* q_evap, void* q_net, void* stream);
+
+/* out = -h·(T - T0)·(prog ? kbar/k(T) : 1).  Convection flux (spectral.py:341,348,376-378). */
+int sgh_convection_flux(int dtype, long long S, const void* T, double h, double T0,
+                        const void* prog, double kbar, void* out, void* stream);
+
+/* ---- volume forcing ------------------------------------------------------
+ * Q = -ρ·L·(f_l(T) - f_l(T_prev))/dt, f_l clamped; ρ = rho, or the blended ρ(T) of
+ * `prog` when prog != NULL.  Replaces compute_source_term_from_temperature
+ * (spectral_cpu_kernels.py:115-140; GPU K5) and MaterialModel.latent_heat_source
+ * (sgh/core/properties.py:375-430; GPU K7). */
+int sgh_latent_source(int dtype, long long n, const void* T, const void* T_prev, double T_S,
+                      double T_L, double rho, double L, double dt, const void* prog,
+                      void* out, void* stream);
+
+/* f = ∇·(k'∇T) - a'(T - T_prev)/dt with k' = k(T) - kbar, a' = ρ(T)c(T) - abar from
+ * `prog`, numpy-gradient stencil (central inside, one-sided on faces).  `work` holds
+ * 3·nz·ny·nx elements.  Replaces MaterialModel.k_prime_a_prime (properties.py:318-373;
+ * GPU K6) + _correction_source (spectral_cpu_kernels.py:165-252; GPU K8). */
+int sgh_property_correction_source(int dtype, int nx, int ny, int nz, const void* T,
+                                   const void* T_prev, const void* prog, double kbar,
+                                   double abar, double dt, double dx, double dy, double dz,
+                                   void* work, void* out, void* stream);
+
+/* Property program layout size in bytes (for validation on the host). */
+int sgh_property_program_bytes(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SGH_MI355X_H */

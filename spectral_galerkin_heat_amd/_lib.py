@@ -1,0 +1,117 @@
+"""ctypes binding of libsgh_mi355x.so (the C ABI declared in include/sgh_mi355x.h).
+
+The library is loaded from the package directory only; there is no fallback. Any
+non-zero return code raises ``SGHError`` carrying ``sgh_last_error()``.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+__all__ = ["SGHError", "lib", "call", "F64", "F32", "dtype_code", "ptr", "stream_ptr"]
+
+F64, F32 = 0, 1
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsgh_mi355x.so")
+
+vp, i32, i64, f64, dp = C.c_void_p, C.c_int, C.c_longlong, C.c_double, C.POINTER(C.c_double)
+
+# name -> argument types (all return int unless listed in _RESTYPES)
+SIGNATURES = {
+    "sgh_version": [],
+    "sgh_last_error": [],
+    "sgh_property_program_bytes": [],
+    "sgh_etd_tables": [i32, i32, i32, i32, f64, f64, f64, f64, f64, f64, vp, vp, vp],
+    "sgh_etd1_update": [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp],
+    "sgh_add_volume_source": [i32, i64, vp, vp, vp, vp],
+    "sgh_add_surface_source": [i32, i32, i32, i32, vp, vp, vp, vp, vp],
+    "sgh_mul_inplace": [i32, i64, vp, vp, vp],
+    "sgh_zcontract": [i32, i32, i32, i32, vp, vp, i32, vp, vp],
+    "sgh_picard_partials_count": [i32, i32, i32, i32],
+    "sgh_picard_iter_fused": [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, f64, vp, i32,
+                              vp, vp, vp, vp],
+    "sgh_reduce_norms": [vp, i32, f64, vp, vp],
+    "sgh_reduce": [i32, i64, vp, i32, vp, vp],
+    "sgh_sep2d": [i32, i32, i32, i32, vp, vp, vp, i32, f64, vp, vp],
+    "sgh_axis_contract": [i32, i64, i32, i64, vp, vp, f64, vp, vp],
+    "sgh_laser_flux": [i32, i32, i32, f64, f64, f64, f64, f64, f64, f64, f64, i32, vp, vp],
+    "sgh_shift2d": [i32, i32, i32, vp, f64, f64, vp, f64, vp, vp],
+    "sgh_surface_forcing": [i32, i64, vp, vp, f64, f64, f64, f64, f64, f64, f64, vp, f64, vp, vp,
+                            vp],
+    "sgh_convection_flux": [i32, i64, vp, f64, f64, vp, f64, vp, vp],
+    "sgh_latent_source": [i32, i64, vp, vp, f64, f64, f64, f64, f64, vp, vp, vp],
+    "sgh_property_correction_source": [i32, i32, i32, i32, vp, vp, vp, f64, f64, f64, f64, f64,
+                                       f64, vp, vp, vp],
+}
+_RESTYPES = {"sgh_last_error": C.c_char_p}
+
+
+class SGHError(RuntimeError):
+    """A libsgh_mi355x call returned an error code."""
+
+
+class _Lib:
+    def __init__(self):
+        self._h = None
+
+    def load(self):
+        if self._h is not None:
+            return self._h
+        if not os.path.exists(LIB_PATH):
+            raise SGHError(
+                f"{LIB_PATH} is missing: build it with `python -m "
+                "spectral_galerkin_heat_amd._build` (hipcc, gfx950). The MI355X backend has "
+                "no CPU fallback.")
+        h = C.CDLL(LIB_PATH)
+        for name, argtypes in SIGNATURES.items():
+            fn = getattr(h, name, None)
+            if fn is None:
+                continue
+            fn.argtypes = argtypes
+            fn.restype = _RESTYPES.get(name, C.c_int)
+        self._h = h
+        return h
+
+    def __getattr__(self, name):
+        return getattr(self.load(), name)
+
+
+lib = _Lib()
+
+
+def last_error() -> str:
+    msg = lib.sgh_last_error()
+    return msg.decode() if msg else ""
+
+
+def call(name: str, *args) -> int:
+    rc = getattr(lib, name)(*args)
+    if rc < 0:
+        raise SGHError(f"{name} failed ({rc}): {last_error()}")
+    return rc
+
+
+def dtype_code(dtype) -> int:
+    import numpy as np
+    import torch
+    if dtype in (torch.float64, np.float64) or str(dtype) in ("float64", "torch.float64"):
+        return F64
+    if dtype in (torch.float32, np.float32) or str(dtype) in ("float32", "torch.float32"):
+        return F32
+    raise ValueError(f"unsupported dtype {dtype!r} (float32 or float64)")
+
+
+def ptr(t):
+    """Device pointer of a tensor (None -> NULL). The tensor must be contiguous."""
+    if t is None:
+        return None
+    if not t.is_contiguous():
+        raise ValueError("MI355X kernels need C-contiguous tensors")
+    return t.data_ptr()
+
+
+def stream_ptr(stream=None):
+    import torch
+    s = torch.cuda.current_stream() if stream is None else stream
+    return s.cuda_stream

@@ -1,0 +1,542 @@
+"""MI355X kernel module: the reference kernel-module API on torch device buffers.
+
+Same ``__all__`` as the reference's kernel modules (``sgh/physics/spectral_cpu_kernels.py:37-48``,
+``spectral_gpu_kernels.py:36-47``) plus the extra module-level names the solver reaches
+(``DCT_II``, ``IDCT_II``, ``shift_flux``, ``compute_evaporation_flux``,
+``reconstruct_volume``, ``project_volume``, ``assemble_property_correction``, ...).
+Every function calls a gfx950 HIP kernel through the C ABI (``_lib``); none falls back
+to host arithmetic. ``picard_step`` is the fused per-step driver the ``SpectralSolver``
+uses on this backend.
+
+Grid mode only (no laser-following fine sub-box): box mode is not implemented here and
+raises ``NotImplementedError`` at state construction.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from .. import _lib
+from .._lib import call, dtype_code, ptr, stream_ptr
+
+__all__ = [
+    "FineMeshState",
+    "SpectralGrid",
+    "SpectralSolverState",
+    "add_bottom_surface_source",
+    "add_source_term_modes",
+    "compute_latent_heat_source",
+    "compute_latent_heat_source_grid",
+    "reconstruct_bottom_temperature",
+    "reconstruct_surface_temperature",
+    "update_modes_etd1",
+]
+
+_DEVICE = "cuda"
+
+
+def _tdtype(dtype):
+    return torch.float64 if np.dtype(dtype) == np.float64 else torch.float32
+
+
+def _dev(arr, dtype):
+    return torch.as_tensor(np.ascontiguousarray(arr, dtype=np.float64)).to(
+        device=_DEVICE, dtype=_tdtype(dtype)).contiguous()
+
+
+def cosine_basis(n: int, L: float) -> np.ndarray:
+    """B[k, i] = C_k·cos(πk(i+½)/n) with C = √(2/L), √(1/L) for k = 0 (FP64, host).
+
+    The surface reconstruction is Bᵧᵀ·A·Bₓ and the projection dx·dy·Bᵧ·q·Bₓᵀ, which equal
+    the reference's recon_scale·IDCT_II and dct_scale·DCT_II (spectral_state.py:115-116).
+    """
+    k = np.arange(n)[:, None]
+    i = np.arange(n)[None, :]
+    ang = np.pi * ((k * (2 * i + 1)) % (4 * n)) / (2.0 * n)
+    C = np.full(n, math.sqrt(2.0 / L))
+    C[0] = math.sqrt(1.0 / L)
+    return C[:, None] * np.cos(ang)
+
+
+def dct_ortho_matrix(n: int) -> np.ndarray:
+    """Ortho DCT-II matrix M[k, j] = s_k·cos(πk(2j+1)/(2n)), s_0 = √(1/n), s_k = √(2/n)."""
+    k = np.arange(n)[:, None]
+    j = np.arange(n)[None, :]
+    ang = np.pi * ((k * (2 * j + 1)) % (4 * n)) / (2.0 * n)
+    s = np.full(n, math.sqrt(2.0 / n))
+    s[0] = math.sqrt(1.0 / n)
+    return s[:, None] * np.cos(ang)
+
+
+_DCT_CACHE: dict = {}
+
+
+def _dct_mats(n, tdtype):
+    key = (n, tdtype)
+    m = _DCT_CACHE.get(key)
+    if m is None:
+        M = dct_ortho_matrix(n)
+        m = (torch.as_tensor(M).to(_DEVICE, tdtype).contiguous(),
+             torch.as_tensor(np.ascontiguousarray(M.T)).to(_DEVICE, tdtype).contiguous())
+        _DCT_CACHE[key] = m
+    return m
+
+
+def _apply_axes(x, mats, scale=1.0):
+    """Contract every axis of a 3-D tensor with its matrix (mats in x, y, z order)."""
+    nz, ny, nx = x.shape
+    code = dtype_code(x.dtype)
+    st = stream_ptr()
+    w1 = torch.empty_like(x)
+    w2 = torch.empty_like(x)
+    call("sgh_axis_contract", code, nz * ny, nx, 1, ptr(mats[0]), ptr(x), 1.0, ptr(w1), st)
+    call("sgh_axis_contract", code, nz, ny, nx, ptr(mats[1]), ptr(w1), 1.0, ptr(w2), st)
+    call("sgh_axis_contract", code, 1, nz, ny * nx, ptr(mats[2]), ptr(w2), float(scale), ptr(w1), st)
+    return w1
+
+
+def _sep2d(X, L, R, scale=1.0, nparts=1, out=None):
+    ny, nx = L.shape[0], R.shape[0]
+    if out is None:
+        out = torch.empty((ny, nx), dtype=X.dtype, device=X.device)
+    call("sgh_sep2d", dtype_code(X.dtype), 1, ny, nx, ptr(L), ptr(R), ptr(X), int(nparts),
+         float(scale), ptr(out), stream_ptr())
+    return out
+
+
+def DCT_II(q):
+    """Ortho DCT-II over every axis (spectral_cpu_kernels.py:374-381), on MFMA-free LDS GEMMs."""
+    q = q.contiguous()
+    td = q.dtype
+    if q.dim() == 2:
+        My, _ = _dct_mats(q.shape[0], td)
+        _, MxT = _dct_mats(q.shape[1], td)
+        return _sep2d(q, My, MxT)
+    if q.dim() == 3:
+        mats = [_dct_mats(n, td)[0] for n in (q.shape[2], q.shape[1], q.shape[0])]
+        return _apply_axes(q, mats)
+    raise ValueError("DCT_II supports 2-D and 3-D fields")
+
+
+def IDCT_II(a):
+    """Ortho DCT-III over every axis, the inverse of DCT_II (spectral_cpu_kernels.py:383-389)."""
+    a = a.contiguous()
+    td = a.dtype
+    if a.dim() == 2:
+        _, MyT = _dct_mats(a.shape[0], td)
+        Mx, _ = _dct_mats(a.shape[1], td)
+        return _sep2d(a, MyT, Mx)
+    if a.dim() == 3:
+        mats = [_dct_mats(n, td)[1] for n in (a.shape[2], a.shape[1], a.shape[0])]
+        return _apply_axes(a, mats)
+    raise ValueError("IDCT_II supports 2-D and 3-D fields")
+
+
+# ---------------------------------------------------------------------------------
+# State
+# ---------------------------------------------------------------------------------
+
+
+class SpectralGrid:
+    """Grid constants and cosine bases on the device (spectral_state.py:71-143)."""
+
+    def __init__(self, geom, dtype=np.float32):
+        self.dtype = dtype
+        td = _tdtype(dtype)
+        nx, ny, nz = (int(v) for v in geom.n)
+        Lx, Ly, Lz = (float(v) for v in geom.size)
+        dx, dy, dz = (float(v) for v in geom.d)
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.x, self.y, self.z = [_dev((np.arange(n) + 0.5) * d, dtype)
+                                  for n, d in ((nx, dx), (ny, dy), (nz, dz))]
+        Cs = []
+        for n, L in ((nx, Lx), (ny, Ly), (nz, Lz)):
+            c = np.full(n, math.sqrt(2.0 / L))
+            c[0] = math.sqrt(1.0 / L)
+            Cs.append(c)
+        self.C = tuple(_dev(c, dtype) for c in Cs)
+        self.dct_scale = dtype((dx * dy) * np.sqrt((nx * ny) / (Lx * Ly)))
+        self.recon_scale = dtype(np.sqrt(nx * ny) / np.sqrt(Lx * Ly))
+        sign = np.power(-1.0, np.arange(nz))
+        self.w_top = _dev(Cs[2] * sign, dtype)
+        self.w_bot = _dev(Cs[2], dtype)
+        self.Cp32_broadcast = self.w_top.view(nz, 1, 1)
+        self.Cp32_broadcast_bottom = self.w_bot.view(nz, 1, 1)
+        self.sqrt_dV = dtype(np.sqrt(dx * dy * dz))
+        self.dx, self.dy, self.dz = dx, dy, dz
+        self.kx = _dev(np.pi * np.arange(nx) / Lx, dtype)
+        self.ky = _dev(np.pi * np.arange(ny) / Ly, dtype)
+        self.kz = _dev(np.pi * np.arange(nz) / Lz, dtype)
+        B = [cosine_basis(n, L) for n, L in ((nx, Lx), (ny, Ly), (nz, Lz))]
+        self.Bx, self.By, self.Bz = (_dev(b, dtype) for b in B)
+        self.BxT, self.ByT, self.BzT = (_dev(np.ascontiguousarray(b.T), dtype) for b in B)
+        self.dV = dx * dy * dz
+        self.B_recon = None
+        self.coords_rec = None
+
+
+class FineMeshState:
+    """Laser-following fine sub-box (box mode). Not implemented on this backend."""
+
+    def __init__(self, *args, **kwargs):
+        raise NotImplementedError(
+            "box mode (fine_mesh) is not implemented in the MI355X backend; omit the "
+            "fine_mesh section to run in grid mode")
+
+
+class SolverBuffers:
+    """Reusable device buffers (spectral_state.py:242-275) plus the fused-step scratch."""
+
+    def __init__(self, num, dtype):
+        td = _tdtype(dtype)
+        nx, ny, nz = num.nx, num.ny, num.nz
+        shp, srf = (nz, ny, nx), (ny, nx)
+        dev = dict(dtype=td, device=_DEVICE)
+        self.a_temp = torch.empty(shp, **dev)
+        self.a_alt = torch.empty(shp, **dev)
+        self.q_evap_old = torch.zeros(srf, **dev)
+        self.q_evap_buffer = torch.zeros(srf, **dev)
+        self.Q_latent_buffer = torch.zeros(shp, **dev)
+        self.n_elements = dtype(nx * ny * nz)
+        self.q_las = torch.zeros(srf, **dev)
+        self.q_tmp = torch.zeros(srf, **dev)
+        self.T_s = torch.zeros(srf, **dev)
+        self.S_top = torch.zeros(srf, **dev)
+        self.S_tmp = torch.zeros(srf, **dev)
+        self.S_bot = torch.zeros(srf, **dev)
+        self.S_bot_raw = torch.zeros(srf, **dev)
+        self.S_zero = torch.zeros(srf, **dev)
+        self.nchunks = max(1, min(nz, 8))
+        self.zc = torch.zeros((self.nchunks, ny, nx), **dev)
+        n_pairs = _lib.lib.sgh_picard_partials_count(nx, ny, nz, self.nchunks)
+        self.n_pairs = n_pairs
+        self.partials = torch.zeros(2 * n_pairs, dtype=torch.float64, device=_DEVICE)
+        self.norms = torch.zeros(3, dtype=torch.float64, device=_DEVICE)
+        self.scal = torch.zeros(4, dtype=torch.float64, device=_DEVICE)
+        self.norms_host = torch.zeros(3, dtype=torch.float64).pin_memory()
+
+
+class SpectralSolverState:
+    """Device state: grid, buffers, modes ``a`` and the E/KK propagators (spectral_state.py:315-365)."""
+
+    def __init__(self, phys, geom, num, fine):
+        if fine is not None:
+            FineMeshState()
+        self.dtype = getattr(num, "dtype", np.float32)
+        self.code = dtype_code(self.dtype)
+        self.xp = None
+        self.grid = SpectralGrid(geom, self.dtype)
+        self.fine_mesh = None
+        self.buffers = SolverBuffers(num, self.dtype)
+        self.K, self.KK = precompute_K_KK(phys, num, geom, self.dtype)
+        self.a = None
+        self.hooks = None
+
+
+def precompute_K_KK(phys, num, geom, dtype=np.float32):
+    """E and KK tables on the device in FP64 math (spectral_state.py:374-421)."""
+    nx, ny, nz = (int(v) for v in geom.n)
+    Lx, Ly, Lz = (float(v) for v in geom.size)
+    td = _tdtype(dtype)
+    K = torch.empty((nz, ny, nx), dtype=td, device=_DEVICE)
+    KK = torch.empty_like(K)
+    alpha = float(phys.k) / (float(phys.rho) * float(phys.Cp))
+    call("sgh_etd_tables", dtype_code(dtype), nx, ny, nz, Lx, Ly, Lz, alpha,
+         float(phys.rho) * float(phys.Cp), float(num.dt), ptr(K), ptr(KK), stream_ptr())
+    return K, KK
+
+
+# ---------------------------------------------------------------------------------
+# Fine-grained kernels (reference names)
+# ---------------------------------------------------------------------------------
+
+
+def _vec(cp):
+    return cp.reshape(-1).contiguous()
+
+
+def update_modes_etd1(aK, KK, Cp_broadcast, B_scaled, a_temp_out):
+    """a_out = aK + KK·Cp[p]·B (spectral_cpu_kernels.py:83-91)."""
+    nz, ny, nx = aK.shape
+    call("sgh_etd1_update", dtype_code(aK.dtype), nx, ny, nz, ptr(aK), ptr(KK),
+         ptr(_vec(Cp_broadcast)), ptr(B_scaled.contiguous()), None, None, None, ptr(a_temp_out),
+         stream_ptr())
+
+
+def add_source_term_modes(a_temp, KK, Q_modes):
+    """a += KK·Q (spectral_cpu_kernels.py:93-103)."""
+    call("sgh_add_volume_source", dtype_code(a_temp.dtype), a_temp.numel(), ptr(a_temp), ptr(KK),
+         ptr(Q_modes.contiguous()), stream_ptr())
+
+
+def add_bottom_surface_source(a_temp, KK, Cp_broadcast_bottom, B_scaled):
+    """a += KK·Cb[p]·B (spectral_cpu_kernels.py:105-113)."""
+    nz, ny, nx = a_temp.shape
+    call("sgh_add_surface_source", dtype_code(a_temp.dtype), nx, ny, nz, ptr(a_temp), ptr(KK),
+         ptr(_vec(Cp_broadcast_bottom)), ptr(B_scaled.contiguous()), stream_ptr())
+
+
+def _recon_face(a, w, SsState, out=None):
+    g, b = SsState.grid, SsState.buffers
+    nz, ny, nx = a.shape
+    code = dtype_code(a.dtype)
+    call("sgh_zcontract", code, nx, ny, nz, ptr(a), ptr(w), b.nchunks, ptr(b.zc), stream_ptr())
+    return _sep2d(b.zc, g.ByT, g.Bx, 1.0, b.nchunks, out=out)
+
+
+def reconstruct_surface_temperature(a, SsState):
+    """Top-surface temperature at z = Lz (spectral_ops.py:210-219)."""
+    return _recon_face(a.contiguous(), SsState.grid.w_top, SsState)
+
+
+def reconstruct_bottom_temperature(a, SsState):
+    """Bottom-surface temperature at z = 0 (spectral_ops.py:222-231)."""
+    return _recon_face(a.contiguous(), SsState.grid.w_bot, SsState)
+
+
+def compute_evaporation_flux(T_surface, q_out, P0, T_boil, DeltaH_LV, R_v, T_liquidus):
+    """Evaporation flux into ``q_out`` (spectral_cpu_kernels.py:144-162)."""
+    scratch = torch.empty_like(q_out)
+    call("sgh_surface_forcing", dtype_code(q_out.dtype), q_out.numel(), ptr(T_surface.contiguous()),
+         None, float(P0), float(T_boil), float(DeltaH_LV), float(R_v), float(T_liquidus), 0.0,
+         0.0, None, 0.0, ptr(q_out), ptr(scratch), stream_ptr())
+
+
+def reconstruct_volume(a, SsState):
+    """T = Σ a·B_z B_y B_x on the cell-centred grid = IDCT(a)/√dV (spectral_ops.py:83-91)."""
+    g = SsState.grid
+    return _apply_axes(a.contiguous(), (g.BxT, g.ByT, g.BzT))
+
+
+def project_volume(field, SsState):
+    """modes = dV·Σ f·B_z B_y B_x = √dV·DCT(f) (spectral_ops.py:94-102)."""
+    g = SsState.grid
+    return _apply_axes(field.contiguous(), (g.Bx, g.By, g.Bz), scale=g.dV)
+
+
+def _latent(Q_buffer, T, T_prev, phys, num):
+    model = getattr(phys, "model", None)
+    prog = _program(model) if (model is not None and not model.is_constant) else None
+    call("sgh_latent_source", dtype_code(Q_buffer.dtype), Q_buffer.numel(), ptr(T.contiguous()),
+         ptr(T_prev.contiguous()), float(phys.T_solidus), float(phys.T_liquidus), float(phys.rho),
+         float(phys.L_f), float(num.dt), ptr(prog), ptr(Q_buffer), stream_ptr())
+
+
+def compute_latent_heat_source_grid(Q_buffer, T_full, T_prev_full, phys, num, SsState):
+    """Grid-mode latent source −ρL(f_l(T) − f_l(T_prev))/dt (spectral_ops.py:270-289)."""
+    _latent(Q_buffer, T_full, T_prev_full, phys, num)
+
+
+def compute_latent_heat_source(Q_buffer, phys, num, SsState):
+    """Box-mode latent source; with no fine box the source is zero (spectral_ops.py:234-267)."""
+    if SsState.fine_mesh is None:
+        Q_buffer.zero_()
+        return
+    raise NotImplementedError("box mode is not implemented in the MI355X backend")
+
+
+def initialize_latent_heat_if_needed(SsState):
+    """No-op in grid mode (spectral_ops.py:182-195)."""
+
+
+def update_latent_heat_history(SsState):
+    """No-op in grid mode (spectral_ops.py:198-207)."""
+
+
+def shift_latent_heat_history(SsState, laser_state, num):
+    """No-op in grid mode (spectral_ops.py:292-306)."""
+
+
+def project_box_to_modes(field_box, SsState):
+    raise RuntimeError("Fine mesh not initialized.")
+
+
+_PROGRAMS: dict = {}
+
+
+def _program(model):
+    """Packed device property program of a MaterialModel (cached per model)."""
+    key = id(model)
+    hit = _PROGRAMS.get(key)
+    if hit is not None and hit[0] is model:
+        return hit[1]
+    blob = model.device_program()
+    t = torch.frombuffer(bytearray(blob), dtype=torch.uint8).to(_DEVICE)
+    _PROGRAMS[key] = (model, t)
+    return t
+
+
+def assemble_property_correction(SsState, a_trial, T_prev_full, dt, model, k_bar, a_bar):
+    """C = project_volume(∇·(k'∇T) − a'∂ₜT) at the trial modes (spectral_ops.py:105-170)."""
+    g = SsState.grid
+    nz, ny, nx = a_trial.shape
+    T = reconstruct_volume(a_trial, SsState)
+    work = torch.empty((3,) + tuple(a_trial.shape), dtype=a_trial.dtype, device=a_trial.device)
+    f = torch.empty_like(T)
+    call("sgh_property_correction_source", dtype_code(a_trial.dtype), nx, ny, nz, ptr(T),
+         ptr(T_prev_full.contiguous()), ptr(_program(model)), float(k_bar), float(a_bar),
+         float(dt), g.dx, g.dy, g.dz, ptr(work), ptr(f), stream_ptr())
+    return project_volume(f, SsState)
+
+
+def shift_flux(field, shift, geom):
+    """Translate a surface flux by ``shift = (Δx, Δy)`` metres (spectral_cpu_kernels.py:392-396)."""
+    dx, dy = shift
+    out = torch.empty_like(field)
+    ny, nx = field.shape
+    call("sgh_shift2d", dtype_code(field.dtype), ny, nx, ptr(field.contiguous()),
+         float(dy) / float(geom.d.y), float(dx) / float(geom.d.x), None, 1.0, ptr(out),
+         stream_ptr())
+    return out
+
+
+def laser_flux(SsState, profile, laser_params, x0, y0, peak, out=None):
+    """Beam flux on the cell centres (laser.py:42-53,65-90,122-135)."""
+    g = SsState.grid
+    if out is None:
+        out = torch.empty((g.ny, g.nx), dtype=_tdtype(SsState.dtype), device=_DEVICE)
+    call("sgh_laser_flux", SsState.code, g.nx, g.ny, g.dx, g.dy, float(x0), float(y0),
+         float(laser_params.r_x), float(laser_params.r_y), float(profile.order), float(peak),
+         int(bool(profile.cell_integrated)), ptr(out), stream_ptr())
+    return out
+
+
+# ---------------------------------------------------------------------------------
+# Fused per-step driver (the plain device form of SpectralSolver.step)
+# ---------------------------------------------------------------------------------
+
+
+def picard_step(solver, t, dt):
+    """One ETD1 step with the relaxed Picard loop, kernel for kernel on the device.
+
+    Follows ``SpectralSolver.step`` (sgh/solvers/spectral.py:219-543): laser flux,
+    decay, warm start (shifted evaporation, latent replay, convection), then per
+    iteration: surface temperature → evaporation/convection flux → projection →
+    [latent and property-correction volume sources] → fused ETD1 + relaxation + FP64
+    norms + next-iterate surface contraction, one host read of the residual ratio.
+    """
+    S = solver.state
+    g, b = S.grid, S.buffers
+    ctx = solver.context
+    mat, num, lp = ctx.mat, ctx.num, ctx.laser
+    code, st = S.code, stream_ptr()
+    nx, ny, nz = g.nx, g.ny, g.nz
+    Ssz, N = nx * ny, nx * ny * nz
+    dxdy = g.dx * g.dy
+
+    ls = ctx.laser_path.get_state(t, dt)
+    vx, vy = ls.v
+    peak = (solver._profile.peak_intensity(float(ls.power), float(lp.absorptivity),
+                                           float(lp.ref_area)) if ls.is_on else 0.0)
+    laser_flux(S, solver._profile, lp, ls.x, ls.y, peak, out=b.q_las)
+    call("sgh_reduce", code, Ssz, ptr(b.q_las), 0, ptr(b.scal), st)
+
+    call("sgh_mul_inplace", code, N, ptr(S.a), ptr(S.K), st)
+    if solver.track_picard_history:
+        solver.picard_history = []
+
+    # warm start: S_top = dx·dy·Bᵧ(q_las − shift(q_evap_old))Bₓᵀ
+    call("sgh_shift2d", code, ny, nx, ptr(b.q_evap_old), float(vy) * dt / g.dy,
+         float(vx) * dt / g.dx, ptr(b.q_las), -1.0, ptr(b.q_tmp), st)
+    _sep2d(b.q_tmp, g.By, g.BxT, dxdy, out=b.S_top)
+    a_cur = b.a_temp
+    replay = solver._grid_latent and solver._Q_latent_modes_prev is not None
+    call("sgh_etd1_update", code, nx, ny, nz, ptr(S.a), ptr(S.KK), ptr(g.w_top), ptr(b.S_top),
+         None, None, ptr(solver._Q_latent_modes_prev) if replay else None, ptr(a_cur), st)
+    h_top, h_bot, T0 = float(mat.h_conv_top), float(mat.h_conv_bottom), float(mat.T0)
+    if h_top > 0:
+        _recon_face(a_cur, g.w_top, S, out=b.T_s)
+        call("sgh_convection_flux", code, Ssz, ptr(b.T_s), h_top, T0, None, 0.0, ptr(b.q_tmp), st)
+        _sep2d(b.q_tmp, g.By, g.BxT, dxdy, out=b.S_tmp)
+        call("sgh_add_surface_source", code, nx, ny, nz, ptr(a_cur), ptr(S.KK), ptr(g.w_top),
+             ptr(b.S_tmp), st)
+    if h_bot > 0:
+        _recon_face(a_cur, g.w_bot, S, out=b.T_s)
+        call("sgh_convection_flux", code, Ssz, ptr(b.T_s), h_bot, T0, None, 0.0, ptr(b.q_tmp), st)
+        _sep2d(b.q_tmp, g.By, g.BxT, dxdy, out=b.S_bot)
+        call("sgh_add_surface_source", code, nx, ny, nz, ptr(a_cur), ptr(S.KK), ptr(g.w_bot),
+             ptr(b.S_bot), st)
+
+    rescale = solver._property_correction
+    prog = _program(mat.model) if rescale else None
+    kbar = float(solver._kbar) if rescale else 0.0
+    have_bot = h_bot > 0
+    if have_bot:
+        _recon_face(a_cur, g.w_bot, S, out=b.T_s)
+        call("sgh_convection_flux", code, Ssz, ptr(b.T_s), h_bot, T0, ptr(prog), kbar,
+             ptr(b.q_tmp), st)
+        _sep2d(b.q_tmp, g.By, g.BxT, dxdy, out=b.S_bot_raw)
+
+    call("sgh_zcontract", code, nx, ny, nz, ptr(a_cur), ptr(g.w_top), b.nchunks, ptr(b.zc), st)
+    omega = float(solver.mixing_omega)
+    tol = float(solver.convergence_tol)
+    iter_k, rel, conv = -1, float("nan"), False
+    Q_modes = None
+    a_raw = None
+    for iter_k in range(solver.max_picard_iter):
+        a_old = a_cur
+        a_new = b.a_alt if a_old is b.a_temp else b.a_temp
+        _sep2d(b.zc, g.ByT, g.Bx, 1.0, b.nchunks, out=b.T_s)
+        call("sgh_surface_forcing", code, Ssz, ptr(b.T_s), ptr(b.q_las), float(mat.Pa),
+             float(mat.T_boil), float(mat.DeltaH_LV), float(mat.R_v), float(mat.T_liquidus),
+             h_top, T0, ptr(prog), kbar, ptr(b.q_evap_buffer), ptr(b.q_tmp), st)
+        _sep2d(b.q_tmp, g.By, g.BxT, dxdy, out=b.S_top)
+        Q_modes = None
+        if solver._grid_latent:
+            T_full = reconstruct_volume(a_old, S)
+            _latent(b.Q_latent_buffer, T_full, solver._T_prev_full, mat, num)
+            Q_modes = project_volume(b.Q_latent_buffer, S)
+        wb = ptr(g.w_bot) if have_bot else None
+        Sb = ptr(b.S_bot_raw) if have_bot else None
+        if rescale:
+            if a_raw is None:
+                a_raw = torch.empty_like(a_cur)
+            call("sgh_etd1_update", code, nx, ny, nz, ptr(S.a), ptr(S.KK), ptr(g.w_top),
+                 ptr(b.S_top), wb, Sb, ptr(Q_modes), ptr(a_raw), st)
+            C = assemble_property_correction(S, a_raw, solver._T_prev_full, num.dt, mat.model,
+                                             solver._kbar, solver._abar)
+            call("sgh_picard_iter_fused", code, nx, ny, nz, ptr(a_raw), ptr(S.KK), ptr(a_old),
+                 ptr(g.w_top), ptr(b.S_zero), None, None, ptr(C), omega, ptr(g.w_top),
+                 b.nchunks, ptr(b.zc), ptr(a_new), ptr(b.partials), st)
+        else:
+            call("sgh_picard_iter_fused", code, nx, ny, nz, ptr(S.a), ptr(S.KK), ptr(a_old),
+                 ptr(g.w_top), ptr(b.S_top), wb, Sb, ptr(Q_modes), omega, ptr(g.w_top),
+                 b.nchunks, ptr(b.zc), ptr(a_new), ptr(b.partials), st)
+        call("sgh_reduce_norms", ptr(b.partials), b.n_pairs, float(N), ptr(b.norms), st)
+        a_cur = a_new
+        b.norms_host.copy_(b.norms, non_blocking=True)
+        torch.cuda.current_stream().synchronize()
+        rms_diff, _, rel = (float(v) for v in b.norms_host)
+        if solver.track_picard_history:
+            rho_k = None
+            if iter_k > 0 and solver.picard_history:
+                prev = solver.picard_history[-1]["rms_diff"]
+                if prev > 0:
+                    rho_k = rms_diff / prev
+            solver.picard_history.append({"iter": iter_k, "rms_diff": rms_diff, "rho": rho_k,
+                                          "true_rel_err": rel})
+        if rel < tol:
+            conv = True
+            break
+    else:
+        solver._note_cap(t, rel)
+
+    T_temp = _recon_face(a_cur, g.w_top, S, out=b.T_s)
+    call("sgh_reduce", code, Ssz, ptr(T_temp), 1, ptr(b.scal[1:]), st)
+    if solver._grid_latent:
+        solver._Q_latent_modes_prev = Q_modes
+    b.q_evap_old.copy_(b.q_evap_buffer)
+    S.a.copy_(a_cur)
+    if solver._property_correction or solver._grid_latent:
+        solver._T_prev_full.copy_(reconstruct_volume(S.a, S))
+    metrics = {
+        "T_surface_max": b.scal[1].clone(),
+        "P_laser": b.scal[0] * dxdy,
+        "n_evap_iter": iter_k + 1,
+        "picard_converged": conv,
+        "picard_rel_err": rel,
+    }
+    return S, metrics

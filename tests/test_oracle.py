@@ -1,0 +1,338 @@
+"""Pin the FP64 oracle before it is trusted as the parity target.
+
+Every check compares the oracle with something it does not restate: the reference's
+own known-answer tests (values and tolerances quoted with their source), closed forms
+(orthonormality, φ₁ identity, energy conservation, the Eagar–Tsai solution), brute-force
+sums on tiny grids, or an independent library routine (scipy.ndimage.shift).
+"""
+
+import copy
+import math
+
+import numpy as np
+import pytest
+import scipy.ndimage
+
+from oracle import sgh_oracle as o
+from spectral_galerkin_heat_amd.core.laser import ConstantVelocityLaser, FixedLaser
+from spectral_galerkin_heat_amd.workloads import MATERIAL_316L, make_context
+
+
+# --- basis and transforms -------------------------------------------------------------
+
+def test_cell_centred_basis_is_orthonormal():
+    # Σ_i C_k cos(πk x_i/L) C_l cos(πl x_i/L) dx = δ_kl on cell centres x_i = (i+½)dx:
+    # fixes both the C norms (spectral_helpers.py:26-46) and the cell-centre convention.
+    n, L = 12, 3.7e-4
+    x = (np.arange(n) + 0.5) * (L / n)
+    B = o.c_coef(n, L)[:, None] * np.cos(np.pi * np.arange(n)[:, None] * x[None, :] / L)
+    np.testing.assert_allclose(B @ B.T * (L / n), np.eye(n), atol=1e-13)
+
+
+def test_dct_matches_bruteforce_cosine_sums():
+    # Ortho DCT-II written out as explicit sums, on an asymmetric 3-D grid.
+    rng = np.random.default_rng(3)
+    nz, ny, nx = 3, 4, 5
+    f = rng.standard_normal((nz, ny, nx))
+
+    def M(n):
+        k, j = np.arange(n)[:, None], np.arange(n)[None, :]
+        s = np.where(k == 0, math.sqrt(1.0 / n), math.sqrt(2.0 / n))
+        return s * np.cos(np.pi * k * (2 * j + 1) / (2 * n))
+
+    ref = np.zeros_like(f)
+    Mz, My, Mx = M(nz), M(ny), M(nx)
+    for p in range(nz):
+        for q in range(ny):
+            for r in range(nx):
+                ref[p, q, r] = sum(Mz[p, a] * My[q, b] * Mx[r, c] * f[a, b, c]
+                                   for a in range(nz) for b in range(ny) for c in range(nx))
+    np.testing.assert_allclose(o.dct2(f), ref, atol=1e-13)
+    np.testing.assert_allclose(o.idct2(o.dct2(f)), f, atol=1e-13)
+
+
+def test_dct_of_constant_is_single_impulse():
+    # tests/unit_tests/physics/test_spectral_cpu_kernels.py:24-32: DCT(1) = n^{3/2} δ.
+    out = o.dct2(np.ones((4, 4, 4)))
+    assert out[0, 0, 0] == pytest.approx(4 ** 1.5, rel=1e-12)
+    out[0, 0, 0] = 0.0
+    assert np.abs(out).max() < 1e-12
+
+
+def test_reconstruct_volume_matches_modal_sum():
+    # tests/unit_tests/physics/test_property_transforms.py:47-66.
+    g = o.OracleGrid((1.0e-3, 0.7e-3, 0.4e-3), (6, 5, 4))
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal((4, 5, 6))
+    T = o.reconstruct_volume(a, g)
+    cx, cy, cz = [g.C[i][:, None] * np.cos(np.pi * np.arange(n)[:, None] * c[None, :] / L)
+                  for i, (n, c, L) in enumerate(((6, g.x, g.Lx), (5, g.y, g.Ly), (4, g.z, g.Lz)))]
+    T_ref = np.einsum("pnm,pk,nj,mi->kji", a, cz, cy, cx)
+    np.testing.assert_allclose(T, T_ref, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(o.project_volume(T, g), a, atol=1e-12)
+
+
+def test_node_reconstruction_matches_direct_modal_sum():
+    # reconstruct_temperature_DCT == reconstruct_temperature_volume (spectral_helpers.py:100-145).
+    g = o.OracleGrid((1.0e-3, 0.6e-3, 0.3e-3), (6, 5, 4))
+    rng = np.random.default_rng(2)
+    a = rng.standard_normal((4, 5, 6))
+    T = o.reconstruct_temperature_DCT(a, g)
+    nodes = [np.arange(n + 1) * (L / n) for n, L in ((6, g.Lx), (5, g.Ly), (4, g.Lz))]
+    Bx, By, Bz = [g.C[i][:, None] * np.cos(np.pi * np.arange(n)[:, None] * c[None, :] / L)
+                  for i, (n, c, L) in enumerate(((6, nodes[0], g.Lx), (5, nodes[1], g.Ly),
+                                                  (4, nodes[2], g.Lz)))]
+    T_ref = np.einsum("pnm,mi,nj,pk->ijk", a, Bx, By, Bz)
+    np.testing.assert_allclose(T, T_ref, atol=1e-11)
+
+
+# --- propagators ---------------------------------------------------------------------
+
+def _tables(Lx=4e-3, Lz=1e-3):
+    return o.precompute_K_KK(20.0, 7900.0, 500.0, 1e-6, (4, 6, 8), (Lz, 1e-3, Lx))
+
+
+def test_K_KK_zero_mode_and_decay():
+    # tests/unit_tests/physics/test_spectral_state.py:35-65.
+    K, KK = _tables()
+    alpha = 20.0 / (7900.0 * 500.0)
+    assert K[0, 0, 0] == 1.0
+    assert KK[0, 0, 0] == pytest.approx(1e-6 / (7900.0 * 500.0), rel=1e-15)
+    assert K[1, 0, 0] == pytest.approx(math.exp(-alpha * (math.pi / 1e-3) ** 2 * 1e-6), rel=1e-14)
+    assert K[0, 0, 1] == pytest.approx(math.exp(-alpha * (math.pi / 4e-3) ** 2 * 1e-6), rel=1e-14)
+    assert K.shape == (4, 6, 8) and K[1, 0, 0] != K[0, 0, 1]
+
+
+def test_KK_is_phi1_over_rhocp():
+    # KK = Δt·φ₁(z)/(ρc) with φ₁(z) = expm1(z)/z, z = −αλΔt, evaluated independently.
+    K, KK = _tables()
+    alpha, dt, rc = 20.0 / (7900.0 * 500.0), 1e-6, 7900.0 * 500.0
+    kz, ky, kx = [np.pi * np.arange(n) / L for n, L in ((4, 1e-3), (6, 1e-3), (8, 4e-3))]
+    lam = kz[:, None, None] ** 2 + ky[None, :, None] ** 2 + kx[None, None, :] ** 2
+    z = -alpha * lam * dt
+    with np.errstate(invalid="ignore", divide="ignore"):
+        phi1 = np.where(lam == 0, 1.0, np.expm1(z) / z)
+    np.testing.assert_allclose(KK, dt * phi1 / rc, rtol=1e-9)
+
+
+# --- pointwise kernels (hand-computed, tests/unit_tests/physics/test_spectral_cpu_kernels.py) --
+
+def test_update_and_accumulators_hand_values():
+    aK = np.ones((2, 2, 2))
+    KK = np.ones((2, 2, 2))
+    out = np.empty((2, 2, 2))
+    o.update_modes_etd1(aK, KK, np.array([2.0, 3.0]).reshape(2, 1, 1), np.full((2, 2), 5.0), out)
+    assert np.all(out[0] == 11.0) and np.all(out[1] == 16.0)
+    a = np.ones((2, 2, 2))
+    o.add_source_term_modes(a, np.full((2, 2, 2), 2.0), np.full((2, 2, 2), 3.0))
+    assert np.all(a == 7.0)
+    a = np.ones((2, 2, 2))
+    o.add_bottom_surface_source(a, KK, np.array([2.0, 4.0]).reshape(2, 1, 1), np.full((2, 2), 5.0))
+    assert np.all(a[0] == 11.0) and np.all(a[1] == 21.0)
+
+
+def test_latent_source_band_sign_magnitude_clamp():
+    T_S, T_L, rho, L, dt = 1700.0, 1800.0, 7900.0, 2.5e5, 1e-6
+    Tc = np.array([[[1600.0, 1750.0, 1900.0]]])
+    Tp = np.array([[[1600.0, 1700.0, 1900.0]]])
+    out = o.latent_source(Tc, Tp, T_S, T_L, rho, L, dt)
+    assert out[0, 0, 0] == 0.0 and out[0, 0, 2] == 0.0
+    assert out[0, 0, 1] == pytest.approx(-rho * L / ((T_L - T_S) * dt) * 50.0, rel=1e-12)
+    out = o.latent_source(np.array([1750.0]), np.array([1000.0]), T_S, T_L, rho, L, dt)
+    assert out[0] == pytest.approx(-rho * L / ((T_L - T_S) * dt) * 50.0, rel=1e-12)
+
+
+def test_evaporation_equals_hertz_knudsen_form():
+    # 0.82·L_v·p_sat(T)·sqrt(M/(2πRT)), p_sat = Pa·exp(L_v·M/R·(1/T_b − 1/T)), R_v = R/M:
+    # an algebraically different statement of the same law; gate at T_liq.
+    R, M, Lv, Pa, Tb = 8.314462618, 0.0558, 7.45e6, 101325.0, 3090.0
+    T = np.array([[1500.0, 2000.0, 2600.0, 3500.0]])
+    q = o.evaporation_flux(T, Pa, Tb, Lv, R / M, 1800.0)
+    psat = Pa * np.exp(Lv * M / R * (1.0 / Tb - 1.0 / T))
+    ref = 0.82 * Lv * psat * np.sqrt(M / (2 * np.pi * R * T))
+    assert q[0, 0] == 0.0
+    np.testing.assert_allclose(q[0, 1:], ref[0, 1:], rtol=1e-12)
+    assert np.all(np.diff(q[0, 1:]) > 0)
+
+
+def test_gaussian_flux_peak_and_integral():
+    # tests/unit_tests/physics/test_spectral_cpu_kernels.py:73-84.
+    nx, dx = 200, 5e-6
+    c = (np.arange(nx) + 0.5) * dx
+    q = o.super_gaussian_flux(c, c, c[nx // 2], c[nx // 2], 1e-4, 1e-4, 2.0, 7.0)
+    assert q.max() == pytest.approx(7.0, rel=1e-12)
+    assert q.sum() * dx * dx == pytest.approx(7.0 * np.pi * 1e-8 / 2, rel=0.02)
+
+
+def test_cell_integrated_flux_deposits_full_power():
+    # A beam narrower than a cell: the erf cell means still integrate to I₀·πr²/2.
+    n, h, r = 40, 50e-6, 10e-6
+    c = (np.arange(n) + 0.5) * h
+    I0 = 3.0
+    q = o.gaussian_cell_integrated_flux(c, c, 1.013e-3, 0.987e-3, r, r, h, h, I0)
+    assert q.sum() * h * h == pytest.approx(I0 * np.pi * r * r / 2, rel=1e-10)
+    assert o.super_gaussian_area_factor(2.0) == pytest.approx(np.pi / 2, rel=1e-14)
+
+
+def test_shift_matches_scipy_ndimage_and_half_pixel_ramp():
+    rng = np.random.default_rng(5)
+    f = rng.standard_normal((9, 13))
+    for s in [(0.3, -0.7), (1.5, 2.25), (-0.16, 0.0), (0.0, 0.5), (3.0, -2.0)]:
+        np.testing.assert_allclose(o.shift_2d(f, *s),
+                                   scipy.ndimage.shift(f, s, order=1, mode="constant", cval=0.0),
+                                   atol=1e-14)
+    ramp = np.tile(np.arange(8.0), (8, 1))
+    out = o.shift_flux(ramp, (0.5, 0.0), 1.0, 1.0)   # test_spectral_cpu_kernels.py:86-94
+    assert out[4, 4] == pytest.approx(3.5) and out[4, 5] == pytest.approx(4.5)
+
+
+def test_correction_source_closed_forms():
+    # Constant k' on T = x² + 2y² + 3z²: interior ∇·(k'∇T) = k'·(2 + 4 + 6) exactly for the
+    # central stencil; k' = a' = 0 gives an exact zero (test_property_transforms.py:151-165).
+    n, d = 9, 0.1
+    z, y, x = np.meshgrid(*(np.arange(n) * d,) * 3, indexing="ij")
+    T = x ** 2 + 2 * y ** 2 + 3 * z ** 2
+    f = o.correction_source(T, np.full_like(T, 1.5), np.zeros_like(T), T, 1.0, d, d, d)
+    np.testing.assert_allclose(f[2:-2, 2:-2, 2:-2], 1.5 * 12.0, rtol=1e-10)
+    f0 = o.correction_source(T, np.zeros_like(T), np.zeros_like(T), T - 1, 1.0, d, d, d)
+    assert np.abs(f0).max() == 0.0
+    fa = o.correction_source(T, np.zeros_like(T), np.full_like(T, 2.0), T - 3.0, 0.5, d, d, d)
+    np.testing.assert_allclose(fa, -2.0 * 3.0 / 0.5)
+
+
+def test_property_expression_evaluator():
+    props = o.OracleProps("9.248 + 0.01571 * T", "12.41 + 0.003279 * T", "7900", "7900",
+                          "458.98 + 0.1328 * T", "769.86", 1674.15, 1697.15)
+    T = np.array([300.0, 1685.65, 2000.0])
+    np.testing.assert_allclose(props.k(T)[[0, 2]], [9.248 + 0.01571 * 300, 12.41 + 0.003279 * 2000])
+    mid = 0.5 * (9.248 + 0.01571 * 1685.65) + 0.5 * (12.41 + 0.003279 * 1685.65)
+    assert props.k(T)[1] == pytest.approx(mid, rel=1e-14)
+    assert o.eval_expr("0.5*(T + 30 - fabs(T - 30))", np.array([10.0, 50.0])).tolist() == [10.0, 30.0]
+
+
+# --- whole steps ----------------------------------------------------------------------
+
+_TINY = {
+    "simulation": {"backend": "mi355x", "duration": 6e-6, "dt": 6e-6, "dtype": "float64"},
+    "domain": {"size": [4e-4, 4e-4, 1e-4], "mesh": [32, 32, 16]},
+    "material": {"name": "316L", "rho": 7850.0, "k": 15.0, "Cp": 500.0, "L_f": 267700.0,
+                 "T_solidus": 1700.0, "T_liquidus": 1800.0, "T0": 293.0, "DeltaH_LV": 7.41e6,
+                 "R_v": 150.774, "Pa": 101325.0, "T_boil": 3090.0},
+    "laser": {"radius": 60.0e-6, "absorptivity": 0.3, "power_nominal": 200.0},
+    "io": {},
+}
+
+
+def _tiny(power=200.0, is_on=True, **mat):
+    cfg = copy.deepcopy(_TINY)
+    cfg["material"].update(mat)
+    return make_context(cfg, FixedLaser(2e-4, 2e-4, power, is_on))
+
+
+def test_initial_condition_reconstructs_T0():
+    s = o.OracleSolver(_tiny()).initialize()
+    assert np.count_nonzero(s.a) == 1
+    np.testing.assert_allclose(s.surface_T(), 293.0, rtol=1e-12)
+    s.set_state(np.full((16, 32, 32), 500.0))
+    np.testing.assert_allclose(s.surface_T(), 500.0, rtol=1e-12)
+
+
+def test_laser_off_stays_ambient_and_power_is_AP():
+    s = o.OracleSolver(_tiny(is_on=False)).initialize()
+    _, m = s.step(0.0, 6e-6)
+    assert m["P_laser"] == 0.0
+    np.testing.assert_allclose(s.volume_T(), 293.0, rtol=1e-12)
+    s = o.OracleSolver(_tiny()).initialize()
+    _, m = s.step(0.0, 6e-6)
+    assert m["P_laser"] == pytest.approx(0.3 * 200.0, rel=0.1)   # test_spectral.py:92-107
+    assert set(m) == {"T_surface_max", "P_laser", "n_evap_iter", "picard_converged",
+                      "picard_rel_err"}
+
+
+def test_energy_conservation_without_evaporation():
+    # Pa = 0 disables evaporation; the mean mode then gains exactly Δt·P/(ρc√V) per step
+    # (E₀₀₀ = 1, KK₀₀₀ = Δt/ρc), i.e. the mean temperature rises by the deposited energy.
+    cfg = copy.deepcopy(_TINY)
+    cfg["material"].update({"Pa": 0.0, "L_f": 0.0})
+    cfg["simulation"].update({"duration": 6e-5, "dt": 6e-6})
+    ctx = make_context(cfg, ConstantVelocityLaser(1e-4, 2e-4, 0.5, 0.0, 150.0))
+    s = o.OracleSolver(ctx).initialize()
+    V = 4e-4 * 4e-4 * 1e-4
+    a0 = s.a[0, 0, 0]
+    E = 0.0
+    for n in range(10):
+        _, m = s.step(n * 6e-6, 6e-6)
+        E += m["P_laser"] * 6e-6
+        assert m["n_evap_iter"] == 1 and m["picard_converged"]
+    assert (s.a[0, 0, 0] - a0) * math.sqrt(V) * 7850.0 * 500.0 == pytest.approx(E, rel=1e-12)
+
+
+def test_nonlinear_step_reduces_to_linear_solver_without_evaporation():
+    cfg = copy.deepcopy(_TINY)
+    cfg["material"].update({"Pa": 0.0, "L_f": 0.0})
+    cfg["simulation"].update({"duration": 3e-5, "dt": 6e-6})
+    lp = ConstantVelocityLaser(1e-4, 2e-4, 0.5, 0.0, 150.0)
+    s = o.OracleSolver(make_context(cfg, lp)).initialize()
+    lin = o.OracleLinearSolver(make_context(cfg, lp)).initialize()
+    for n in range(5):
+        s.step(n * 6e-6, 6e-6)
+        lin.step(n * 6e-6, 6e-6)
+    np.testing.assert_allclose(s.a, lin.a, rtol=1e-13, atol=1e-9)
+
+
+def eagar_tsai_field(rho, k, Cp, T0, A, P, r_b, Lx, Ly, Lz, nx, ny, nz, x_end, y_laser, v,
+                     t_total, n_tau=400, tau_min=1e-10):
+    """Moving Gaussian source on a semi-infinite body with the five insulating images,
+    node-centred [z, y, x] (the closed form of tests/integration/eagar_tsai.py:38-180)."""
+    alpha = k / (rho * Cp)
+    tau = np.geomspace(tau_min, t_total, n_tau)
+    x, y, z = (np.linspace(0.0, L, n + 1) for L, n in ((Lx, nx), (Ly, ny), (Lz, nz)))
+    xi, eta, depth = x - x_end, y - y_laser, Lz - z
+    s2 = r_b ** 2 / 2.0
+    C = A * P / (np.pi * rho * Cp * np.sqrt(np.pi * alpha))
+    T = np.zeros((nz + 1, ny + 1, nx + 1))
+    for i in range(n_tau - 1):
+        t = 0.5 * (tau[i] + tau[i + 1])
+        w = tau[i + 1] - tau[i]
+        dxy, dz = 4 * alpha * t + s2, 4 * alpha * t
+        ex = lambda u: np.exp(-u * u / dxy)
+        fx = [ex(xi + v * t), ex(xi + 2 * x_end - v * t), ex(xi + 2 * x_end - 2 * Lx - v * t)]
+        fy = [ex(eta), ex(eta + 2 * y_laser), ex(eta - 2 * (Ly - y_laser))]
+        primary = np.outer(fy[0], fx[0])
+        side_images = (np.outer(fy[0], fx[1]) + np.outer(fy[0], fx[2])
+                       + np.outer(fy[1], fx[0]) + np.outer(fy[2], fx[0]))
+        ez = np.exp(-depth ** 2 / dz)
+        ez_bottom_image = np.exp(-(2 * Lz - depth) ** 2 / dz)
+        T += (w / (np.sqrt(t) * dxy)) * (ez[:, None, None] * (primary + side_images)[None]
+                                          + ez_bottom_image[:, None, None] * primary[None])
+    return T * C + T0
+
+
+@pytest.mark.slow
+def test_linear_step_matches_eagar_tsai():
+    # tests/integration/test_validation.py:76-120: 128×64×160 modes, 100 steps, gate
+    # L2_rel < 0.2 %, Linf/peak < 10 %, peak within 2 cells; ETD1 time-centring dt/2.
+    rho, k, cp, T0 = 7958.0, 13.85, 498.0, 293.0
+    Lx, Ly, Lz = 1.0e-3, 0.5e-3, 0.25e-3
+    A, P, rb, v, ttot = 0.30, 24.0, 30e-6, 0.15, 1.0e-3
+    nx, ny, nz, ns = 128, 64, 160, 100
+    mat = dict(MATERIAL_316L, rho=rho, k=k, Cp=cp, T0=T0)
+    cfg = {"simulation": {"dt": ttot / ns, "duration": ttot, "dtype": "float64"},
+           "domain": {"size": [Lx, Ly, Lz], "mesh": [nx, ny, nz]}, "material": mat,
+           "laser": {"radius": rb, "absorptivity": A, "power_nominal": P}, "io": {}}
+    ctx = make_context(cfg, ConstantVelocityLaser(Lx / 4, Ly / 2, v, 0.0, P))
+    s = o.OracleLinearSolver(ctx).initialize()
+    dt = ctx.num.dt
+    for n in range(ns):
+        s.step(n * dt, dt)
+    T = np.ascontiguousarray(o.reconstruct_temperature_DCT(s.a, s.grid).transpose(2, 1, 0))
+    T_ref = eagar_tsai_field(rho, k, cp, T0, A, P, rb, Lx, Ly, Lz, nx, ny, nz,
+                             Lx / 4 + v * ttot - 0.5 * v * dt, Ly / 2, v, ttot)
+    x, y, z = (np.linspace(0.0, L, n + 1) for L, n in ((Lx, nx), (Ly, ny), (Lz, nz)))
+    res = o.compute_L2_structured(T_ref, T, x, y, z)
+    peak = float(T.max() - T0)
+    assert res["L2_rel"] < 0.002, res
+    assert res["Linf"] / peak < 0.10
+    pk = np.unravel_index(int(np.argmax(T)), T.shape)
+    pr = np.unravel_index(int(np.argmax(T_ref)), T_ref.shape)
+    assert max(abs(a - b) for a, b in zip(pk, pr)) <= 2
