@@ -1,0 +1,247 @@
+"""Benchmark: mode-steps/s of the batched mixed-precision engine vs the plain FP64 GPU path.
+
+Workload: 64 independent cuboids, seeded (numpy default_rng(0)); per-axis mode counts from
+{32,48,64,96,128} (z from {16,24,32}); extents 0.5-4 mm (x, y), 0.25-1 mm (z); P 100-400 W,
+radius 35-70 µm, speed 0.2-1.5 m/s, one track at a random y in the middle 60 %; 316L-like
+constants, evaporation only; Δt = 5e-6 s; Picard tol 1e-8, cap 20, ω = 0.1.
+
+A step advances all 64 cuboids by one ETD1 step (with its Picard loop). ``value`` is
+Σ_i N_i × steps / seconds of the mixed-precision batched engine with its state resident in
+HBM, timed between device synchronisations. Printed alongside (in ``config``): the plain
+FP64 GPU path (SpectralSolver on the mi355x backend, cuboid by cuboid, one residual read
+per Picard iteration), the FP64 batched engine, the relative L2 of the mixed result against
+the FP64 batched result (itself pinned to the FP64 oracle at 1e-9 by the parity tests),
+a roofline entry for the dominant kernel and the FP64 NumPy oracle timed on host cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "mode-steps/s vs plain FP64 GPU at ≤1e-4 rel L2; p50/p99 solve latency"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-cuboids", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    return ap.parse_args()
+
+
+def surface_flops_per_launch(ctxs):
+    """Algorithmic flops of one batched 2-D transform launch: Σ 2·S·(nx + ny)."""
+    tot = 0
+    for c in ctxs:
+        nx, ny = int(c.geom.n.x), int(c.geom.n.y)
+        tot += 2 * nx * ny * (nx + ny)
+    return tot
+
+
+def volume_bytes_per_launch(ctxs, word):
+    """Algorithmic HBM bytes of one volume pass: read + write the state, plus the
+    surface reads/writes (G in, ZE and U out, q_evap copied)."""
+    tot = 0
+    for c in ctxs:
+        nx, ny, nz = int(c.geom.n.x), int(c.geom.n.y), int(c.geom.n.z)
+        tot += 2 * word * nx * ny * nz + 5 * word * nx * ny
+    return tot
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (see DESIGN.md, "
+                          "'More than one GPU')"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from spectral_galerkin_heat_amd.engine.batch import BatchEngine
+    from spectral_galerkin_heat_amd.workloads import batch_uneven_cuboids, make_context
+
+    assert torch.cuda.is_available(), "bench.py needs the MI355X"
+    W, K = args.warmup, args.steps
+    total_steps = W + K
+    cases = batch_uneven_cuboids(n=64, seed=0, n_steps=total_steps)
+    ctxs = [make_context(c, lp) for c, lp in cases]
+    N_total = sum(int(np.prod(tuple(c.geom.n))) for c in ctxs)
+    log(f"workload: 64 cuboids, sum N = {N_total}, warmup {W}, timed {K}")
+
+    def timed_engine(precision):
+        eng = BatchEngine(ctxs, precision=precision)
+        eng.run(W)
+        eng.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eng.run(K)
+        eng.synchronize()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        return eng, dt
+
+    eng_mx, t_mx = timed_engine("mixed")
+    value = N_total * K / t_mx
+    log(f"mixed batched: {t_mx * 1e3:.1f} ms for {K} steps -> {value:.3e} mode-steps/s")
+    met = eng_mx.metrics()
+    n_it_mean = float(met[W:, :, 2].mean())
+
+    eng_64, t_64 = timed_engine("fp64")
+    fp64_rate = N_total * K / t_64
+    log(f"fp64 batched: {t_64 * 1e3:.1f} ms -> {fp64_rate:.3e} mode-steps/s")
+    num, den = 0.0, 0.0
+    rel_max = 0.0
+    for i in range(64):
+        a = eng_mx.state(i).double()
+        b = eng_64.state(i)
+        d2 = float(torch.sum((a - b) ** 2))
+        n2 = float(torch.sum(b ** 2))
+        num += d2
+        den += n2
+        rel_max = max(rel_max, (d2 / n2) ** 0.5)
+    rel_batch = (num / den) ** 0.5
+    log(f"mixed vs fp64: batch rel-L2 {rel_batch:.2e}, worst cuboid {rel_max:.2e}")
+    eng_64.close()
+
+    # live per-kernel timing (HIP events on the engine stream) over an instrumented segment
+    instr = min(K, 40)
+    eng_mx.set_timing(True)
+    eng_mx.run(instr, use_graph=False)
+    eng_mx.synchronize()
+    tm = eng_mx.timing()
+    eng_mx.set_timing(False)
+    surf_ms = tm["surface_ms"] / max(tm["surface_launches"], 1)
+    vol_ms = tm["volume_ms"] / max(tm["volume_launches"], 1)
+    step_ms = t_mx / K * 1e3
+    share_surf = tm["surface_ms"] / instr / step_ms
+    share_vol = tm["volume_ms"] / instr / step_ms
+    log(f"kernel avg: surface {surf_ms * 1e3:.1f} us ({share_surf:.0%} of step), "
+        f"volume {vol_ms * 1e3:.1f} us ({share_vol:.0%})")
+    flops = surface_flops_per_launch(ctxs)
+    vbytes = volume_bytes_per_launch(ctxs, 4)
+    if share_surf >= share_vol:
+        achieved = flops / (surf_ms * 1e-3) / 1e12
+        roofline = {"kernel": "k_b_surface (batched 2-D cosine transform + Picard epilogue)",
+                    "bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
+                    "traffic": None, "avg_launch_us": round(surf_ms * 1e3, 2),
+                    "algorithmic_flops_per_launch": flops}
+    else:
+        achieved = vbytes / (vol_ms * 1e-3) / 1e9
+        roofline = {"kernel": "k_b_volume (ETD1 state pass)", "bound": "hbm",
+                    "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "avg_launch_us": round(vol_ms * 1e3, 2),
+                    "algorithmic_bytes_per_launch": vbytes}
+    vol_gbs = vbytes / (vol_ms * 1e-3) / 1e9
+    traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            tr = json.load(f)
+        roofline["traffic"] = tr.get(roofline["kernel"].split()[0])
+    eng_mx.close()
+
+    # plain FP64 GPU path: SpectralSolver per cuboid
+    plain_rate = None
+    if not args.no_plain:
+        from spectral_galerkin_heat_amd.backends import get_backend
+        from spectral_galerkin_heat_amd.solvers.spectral import SpectralSolver
+        backend = get_backend("mi355x")
+        solvers = []
+        for c, lp in cases:
+            s = SpectralSolver(backend, make_context(c, lp))
+            s.initialize()
+            solvers.append(s)
+        for s in solvers:
+            s.step(0.0, s.context.num.dt)
+        torch.cuda.synchronize()
+        P = args.plain_steps
+        t0 = time.perf_counter()
+        for s in solvers:
+            dt = s.context.num.dt
+            for n in range(1, 1 + P):
+                s.step(n * dt, dt)
+        torch.cuda.synchronize()
+        t_plain = time.perf_counter() - t0
+        plain_rate = N_total * P / t_plain
+        log(f"plain fp64 path: {t_plain:.2f} s for {P} steps x 64 cuboids -> {plain_rate:.3e}")
+        del solvers
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle.sgh_oracle import OracleSolver
+        cores = len(os.sched_getaffinity(0))
+        nc, ns = args.cpu_cuboids, args.cpu_steps
+        refs = [OracleSolver(make_context(c, lp)).initialize() for c, lp in cases[:nc]]
+        t0 = time.perf_counter()
+        for r in refs:
+            dt = r.ctx.num.dt
+            for n in range(ns):
+                r.step(n * dt, dt)
+        t_cpu = time.perf_counter() - t0
+        n_sample = sum(int(r.a.size) for r in refs)
+        cpu = {"value": round(n_sample * ns / t_cpu, 1), "unit": "mode-steps/s", "cores": cores,
+               "kind": "port",
+               "sample": f"FP64 NumPy/SciPy oracle (restated reference step, no Numba), first "
+                         f"{nc} of the 64 cuboids x {ns} steps ({t_cpu:.1f} s), scipy.fft workers=-1"}
+        log(f"cpu oracle: {t_cpu:.1f} s -> {cpu['value']:.3e} mode-steps/s on {cores} cores")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "mode-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(step_ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (state, transforms, forcing) + f64 (E/KK tables, Picard norms, reductions)",
+        "data": "synthetic",
+        "config": {
+            "workload": "64 uneven cuboids in one batch (seed 0), constant-property single "
+                        "tracks with evaporation, dt 5e-6 s, Picard tol 1e-8 cap 20, omega 0.1",
+            "precision": "mixed",
+            "sum_modes": N_total,
+            "picard_iterations_mean": round(n_it_mean, 2),
+            "plain_fp64_gpu_mode_steps_per_s": None if plain_rate is None else round(plain_rate, 1),
+            "speedup_vs_plain_fp64_gpu": None if plain_rate is None else round(value / plain_rate, 2),
+            "fp64_batched_mode_steps_per_s": round(fp64_rate, 1),
+            "rel_l2_mixed_vs_fp64_batch": float(f"{rel_batch:.3e}"),
+            "rel_l2_mixed_vs_fp64_worst_cuboid": float(f"{rel_max:.3e}"),
+            "volume_pass_GBps": round(vol_gbs, 1),
+            "kernel_share_of_step": {"surface_iterations": round(share_surf, 3),
+                                     "volume_pass": round(share_vol, 3)},
+            "parallelism": "single GPU",
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

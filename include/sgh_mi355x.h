@@ -157,6 +157,52 @@ int sgh_property_correction_source(int dtype, int nx, int ny, int nz, const void
 /* Property program layout size in bytes (for validation on the host). */
 int sgh_property_program_bytes(void);
 
+/* ---- batched surface-collapsed engine ------------------------------------
+ * Many independent cuboids with top-surface forcing only (laser + evaporation,
+ * constant properties, no latent heat, no convection) advance together, each with the
+ * reference's step semantics (sgh/solvers/spectral.py:219-543): warm start from the
+ * shifted previous evaporation flux, relaxed Picard loop with its own ω, tol and cap,
+ * one step per sgh_batch_run iteration. Every iterate is aE + KK·(w⊗G), so the loop
+ * runs on surface-sized arrays (exact norm identities) and one volume pass per step
+ * writes the state; E and KK are regenerated in FP64 from 1-D tables.
+ * The laser state for every step comes from a host table copied at creation, so a
+ * step replays as one captured hipGraph without parameter updates. */
+typedef struct sgh_cuboid_spec {
+  int nx, ny, nz;         /* modes; nx, ny <= 256                                  */
+  int max_it;             /* Picard cap                                            */
+  int n_steps;            /* rows of `laser`; the cuboid stops after that many steps */
+  int cell_integrated;    /* Gaussian cell-mean flux (order 2 only)                */
+  double Lx, Ly, Lz;      /* extents (m)                                           */
+  double k, rho, cp;      /* constant conductivity, density, heat capacity         */
+  double Pa, T_boil, dH, R_v, T_liq;  /* evaporation law                          */
+  double absorptivity, rx, ry, order; /* beam                                     */
+  double dt, omega, tol;  /* time step, relaxation, Picard tolerance               */
+  const double* laser;    /* host: n_steps × {x, y, power, on, vx, vy}             */
+} sgh_cuboid_spec;
+
+/* Creates the plan (allocates the packed device state; the only allocating call) and
+ * copies every table to the device. State starts at zero: set it with sgh_batch_set_state.
+ * `metrics_rows` steps of per-cuboid metrics {P_laser, T_surface_max, n_it, converged,
+ * rel_err} are kept on the device. Replaces a loop of SpectralSolver.step calls. */
+int sgh_batch_create(int dtype, int n, const sgh_cuboid_spec* specs, int metrics_rows,
+                     void** plan, void* stream);
+int sgh_batch_destroy(void* plan);
+/* Advance every live cuboid by n_steps steps (one captured graph per step if use_graph). */
+int sgh_batch_run(void* plan, int n_steps, int use_graph, void* stream);
+/* Device-to-device copy of cuboid c's modal state [nz, ny, nx] in the plan's dtype. */
+int sgh_batch_set_state(void* plan, int c, const void* src, void* stream);
+int sgh_batch_get_state(void* plan, int c, void* dst, void* stream);
+int sgh_batch_state_layout(void* plan, int c, long long* a_offset, int* nx, int* ny, int* nz);
+/* Copies rows × n × 5 metrics to host memory (synchronises the stream). */
+int sgh_batch_read_metrics(void* plan, double* host_out, int rows, void* stream);
+/* Event timing of the surface-iteration and volume kernels (disables graphs while on):
+ * out4 = {surface ms, surface launches, volume ms, volume launches}. */
+int sgh_batch_set_timing(void* plan, int enable);
+int sgh_batch_timing(void* plan, double* out4);
+/* out6 = {packed volume elements, packed surface elements, surface items, volume items,
+ *         max Picard cap, LDS bytes per surface workgroup}. */
+int sgh_batch_info(void* plan, long long* out6);
+
 #ifdef __cplusplus
 }
 #endif

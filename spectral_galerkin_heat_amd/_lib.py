@@ -43,7 +43,37 @@ SIGNATURES = {
     "sgh_latent_source": [i32, i64, vp, vp, f64, f64, f64, f64, f64, vp, vp, vp],
     "sgh_property_correction_source": [i32, i32, i32, i32, vp, vp, vp, f64, f64, f64, f64, f64,
                                        f64, vp, vp, vp],
+    "sgh_batch_create": [i32, i32, vp, i32, C.POINTER(vp), vp],
+    "sgh_batch_destroy": [vp],
+    "sgh_batch_run": [vp, i32, i32, vp],
+    "sgh_batch_set_state": [vp, i32, vp, vp],
+    "sgh_batch_get_state": [vp, i32, vp, vp],
+    "sgh_batch_state_layout": [vp, i32, C.POINTER(i64), C.POINTER(i32), C.POINTER(i32),
+                               C.POINTER(i32)],
+    "sgh_batch_read_metrics": [vp, vp, i32, vp],
+    "sgh_batch_set_timing": [vp, i32],
+    "sgh_batch_timing": [vp, vp],
+    "sgh_batch_info": [vp, vp],
 }
+
+
+class CuboidSpec(C.Structure):
+    """Mirror of sgh_cuboid_spec (include/sgh_mi355x.h)."""
+
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("max_it", C.c_int),
+        ("n_steps", C.c_int), ("cell_integrated", C.c_int),
+        ("Lx", C.c_double), ("Ly", C.c_double), ("Lz", C.c_double),
+        ("k", C.c_double), ("rho", C.c_double), ("cp", C.c_double),
+        ("Pa", C.c_double), ("T_boil", C.c_double), ("dH", C.c_double), ("R_v", C.c_double),
+        ("T_liq", C.c_double),
+        ("absorptivity", C.c_double), ("rx", C.c_double), ("ry", C.c_double),
+        ("order", C.c_double),
+        ("dt", C.c_double), ("omega", C.c_double), ("tol", C.c_double),
+        ("laser", C.POINTER(C.c_double)),
+    ]
+
+
 _RESTYPES = {"sgh_last_error": C.c_char_p}
 
 

@@ -1,0 +1,902 @@
+// Batched surface-collapsed ETD1 + Picard engine for many independent cuboids.
+//
+// For a cuboid whose forcing is a top-surface flux only (laser + evaporation; constant
+// properties, no latent heat, no convection) every Picard iterate of the reference's
+// step (sgh/solvers/spectral.py:320-465) has the form
+//     a = aE + KK ∘ (w ⊗ G),   aE = a_n·E,  w[p] = C_p(−1)^p,  G a surface field,
+// so the whole iteration reduces to surface-sized work:
+//     T_s      = Bᵧᵀ (ZE + G∘Ztt) Bₓ             (ZE = Σ_p w·aE,  Ztt = Σ_p w²·KK)
+//     S_raw    = dx·dy · Bᵧ (q_las − q_evap(T_s)) Bₓᵀ
+//     D = S_raw − G,   G ← G + ω·D
+//     r·r      = Σ D²·Wtt                          (Wtt = Σ_p w²·KK²)
+//     aₒ·aₒ    = AA + 2 Σ G·U + Σ G²·Wtt          (AA = Σ aE², U = Σ_p aE·KK·w)
+// and one volume pass per step writes a_{n+1} = aE + KK∘(w⊗G) and already gathers
+// ZE, U, AA of the next step. E and KK are regenerated in FP64 from 1-D tables.
+//
+// Work decomposition: every launch covers all cuboids through a packed work list of
+// (cuboid, 8-row block) items for surface transforms and (cuboid, 256-column block)
+// items for the volume pass. Partial sums go to fixed per-item slots and are summed in
+// a fixed order by the next kernel, so a cuboid's arithmetic does not depend on what
+// else is in the batch, and no block ever waits on another inside a kernel.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "sgh_common.h"
+
+static_assert(sizeof(sgh_cuboid_spec) == 176, "sgh_cuboid_spec layout (mirrored in _lib.py)");
+
+namespace sgh {
+
+constexpr int kBT = 256;        // threads per block
+constexpr int kBRB = 8;         // surface rows per transform item
+constexpr int kBKC = 16;        // LDS contraction chunk
+constexpr int kBMaxN = 256;     // widest surface axis
+constexpr int kBMaxO = kBRB * kBMaxN / kBT;
+constexpr int kStatusStride = 4;  // n_it, converged, rel, pad
+constexpr int kMetricStride = 5;  // P_laser, T_max, n_it, converged, rel
+
+struct CuboidDesc {
+  int nx, ny, nz, S;
+  long long a_off;
+  int s_off;
+  int bx_off, by_off;      // offsets of B̂ (n×n) and B̂ᵀ for x and y in the matrix table
+  int bxT_off, byT_off;
+  int t1_off;              // offset of the 1-D tables (ex, kx2 | ey, ky2 | ez, kz2)
+  int row_item0, nrowblk;  // first transform item and count
+  int col_item0, ncolblk;  // first volume item and count
+  int max_it;
+  int n_steps;             // laser-table length
+  long long laser_off;     // row offset into the laser table (6 doubles per row)
+  double dx, dy, inv_sqrt_LxLy, Lz;
+  double alpha, rho_cp, dt;
+  double omega, tol, n_elem;
+  double f1, f2, T_boil, T_liq;
+  double absorptivity, rx, ry, order, area_factor;
+  int cell_integrated, pad;
+};
+
+struct Item {
+  int c, blk;
+};
+
+template <typename F>
+struct BatchPtrs {
+  F* a;            // packed modal state
+  F* G;            // surface forcing estimate (modal)
+  F* ZE;           // Σ w·aE (modal)
+  F* U;            // Σ aE·KK·w (modal)
+  F* Ztt;          // Σ w²·KK
+  F* Wtt;          // Σ w²·KK²
+  F* q_las;        // laser flux (physical)
+  F* q_net;        // net top flux (physical)
+  F* q_evap;       // evaporation flux of the current iterate (physical)
+  F* q_evap_old;   // previous step's evaporation flux
+  const F* mats;   // B̂ and B̂ᵀ matrices
+  const double* t1;        // 1-D tables
+  const double* laser;     // laser rows
+  const CuboidDesc* desc;
+  const Item* row_items;
+  const Item* col_items;
+  double* row_slots;       // per row item: rr, gu, g2w  (3 doubles)
+  double* col_slots;       // per col item: AA
+  double* aux_slots;       // per row item: P_laser, T_max (2 doubles)
+  int* done_at;            // per cuboid: iteration at which the step finished, −1 active
+  double* status;          // per cuboid: n_it, converged, rel
+  int* step_idx;           // per cuboid: step counter
+  double* metrics;         // [step][cuboid][5]
+  int metrics_rows;        // capacity in steps
+  int n_cuboids;
+};
+
+__device__ __forceinline__ double dexp_(double x) { return exp(x); }
+
+// ---------------------------------------------------------------------------------
+// setup: Ztt, Wtt (constant per cuboid)
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ double w_top(int p, double Lz) {
+  const double c = p == 0 ? sqrt(1.0 / Lz) : sqrt(2.0 / Lz);
+  return (p & 1) ? -c : c;
+}
+
+__device__ __forceinline__ double kk_of(double e, double lam, const CuboidDesc& d) {
+  const double denom = d.alpha * lam;
+  return denom == 0.0 ? d.dt / d.rho_cp : ((1.0 - e) / denom) / d.rho_cp;
+}
+
+template <typename F>
+__global__ void __launch_bounds__(kBT) k_b_setup(BatchPtrs<F> P) {
+  const Item it = P.col_items[blockIdx.x];
+  const CuboidDesc d = P.desc[it.c];
+  const int s = it.blk * kBT + threadIdx.x;
+  if (s >= d.S) return;
+  const int n = s / d.nx, m = s - n * d.nx;
+  const double* ex = P.t1 + d.t1_off;
+  const double* kx2 = ex + d.nx;
+  const double* ey = kx2 + d.nx;
+  const double* ky2 = ey + d.ny;
+  const double* ez = ky2 + d.ny;
+  const double* kz2 = ez + d.nz;
+  const double eyx = ey[n] * ex[m];
+  const double kyx = ky2[n] + kx2[m];
+  double z = 0.0, w2 = 0.0;
+  for (int p = 0; p < d.nz; ++p) {
+    const double e = ez[p] * eyx;
+    const double kk = kk_of(e, kz2[p] + kyx, d);
+    const double w = w_top(p, d.Lz);
+    z += w * w * kk;
+    w2 += w * w * kk * kk;
+  }
+  P.Ztt[d.s_off + s] = (F)z;
+  P.Wtt[d.s_off + s] = (F)w2;
+}
+
+// ---------------------------------------------------------------------------------
+// volume pass: finalize step n (a ← aE + KK·w·G) and gather ZE, U, AA of step n+1
+// ---------------------------------------------------------------------------------
+template <typename F>
+__global__ void __launch_bounds__(kBT) k_b_volume(BatchPtrs<F> P, int finalize) {
+  __shared__ double sh[kBT / kWave];
+  const Item it = P.col_items[blockIdx.x];
+  const CuboidDesc d = P.desc[it.c];
+  const int s = it.blk * kBT + threadIdx.x;
+  const bool active_cuboid = P.step_idx[it.c] < d.n_steps || !finalize;
+  double aa = 0.0;
+  if (s < d.S && active_cuboid) {
+    const int n = s / d.nx, m = s - n * d.nx;
+    const double* ex = P.t1 + d.t1_off;
+    const double* kx2 = ex + d.nx;
+    const double* ey = kx2 + d.nx;
+    const double* ky2 = ey + d.ny;
+    const double* ez = ky2 + d.ny;
+    const double* kz2 = ez + d.nz;
+    const double eyx = ey[n] * ex[m];
+    const double kyx = ky2[n] + kx2[m];
+    const double g = finalize ? (double)P.G[d.s_off + s] : 0.0;
+    F* a = P.a + d.a_off + s;
+    double ze = 0.0, u = 0.0;
+    for (int p = 0; p < d.nz; ++p) {
+      const double e = ez[p] * eyx;
+      const double kk = kk_of(e, kz2[p] + kyx, d);
+      const double w = w_top(p, d.Lz);
+      double av = (double)a[(long long)p * d.S];
+      if (finalize) {
+        const F nv = (F)(av * e + kk * (w * g));
+        a[(long long)p * d.S] = nv;
+        av = (double)nv;
+      }
+      const double ae = av * e;
+      ze += w * ae;
+      u += ae * kk * w;
+      aa += ae * ae;
+    }
+    P.ZE[d.s_off + s] = (F)ze;
+    P.U[d.s_off + s] = (F)u;
+    if (finalize) P.q_evap_old[d.s_off + s] = P.q_evap[d.s_off + s];
+  }
+  const double t = block_sum(aa, sh);
+  if (threadIdx.x == 0) P.col_slots[blockIdx.x] = t;
+}
+
+// ---------------------------------------------------------------------------------
+// step preparation: laser flux, shifted warm-start flux, per-step resets
+// ---------------------------------------------------------------------------------
+template <typename F>
+__global__ void __launch_bounds__(kBT) k_b_prep(BatchPtrs<F> P) {
+  __shared__ double sh[kBT / kWave];
+  const Item it = P.row_items[blockIdx.x];
+  const CuboidDesc d = P.desc[it.c];
+  const int step = P.step_idx[it.c];
+  const bool live = step < d.n_steps;
+  if (it.blk == 0 && threadIdx.x == 0) P.done_at[it.c] = live ? -1 : 0x7fffffff;
+  double psum = 0.0;
+  if (live) {
+    const double* L = P.laser + (d.laser_off + step) * 6;
+    const double x0 = L[0], y0 = L[1], power = L[2], on = L[3], vx = L[4], vy = L[5];
+    const double I0 = on != 0.0 ? d.absorptivity * power / (d.area_factor * d.rx * d.ry) : 0.0;
+    const double sy = vy * d.dt / d.dy, sx = vx * d.dt / d.dx;
+    const int r0 = it.blk * kBRB;
+    const int rows = min(kBRB, d.ny - r0);
+    const F* qo = P.q_evap_old + d.s_off;
+    for (int e = threadIdx.x; e < rows * d.nx; e += kBT) {
+      const int j = r0 + e / d.nx, i = e % d.nx;
+      const int s = j * d.nx + i;
+      const double x = (i + 0.5) * d.dx, y = (j + 0.5) * d.dy;
+      double q;
+      if (d.cell_integrated) {
+        const double s2 = sqrt(2.0), cc = sqrt(M_PI / 8.0);
+        const double ax = (x - x0 - d.dx / 2.0) * s2 / d.rx, bx = (x - x0 + d.dx / 2.0) * s2 / d.rx;
+        const double ay = (y - y0 - d.dy / 2.0) * s2 / d.ry, by = (y - y0 + d.dy / 2.0) * s2 / d.ry;
+        q = I0 * ((d.ry * cc * (erf(by) - erf(ay)) / d.dy) * (d.rx * cc * (erf(bx) - erf(ax)) / d.dx));
+      } else {
+        const double ux = (x - x0) / d.rx, uy = (y - y0) / d.ry;
+        const double r2 = ux * ux + uy * uy;
+        q = I0 * (d.order == 2.0 ? exp(-2.0 * r2) : exp(-2.0 * pow(r2, d.order / 2.0)));
+      }
+      const F ql = (F)q;
+      // bilinear shift of the previous evaporation flux, scipy order-1 'constant'
+      const double py = j - sy, px = i - sx;
+      const double fy = floor(py), fx = floor(px);
+      const double ty = py - fy, tx = px - fx;
+      const int iy0 = (int)fy, ix0 = (int)fx;
+      const bool inr = py >= 0.0 && py <= d.ny - 1 && px >= 0.0 && px <= d.nx - 1;
+      double sh_v = 0.0;
+      if (inr) {
+        const int a0 = min(max(iy0, 0), d.ny - 1), b0 = min(max(iy0 + 1, 0), d.ny - 1);
+        const int c0 = min(max(ix0, 0), d.nx - 1), d0 = min(max(ix0 + 1, 0), d.nx - 1);
+        sh_v = (1.0 - ty) * ((1.0 - tx) * (double)qo[a0 * d.nx + c0] + tx * (double)qo[a0 * d.nx + d0]) +
+               ty * ((1.0 - tx) * (double)qo[b0 * d.nx + c0] + tx * (double)qo[b0 * d.nx + d0]);
+      }
+      P.q_las[d.s_off + s] = ql;
+      P.q_net[d.s_off + s] = (F)((double)ql - (double)(F)sh_v);
+      psum += (double)ql;
+    }
+  }
+  const double t = block_sum(psum, sh);
+  if (threadIdx.x == 0) P.aux_slots[2 * blockIdx.x] = t;
+}
+
+// Picard decision for cuboid c before iteration k (k = 0: always run).
+// Returns true if the cuboid is still iterating.
+template <typename F>
+__device__ bool picard_active(const BatchPtrs<F>& P, const CuboidDesc& d, int c, int k,
+                              bool record) {
+  if (P.done_at[c] >= 0) return false;
+  bool done = false, conv = false;
+  double rel = __builtin_nan("");
+  if (k > 0) {
+    double rr = 0.0, gu = 0.0, g2w = 0.0, aa = 0.0;
+    for (int b = 0; b < d.nrowblk; ++b) {
+      const double* sl = P.row_slots + 3 * (d.row_item0 + b);
+      rr += sl[0];
+      gu += sl[1];
+      g2w += sl[2];
+    }
+    for (int b = 0; b < d.ncolblk; ++b) aa += P.col_slots[d.col_item0 + b];
+    aa += 2.0 * gu + g2w;
+    const double rms_diff = sqrt(rr / d.n_elem);
+    const double rms_old = sqrt(aa / d.n_elem);
+    rel = rms_diff / (rms_old > 1e-9 ? rms_old : 1e-9);
+    conv = rel < d.tol;
+    done = conv;
+  }
+  if (k >= d.max_it) done = true;
+  if (done && record) {
+    P.done_at[c] = k;
+    double* st = P.status + kStatusStride * c;
+    st[0] = k;
+    st[1] = conv ? 1.0 : 0.0;
+    st[2] = rel;
+  }
+  return !done;
+}
+
+// ---------------------------------------------------------------------------------
+// surface transforms (one kernel, three roles)
+//   MODE 0 (warm start): G = dx·dy·Bᵧ q_net Bₓᵀ / √(LxLy)
+//   MODE 1 (iteration, forward half): D = S_raw − G, norm slots, G += ω D
+//   MODE 2 (iteration, inverse half): T_s = Bᵧᵀ(ZE + G∘Ztt)Bₓ/√(LxLy) → q_evap, q_net
+//   MODE 3 (step end): T_s of the final iterate → T_max slot
+// ---------------------------------------------------------------------------------
+template <typename F, int MODE>
+__global__ void __launch_bounds__(kBT) k_b_surface(BatchPtrs<F> P, int k) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ double sh[kBT / kWave];
+  __shared__ int s_active;
+  const Item it = P.row_items[blockIdx.x];
+  const CuboidDesc d = P.desc[it.c];
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    bool act;
+    if (MODE == 0) act = P.step_idx[it.c] < d.n_steps;
+    else if (MODE == 1) act = P.done_at[it.c] < 0;
+    else if (MODE == 2) act = picard_active<F>(P, d, it.c, k, it.blk == 0);
+    else {
+      picard_active<F>(P, d, it.c, k, it.blk == 0);
+      act = P.step_idx[it.c] < d.n_steps;
+    }
+    s_active = act ? 1 : 0;
+  }
+  __syncthreads();
+  if (!s_active) return;
+
+  const int nx = d.nx, ny = d.ny;
+  F* Bs = reinterpret_cast<F*>(smem_raw);   // [kBKC][nx]
+  F* Ys = Bs + kBKC * nx;                    // [kBRB][nx]
+  F* As = Ys + kBRB * nx;                    // [kBRB][kBKC]
+  const F* Lm = P.mats + (MODE == 0 || MODE == 1 ? d.by_off : d.byT_off);
+  const F* Rm = P.mats + (MODE == 0 || MODE == 1 ? d.bxT_off : d.bx_off);
+  const int r0 = it.blk * kBRB;
+  const int rows = min(kBRB, ny - r0);
+  const int nout = rows * nx;
+  const int so = d.s_off;
+
+  int ro[kBMaxO], co[kBMaxO];
+  F acc[kBMaxO];
+#pragma unroll
+  for (int j = 0; j < kBMaxO; ++j) {
+    const int o = tid + j * kBT;
+    ro[j] = o / nx;
+    co[j] = o - ro[j] * nx;
+    acc[j] = F(0);
+  }
+  for (int k0 = 0; k0 < ny; k0 += kBKC) {
+    const int kc = min(kBKC, ny - k0);
+    for (int e = tid; e < kc * nx; e += kBT) {
+      const int kk = e / nx, c = e - kk * nx;
+      const int s = so + (k0 + kk) * nx + c;
+      F v;
+      if (MODE <= 1) v = P.q_net[s];
+      else v = P.ZE[s] + P.G[s] * P.Ztt[s];
+      Bs[kk * nx + c] = v;
+    }
+    for (int e = tid; e < rows * kBKC; e += kBT) {
+      const int r = e / kBKC, kk = e - r * kBKC;
+      As[r * kBKC + kk] = kk < kc ? Lm[(r0 + r) * ny + k0 + kk] : F(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kBMaxO; ++j) {
+      if (tid + j * kBT < nout) {
+        const F* a = As + ro[j] * kBKC;
+        const F* bc = Bs + co[j];
+        F s = acc[j];
+        for (int kk = 0; kk < kc; ++kk) s += a[kk] * bc[kk * nx];
+        acc[j] = s;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < kBMaxO; ++j) {
+    if (tid + j * kBT < nout) Ys[tid + j * kBT] = acc[j];
+    acc[j] = F(0);
+  }
+  __syncthreads();
+  for (int k0 = 0; k0 < nx; k0 += kBKC) {
+    const int kc = min(kBKC, nx - k0);
+    for (int e = tid; e < kc * nx; e += kBT) {
+      const int kk = e / nx, c = e - kk * nx;
+      Bs[kk * nx + c] = Rm[(k0 + kk) * nx + c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kBMaxO; ++j) {
+      if (tid + j * kBT < nout) {
+        const F* y = Ys + ro[j] * nx + k0;
+        const F* bc = Bs + co[j];
+        F s = acc[j];
+        for (int kk = 0; kk < kc; ++kk) s += y[kk] * bc[kk * nx];
+        acc[j] = s;
+      }
+    }
+    __syncthreads();
+  }
+
+  // epilogues
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+  if (MODE == 3) v0 = -1.0 / 0.0;
+  const double fwd_scale = d.dx * d.dy * d.inv_sqrt_LxLy;
+#pragma unroll
+  for (int j = 0; j < kBMaxO; ++j) {
+    if (tid + j * kBT < nout) {
+      const int s = so + (r0 + ro[j]) * nx + co[j];
+      if (MODE == 0) {
+        P.G[s] = (F)((double)acc[j] * fwd_scale);
+      } else if (MODE == 1) {
+        const F Sr = (F)((double)acc[j] * fwd_scale);
+        const F g = P.G[s];
+        const F dd = Sr - g;
+        const double w = (double)P.Wtt[s];
+        v0 += (double)dd * (double)dd * w;
+        v1 += (double)g * (double)P.U[s];
+        v2 += (double)g * (double)g * w;
+        P.G[s] = g + (F)d.omega * dd;
+      } else {
+        const F T = (F)((double)acc[j] * d.inv_sqrt_LxLy);
+        if (MODE == 2) {
+          F qe = F(0);
+          if (T >= (F)d.T_liq)
+            qe = (F)d.f1 * ((F(1) / sqrt(T)) * exp((F)d.f2 * (F(1) - (F)d.T_boil / T)));
+          P.q_evap[s] = qe;
+          P.q_net[s] = P.q_las[s] - qe;
+        } else {
+          v0 = (double)T > v0 ? (double)T : v0;
+        }
+      }
+    }
+  }
+  if (MODE == 1) {
+    const double a0 = block_sum(v0, sh);
+    const double a1 = block_sum(v1, sh);
+    const double a2 = block_sum(v2, sh);
+    if (tid == 0) {
+      double* sl = P.row_slots + 3 * blockIdx.x;
+      sl[0] = a0;
+      sl[1] = a1;
+      sl[2] = a2;
+    }
+  } else if (MODE == 3) {
+    const double m = block_max(v0, sh);
+    if (tid == 0) P.aux_slots[2 * blockIdx.x + 1] = m;
+  }
+}
+
+// Per-step metrics and step-counter advance (single block).
+template <typename F>
+__global__ void __launch_bounds__(kBT) k_b_metrics(BatchPtrs<F> P) {
+  for (int c = threadIdx.x; c < P.n_cuboids; c += kBT) {
+    const CuboidDesc d = P.desc[c];
+    const int step = P.step_idx[c];
+    if (step >= d.n_steps) continue;
+    double pl = 0.0, tm = -1.0 / 0.0;
+    for (int b = 0; b < d.nrowblk; ++b) {
+      pl += P.aux_slots[2 * (d.row_item0 + b)];
+      const double t = P.aux_slots[2 * (d.row_item0 + b) + 1];
+      tm = t > tm ? t : tm;
+    }
+    if (step < P.metrics_rows) {
+      double* m = P.metrics + ((long long)step * P.n_cuboids + c) * kMetricStride;
+      const double* st = P.status + kStatusStride * c;
+      m[0] = pl * d.dx * d.dy;
+      m[1] = tm;
+      m[2] = st[0];
+      m[3] = st[1];
+      m[4] = st[2];
+    }
+    P.step_idx[c] = step + 1;
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Plan
+// ---------------------------------------------------------------------------------
+struct Plan {
+  int dtype = 0;
+  int n = 0;
+  int max_it = 0;
+  size_t elem = 8;
+  std::vector<CuboidDesc> hdesc;
+  std::vector<Item> hrow, hcol;
+  int nx_max = 0;
+  size_t shm = 0;
+  long long total_N = 0, total_S = 0;
+  // device allocations
+  void* a = nullptr;
+  void* surf = nullptr;     // 10 surface arrays of total_S
+  void* mats = nullptr;
+  double* t1 = nullptr;
+  double* laser = nullptr;
+  CuboidDesc* desc = nullptr;
+  Item* row_items = nullptr;
+  Item* col_items = nullptr;
+  double* row_slots = nullptr;
+  double* col_slots = nullptr;
+  double* aux_slots = nullptr;
+  int* done_at = nullptr;
+  double* status = nullptr;
+  int* step_idx = nullptr;
+  double* metrics = nullptr;
+  int metrics_rows = 0;
+  bool primed = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  hipStream_t graph_stream = nullptr;
+  // timing of the dominant kernels (non-graph runs)
+  bool timing = false;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  double t_surface_ms = 0.0, t_volume_ms = 0.0;
+  long long n_surface = 0, n_volume = 0;
+};
+
+template <typename F>
+BatchPtrs<F> ptrs(Plan* p) {
+  BatchPtrs<F> P;
+  F* s = (F*)p->surf;
+  const long long T = p->total_S;
+  P.a = (F*)p->a;
+  P.G = s + 0 * T;
+  P.ZE = s + 1 * T;
+  P.U = s + 2 * T;
+  P.Ztt = s + 3 * T;
+  P.Wtt = s + 4 * T;
+  P.q_las = s + 5 * T;
+  P.q_net = s + 6 * T;
+  P.q_evap = s + 7 * T;
+  P.q_evap_old = s + 8 * T;
+  P.mats = (const F*)p->mats;
+  P.t1 = p->t1;
+  P.laser = p->laser;
+  P.desc = p->desc;
+  P.row_items = p->row_items;
+  P.col_items = p->col_items;
+  P.row_slots = p->row_slots;
+  P.col_slots = p->col_slots;
+  P.aux_slots = p->aux_slots;
+  P.done_at = p->done_at;
+  P.status = p->status;
+  P.step_idx = p->step_idx;
+  P.metrics = p->metrics;
+  P.metrics_rows = p->metrics_rows;
+  P.n_cuboids = p->n;
+  return P;
+}
+
+template <typename F>
+int enqueue_step(Plan* p, hipStream_t st) {
+  BatchPtrs<F> P = ptrs<F>(p);
+  const unsigned nrow = (unsigned)p->hrow.size(), ncol = (unsigned)p->hcol.size();
+  const bool tm = p->timing;
+  hipLaunchKernelGGL(k_b_prep<F>, dim3(nrow), dim3(kBT), 0, st, P);
+  hipLaunchKernelGGL((k_b_surface<F, 0>), dim3(nrow), dim3(kBT), p->shm, st, P, 0);
+  if (tm) hipEventRecord(p->ev[0], st);
+  for (int k = 0; k < p->max_it; ++k) {
+    hipLaunchKernelGGL((k_b_surface<F, 2>), dim3(nrow), dim3(kBT), p->shm, st, P, k);
+    hipLaunchKernelGGL((k_b_surface<F, 1>), dim3(nrow), dim3(kBT), p->shm, st, P, k);
+  }
+  if (tm) hipEventRecord(p->ev[1], st);
+  hipLaunchKernelGGL((k_b_surface<F, 3>), dim3(nrow), dim3(kBT), p->shm, st, P, p->max_it);
+  if (tm) hipEventRecord(p->ev[2], st);
+  hipLaunchKernelGGL(k_b_volume<F>, dim3(ncol), dim3(kBT), 0, st, P, 1);
+  if (tm) hipEventRecord(p->ev[3], st);
+  hipLaunchKernelGGL(k_b_metrics<F>, dim3(1), dim3(kBT), 0, st, P);
+  SGH_CHECK_LAUNCH("sgh_batch step");
+  if (tm) {
+    hipError_t e = hipEventSynchronize(p->ev[3]);
+    if (e != hipSuccess) return fail_hip(e, "sgh_batch timing");
+    float a = 0.f, b = 0.f;
+    hipEventElapsedTime(&a, p->ev[0], p->ev[1]);
+    hipEventElapsedTime(&b, p->ev[2], p->ev[3]);
+    p->t_surface_ms += a;
+    p->n_surface += 2LL * p->max_it;
+    p->t_volume_ms += b;
+    p->n_volume += 1;
+  }
+  return SGH_OK;
+}
+
+template <typename F>
+int prime(Plan* p, hipStream_t st) {
+  BatchPtrs<F> P = ptrs<F>(p);
+  hipLaunchKernelGGL(k_b_setup<F>, dim3((unsigned)p->hcol.size()), dim3(kBT), 0, st, P);
+  hipLaunchKernelGGL(k_b_volume<F>, dim3((unsigned)p->hcol.size()), dim3(kBT), 0, st, P, 0);
+  SGH_CHECK_LAUNCH("sgh_batch prime");
+  return SGH_OK;
+}
+
+// B̂[k, i] = (k == 0 ? 1 : √2)·cos(πk(2i+1)/(2n)); the basis of a cuboid of extent L is
+// B̂/√L, so one matrix per distinct mode count serves every cuboid.
+static std::vector<double> unit_basis(int n) {
+  std::vector<double> B((size_t)n * n);
+  for (int k = 0; k < n; ++k)
+    for (int i = 0; i < n; ++i) {
+      const long long num = ((long long)k * (2 * i + 1)) % (4LL * n);
+      B[(size_t)k * n + i] = (k == 0 ? 1.0 : sqrt(2.0)) * cos(M_PI * (double)num / (2.0 * n));
+    }
+  return B;
+}
+
+static void free_plan(Plan* p) {
+  if (!p) return;
+  if (p->graph_exec) hipGraphExecDestroy(p->graph_exec);
+  if (p->graph) hipGraphDestroy(p->graph);
+  for (auto& e : p->ev)
+    if (e) hipEventDestroy(e);
+  void* bufs[] = {p->a, p->surf, p->mats, p->t1, p->laser, p->desc, p->row_items, p->col_items,
+                  p->row_slots, p->col_slots, p->aux_slots, p->done_at, p->status, p->step_idx,
+                  p->metrics};
+  for (void* b : bufs)
+    if (b) hipFree(b);
+  delete p;
+}
+
+template <typename T>
+static hipError_t dmalloc(T** ptr, size_t bytes) {
+  return hipMalloc((void**)ptr, bytes < 16 ? 16 : bytes);
+}
+
+}  // namespace sgh
+
+using namespace sgh;
+
+extern "C" {
+
+int sgh_batch_create(int dtype, int n, const sgh_cuboid_spec* specs, int metrics_rows,
+                     void** plan_out, void* stream) {
+  SGH_REQUIRE(plan_out && specs && n > 0, SGH_ERR_ARG, "sgh_batch_create: bad args");
+  SGH_REQUIRE(dtype == SGH_F64 || dtype == SGH_F32, SGH_ERR_ARG, "sgh_batch_create: dtype");
+  SGH_REQUIRE(metrics_rows >= 0, SGH_ERR_ARG, "sgh_batch_create: metrics_rows");
+  *plan_out = nullptr;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  Plan* p = new Plan();
+  p->dtype = dtype;
+  p->n = n;
+  p->elem = dtype == SGH_F64 ? 8 : 4;
+  p->metrics_rows = metrics_rows;
+
+  // matrices per distinct mode count
+  std::vector<int> sizes;
+  std::vector<long long> mat_off;
+  std::vector<double> mats;
+  auto mat_for = [&](int m) -> long long {
+    for (size_t i = 0; i < sizes.size(); ++i)
+      if (sizes[i] == m) return mat_off[i];
+    const long long off = (long long)mats.size();
+    std::vector<double> B = unit_basis(m);
+    mats.insert(mats.end(), B.begin(), B.end());
+    for (int r = 0; r < m; ++r)
+      for (int c = 0; c < m; ++c) mats.push_back(B[(size_t)c * m + r]);
+    sizes.push_back(m);
+    mat_off.push_back(off);
+    return off;
+  };
+
+  std::vector<double> t1, laser;
+  long long a_off = 0, s_off = 0;
+  for (int c = 0; c < n; ++c) {
+    const sgh_cuboid_spec& sp = specs[c];
+    if (sp.nx < 2 || sp.ny < 1 || sp.nz < 1 || sp.nx > kBMaxN || sp.ny > kBMaxN ||
+        sp.Lx <= 0 || sp.Ly <= 0 || sp.Lz <= 0 || sp.k <= 0 || sp.rho <= 0 || sp.cp <= 0 ||
+        sp.dt <= 0 || sp.rx <= 0 || sp.ry <= 0 || sp.R_v <= 0 || sp.T_boil <= 0 ||
+        sp.n_steps < 0 || sp.max_it < 0 || (sp.n_steps > 0 && sp.laser == nullptr) ||
+        (sp.cell_integrated && sp.order != 2.0)) {
+      free_plan(p);
+      set_error("sgh_batch_create: invalid spec for cuboid %d", c);
+      return SGH_ERR_ARG;
+    }
+    CuboidDesc d{};
+    d.nx = sp.nx;
+    d.ny = sp.ny;
+    d.nz = sp.nz;
+    d.S = sp.nx * sp.ny;
+    d.a_off = a_off;
+    d.s_off = (int)s_off;
+    const long long N = (long long)d.S * d.nz;
+    a_off += (N + 63) / 64 * 64;
+    s_off += (d.S + 63) / 64 * 64;
+    d.bx_off = (int)mat_for(sp.nx);
+    d.bxT_off = d.bx_off + sp.nx * sp.nx;
+    d.by_off = (int)mat_for(sp.ny);
+    d.byT_off = d.by_off + sp.ny * sp.ny;
+    d.t1_off = (int)t1.size();
+    const double alpha = sp.k / (sp.rho * sp.cp);
+    const double Ls[3] = {sp.Lx, sp.Ly, sp.Lz};
+    const int ns[3] = {sp.nx, sp.ny, sp.nz};
+    for (int ax = 0; ax < 3; ++ax) {
+      std::vector<double> e(ns[ax]), k2(ns[ax]);
+      for (int m = 0; m < ns[ax]; ++m) {
+        const double kk = M_PI * m / Ls[ax];
+        k2[m] = kk * kk;
+        e[m] = exp(-alpha * k2[m] * sp.dt);
+      }
+      t1.insert(t1.end(), e.begin(), e.end());
+      t1.insert(t1.end(), k2.begin(), k2.end());
+    }
+    d.row_item0 = (int)p->hrow.size();
+    d.nrowblk = (sp.ny + kBRB - 1) / kBRB;
+    for (int b = 0; b < d.nrowblk; ++b) p->hrow.push_back({c, b});
+    d.col_item0 = (int)p->hcol.size();
+    d.ncolblk = (d.S + kBT - 1) / kBT;
+    for (int b = 0; b < d.ncolblk; ++b) p->hcol.push_back({c, b});
+    d.max_it = sp.max_it;
+    d.n_steps = sp.n_steps;
+    d.laser_off = (long long)laser.size() / 6;
+    if (sp.n_steps > 0) laser.insert(laser.end(), sp.laser, sp.laser + 6LL * sp.n_steps);
+    d.dx = sp.Lx / sp.nx;
+    d.dy = sp.Ly / sp.ny;
+    d.inv_sqrt_LxLy = 1.0 / sqrt(sp.Lx * sp.Ly);
+    d.Lz = sp.Lz;
+    d.alpha = alpha;
+    d.rho_cp = sp.rho * sp.cp;
+    d.dt = sp.dt;
+    d.omega = sp.omega;
+    d.tol = sp.tol;
+    d.n_elem = (double)N;
+    d.f1 = 0.82 * sp.dH * sp.Pa / sqrt(2.0 * M_PI * sp.R_v);
+    d.f2 = sp.dH / (sp.R_v * sp.T_boil);
+    d.T_boil = sp.T_boil;
+    d.T_liq = sp.T_liq;
+    d.absorptivity = sp.absorptivity;
+    d.rx = sp.rx;
+    d.ry = sp.ry;
+    d.order = sp.order;
+    const double nn = sp.order;
+    d.area_factor = M_PI * pow(2.0, 1.0 - 2.0 / nn) * tgamma(2.0 / nn) / nn;
+    d.cell_integrated = sp.cell_integrated;
+    p->hdesc.push_back(d);
+    p->max_it = sp.max_it > p->max_it ? sp.max_it : p->max_it;
+    p->nx_max = sp.nx > p->nx_max ? sp.nx : p->nx_max;
+  }
+  p->total_N = a_off;
+  p->total_S = s_off;
+  p->shm = (size_t)(kBKC * p->nx_max + kBRB * p->nx_max + kBRB * kBKC) * p->elem;
+
+  hipError_t e = hipSuccess;
+  auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
+  const size_t E = p->elem;
+  chk(dmalloc(&p->a, p->total_N * E));
+  chk(dmalloc(&p->surf, 9 * p->total_S * E));
+  chk(dmalloc(&p->mats, mats.size() * E));
+  chk(dmalloc(&p->t1, t1.size() * sizeof(double)));
+  chk(dmalloc(&p->laser, laser.size() * sizeof(double)));
+  chk(dmalloc(&p->desc, n * sizeof(CuboidDesc)));
+  chk(dmalloc(&p->row_items, p->hrow.size() * sizeof(Item)));
+  chk(dmalloc(&p->col_items, p->hcol.size() * sizeof(Item)));
+  chk(dmalloc(&p->row_slots, 3 * p->hrow.size() * sizeof(double)));
+  chk(dmalloc(&p->col_slots, p->hcol.size() * sizeof(double)));
+  chk(dmalloc(&p->aux_slots, 2 * p->hrow.size() * sizeof(double)));
+  chk(dmalloc(&p->done_at, n * sizeof(int)));
+  chk(dmalloc(&p->status, kStatusStride * n * sizeof(double)));
+  chk(dmalloc(&p->step_idx, n * sizeof(int)));
+  chk(dmalloc(&p->metrics, (size_t)metrics_rows * n * kMetricStride * sizeof(double)));
+  if (e != hipSuccess) {
+    free_plan(p);
+    set_error("sgh_batch_create: device allocation failed: %s", hipGetErrorString(e));
+    return SGH_ERR_NOMEM;
+  }
+  std::vector<unsigned char> mbuf(mats.size() * E);
+  for (size_t i = 0; i < mats.size(); ++i) {
+    if (E == 8) reinterpret_cast<double*>(mbuf.data())[i] = mats[i];
+    else reinterpret_cast<float*>(mbuf.data())[i] = (float)mats[i];
+  }
+  chk(hipMemcpyAsync(p->mats, mbuf.data(), mbuf.size(), hipMemcpyHostToDevice, st));
+  chk(hipMemcpyAsync(p->t1, t1.data(), t1.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  if (!laser.empty())
+    chk(hipMemcpyAsync(p->laser, laser.data(), laser.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  chk(hipMemcpyAsync(p->desc, p->hdesc.data(), n * sizeof(CuboidDesc), hipMemcpyHostToDevice, st));
+  chk(hipMemcpyAsync(p->row_items, p->hrow.data(), p->hrow.size() * sizeof(Item), hipMemcpyHostToDevice, st));
+  chk(hipMemcpyAsync(p->col_items, p->hcol.data(), p->hcol.size() * sizeof(Item), hipMemcpyHostToDevice, st));
+  chk(hipMemsetAsync(p->surf, 0, 9 * p->total_S * E, st));
+  chk(hipMemsetAsync(p->a, 0, p->total_N * E, st));
+  chk(hipMemsetAsync(p->step_idx, 0, n * sizeof(int), st));
+  chk(hipMemsetAsync(p->status, 0, kStatusStride * n * sizeof(double), st));
+  chk(hipMemsetAsync(p->done_at, 0, n * sizeof(int), st));
+  chk(hipStreamSynchronize(st));
+  if (e != hipSuccess) {
+    free_plan(p);
+    return fail_hip(e, "sgh_batch_create upload");
+  }
+  *plan_out = p;
+  return SGH_OK;
+}
+
+int sgh_batch_destroy(void* plan) {
+  if (!plan) return SGH_OK;
+  hipDeviceSynchronize();
+  free_plan(static_cast<Plan*>(plan));
+  return SGH_OK;
+}
+
+int sgh_batch_state_layout(void* plan, int c, long long* a_offset, int* nx, int* ny, int* nz) {
+  Plan* p = static_cast<Plan*>(plan);
+  SGH_REQUIRE(p && c >= 0 && c < p->n, SGH_ERR_ARG, "sgh_batch_state_layout: bad cuboid");
+  const CuboidDesc& d = p->hdesc[c];
+  if (a_offset) *a_offset = d.a_off;
+  if (nx) *nx = d.nx;
+  if (ny) *ny = d.ny;
+  if (nz) *nz = d.nz;
+  return SGH_OK;
+}
+
+int sgh_batch_set_state(void* plan, int c, const void* src, void* stream) {
+  Plan* p = static_cast<Plan*>(plan);
+  SGH_REQUIRE(p && src && c >= 0 && c < p->n, SGH_ERR_ARG, "sgh_batch_set_state: bad args");
+  const CuboidDesc& d = p->hdesc[c];
+  hipError_t e = hipMemcpyAsync((char*)p->a + d.a_off * p->elem, src, (size_t)d.S * d.nz * p->elem,
+                                hipMemcpyDeviceToDevice, reinterpret_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return fail_hip(e, "sgh_batch_set_state");
+  p->primed = false;
+  return SGH_OK;
+}
+
+int sgh_batch_get_state(void* plan, int c, void* dst, void* stream) {
+  Plan* p = static_cast<Plan*>(plan);
+  SGH_REQUIRE(p && dst && c >= 0 && c < p->n, SGH_ERR_ARG, "sgh_batch_get_state: bad args");
+  const CuboidDesc& d = p->hdesc[c];
+  hipError_t e = hipMemcpyAsync(dst, (char*)p->a + d.a_off * p->elem, (size_t)d.S * d.nz * p->elem,
+                                hipMemcpyDeviceToDevice, reinterpret_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return fail_hip(e, "sgh_batch_get_state");
+  return SGH_OK;
+}
+
+int sgh_batch_run(void* plan, int n_steps, int use_graph, void* stream) {
+  Plan* p = static_cast<Plan*>(plan);
+  SGH_REQUIRE(p && n_steps >= 0, SGH_ERR_ARG, "sgh_batch_run: bad args");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  int rc = SGH_OK;
+  if (!p->primed) {
+    rc = p->dtype == SGH_F64 ? prime<double>(p, st) : prime<float>(p, st);
+    if (rc) return rc;
+    p->primed = true;
+  }
+  if (p->timing) use_graph = 0;
+  if (use_graph) {
+    if (p->graph_exec && p->graph_stream != st) {
+      hipGraphExecDestroy(p->graph_exec);
+      hipGraphDestroy(p->graph);
+      p->graph_exec = nullptr;
+      p->graph = nullptr;
+    }
+    if (!p->graph_exec) {
+      hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+      if (e != hipSuccess) return fail_hip(e, "sgh_batch_run capture");
+      rc = p->dtype == SGH_F64 ? enqueue_step<double>(p, st) : enqueue_step<float>(p, st);
+      hipGraph_t g = nullptr;
+      e = hipStreamEndCapture(st, &g);
+      if (rc) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return fail_hip(e, "sgh_batch_run end capture");
+      e = hipGraphInstantiate(&p->graph_exec, g, nullptr, nullptr, 0);
+      if (e != hipSuccess) {
+        hipGraphDestroy(g);
+        return fail_hip(e, "sgh_batch_run instantiate");
+      }
+      p->graph = g;
+      p->graph_stream = st;
+    }
+    for (int s = 0; s < n_steps; ++s) {
+      hipError_t e = hipGraphLaunch(p->graph_exec, st);
+      if (e != hipSuccess) return fail_hip(e, "sgh_batch_run graph launch");
+    }
+    return SGH_OK;
+  }
+  for (int s = 0; s < n_steps; ++s) {
+    rc = p->dtype == SGH_F64 ? enqueue_step<double>(p, st) : enqueue_step<float>(p, st);
+    if (rc) return rc;
+  }
+  return SGH_OK;
+}
+
+int sgh_batch_read_metrics(void* plan, double* host_out, int rows, void* stream) {
+  Plan* p = static_cast<Plan*>(plan);
+  SGH_REQUIRE(p && host_out && rows >= 0 && rows <= p->metrics_rows, SGH_ERR_ARG,
+              "sgh_batch_read_metrics: bad args");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  hipError_t e = hipMemcpyAsync(host_out, p->metrics, (size_t)rows * p->n * kMetricStride * sizeof(double),
+                                hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return fail_hip(e, "sgh_batch_read_metrics");
+  return SGH_OK;
+}
+
+int sgh_batch_set_timing(void* plan, int enable) {
+  Plan* p = static_cast<Plan*>(plan);
+  SGH_REQUIRE(p, SGH_ERR_ARG, "sgh_batch_set_timing: null plan");
+  if (enable && !p->ev[0])
+    for (auto& e : p->ev) {
+      hipError_t r = hipEventCreate(&e);
+      if (r != hipSuccess) return fail_hip(r, "sgh_batch_set_timing");
+    }
+  p->timing = enable != 0;
+  p->t_surface_ms = p->t_volume_ms = 0.0;
+  p->n_surface = p->n_volume = 0;
+  return SGH_OK;
+}
+
+int sgh_batch_timing(void* plan, double* out4) {
+  Plan* p = static_cast<Plan*>(plan);
+  SGH_REQUIRE(p && out4, SGH_ERR_ARG, "sgh_batch_timing: bad args");
+  out4[0] = p->t_surface_ms;
+  out4[1] = (double)p->n_surface;
+  out4[2] = p->t_volume_ms;
+  out4[3] = (double)p->n_volume;
+  return SGH_OK;
+}
+
+int sgh_batch_info(void* plan, long long* out6) {
+  Plan* p = static_cast<Plan*>(plan);
+  SGH_REQUIRE(p && out6, SGH_ERR_ARG, "sgh_batch_info: bad args");
+  out6[0] = p->total_N;
+  out6[1] = p->total_S;
+  out6[2] = (long long)p->hrow.size();
+  out6[3] = (long long)p->hcol.size();
+  out6[4] = p->max_it;
+  out6[5] = (long long)p->shm;
+  return SGH_OK;
+}
+
+}  // extern "C"
