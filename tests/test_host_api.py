@@ -1,0 +1,179 @@
+"""Host-side layer: config parsing, units, property expressions, registry, solver selection.
+
+Behaviour mirrors the reference's unit tests for the same layer
+(tests/unit_tests/core/test_parameters.py, test_property_expr_precision.py,
+test_properties.py, tests/unit_tests/backends/test_base.py,
+tests/unit_tests/solvers/test_build_solver.py); no GPU is touched.
+"""
+
+import copy
+import math
+
+import numpy as np
+import pytest
+
+from spectral_galerkin_heat_amd.core import units
+from spectral_galerkin_heat_amd.core.laser import (RasterLaserPath, SingleTrackLaser,
+                                                   build_laser_profile,
+                                                   super_gaussian_area_factor)
+from spectral_galerkin_heat_amd.core.parameters import SimulationContext
+from spectral_galerkin_heat_amd.core.properties import (PROGRAM_BYTES, MaterialModel,
+                                                        liquid_fraction)
+from spectral_galerkin_heat_amd.core.property_expr import PropertyExpr
+
+BASE = {
+    "simulation": {"backend": "mi355x", "duration": 1e-4, "dt": 3e-6},
+    "domain": {"size": [4e-4, 4e-4, 1e-4], "mesh": [32, 32, 16]},
+    "material": {"rho": 7850.0, "k": 15.0, "Cp": 500.0, "T0": 293.0, "Pa": 101325.0,
+                 "T_boil": 3090.0, "DeltaH_LV": 7.41e6, "R_v": 150.774},
+    "laser": {"radius": 60e-6, "absorptivity": 0.3, "power_nominal": 200.0},
+    "io": {},
+}
+
+
+def cfg(**over):
+    c = copy.deepcopy(BASE)
+    for k, v in over.items():
+        c[k].update(v)
+    return c
+
+
+def test_step_rounding_and_dtype():
+    ctx = SimulationContext.from_dict(cfg())
+    assert ctx.num.n_steps == round(1e-4 / 3e-6)
+    assert ctx.num.dt * ctx.num.n_steps == pytest.approx(1e-4, rel=1e-15)
+    assert ctx.num.dtype is np.float32
+    assert isinstance(ctx.mat.k, np.float32)
+    ctx64 = SimulationContext.from_dict(cfg(simulation={"dtype": "float64"}))
+    assert ctx64.num.dtype is np.float64 and isinstance(ctx64.mat.k, np.float64)
+    with pytest.raises(ValueError):
+        SimulationContext.from_dict(cfg(simulation={"dtype": "float16"}))
+
+
+def test_unit_conversion_and_mismatch():
+    c = cfg(domain={"size": {"value": [0.4, 0.4, 0.1], "unit": "mm"}},
+            laser={"radius": {"value": 60, "unit": "um"}},
+            material={"rho": {"value": 7.85, "unit": "g/cm**3"},
+                      "h_conv_top": {"value": 10.0, "unit": "W/(m2.K)"}})
+    ctx = SimulationContext.from_dict(c)
+    assert ctx.geom.size.x == pytest.approx(4e-4)
+    assert ctx.laser.radius == pytest.approx(60e-6)
+    assert float(ctx.mat.rho) == pytest.approx(7850.0)
+    assert float(ctx.mat.h_conv_top) == pytest.approx(10.0)
+    with pytest.raises(units.DimensionalityError):
+        SimulationContext.from_dict(cfg(laser={"radius": {"value": 60, "unit": "s"}}))
+    assert units.convert(1.0, "kg/m^3", "kg / m**3") == 1.0
+    assert units.convert(2.0, "mm", "m") == pytest.approx(2e-3)
+    with pytest.raises(units.UnitError):
+        units.parse_unit("furlong")
+
+
+def test_unknown_keys_and_sections_rejected():
+    with pytest.raises(ValueError, match="Unknown key"):
+        SimulationContext.from_dict(cfg(material={"conductivity": 1.0}))
+    bad = cfg()
+    bad["solver"] = {}
+    with pytest.raises(ValueError, match="Unknown config section"):
+        SimulationContext.from_dict(bad)
+    with pytest.raises(ValueError, match="laser.path"):
+        SimulationContext.from_dict(cfg(laser={"path": {"type": "gcode", "speed": 1}}))
+
+
+def test_picard_overrides_are_parsed():
+    ctx = SimulationContext.from_dict(cfg(simulation={"max_picard_iter": 55, "picard_tol": 1e-6,
+                                                      "picard_omega": 0.25}))
+    assert (ctx.num.max_picard_iter, ctx.num.picard_tol, ctx.num.picard_omega) == (55, 1e-6, 0.25)
+
+
+def test_tdep_property_requires_reference():
+    mat = {"k": {"solid": "9.248 + 0.01571 * T", "liquid": "12.41 + 0.003279 * T"},
+           "T_solidus": 1674.15, "T_liquidus": 1697.15}
+    with pytest.raises(ValueError, match="reference"):
+        SimulationContext.from_dict(cfg(material=mat))
+    mat["k"]["reference"] = 24.5
+    ctx = SimulationContext.from_dict(cfg(material=mat))
+    assert float(ctx.mat.k) == pytest.approx(24.5)
+    assert not ctx.mat.model.is_constant
+    # constant branches keep the constant fast path
+    same = {"k": {"solid": "15", "liquid": "15"}}
+    assert SimulationContext.from_dict(cfg(material=same)).mat.model.is_constant
+
+
+def test_property_expressions():
+    assert PropertyExpr("42").is_constant
+    e = PropertyExpr("0.5*(T + 30 - fabs(T - 30))")
+    np.testing.assert_allclose(e(np.array([10.0, 50.0])), [10.0, 30.0])
+    for bad in ("__import__('os')", "T.real", "T if T else 1", "max(T, 1)", "x + 1", "'a'"):
+        with pytest.raises(ValueError):
+            PropertyExpr(bad)
+    ops, consts = PropertyExpr("8084.2 - 0.42086 * T - 3.8942e-5 * T**2").to_program()
+    assert consts == [8084.2, 0.42086, 3.8942e-5, 2.0] or 2.0 not in consts
+
+
+def test_liquid_fraction_and_model_program():
+    assert liquid_fraction(1600.0, 1650.0, 1700.0) == 0.0
+    assert liquid_fraction(1675.0, 1650.0, 1700.0) == pytest.approx(0.5)
+    assert liquid_fraction(1650.0, 1650.0, 1650.0) == 1.0
+    model = MaterialModel.from_config(
+        {"k": {"solid": "9 + 0.01*T", "liquid": "20"}, "rho": 7900.0, "Cp": 500.0}, 1650.0, 1700.0)
+    assert len(model.device_program()) == PROGRAM_BYTES
+    k_ref, rho_ref, c_ref = model.recommended_references(293.0, 3090.0)
+    assert rho_ref == pytest.approx(7900.0) and c_ref == pytest.approx(500.0)
+
+
+def test_laser_profiles_and_paths():
+    assert build_laser_profile("gaussian").order == 2.0
+    assert build_laser_profile("flat_top").order == 12.0
+    assert build_laser_profile("super_gaussian", 6.0).order == 6.0
+    with pytest.raises(ValueError):
+        build_laser_profile("donut")
+    assert super_gaussian_area_factor(2.0) == pytest.approx(math.pi / 2)
+    assert super_gaussian_area_factor(2000.0) == pytest.approx(math.pi, rel=0.01)
+    p = build_laser_profile("gaussian")
+    assert p.peak_intensity(200.0, 0.35, 50e-6 ** 2) == pytest.approx(2 * 0.35 * 200 / (math.pi * 50e-6 ** 2))
+    tr = SingleTrackLaser(0.2e-3, 1.8e-3, 0.5e-3, 0.5, 200.0)
+    s = tr.get_state(1e-3, 1e-5)
+    assert s.is_on and s.x == pytest.approx(0.7e-3) and s.v == (0.5, 0.0)
+    assert not tr.get_state(4e-3, 1e-5).is_on
+    r = RasterLaserPath(3, 1e-3, 2e-3, 1e-3, 80e-6, 1.0, 250.0, 0.5e-3)
+    s0, s_dwell, s1 = r.get_state(0.5e-3, 1e-5), r.get_state(1.2e-3, 1e-5), r.get_state(1.6e-3, 1e-5)
+    assert s0.is_on and s0.v[0] > 0 and s0.y == pytest.approx(1e-3 - 80e-6)
+    assert not s_dwell.is_on
+    assert s1.is_on and s1.v[0] < 0 and s1.y == pytest.approx(1e-3)
+    assert not r.get_state(10e-3, 1e-5).is_on
+
+
+def test_registry_and_build_solver():
+    from spectral_galerkin_heat_amd.backends import (get_backend, register_backend,
+                                                     registered_backends)
+    from spectral_galerkin_heat_amd.solvers import build_solver
+    assert "mi355x" in registered_backends()
+    with pytest.raises(ValueError, match="Unknown backend"):
+        get_backend("cupy")
+    with pytest.raises(ValueError, match="already registered"):
+        register_backend("mi355x")(lambda: None)
+    ctx = SimulationContext.from_dict(cfg(simulation={"backend": "cpu_linear"}))
+    with pytest.raises(ValueError, match="Unknown backend"):
+        build_solver(ctx)
+
+
+def test_batch_rejects_volume_forcing():
+    from spectral_galerkin_heat_amd.engine.batch import check_batchable
+    from spectral_galerkin_heat_amd.workloads import make_context, single_track_constant, single_track_tdep
+    c, lp = single_track_constant()
+    check_batchable(make_context(c, lp))
+    c, lp = single_track_tdep()
+    with pytest.raises(ValueError, match="latent heat"):
+        check_batchable(make_context(c, lp))
+
+
+def test_gcode_path(tmp_path):
+    from spectral_galerkin_heat_amd.io_utils.gcode_path import GCodeLaserPath
+    f = tmp_path / "t.gcode"
+    f.write_text("G21\nG0 X0 Y0\nM3 S200\nG1 X10 Y0 F600\nM5\nG0 X0 Y1\n")
+    p = GCodeLaserPath(str(f))
+    assert len(p.segments) == 1
+    s = p.get_state(0.5, 0.01)
+    assert s.is_on and float(s.x) == pytest.approx(5e-3, rel=1e-6) and float(s.power) == 200.0
+    assert float(s.v[0]) == pytest.approx(0.01, rel=1e-3)
+    assert not p.get_state(2.0, 0.01).is_on
