@@ -76,15 +76,19 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (see DESIGN.md, "
                           "'More than one GPU')"}))
         sys.exit(2)
+    import logging
+
     import numpy as np
     import torch
 
+    logging.getLogger("spectral_galerkin_heat_amd").setLevel(logging.ERROR)
     from spectral_galerkin_heat_amd.engine.batch import BatchEngine
     from spectral_galerkin_heat_amd.workloads import batch_uneven_cuboids, make_context
 
     assert torch.cuda.is_available(), "bench.py needs the MI355X"
     W, K = args.warmup, args.steps
-    total_steps = W + K
+    instr = min(K, 40)
+    total_steps = W + K + instr
     cases = batch_uneven_cuboids(n=64, seed=0, n_steps=total_steps)
     ctxs = [make_context(c, lp) for c, lp in cases]
     N_total = sum(int(np.prod(tuple(c.geom.n))) for c in ctxs)
@@ -126,7 +130,7 @@ def main():
     eng_64.close()
 
     # live per-kernel timing (HIP events on the engine stream) over an instrumented segment
-    instr = min(K, 40)
+    # of the same workload (steps W+K .. W+K+instr, all cuboids still live)
     eng_mx.set_timing(True)
     eng_mx.run(instr, use_graph=False)
     eng_mx.synchronize()

@@ -14,13 +14,14 @@
 // ZE, U, AA of the next step. E and KK are regenerated in FP64 from 1-D tables.
 //
 // Work decomposition: every launch covers all cuboids through a packed work list of
-// (cuboid, 8-row block) items for surface transforms and (cuboid, 256-column block)
+// (cuboid, 16-row block) items for surface transforms and (cuboid, 256-column block)
 // items for the volume pass. Partial sums go to fixed per-item slots and are summed in
 // a fixed order by the next kernel, so a cuboid's arithmetic does not depend on what
 // else is in the batch, and no block ever waits on another inside a kernel.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -31,12 +32,13 @@ static_assert(sizeof(sgh_cuboid_spec) == 176, "sgh_cuboid_spec layout (mirrored 
 namespace sgh {
 
 constexpr int kBT = 256;        // threads per block
-constexpr int kBRB = 8;         // surface rows per transform item
-constexpr int kBKC = 16;        // LDS contraction chunk
+constexpr int kBRB = 16;        // surface rows per transform item (one MFMA M tile)
 constexpr int kBMaxN = 256;     // widest surface axis
-constexpr int kBMaxO = kBRB * kBMaxN / kBT;
 constexpr int kStatusStride = 4;  // n_it, converged, rel, pad
 constexpr int kMetricStride = 5;  // P_laser, T_max, n_it, converged, rel
+constexpr int kSurfArrays = 10;
+constexpr int kLoadAhead = 8;   // MFMA steps whose operands are loaded before use
+constexpr int kXcds = 8;          // MI355X: workgroup b runs on XCD b mod 8
 
 struct CuboidDesc {
   int nx, ny, nz, S;
@@ -45,13 +47,14 @@ struct CuboidDesc {
   int bx_off, by_off;      // offsets of B̂ (n×n) and B̂ᵀ for x and y in the matrix table
   int bxT_off, byT_off;
   int t1_off;              // offset of the 1-D tables (ex, kx2 | ey, ky2 | ez, kz2)
-  int row_item0, nrowblk;  // first transform item and count
-  int col_item0, ncolblk;  // first volume item and count
+  int row_item0, nrowblk;  // first per-cuboid transform slot and block count
+  int col_item0, ncolblk;  // first per-cuboid volume slot and block count
   int max_it;
   int n_steps;             // laser-table length
   long long laser_off;     // row offset into the laser table (6 doubles per row)
   double dx, dy, inv_sqrt_LxLy, Lz;
   double alpha, rho_cp, dt;
+  double w0, w1, inv_arc;  // √(1/Lz), √(2/Lz), 1/(α·ρc)
   double omega, tol, n_elem;
   double f1, f2, T_boil, T_liq;
   double absorptivity, rx, ry, order, area_factor;
@@ -74,6 +77,7 @@ struct BatchPtrs {
   F* q_net;        // net top flux (physical)
   F* q_evap;       // evaporation flux of the current iterate (physical)
   F* q_evap_old;   // previous step's evaporation flux
+  F* Xinv;         // ZE + G∘Ztt: modal input of the inverse surface transform
   const F* mats;   // B̂ and B̂ᵀ matrices
   const double* t1;        // 1-D tables
   const double* laser;     // laser rows
@@ -96,19 +100,24 @@ __device__ __forceinline__ double dexp_(double x) { return exp(x); }
 // ---------------------------------------------------------------------------------
 // setup: Ztt, Wtt (constant per cuboid)
 // ---------------------------------------------------------------------------------
-__device__ __forceinline__ double w_top(int p, double Lz) {
-  const double c = p == 0 ? sqrt(1.0 / Lz) : sqrt(2.0 / Lz);
+__device__ __forceinline__ double w_top(int p, const CuboidDesc& d) {
+  const double c = p == 0 ? d.w0 : d.w1;
   return (p & 1) ? -c : c;
 }
 
+// KK = (1 − E)/(αλ)/(ρc), with 1/λ from v_rcp_f64 refined by two Newton steps.
 __device__ __forceinline__ double kk_of(double e, double lam, const CuboidDesc& d) {
-  const double denom = d.alpha * lam;
-  return denom == 0.0 ? d.dt / d.rho_cp : ((1.0 - e) / denom) / d.rho_cp;
+  if (lam == 0.0) return d.dt / d.rho_cp;
+  double r = __builtin_amdgcn_rcp(lam);
+  r = fma(r, fma(-lam, r, 1.0), r);
+  r = fma(r, fma(-lam, r, 1.0), r);
+  return (1.0 - e) * d.inv_arc * r;
 }
 
 template <typename F>
 __global__ void __launch_bounds__(kBT) k_b_setup(BatchPtrs<F> P) {
   const Item it = P.col_items[blockIdx.x];
+  if (it.c < 0) return;
   const CuboidDesc d = P.desc[it.c];
   const int s = it.blk * kBT + threadIdx.x;
   if (s >= d.S) return;
@@ -125,7 +134,7 @@ __global__ void __launch_bounds__(kBT) k_b_setup(BatchPtrs<F> P) {
   for (int p = 0; p < d.nz; ++p) {
     const double e = ez[p] * eyx;
     const double kk = kk_of(e, kz2[p] + kyx, d);
-    const double w = w_top(p, d.Lz);
+    const double w = w_top(p, d);
     z += w * w * kk;
     w2 += w * w * kk * kk;
   }
@@ -138,8 +147,10 @@ __global__ void __launch_bounds__(kBT) k_b_setup(BatchPtrs<F> P) {
 // ---------------------------------------------------------------------------------
 template <typename F>
 __global__ void __launch_bounds__(kBT) k_b_volume(BatchPtrs<F> P, int finalize) {
+  constexpr int PC = 8;  // z-planes loaded ahead per column
   __shared__ double sh[kBT / kWave];
   const Item it = P.col_items[blockIdx.x];
+  if (it.c < 0) return;
   const CuboidDesc d = P.desc[it.c];
   const int s = it.blk * kBT + threadIdx.x;
   const bool active_cuboid = P.step_idx[it.c] < d.n_steps || !finalize;
@@ -155,29 +166,44 @@ __global__ void __launch_bounds__(kBT) k_b_volume(BatchPtrs<F> P, int finalize) 
     const double eyx = ey[n] * ex[m];
     const double kyx = ky2[n] + kx2[m];
     const double g = finalize ? (double)P.G[d.s_off + s] : 0.0;
-    F* a = P.a + d.a_off + s;
+    F* __restrict__ a = P.a + d.a_off + s;
+    const long long S = d.S;
     double ze = 0.0, u = 0.0;
-    for (int p = 0; p < d.nz; ++p) {
-      const double e = ez[p] * eyx;
-      const double kk = kk_of(e, kz2[p] + kyx, d);
-      const double w = w_top(p, d.Lz);
-      double av = (double)a[(long long)p * d.S];
-      if (finalize) {
-        const F nv = (F)(av * e + kk * (w * g));
-        a[(long long)p * d.S] = nv;
-        av = (double)nv;
+    for (int p0 = 0; p0 < d.nz; p0 += PC) {
+      F buf[PC];
+#pragma unroll
+      for (int q = 0; q < PC; ++q)
+        buf[q] = p0 + q < d.nz ? a[(long long)(p0 + q) * S] : F(0);
+#pragma unroll
+      for (int q = 0; q < PC; ++q) {
+        const int p = p0 + q;
+        if (p >= d.nz) break;
+        const double e = ez[p] * eyx;
+        const double kk = kk_of(e, kz2[p] + kyx, d);
+        const double w = w_top(p, d);
+        double av = (double)buf[q];
+        if (finalize) {
+          const F nv = (F)(av * e + kk * (w * g));
+          buf[q] = nv;
+          av = (double)nv;
+        }
+        const double ae = av * e;
+        ze += w * ae;
+        u += ae * kk * w;
+        aa += ae * ae;
       }
-      const double ae = av * e;
-      ze += w * ae;
-      u += ae * kk * w;
-      aa += ae * ae;
+      if (finalize) {
+#pragma unroll
+        for (int q = 0; q < PC; ++q)
+          if (p0 + q < d.nz) a[(long long)(p0 + q) * S] = buf[q];
+      }
     }
     P.ZE[d.s_off + s] = (F)ze;
     P.U[d.s_off + s] = (F)u;
     if (finalize) P.q_evap_old[d.s_off + s] = P.q_evap[d.s_off + s];
   }
   const double t = block_sum(aa, sh);
-  if (threadIdx.x == 0) P.col_slots[blockIdx.x] = t;
+  if (threadIdx.x == 0) P.col_slots[d.col_item0 + it.blk] = t;
 }
 
 // ---------------------------------------------------------------------------------
@@ -187,6 +213,7 @@ template <typename F>
 __global__ void __launch_bounds__(kBT) k_b_prep(BatchPtrs<F> P) {
   __shared__ double sh[kBT / kWave];
   const Item it = P.row_items[blockIdx.x];
+  if (it.c < 0) return;
   const CuboidDesc d = P.desc[it.c];
   const int step = P.step_idx[it.c];
   const bool live = step < d.n_steps;
@@ -235,7 +262,7 @@ __global__ void __launch_bounds__(kBT) k_b_prep(BatchPtrs<F> P) {
     }
   }
   const double t = block_sum(psum, sh);
-  if (threadIdx.x == 0) P.aux_slots[2 * blockIdx.x] = t;
+  if (threadIdx.x == 0) P.aux_slots[2 * (d.row_item0 + it.blk)] = t;
 }
 
 // Picard decision for cuboid c before iteration k (k = 0: always run).
@@ -274,18 +301,51 @@ __device__ bool picard_active(const BatchPtrs<F>& P, const CuboidDesc& d, int c,
 }
 
 // ---------------------------------------------------------------------------------
-// surface transforms (one kernel, three roles)
+// MFMA tiles. Lane layouts (checked on gfx950): for both instructions A[i][k] sits in
+// lane i + 16k and B[k][j] in lane j + 16k; the 16×16 accumulator holds D[i][j] in lane
+// j + 16(i/4), register i%4 for FP32, and in lane j + 16(i%4), register i/4 for FP64.
+// ---------------------------------------------------------------------------------
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+template <typename F>
+struct Mfma;
+template <>
+struct Mfma<float> {
+  using V = f4v;
+  static __device__ __forceinline__ V mma(float a, float b, V c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+template <>
+struct Mfma<double> {
+  using V = d4v;
+  static __device__ __forceinline__ V mma(double a, double b, V c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int v) { return 4 * v + (lane >> 4); }
+};
+
+// ---------------------------------------------------------------------------------
+// surface transforms (one kernel, four roles), 16 output rows per item:
 //   MODE 0 (warm start): G = dx·dy·Bᵧ q_net Bₓᵀ / √(LxLy)
 //   MODE 1 (iteration, forward half): D = S_raw − G, norm slots, G += ω D
 //   MODE 2 (iteration, inverse half): T_s = Bᵧᵀ(ZE + G∘Ztt)Bₓ/√(LxLy) → q_evap, q_net
 //   MODE 3 (step end): T_s of the final iterate → T_max slot
+// Stage 1, Y = L[rows,:]·X, streams X from L2 straight into MFMA B operands (each
+// element is read by exactly one wave of the item); Y goes through LDS; stage 2,
+// out = Y·R, streams R the same way. Column tiles of 16 are dealt to the 4 waves.
 // ---------------------------------------------------------------------------------
 template <typename F, int MODE>
 __global__ void __launch_bounds__(kBT) k_b_surface(BatchPtrs<F> P, int k) {
+  using M = Mfma<F>;
+  using V = typename M::V;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double sh[kBT / kWave];
   __shared__ int s_active;
   const Item it = P.row_items[blockIdx.x];
+  if (it.c < 0) return;
   const CuboidDesc d = P.desc[it.c];
   const int tid = threadIdx.x;
   if (tid == 0) {
@@ -302,100 +362,88 @@ __global__ void __launch_bounds__(kBT) k_b_surface(BatchPtrs<F> P, int k) {
   __syncthreads();
   if (!s_active) return;
 
-  const int nx = d.nx, ny = d.ny;
-  F* Bs = reinterpret_cast<F*>(smem_raw);   // [kBKC][nx]
-  F* Ys = Bs + kBKC * nx;                    // [kBRB][nx]
-  F* As = Ys + kBRB * nx;                    // [kBRB][kBKC]
+  const int nx = d.nx, ny = d.ny, so = d.s_off;
+  const int ldy = nx + 1;
+  F* Ys = reinterpret_cast<F*>(smem_raw);  // [kBRB][nx + 1]
   const F* Lm = P.mats + (MODE == 0 || MODE == 1 ? d.by_off : d.byT_off);
   const F* Rm = P.mats + (MODE == 0 || MODE == 1 ? d.bxT_off : d.bx_off);
   const int r0 = it.blk * kBRB;
   const int rows = min(kBRB, ny - r0);
-  const int nout = rows * nx;
-  const int so = d.s_off;
+  const int lane = tid & (kWave - 1), wave = tid / kWave;
+  const int li = lane & 15, lk = lane >> 4;
+  const int ntile = (nx + 15) >> 4;
+  const int lrow = r0 + li;
+  const F* Lrow = Lm + (long long)(lrow < ny ? lrow : 0) * ny;
 
-  int ro[kBMaxO], co[kBMaxO];
-  F acc[kBMaxO];
+  // stage 1: Y = L[rows, :] · X, loads issued 8 MFMA steps ahead
+  const F* Xsrc = (MODE <= 1 ? P.q_net : P.Xinv) + so;
+  for (int jt = wave; jt < ntile; jt += kBT / kWave) {
+    const int col = jt * 16 + li;
+    const bool cin = col < nx;
+    V acc = {0, 0, 0, 0};
+    for (int k0 = 0; k0 < ny; k0 += 4 * kLoadAhead) {
+      F a[kLoadAhead], b[kLoadAhead];
 #pragma unroll
-  for (int j = 0; j < kBMaxO; ++j) {
-    const int o = tid + j * kBT;
-    ro[j] = o / nx;
-    co[j] = o - ro[j] * nx;
-    acc[j] = F(0);
-  }
-  for (int k0 = 0; k0 < ny; k0 += kBKC) {
-    const int kc = min(kBKC, ny - k0);
-    for (int e = tid; e < kc * nx; e += kBT) {
-      const int kk = e / nx, c = e - kk * nx;
-      const int s = so + (k0 + kk) * nx + c;
-      F v;
-      if (MODE <= 1) v = P.q_net[s];
-      else v = P.ZE[s] + P.G[s] * P.Ztt[s];
-      Bs[kk * nx + c] = v;
-    }
-    for (int e = tid; e < rows * kBKC; e += kBT) {
-      const int r = e / kBKC, kk = e - r * kBKC;
-      As[r * kBKC + kk] = kk < kc ? Lm[(r0 + r) * ny + k0 + kk] : F(0);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < kBMaxO; ++j) {
-      if (tid + j * kBT < nout) {
-        const F* a = As + ro[j] * kBKC;
-        const F* bc = Bs + co[j];
-        F s = acc[j];
-        for (int kk = 0; kk < kc; ++kk) s += a[kk] * bc[kk * nx];
-        acc[j] = s;
+      for (int q = 0; q < kLoadAhead; ++q) {
+        const int kk = k0 + 4 * q + lk;
+        const bool kin = kk < ny;
+        a[q] = (kin && lrow < ny) ? Lrow[kk] : F(0);
+        b[q] = (kin && cin) ? Xsrc[kk * nx + col] : F(0);
       }
-    }
-    __syncthreads();
-  }
 #pragma unroll
-  for (int j = 0; j < kBMaxO; ++j) {
-    if (tid + j * kBT < nout) Ys[tid + j * kBT] = acc[j];
-    acc[j] = F(0);
+      for (int q = 0; q < kLoadAhead; ++q) acc = M::mma(a[q], b[q], acc);
+    }
+    if (cin) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Ys[M::row(lane, v) * ldy + col] = acc[v];
+    }
   }
   __syncthreads();
-  for (int k0 = 0; k0 < nx; k0 += kBKC) {
-    const int kc = min(kBKC, nx - k0);
-    for (int e = tid; e < kc * nx; e += kBT) {
-      const int kk = e / nx, c = e - kk * nx;
-      Bs[kk * nx + c] = Rm[(k0 + kk) * nx + c];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < kBMaxO; ++j) {
-      if (tid + j * kBT < nout) {
-        const F* y = Ys + ro[j] * nx + k0;
-        const F* bc = Bs + co[j];
-        F s = acc[j];
-        for (int kk = 0; kk < kc; ++kk) s += y[kk] * bc[kk * nx];
-        acc[j] = s;
-      }
-    }
-    __syncthreads();
-  }
 
-  // epilogues
+  // stage 2: out = Y · R, then the epilogue of each role
   double v0 = 0.0, v1 = 0.0, v2 = 0.0;
   if (MODE == 3) v0 = -1.0 / 0.0;
   const double fwd_scale = d.dx * d.dy * d.inv_sqrt_LxLy;
+  for (int jt = wave; jt < ntile; jt += kBT / kWave) {
+    const int col = jt * 16 + li;
+    const bool cin = col < nx;
+    V acc = {0, 0, 0, 0};
+    for (int k0 = 0; k0 < nx; k0 += 4 * kLoadAhead) {
+      F a[kLoadAhead], b[kLoadAhead];
 #pragma unroll
-  for (int j = 0; j < kBMaxO; ++j) {
-    if (tid + j * kBT < nout) {
-      const int s = so + (r0 + ro[j]) * nx + co[j];
+      for (int q = 0; q < kLoadAhead; ++q) {
+        const int kk = k0 + 4 * q + lk;
+        const bool kin = kk < nx;
+        a[q] = kin ? Ys[li * ldy + kk] : F(0);
+        b[q] = (kin && cin) ? Rm[kk * nx + col] : F(0);
+      }
+#pragma unroll
+      for (int q = 0; q < kLoadAhead; ++q) acc = M::mma(a[q], b[q], acc);
+    }
+    if (!cin) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = M::row(lane, v);
+      if (row >= rows) continue;
+      const int s = so + (r0 + row) * nx + col;
+      const F val = acc[v];
       if (MODE == 0) {
-        P.G[s] = (F)((double)acc[j] * fwd_scale);
+        const F g = (F)((double)val * fwd_scale);
+        P.G[s] = g;
+        P.Xinv[s] = P.ZE[s] + g * P.Ztt[s];
       } else if (MODE == 1) {
-        const F Sr = (F)((double)acc[j] * fwd_scale);
+        const F Sr = (F)((double)val * fwd_scale);
         const F g = P.G[s];
         const F dd = Sr - g;
         const double w = (double)P.Wtt[s];
         v0 += (double)dd * (double)dd * w;
         v1 += (double)g * (double)P.U[s];
         v2 += (double)g * (double)g * w;
-        P.G[s] = g + (F)d.omega * dd;
+        const F gn = g + (F)d.omega * dd;
+        P.G[s] = gn;
+        P.Xinv[s] = P.ZE[s] + gn * P.Ztt[s];
       } else {
-        const F T = (F)((double)acc[j] * d.inv_sqrt_LxLy);
+        const F T = (F)((double)val * d.inv_sqrt_LxLy);
         if (MODE == 2) {
           F qe = F(0);
           if (T >= (F)d.T_liq)
@@ -413,14 +461,14 @@ __global__ void __launch_bounds__(kBT) k_b_surface(BatchPtrs<F> P, int k) {
     const double a1 = block_sum(v1, sh);
     const double a2 = block_sum(v2, sh);
     if (tid == 0) {
-      double* sl = P.row_slots + 3 * blockIdx.x;
+      double* sl = P.row_slots + 3 * (d.row_item0 + it.blk);
       sl[0] = a0;
       sl[1] = a1;
       sl[2] = a2;
     }
   } else if (MODE == 3) {
     const double m = block_max(v0, sh);
-    if (tid == 0) P.aux_slots[2 * blockIdx.x + 1] = m;
+    if (tid == 0) P.aux_slots[2 * (d.row_item0 + it.blk) + 1] = m;
   }
 }
 
@@ -460,6 +508,7 @@ struct Plan {
   size_t elem = 8;
   std::vector<CuboidDesc> hdesc;
   std::vector<Item> hrow, hcol;
+  int n_row_slots = 0, n_col_slots = 0;
   int nx_max = 0;
   size_t shm = 0;
   long long total_N = 0, total_S = 0;
@@ -506,6 +555,7 @@ BatchPtrs<F> ptrs(Plan* p) {
   P.q_net = s + 6 * T;
   P.q_evap = s + 7 * T;
   P.q_evap_old = s + 8 * T;
+  P.Xinv = s + 9 * T;
   P.mats = (const F*)p->mats;
   P.t1 = p->t1;
   P.laser = p->laser;
@@ -578,6 +628,37 @@ static std::vector<double> unit_basis(int n) {
   return B;
 }
 
+// XCD-aware work lists: each cuboid is pinned to one of the 8 XCDs (greedy balance of
+// its transform work) and its items are placed at list positions ≡ xcd (mod 8), so all
+// workgroups of a cuboid share one L2. Short bins are padded with empty items (c = −1).
+static void build_xcd_items(Plan* p) {
+  std::vector<double> load(kXcds, 0.0);
+  std::vector<std::vector<Item>> rb(kXcds), cb(kXcds);
+  std::vector<int> order(p->n);
+  for (int c = 0; c < p->n; ++c) order[c] = c;
+  auto cost = [&](int c) {
+    const CuboidDesc& d = p->hdesc[c];
+    return (double)d.nrowblk * d.nx * (d.nx + d.ny) + 0.05 * (double)d.S * d.nz;
+  };
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+  for (int c : order) {
+    int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[x] += cost(c);
+    const CuboidDesc& d = p->hdesc[c];
+    for (int b = 0; b < d.nrowblk; ++b) rb[x].push_back({c, b});
+    for (int b = 0; b < d.ncolblk; ++b) cb[x].push_back({c, b});
+  }
+  auto interleave = [](std::vector<std::vector<Item>>& bins, std::vector<Item>& out) {
+    size_t m = 0;
+    for (auto& b : bins) m = std::max(m, b.size());
+    out.clear();
+    for (size_t r = 0; r < m; ++r)
+      for (int x = 0; x < kXcds; ++x) out.push_back(r < bins[x].size() ? bins[x][r] : Item{-1, 0});
+  };
+  interleave(rb, p->hrow);
+  interleave(cb, p->hcol);
+}
+
 static void free_plan(Plan* p) {
   if (!p) return;
   if (p->graph_exec) hipGraphExecDestroy(p->graph_exec);
@@ -635,6 +716,7 @@ int sgh_batch_create(int dtype, int n, const sgh_cuboid_spec* specs, int metrics
 
   std::vector<double> t1, laser;
   long long a_off = 0, s_off = 0;
+  int n_row_slots = 0, n_col_slots = 0;
   for (int c = 0; c < n; ++c) {
     const sgh_cuboid_spec& sp = specs[c];
     if (sp.nx < 2 || sp.ny < 1 || sp.nz < 1 || sp.nx > kBMaxN || sp.ny > kBMaxN ||
@@ -674,12 +756,12 @@ int sgh_batch_create(int dtype, int n, const sgh_cuboid_spec* specs, int metrics
       t1.insert(t1.end(), e.begin(), e.end());
       t1.insert(t1.end(), k2.begin(), k2.end());
     }
-    d.row_item0 = (int)p->hrow.size();
+    d.row_item0 = n_row_slots;
     d.nrowblk = (sp.ny + kBRB - 1) / kBRB;
-    for (int b = 0; b < d.nrowblk; ++b) p->hrow.push_back({c, b});
-    d.col_item0 = (int)p->hcol.size();
+    n_row_slots += d.nrowblk;
+    d.col_item0 = n_col_slots;
     d.ncolblk = (d.S + kBT - 1) / kBT;
-    for (int b = 0; b < d.ncolblk; ++b) p->hcol.push_back({c, b});
+    n_col_slots += d.ncolblk;
     d.max_it = sp.max_it;
     d.n_steps = sp.n_steps;
     d.laser_off = (long long)laser.size() / 6;
@@ -690,6 +772,9 @@ int sgh_batch_create(int dtype, int n, const sgh_cuboid_spec* specs, int metrics
     d.Lz = sp.Lz;
     d.alpha = alpha;
     d.rho_cp = sp.rho * sp.cp;
+    d.w0 = sqrt(1.0 / sp.Lz);
+    d.w1 = sqrt(2.0 / sp.Lz);
+    d.inv_arc = 1.0 / (alpha * d.rho_cp);
     d.dt = sp.dt;
     d.omega = sp.omega;
     d.tol = sp.tol;
@@ -710,23 +795,26 @@ int sgh_batch_create(int dtype, int n, const sgh_cuboid_spec* specs, int metrics
     p->nx_max = sp.nx > p->nx_max ? sp.nx : p->nx_max;
   }
   p->total_N = a_off;
+  p->n_row_slots = n_row_slots;
+  p->n_col_slots = n_col_slots;
+  build_xcd_items(p);
   p->total_S = s_off;
-  p->shm = (size_t)(kBKC * p->nx_max + kBRB * p->nx_max + kBRB * kBKC) * p->elem;
+  p->shm = (size_t)(kBRB * (p->nx_max + 1)) * p->elem;
 
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
   const size_t E = p->elem;
   chk(dmalloc(&p->a, p->total_N * E));
-  chk(dmalloc(&p->surf, 9 * p->total_S * E));
+  chk(dmalloc(&p->surf, kSurfArrays * p->total_S * E));
   chk(dmalloc(&p->mats, mats.size() * E));
   chk(dmalloc(&p->t1, t1.size() * sizeof(double)));
   chk(dmalloc(&p->laser, laser.size() * sizeof(double)));
   chk(dmalloc(&p->desc, n * sizeof(CuboidDesc)));
   chk(dmalloc(&p->row_items, p->hrow.size() * sizeof(Item)));
   chk(dmalloc(&p->col_items, p->hcol.size() * sizeof(Item)));
-  chk(dmalloc(&p->row_slots, 3 * p->hrow.size() * sizeof(double)));
-  chk(dmalloc(&p->col_slots, p->hcol.size() * sizeof(double)));
-  chk(dmalloc(&p->aux_slots, 2 * p->hrow.size() * sizeof(double)));
+  chk(dmalloc(&p->row_slots, 3 * (size_t)p->n_row_slots * sizeof(double)));
+  chk(dmalloc(&p->col_slots, (size_t)p->n_col_slots * sizeof(double)));
+  chk(dmalloc(&p->aux_slots, 2 * (size_t)p->n_row_slots * sizeof(double)));
   chk(dmalloc(&p->done_at, n * sizeof(int)));
   chk(dmalloc(&p->status, kStatusStride * n * sizeof(double)));
   chk(dmalloc(&p->step_idx, n * sizeof(int)));
@@ -748,7 +836,7 @@ int sgh_batch_create(int dtype, int n, const sgh_cuboid_spec* specs, int metrics
   chk(hipMemcpyAsync(p->desc, p->hdesc.data(), n * sizeof(CuboidDesc), hipMemcpyHostToDevice, st));
   chk(hipMemcpyAsync(p->row_items, p->hrow.data(), p->hrow.size() * sizeof(Item), hipMemcpyHostToDevice, st));
   chk(hipMemcpyAsync(p->col_items, p->hcol.data(), p->hcol.size() * sizeof(Item), hipMemcpyHostToDevice, st));
-  chk(hipMemsetAsync(p->surf, 0, 9 * p->total_S * E, st));
+  chk(hipMemsetAsync(p->surf, 0, kSurfArrays * p->total_S * E, st));
   chk(hipMemsetAsync(p->a, 0, p->total_N * E, st));
   chk(hipMemsetAsync(p->step_idx, 0, n * sizeof(int), st));
   chk(hipMemsetAsync(p->status, 0, kStatusStride * n * sizeof(double), st));
