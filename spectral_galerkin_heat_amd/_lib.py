@@ -88,6 +88,9 @@ class _Lib:
     def load(self):
         if self._h is not None:
             return self._h
+        # torch's HIP runtime must be the one in the process before this library binds
+        # libamdhip64, so device buffers and streams are shared with torch.
+        import torch  # noqa: F401
         if not os.path.exists(LIB_PATH):
             raise SGHError(
                 f"{LIB_PATH} is missing: build it with `python -m "

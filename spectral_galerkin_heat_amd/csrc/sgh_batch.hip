@@ -32,12 +32,16 @@ static_assert(sizeof(sgh_cuboid_spec) == 176, "sgh_cuboid_spec layout (mirrored 
 namespace sgh {
 
 constexpr int kBT = 256;        // threads per block
-constexpr int kBRB = 16;        // surface rows per transform item (one MFMA M tile)
+constexpr int kBRB = 16;        // surface rows per transform item (one MFMA row tile)
+constexpr int kMT = kBRB / 16;   // MFMA row tiles per item
 constexpr int kBMaxN = 256;     // widest surface axis
 constexpr int kStatusStride = 4;  // n_it, converged, rel, pad
 constexpr int kMetricStride = 5;  // P_laser, T_max, n_it, converged, rel
 constexpr int kSurfArrays = 10;
-constexpr int kLoadAhead = 8;   // MFMA steps whose operands are loaded before use
+constexpr int kST = 512;         // threads of a surface-transform workgroup
+constexpr int kStageAhead = 8;   // global loads in flight per thread while staging into LDS
+constexpr int kTilesPerWave = 2; // 16-column tiles per wave: 16 tiles (nx = 256) over 8 waves
+constexpr int kSurfLds = 150 * 1024;  // LDS bytes per surface workgroup (gfx950 allows 160 KiB)
 constexpr int kXcds = 8;          // MI355X: workgroup b runs on XCD b mod 8
 
 struct CuboidDesc {
@@ -337,12 +341,72 @@ struct Mfma<double> {
 // element is read by exactly one wave of the item); Y goes through LDS; stage 2,
 // out = Y·R, streams R the same way. Column tiles of 16 are dealt to the 4 waves.
 // ---------------------------------------------------------------------------------
+// LDS leading dimensions (in elements) chosen for conflict-free MFMA operand reads: the
+// A operand (16 rows × 4 k) wants a row stride ≡ 2 words (mod 32), the B operand
+// (4 k-rows × 16 columns) a row stride ≡ 16 words (mod 32); a double is 2 words.
+template <typename F>
+__host__ __device__ constexpr int pad_to_mod(int n, int target_words) {
+  const int wpe = (int)(sizeof(F) / 4);
+  const int words = n * wpe;
+  const int add = ((target_words - words % 32) % 32 + 32) % 32;
+  return n + (add + wpe - 1) / wpe;
+}
+template <typename F>
+__host__ __device__ constexpr int pad_a(int n) { return pad_to_mod<F>(n, 2 * (int)(sizeof(F) / 4)); }
+template <typename F>
+__host__ __device__ constexpr int pad_b(int n) { return pad_to_mod<F>(n, 16); }
+
+
+// Copy a rows × cols block (source row stride src_ld, starting at src) into LDS
+// (row stride dst_ld), zero-filling rows >= valid_rows and columns >= valid_cols.
+// Each thread keeps kStageAhead global loads in flight; indices advance incrementally
+// (no per-element division).
+template <typename F, int NT>
+__device__ __forceinline__ void stage_block(F* __restrict__ dst, int dst_ld, const F* __restrict__ src,
+                                            int src_ld, int rows, int cols, int valid_rows,
+                                            int valid_cols) {
+  constexpr int A = kStageAhead;
+  const int total = rows * cols;
+  const int step_r = NT / cols, step_c = NT - (NT / cols) * cols;
+  int r = threadIdx.x / cols, c = threadIdx.x - (threadIdx.x / cols) * cols;
+  for (int e0 = threadIdx.x; e0 < total; e0 += A * NT) {
+    F tmp[A];
+    int rr[A], cc[A];
+#pragma unroll
+    for (int q = 0; q < A; ++q) {
+      rr[q] = r;
+      cc[q] = c;
+      const bool in = e0 + q * NT < total;
+      tmp[q] = (in && r < valid_rows && c < valid_cols) ? src[(long long)r * src_ld + c] : F(0);
+      r += step_r;
+      c += step_c;
+      if (c >= cols) { c -= cols; ++r; }
+    }
+#pragma unroll
+    for (int q = 0; q < A; ++q)
+      if (e0 + q * NT < total) dst[rr[q] * dst_ld + cc[q]] = tmp[q];
+  }
+}
+
+#ifdef SGH_BLOCK_TIMESTAMPS
+__device__ unsigned long long g_block_ts[4 * 8192 * 6];
+#define SGH_TS(slot)                                                                  \
+  do {                                                                                \
+    if (threadIdx.x == 0 && blockIdx.x < 8192)                                        \
+      g_block_ts[(MODE * 8192 + blockIdx.x) * 6 + (slot)] = wall_clock64();           \
+  } while (0)
+#else
+#define SGH_TS(slot) do {} while (0)
+#endif
+
 template <typename F, int MODE>
-__global__ void __launch_bounds__(kBT) k_b_surface(BatchPtrs<F> P, int k) {
+__global__ void __launch_bounds__(kST) k_b_surface(BatchPtrs<F> P, int k, int bud) {
+  SGH_TS(0);
   using M = Mfma<F>;
   using V = typename M::V;
+  constexpr int NW = kST / kWave;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  __shared__ double sh[kBT / kWave];
+  __shared__ double sh[kST / kWave];
   __shared__ int s_active;
   const Item it = P.row_items[blockIdx.x];
   if (it.c < 0) return;
@@ -361,101 +425,170 @@ __global__ void __launch_bounds__(kBT) k_b_surface(BatchPtrs<F> P, int k) {
   }
   __syncthreads();
   if (!s_active) return;
+  SGH_TS(1);
 
   const int nx = d.nx, ny = d.ny, so = d.s_off;
-  const int ldy = nx + 1;
-  F* Ys = reinterpret_cast<F*>(smem_raw);  // [kBRB][nx + 1]
-  const F* Lm = P.mats + (MODE == 0 || MODE == 1 ? d.by_off : d.byT_off);
-  const F* Rm = P.mats + (MODE == 0 || MODE == 1 ? d.bxT_off : d.bx_off);
   const int r0 = it.blk * kBRB;
   const int rows = min(kBRB, ny - r0);
   const int lane = tid & (kWave - 1), wave = tid / kWave;
   const int li = lane & 15, lk = lane >> 4;
   const int ntile = (nx + 15) >> 4;
-  const int lrow = r0 + li;
-  const F* Lrow = Lm + (long long)(lrow < ny ? lrow : 0) * ny;
-
-  // stage 1: Y = L[rows, :] · X, loads issued 8 MFMA steps ahead
+  const int ny4 = (ny + 3) & ~3, nx4 = (nx + 3) & ~3;
+  // LDS layout: Ls[kBRB][ldl] | Ys[kBRB][ldy] | Bb[bud]
+  const int ldl = pad_a<F>(ny4), ldy = pad_a<F>(nx4);
+  F* Ls = reinterpret_cast<F*>(smem_raw);
+  F* Ys = Ls + kBRB * ldl;
+  F* Bb = Ys + kBRB * ldy;
+  const F* Lm = P.mats + (MODE == 0 || MODE == 1 ? d.by_off : d.byT_off);
+  const F* Rm = P.mats + (MODE == 0 || MODE == 1 ? d.bxT_off : d.bx_off);
   const F* Xsrc = (MODE <= 1 ? P.q_net : P.Xinv) + so;
-  for (int jt = wave; jt < ntile; jt += kBT / kWave) {
-    const int col = jt * 16 + li;
-    const bool cin = col < nx;
-    V acc = {0, 0, 0, 0};
-    for (int k0 = 0; k0 < ny; k0 += 4 * kLoadAhead) {
-      F a[kLoadAhead], b[kLoadAhead];
+
+  // L rows of this item (zero beyond ny and beyond the last row)
+  stage_block<F, kST>(Ls, ldl, Lm + (long long)r0 * ny, ny, kBRB, ny4, rows, ny);
+
+  // stage 1: Y = L[rows, :] · X, X staged in chunks of kr rows (usually all of it)
+  const int ldb1 = pad_b<F>(nx);
+  const int kr = min(ny4, (bud / ldb1) & ~3);
+  if (kr < 4) return;  // the host guarantees bud >= 4·ldb1; never taken
+  V acc[kMT][kTilesPerWave];
 #pragma unroll
-      for (int q = 0; q < kLoadAhead; ++q) {
-        const int kk = k0 + 4 * q + lk;
-        const bool kin = kk < ny;
-        a[q] = (kin && lrow < ny) ? Lrow[kk] : F(0);
-        b[q] = (kin && cin) ? Xsrc[kk * nx + col] : F(0);
+  for (int m = 0; m < kMT; ++m)
+#pragma unroll
+    for (int t = 0; t < kTilesPerWave; ++t) acc[m][t] = V{0, 0, 0, 0};
+  for (int kc0 = 0; kc0 < ny4; kc0 += kr) {
+    const int kn = min(kr, ny4 - kc0);
+    if (kc0 > 0) __syncthreads();
+    stage_block<F, kST>(Bb, ldb1, Xsrc + (long long)kc0 * nx, nx, kn, nx, ny - kc0, nx);
+    __syncthreads();
+    SGH_TS(2);
+#pragma unroll
+    for (int t = 0; t < kTilesPerWave; ++t) {
+      const int jt = wave + NW * t;
+      if (jt >= ntile) break;
+      const int col = jt * 16 + li;
+      const bool cin = col < nx;
+#pragma unroll 4
+      for (int kk = 0; kk < kn; kk += 4) {
+        const F bv = cin ? Bb[(kk + lk) * ldb1 + col] : F(0);
+#pragma unroll
+        for (int m = 0; m < kMT; ++m)
+          acc[m][t] = M::mma(Ls[(16 * m + li) * ldl + kc0 + kk + lk], bv, acc[m][t]);
       }
-#pragma unroll
-      for (int q = 0; q < kLoadAhead; ++q) acc = M::mma(a[q], b[q], acc);
-    }
-    if (cin) {
-#pragma unroll
-      for (int v = 0; v < 4; ++v) Ys[M::row(lane, v) * ldy + col] = acc[v];
     }
   }
-  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < kTilesPerWave; ++t) {
+    const int jt = wave + NW * t;
+    const int col = jt * 16 + li;
+#pragma unroll
+    for (int m = 0; m < kMT; ++m) {
+      if (jt < ntile && col < nx) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Ys[(16 * m + M::row(lane, v)) * ldy + col] = acc[m][t][v];
+      }
+      acc[m][t] = V{0, 0, 0, 0};
+    }
+  }
+  if (nx4 > nx)  // zero the k-padding columns of Y
+    for (int e = tid; e < kBRB * (nx4 - nx); e += kST) {
+      const int r = e / (nx4 - nx), c = nx + e % (nx4 - nx);
+      Ys[r * ldy + c] = F(0);
+    }
 
-  // stage 2: out = Y · R, then the epilogue of each role
+  SGH_TS(3);
+  // epilogue operands of this lane's outputs, loaded before stage 2 waits
+  constexpr int NE = 4 * kMT * kTilesPerWave;
+  F e0[NE], e1[NE], e2[NE];
+#pragma unroll
+  for (int t = 0; t < kTilesPerWave; ++t) {
+    const int jt = wave + NW * t;
+    const int col = jt * 16 + li;
+#pragma unroll
+    for (int m = 0; m < kMT; ++m)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int j = (t * kMT + m) * 4 + v;
+        const int row = 16 * m + M::row(lane, v);
+        e0[j] = e1[j] = e2[j] = F(0);
+        if (jt < ntile && col < nx && row < rows) {
+          const int s = so + (r0 + row) * nx + col;
+          if (MODE == 0) { e0[j] = P.ZE[s]; e1[j] = P.Ztt[s]; }
+          if (MODE == 2) { e0[j] = P.q_las[s]; }
+        }
+      }
+  }
+
+  // stage 2: out = Y · R, R staged in column chunks of cw (usually all of it)
+  int cw = min(ntile * 16, max(16, ((bud / nx4) / 16) * 16));
+  while (cw > 16 && nx4 * pad_b<F>(cw) > bud) cw -= 16;
+  const int ldb2 = pad_b<F>(cw);
   double v0 = 0.0, v1 = 0.0, v2 = 0.0;
   if (MODE == 3) v0 = -1.0 / 0.0;
   const double fwd_scale = d.dx * d.dy * d.inv_sqrt_LxLy;
-  for (int jt = wave; jt < ntile; jt += kBT / kWave) {
-    const int col = jt * 16 + li;
-    const bool cin = col < nx;
-    V acc = {0, 0, 0, 0};
-    for (int k0 = 0; k0 < nx; k0 += 4 * kLoadAhead) {
-      F a[kLoadAhead], b[kLoadAhead];
+  for (int c0 = 0; c0 < nx; c0 += cw) {
+    const int cn = min(cw, nx - c0);
+    __syncthreads();
+    stage_block<F, kST>(Bb, ldb2, Rm + c0, nx, nx4, cn, nx, cn);
+    __syncthreads();
+    SGH_TS(4);
 #pragma unroll
-      for (int q = 0; q < kLoadAhead; ++q) {
-        const int kk = k0 + 4 * q + lk;
-        const bool kin = kk < nx;
-        a[q] = kin ? Ys[li * ldy + kk] : F(0);
-        b[q] = (kin && cin) ? Rm[kk * nx + col] : F(0);
+    for (int t = 0; t < kTilesPerWave; ++t) {
+      const int jt = wave + NW * t;
+      if (jt >= ntile) break;
+      const int col = jt * 16 + li;
+      if (col - li < c0 || col - li >= c0 + cn) continue;  // tile not in this chunk
+      const bool cin = col < nx;
+      V a_[kMT];
+#pragma unroll
+      for (int m = 0; m < kMT; ++m) a_[m] = V{0, 0, 0, 0};
+#pragma unroll 4
+      for (int kk = 0; kk < nx4; kk += 4) {
+        const F bv = cin ? Bb[(kk + lk) * ldb2 + (col - c0)] : F(0);
+#pragma unroll
+        for (int m = 0; m < kMT; ++m)
+          a_[m] = M::mma(Ys[(16 * m + li) * ldy + kk + lk], bv, a_[m]);
       }
+      if (!cin) continue;
 #pragma unroll
-      for (int q = 0; q < kLoadAhead; ++q) acc = M::mma(a[q], b[q], acc);
-    }
-    if (!cin) continue;
+      for (int m = 0; m < kMT; ++m)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int row = M::row(lane, v);
-      if (row >= rows) continue;
-      const int s = so + (r0 + row) * nx + col;
-      const F val = acc[v];
-      if (MODE == 0) {
-        const F g = (F)((double)val * fwd_scale);
-        P.G[s] = g;
-        P.Xinv[s] = P.ZE[s] + g * P.Ztt[s];
-      } else if (MODE == 1) {
-        const F Sr = (F)((double)val * fwd_scale);
-        const F g = P.G[s];
-        const F dd = Sr - g;
-        const double w = (double)P.Wtt[s];
-        v0 += (double)dd * (double)dd * w;
-        v1 += (double)g * (double)P.U[s];
-        v2 += (double)g * (double)g * w;
-        const F gn = g + (F)d.omega * dd;
-        P.G[s] = gn;
-        P.Xinv[s] = P.ZE[s] + gn * P.Ztt[s];
-      } else {
-        const F T = (F)((double)val * d.inv_sqrt_LxLy);
-        if (MODE == 2) {
-          F qe = F(0);
-          if (T >= (F)d.T_liq)
-            qe = (F)d.f1 * ((F(1) / sqrt(T)) * exp((F)d.f2 * (F(1) - (F)d.T_boil / T)));
-          P.q_evap[s] = qe;
-          P.q_net[s] = P.q_las[s] - qe;
-        } else {
-          v0 = (double)T > v0 ? (double)T : v0;
+        for (int v = 0; v < 4; ++v) {
+          const int row = 16 * m + M::row(lane, v);
+          if (row >= rows) continue;
+          const int s = so + (r0 + row) * nx + col;
+          const int j = (t * kMT + m) * 4 + v;
+          const F val = a_[m][v];
+          if (MODE == 0) {
+            const F g = (F)((double)val * fwd_scale);
+            P.G[s] = g;
+            P.Xinv[s] = e0[j] + g * e1[j];
+          } else if (MODE == 1) {
+            const F Sr = (F)((double)val * fwd_scale);
+            const F g = P.G[s];
+            const F dd = Sr - g;
+            const double w = (double)P.Wtt[s];
+            v0 += (double)dd * (double)dd * w;
+            v1 += (double)g * (double)P.U[s];
+            v2 += (double)g * (double)g * w;
+            const F gn = g + (F)d.omega * dd;
+            P.G[s] = gn;
+            P.Xinv[s] = P.ZE[s] + gn * P.Ztt[s];
+          } else {
+            const F T = (F)((double)val * d.inv_sqrt_LxLy);
+            if (MODE == 2) {
+              F qe = F(0);
+              if (T >= (F)d.T_liq)
+                qe = (F)d.f1 * ((F(1) / sqrt(T)) * exp((F)d.f2 * (F(1) - (F)d.T_boil / T)));
+              P.q_evap[s] = qe;
+              P.q_net[s] = e0[j] - qe;
+            } else {
+              v0 = (double)T > v0 ? (double)T : v0;
+            }
+          }
         }
-      }
     }
   }
+  SGH_TS(5);
   if (MODE == 1) {
     const double a0 = block_sum(v0, sh);
     const double a1 = block_sum(v1, sh);
@@ -509,8 +642,9 @@ struct Plan {
   std::vector<CuboidDesc> hdesc;
   std::vector<Item> hrow, hcol;
   int n_row_slots = 0, n_col_slots = 0;
-  int nx_max = 0;
+  int nx_max = 0, ny_max = 0;
   size_t shm = 0;
+  int bud = 0;
   long long total_N = 0, total_S = 0;
   // device allocations
   void* a = nullptr;
@@ -580,14 +714,14 @@ int enqueue_step(Plan* p, hipStream_t st) {
   const unsigned nrow = (unsigned)p->hrow.size(), ncol = (unsigned)p->hcol.size();
   const bool tm = p->timing;
   hipLaunchKernelGGL(k_b_prep<F>, dim3(nrow), dim3(kBT), 0, st, P);
-  hipLaunchKernelGGL((k_b_surface<F, 0>), dim3(nrow), dim3(kBT), p->shm, st, P, 0);
+  hipLaunchKernelGGL((k_b_surface<F, 0>), dim3(nrow), dim3(kST), p->shm, st, P, 0, p->bud);
   if (tm) hipEventRecord(p->ev[0], st);
   for (int k = 0; k < p->max_it; ++k) {
-    hipLaunchKernelGGL((k_b_surface<F, 2>), dim3(nrow), dim3(kBT), p->shm, st, P, k);
-    hipLaunchKernelGGL((k_b_surface<F, 1>), dim3(nrow), dim3(kBT), p->shm, st, P, k);
+    hipLaunchKernelGGL((k_b_surface<F, 2>), dim3(nrow), dim3(kST), p->shm, st, P, k, p->bud);
+    hipLaunchKernelGGL((k_b_surface<F, 1>), dim3(nrow), dim3(kST), p->shm, st, P, k, p->bud);
   }
   if (tm) hipEventRecord(p->ev[1], st);
-  hipLaunchKernelGGL((k_b_surface<F, 3>), dim3(nrow), dim3(kBT), p->shm, st, P, p->max_it);
+  hipLaunchKernelGGL((k_b_surface<F, 3>), dim3(nrow), dim3(kST), p->shm, st, P, p->max_it, p->bud);
   if (tm) hipEventRecord(p->ev[2], st);
   hipLaunchKernelGGL(k_b_volume<F>, dim3(ncol), dim3(kBT), 0, st, P, 1);
   if (tm) hipEventRecord(p->ev[3], st);
@@ -605,6 +739,20 @@ int enqueue_step(Plan* p, hipStream_t st) {
     p->n_volume += 1;
   }
   return SGH_OK;
+}
+
+template <typename F>
+static hipError_t allow_large_lds(size_t bytes) {
+  hipError_t e = hipSuccess;
+  const void* fns[] = {reinterpret_cast<const void*>(&k_b_surface<F, 0>),
+                       reinterpret_cast<const void*>(&k_b_surface<F, 1>),
+                       reinterpret_cast<const void*>(&k_b_surface<F, 2>),
+                       reinterpret_cast<const void*>(&k_b_surface<F, 3>)};
+  for (const void* f : fns) {
+    hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e == hipSuccess) e = r;
+  }
+  return e;
 }
 
 template <typename F>
@@ -793,17 +941,35 @@ int sgh_batch_create(int dtype, int n, const sgh_cuboid_spec* specs, int metrics
     p->hdesc.push_back(d);
     p->max_it = sp.max_it > p->max_it ? sp.max_it : p->max_it;
     p->nx_max = sp.nx > p->nx_max ? sp.nx : p->nx_max;
+    p->ny_max = sp.ny > p->ny_max ? sp.ny : p->ny_max;
   }
   p->total_N = a_off;
   p->n_row_slots = n_row_slots;
   p->n_col_slots = n_col_slots;
   build_xcd_items(p);
   p->total_S = s_off;
-  p->shm = (size_t)(kBRB * (p->nx_max + 1)) * p->elem;
+  {
+    // LDS of a surface workgroup: Ls[16][ldl] + Ys[16][ldy] + a staging buffer of `bud`
+    // elements that holds X (or R) whole when it fits, else in chunks.
+    const int ldl = dtype == SGH_F64 ? pad_a<double>((p->ny_max + 3) & ~3) : pad_a<float>((p->ny_max + 3) & ~3);
+    const int ldy = dtype == SGH_F64 ? pad_a<double>((p->nx_max + 3) & ~3) : pad_a<float>((p->nx_max + 3) & ~3);
+    const long long elems = (long long)kSurfLds / (long long)p->elem;
+    p->bud = (int)(elems - (long long)kBRB * ldl - (long long)kBRB * ldy);
+    p->shm = (size_t)kSurfLds;
+    const int nx4 = (p->nx_max + 3) & ~3;
+    const long long need1 = 4LL * (dtype == SGH_F64 ? pad_b<double>(p->nx_max) : pad_b<float>(p->nx_max));
+    const long long need2 = (long long)nx4 * (dtype == SGH_F64 ? pad_b<double>(16) : pad_b<float>(16));
+    if (p->bud < need1 || p->bud < need2) {
+      free_plan(p);
+      set_error("sgh_batch_create: surface too wide for the LDS staging buffer");
+      return SGH_ERR_ARG;
+    }
+  }
 
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
   const size_t E = p->elem;
+  chk(dtype == SGH_F64 ? allow_large_lds<double>(p->shm) : allow_large_lds<float>(p->shm));
   chk(dmalloc(&p->a, p->total_N * E));
   chk(dmalloc(&p->surf, kSurfArrays * p->total_S * E));
   chk(dmalloc(&p->mats, mats.size() * E));
@@ -974,6 +1140,12 @@ int sgh_batch_timing(void* plan, double* out4) {
   out4[3] = (double)p->n_volume;
   return SGH_OK;
 }
+
+#ifdef SGH_BLOCK_TIMESTAMPS
+int sgh_debug_block_timestamps(unsigned long long* host_out, int n) {
+  return hipMemcpyFromSymbol(host_out, HIP_SYMBOL(g_block_ts), (size_t)4 * 8192 * 6 * 8) == hipSuccess ? 0 : -2;
+}
+#endif
 
 int sgh_batch_info(void* plan, long long* out6) {
   Plan* p = static_cast<Plan*>(plan);
