@@ -11,7 +11,8 @@
  *     the stream has been synchronised.
  *   - Return 0 on success or a negative SGH_ERR_* code; sgh_last_error() gives a
  *     per-thread message. No C++ exception crosses the ABI. No call allocates
- *     device memory except sgh_batch_create (which owns its packed state).
+ *     device memory except sgh_batch_pool_create / sgh_batch_create (a plan owns its
+ *     packed state).
  *
  * Each entry point cites the reference function it replaces (paths relative to
  * the reference checkout; sgh/ = src/spectral_galerkin_heat/).
@@ -32,7 +33,7 @@ enum {
   SGH_ERR_ARG = -1,   /* bad shape, null pointer, unsupported dtype           */
   SGH_ERR_HIP = -2,   /* HIP runtime / launch failure                         */
   SGH_ERR_STATE = -3, /* misuse of a plan (wrong order, destroyed handle)     */
-  SGH_ERR_NOMEM = -4  /* device allocation failed (sgh_batch_create only)     */
+  SGH_ERR_NOMEM = -4  /* device allocation failed, or a plan's pool is full    */
 };
 
 /* ---- library ------------------------------------------------------------- */
@@ -180,12 +181,22 @@ typedef struct sgh_cuboid_spec {
   const double* laser;    /* host: n_steps × {x, y, power, on, vx, vy}             */
 } sgh_cuboid_spec;
 
-/* Creates the plan (allocates the packed device state; the only allocating call) and
- * copies every table to the device. State starts at zero: set it with sgh_batch_set_state.
- * `metrics_rows` steps of per-cuboid metrics {P_laser, T_surface_max, n_it, converged,
- * rel_err} are kept on the device. Replaces a loop of SpectralSolver.step calls. */
+/* A plan is a pool of cuboid slots over fixed-capacity packed device arrays (the only
+ * allocating calls are the two constructors). Capacities: state elements, surface
+ * elements, laser-table rows, largest nz; `metrics_rows` steps of per-cuboid metrics
+ * {P_laser, T_surface_max, n_it, converged, rel_err} are kept per slot.
+ * Replaces a loop of SpectralSolver.step calls over independent runs. */
+int sgh_batch_pool_create(int dtype, int max_cuboids, long long state_capacity,
+                          long long surface_capacity, long long laser_capacity, int max_nz,
+                          int metrics_rows, void** plan);
+/* Pool sized exactly for `n` cuboids, all added (slots 0..n-1). */
 int sgh_batch_create(int dtype, int n, const sgh_cuboid_spec* specs, int metrics_rows,
                      void** plan, void* stream);
+/* Continuous batching: place a cuboid in a free slot (state zero, step 0; copies the
+ * laser table) or free a slot. Both synchronise the stream. Returns SGH_ERR_NOMEM when
+ * the pool has no room (the caller keeps the request queued). */
+int sgh_batch_add(void* plan, const sgh_cuboid_spec* spec, int* slot, void* stream);
+int sgh_batch_remove(void* plan, int slot, void* stream);
 int sgh_batch_destroy(void* plan);
 /* Advance every live cuboid by n_steps steps (one captured graph per step if use_graph). */
 int sgh_batch_run(void* plan, int n_steps, int use_graph, void* stream);
@@ -193,13 +204,15 @@ int sgh_batch_run(void* plan, int n_steps, int use_graph, void* stream);
 int sgh_batch_set_state(void* plan, int c, const void* src, void* stream);
 int sgh_batch_get_state(void* plan, int c, void* dst, void* stream);
 int sgh_batch_state_layout(void* plan, int c, long long* a_offset, int* nx, int* ny, int* nz);
-/* Copies rows × n × 5 metrics to host memory (synchronises the stream). */
-int sgh_batch_read_metrics(void* plan, double* host_out, int rows, void* stream);
+/* Copies rows × 5 metrics of slot c to host memory (synchronises the stream). */
+int sgh_batch_read_metrics(void* plan, int c, double* host_out, int rows, void* stream);
+/* Per-slot step counters (max_cuboids ints) to host memory (synchronises the stream). */
+int sgh_batch_steps_done(void* plan, int* host_out, void* stream);
 /* Event timing of the surface-iteration and volume kernels (disables graphs while on):
  * out4 = {surface ms, surface launches, volume ms, volume launches}. */
 int sgh_batch_set_timing(void* plan, int enable);
 int sgh_batch_timing(void* plan, double* out4);
-/* out6 = {packed volume elements, packed surface elements, surface items, volume items,
+/* out6 = {live state elements, surface capacity, surface items, volume items,
  *         max Picard cap, LDS bytes per surface workgroup}. */
 int sgh_batch_info(void* plan, long long* out6);
 

@@ -43,14 +43,18 @@ SIGNATURES = {
     "sgh_latent_source": [i32, i64, vp, vp, f64, f64, f64, f64, f64, vp, vp, vp],
     "sgh_property_correction_source": [i32, i32, i32, i32, vp, vp, vp, f64, f64, f64, f64, f64,
                                        f64, vp, vp, vp],
+    "sgh_batch_pool_create": [i32, i32, i64, i64, i64, i32, i32, C.POINTER(vp)],
     "sgh_batch_create": [i32, i32, vp, i32, C.POINTER(vp), vp],
+    "sgh_batch_add": [vp, vp, C.POINTER(i32), vp],
+    "sgh_batch_remove": [vp, i32, vp],
+    "sgh_batch_steps_done": [vp, vp, vp],
     "sgh_batch_destroy": [vp],
     "sgh_batch_run": [vp, i32, i32, vp],
     "sgh_batch_set_state": [vp, i32, vp, vp],
     "sgh_batch_get_state": [vp, i32, vp, vp],
     "sgh_batch_state_layout": [vp, i32, C.POINTER(i64), C.POINTER(i32), C.POINTER(i32),
                                C.POINTER(i32)],
-    "sgh_batch_read_metrics": [vp, vp, i32, vp],
+    "sgh_batch_read_metrics": [vp, i32, vp, i32, vp],
     "sgh_batch_set_timing": [vp, i32],
     "sgh_batch_timing": [vp, vp],
     "sgh_batch_info": [vp, vp],

@@ -62,7 +62,7 @@ struct CuboidDesc {
   double omega, tol, n_elem;
   double f1, f2, T_boil, T_liq;
   double absorptivity, rx, ry, order, area_factor;
-  int cell_integrated, pad;
+  int cell_integrated, live;  // live: the slot holds a cuboid
 };
 
 struct Item {
@@ -610,6 +610,7 @@ template <typename F>
 __global__ void __launch_bounds__(kBT) k_b_metrics(BatchPtrs<F> P) {
   for (int c = threadIdx.x; c < P.n_cuboids; c += kBT) {
     const CuboidDesc d = P.desc[c];
+    if (!d.live) continue;
     const int step = P.step_idx[c];
     if (step >= d.n_steps) continue;
     double pl = 0.0, tm = -1.0 / 0.0;
@@ -619,7 +620,7 @@ __global__ void __launch_bounds__(kBT) k_b_metrics(BatchPtrs<F> P) {
       tm = t > tm ? t : tm;
     }
     if (step < P.metrics_rows) {
-      double* m = P.metrics + ((long long)step * P.n_cuboids + c) * kMetricStride;
+      double* m = P.metrics + ((long long)c * P.metrics_rows + step) * kMetricStride;
       const double* st = P.status + kStatusStride * c;
       m[0] = pl * d.dx * d.dy;
       m[1] = tm;
@@ -632,29 +633,81 @@ __global__ void __launch_bounds__(kBT) k_b_metrics(BatchPtrs<F> P) {
 }
 
 // ---------------------------------------------------------------------------------
-// Plan
+// Plan: a pool of cuboid slots over fixed-capacity packed arrays. Cuboids are added and
+// removed between steps (continuous batching); work lists and the captured step graph
+// are rebuilt when membership changes.
 // ---------------------------------------------------------------------------------
+constexpr int kMaxRowBlk = kBMaxN / kBRB;          // row blocks per cuboid (ny <= 256)
+constexpr int kMaxColBlk = kBMaxN * kBMaxN / kBT;  // column blocks per cuboid (S <= 65536)
+constexpr int kMaxSizes = 32;                      // distinct mode counts in the matrix table
+
+// First-fit allocator over [0, capacity) with coalescing free list.
+struct Arena {
+  long long cap = 0;
+  std::vector<std::pair<long long, long long>> free_;  // (offset, size), sorted
+  void init(long long c) { cap = c; free_.assign(1, {0, c}); }
+  long long alloc(long long n, long long align) {
+    for (size_t i = 0; i < free_.size(); ++i) {
+      const long long o = (free_[i].first + align - 1) / align * align;
+      const long long end = free_[i].first + free_[i].second;
+      if (o + n <= end) {
+        const long long head = o - free_[i].first, tail = end - (o + n);
+        std::vector<std::pair<long long, long long>> repl;
+        if (head > 0) repl.push_back({free_[i].first, head});
+        if (tail > 0) repl.push_back({o + n, tail});
+        free_.erase(free_.begin() + i);
+        free_.insert(free_.begin() + i, repl.begin(), repl.end());
+        return o;
+      }
+    }
+    return -1;
+  }
+  void release(long long o, long long n) {
+    free_.push_back({o, n});
+    std::sort(free_.begin(), free_.end());
+    std::vector<std::pair<long long, long long>> m;
+    for (auto& f : free_) {
+      if (!m.empty() && m.back().first + m.back().second >= f.first)
+        m.back().second = std::max(m.back().second, f.first + f.second - m.back().first);
+      else
+        m.push_back(f);
+    }
+    free_ = m;
+  }
+};
+
+struct SlotAlloc {
+  long long a_off = 0, a_len = 0, s_off = 0, s_len = 0, l_off = 0, l_len = 0;
+};
+
 struct Plan {
   int dtype = 0;
-  int n = 0;
+  int n = 0;         // slot capacity
+  int n_live = 0;
   int max_it = 0;
   size_t elem = 8;
   std::vector<CuboidDesc> hdesc;
+  std::vector<SlotAlloc> halloc;
   std::vector<Item> hrow, hcol;
-  int n_row_slots = 0, n_col_slots = 0;
-  int nx_max = 0, ny_max = 0;
+  Arena arena_a, arena_s, arena_l;
+  std::vector<int> sizes;
+  std::vector<int> size_off;
+  int t1_stride = 0;
   size_t shm = 0;
   int bud = 0;
   long long total_N = 0, total_S = 0;
   // device allocations
   void* a = nullptr;
-  void* surf = nullptr;     // 10 surface arrays of total_S
+  void* surf = nullptr;  // kSurfArrays arrays of total_S
   void* mats = nullptr;
+  long long mats_cap = 0, mats_used = 0;
   double* t1 = nullptr;
   double* laser = nullptr;
   CuboidDesc* desc = nullptr;
   Item* row_items = nullptr;
   Item* col_items = nullptr;
+  Item* prime_items = nullptr;
+  long long row_cap = 0, col_cap = 0;
   double* row_slots = nullptr;
   double* col_slots = nullptr;
   double* aux_slots = nullptr;
@@ -663,11 +716,11 @@ struct Plan {
   int* step_idx = nullptr;
   double* metrics = nullptr;
   int metrics_rows = 0;
-  bool primed = false;
+  std::vector<int> pending_prime;  // slots added since the last run
+  bool dirty = true;               // work lists / graph out of date
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   hipStream_t graph_stream = nullptr;
-  // timing of the dominant kernels (non-graph runs)
   bool timing = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double t_surface_ms = 0.0, t_volume_ms = 0.0;
@@ -709,9 +762,24 @@ BatchPtrs<F> ptrs(Plan* p) {
 }
 
 template <typename F>
+static hipError_t allow_large_lds(size_t bytes) {
+  hipError_t e = hipSuccess;
+  const void* fns[] = {reinterpret_cast<const void*>(&k_b_surface<F, 0>),
+                       reinterpret_cast<const void*>(&k_b_surface<F, 1>),
+                       reinterpret_cast<const void*>(&k_b_surface<F, 2>),
+                       reinterpret_cast<const void*>(&k_b_surface<F, 3>)};
+  for (const void* f : fns) {
+    hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e == hipSuccess) e = r;
+  }
+  return e;
+}
+
+template <typename F>
 int enqueue_step(Plan* p, hipStream_t st) {
   BatchPtrs<F> P = ptrs<F>(p);
   const unsigned nrow = (unsigned)p->hrow.size(), ncol = (unsigned)p->hcol.size();
+  if (nrow == 0) return SGH_OK;
   const bool tm = p->timing;
   hipLaunchKernelGGL(k_b_prep<F>, dim3(nrow), dim3(kBT), 0, st, P);
   hipLaunchKernelGGL((k_b_surface<F, 0>), dim3(nrow), dim3(kST), p->shm, st, P, 0, p->bud);
@@ -741,25 +809,17 @@ int enqueue_step(Plan* p, hipStream_t st) {
   return SGH_OK;
 }
 
+// Ztt, Wtt and the first step's ZE, U, AA for newly added cuboids only.
 template <typename F>
-static hipError_t allow_large_lds(size_t bytes) {
-  hipError_t e = hipSuccess;
-  const void* fns[] = {reinterpret_cast<const void*>(&k_b_surface<F, 0>),
-                       reinterpret_cast<const void*>(&k_b_surface<F, 1>),
-                       reinterpret_cast<const void*>(&k_b_surface<F, 2>),
-                       reinterpret_cast<const void*>(&k_b_surface<F, 3>)};
-  for (const void* f : fns) {
-    hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e == hipSuccess) e = r;
-  }
-  return e;
-}
-
-template <typename F>
-int prime(Plan* p, hipStream_t st) {
+int prime_new(Plan* p, const std::vector<Item>& items, hipStream_t st) {
+  if (items.empty()) return SGH_OK;
+  hipError_t e = hipMemcpyAsync(p->prime_items, items.data(), items.size() * sizeof(Item),
+                                hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) return fail_hip(e, "sgh_batch prime upload");
   BatchPtrs<F> P = ptrs<F>(p);
-  hipLaunchKernelGGL(k_b_setup<F>, dim3((unsigned)p->hcol.size()), dim3(kBT), 0, st, P);
-  hipLaunchKernelGGL(k_b_volume<F>, dim3((unsigned)p->hcol.size()), dim3(kBT), 0, st, P, 0);
+  P.col_items = p->prime_items;
+  hipLaunchKernelGGL(k_b_setup<F>, dim3((unsigned)items.size()), dim3(kBT), 0, st, P);
+  hipLaunchKernelGGL(k_b_volume<F>, dim3((unsigned)items.size()), dim3(kBT), 0, st, P, 0);
   SGH_CHECK_LAUNCH("sgh_batch prime");
   return SGH_OK;
 }
@@ -776,21 +836,25 @@ static std::vector<double> unit_basis(int n) {
   return B;
 }
 
-// XCD-aware work lists: each cuboid is pinned to one of the 8 XCDs (greedy balance of
-// its transform work) and its items are placed at list positions ≡ xcd (mod 8), so all
+// XCD-aware work lists: each live cuboid is pinned to one of the 8 XCDs (greedy balance of
+// its transform work) and its items sit at list positions ≡ xcd (mod 8), so all
 // workgroups of a cuboid share one L2. Short bins are padded with empty items (c = −1).
 static void build_xcd_items(Plan* p) {
   std::vector<double> load(kXcds, 0.0);
   std::vector<std::vector<Item>> rb(kXcds), cb(kXcds);
-  std::vector<int> order(p->n);
-  for (int c = 0; c < p->n; ++c) order[c] = c;
+  std::vector<int> order;
+  for (int c = 0; c < p->n; ++c)
+    if (p->hdesc[c].live) order.push_back(c);
   auto cost = [&](int c) {
     const CuboidDesc& d = p->hdesc[c];
     return (double)d.nrowblk * d.nx * (d.nx + d.ny) + 0.05 * (double)d.S * d.nz;
   };
-  std::sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+  std::sort(order.begin(), order.end(), [&](int a, int b) {
+    const double ca = cost(a), cb_ = cost(b);
+    return ca != cb_ ? ca > cb_ : a < b;
+  });
   for (int c : order) {
-    int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
     load[x] += cost(c);
     const CuboidDesc& d = p->hdesc[c];
     for (int b = 0; b < d.nrowblk; ++b) rb[x].push_back({c, b});
@@ -805,17 +869,25 @@ static void build_xcd_items(Plan* p) {
   };
   interleave(rb, p->hrow);
   interleave(cb, p->hcol);
+  p->max_it = 0;
+  for (int c : order) p->max_it = std::max(p->max_it, p->hdesc[c].max_it);
+}
+
+static void drop_graph(Plan* p) {
+  if (p->graph_exec) hipGraphExecDestroy(p->graph_exec);
+  if (p->graph) hipGraphDestroy(p->graph);
+  p->graph_exec = nullptr;
+  p->graph = nullptr;
 }
 
 static void free_plan(Plan* p) {
   if (!p) return;
-  if (p->graph_exec) hipGraphExecDestroy(p->graph_exec);
-  if (p->graph) hipGraphDestroy(p->graph);
+  drop_graph(p);
   for (auto& e : p->ev)
     if (e) hipEventDestroy(e);
   void* bufs[] = {p->a, p->surf, p->mats, p->t1, p->laser, p->desc, p->row_items, p->col_items,
-                  p->row_slots, p->col_slots, p->aux_slots, p->done_at, p->status, p->step_idx,
-                  p->metrics};
+                  p->prime_items, p->row_slots, p->col_slots, p->aux_slots, p->done_at,
+                  p->status, p->step_idx, p->metrics};
   for (void* b : bufs)
     if (b) hipFree(b);
   delete p;
@@ -826,191 +898,331 @@ static hipError_t dmalloc(T** ptr, size_t bytes) {
   return hipMalloc((void**)ptr, bytes < 16 ? 16 : bytes);
 }
 
+static long long mats_offset(Plan* p, int m, hipStream_t st, int* rc) {
+  for (size_t i = 0; i < p->sizes.size(); ++i)
+    if (p->sizes[i] == m) return p->size_off[i];
+  const long long need = 2LL * m * m;
+  if ((int)p->sizes.size() >= kMaxSizes || p->mats_used + need > p->mats_cap) {
+    *rc = SGH_ERR_NOMEM;
+    set_error("sgh_batch: matrix table full (%d distinct mode counts)", (int)p->sizes.size());
+    return -1;
+  }
+  std::vector<double> B = unit_basis(m);
+  std::vector<unsigned char> buf((size_t)need * p->elem);
+  for (int r = 0; r < m; ++r)
+    for (int c = 0; c < m; ++c) {
+      const double v = B[(size_t)r * m + c], vt = B[(size_t)c * m + r];
+      const size_t i0 = (size_t)r * m + c, i1 = (size_t)m * m + i0;
+      if (p->elem == 8) {
+        reinterpret_cast<double*>(buf.data())[i0] = v;
+        reinterpret_cast<double*>(buf.data())[i1] = vt;
+      } else {
+        reinterpret_cast<float*>(buf.data())[i0] = (float)v;
+        reinterpret_cast<float*>(buf.data())[i1] = (float)vt;
+      }
+    }
+  const long long off = p->mats_used;
+  // synchronous: the host buffer goes out of scope
+  hipError_t e = hipMemcpy((char*)p->mats + off * p->elem, buf.data(), buf.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    *rc = fail_hip(e, "sgh_batch matrix upload");
+    return -1;
+  }
+  (void)st;
+  p->mats_used += need;
+  p->sizes.push_back(m);
+  p->size_off.push_back(off);
+  return off;
+}
+
+static bool spec_ok(const sgh_cuboid_spec& sp) {
+  return !(sp.nx < 2 || sp.ny < 1 || sp.nz < 1 || sp.nx > kBMaxN || sp.ny > kBMaxN ||
+           sp.nz > 4096 || sp.Lx <= 0 || sp.Ly <= 0 || sp.Lz <= 0 || sp.k <= 0 || sp.rho <= 0 ||
+           sp.cp <= 0 || sp.dt <= 0 || sp.rx <= 0 || sp.ry <= 0 || sp.R_v <= 0 ||
+           sp.T_boil <= 0 || sp.n_steps < 0 || sp.max_it < 0 || sp.order <= 0 ||
+           (sp.n_steps > 0 && sp.laser == nullptr) || (sp.cell_integrated && sp.order != 2.0));
+}
+
+static long long spec_state_elems(const sgh_cuboid_spec& sp) {
+  return (long long)sp.nx * sp.ny * sp.nz;
+}
+
+// Fill slot c with a cuboid (host tables copied synchronously; the caller's laser table
+// may be freed on return).
+static int add_cuboid(Plan* p, const sgh_cuboid_spec& sp, int c, hipStream_t st) {
+  CuboidDesc d{};
+  SlotAlloc al;
+  d.nx = sp.nx;
+  d.ny = sp.ny;
+  d.nz = sp.nz;
+  d.S = sp.nx * sp.ny;
+  const long long N = (long long)d.S * d.nz;
+  al.a_len = N;
+  al.s_len = d.S;
+  al.l_len = std::max(1, sp.n_steps);
+  al.a_off = p->arena_a.alloc(al.a_len, 64);
+  al.s_off = al.a_off < 0 ? -1 : p->arena_s.alloc(al.s_len, 64);
+  al.l_off = al.s_off < 0 ? -1 : p->arena_l.alloc(al.l_len, 1);
+  if (al.a_off < 0 || al.s_off < 0 || al.l_off < 0) {
+    if (al.a_off >= 0) p->arena_a.release(al.a_off, al.a_len);
+    if (al.s_off >= 0) p->arena_s.release(al.s_off, al.s_len);
+    set_error("sgh_batch_add: pool capacity exhausted");
+    return SGH_ERR_NOMEM;
+  }
+  int rc = SGH_OK;
+  const long long bx = mats_offset(p, sp.nx, st, &rc);
+  const long long by = rc ? -1 : mats_offset(p, sp.ny, st, &rc);
+  if (rc) {
+    p->arena_a.release(al.a_off, al.a_len);
+    p->arena_s.release(al.s_off, al.s_len);
+    p->arena_l.release(al.l_off, al.l_len);
+    return rc;
+  }
+  d.a_off = al.a_off;
+  d.s_off = (int)al.s_off;
+  d.bx_off = (int)bx;
+  d.bxT_off = (int)bx + sp.nx * sp.nx;
+  d.by_off = (int)by;
+  d.byT_off = (int)by + sp.ny * sp.ny;
+  d.t1_off = c * p->t1_stride;
+  const double alpha = sp.k / (sp.rho * sp.cp);
+  std::vector<double> t1;
+  const double Ls[3] = {sp.Lx, sp.Ly, sp.Lz};
+  const int ns[3] = {sp.nx, sp.ny, sp.nz};
+  for (int ax = 0; ax < 3; ++ax) {
+    std::vector<double> e(ns[ax]), k2(ns[ax]);
+    for (int m = 0; m < ns[ax]; ++m) {
+      const double kk = M_PI * m / Ls[ax];
+      k2[m] = kk * kk;
+      e[m] = exp(-alpha * k2[m] * sp.dt);
+    }
+    t1.insert(t1.end(), e.begin(), e.end());
+    t1.insert(t1.end(), k2.begin(), k2.end());
+  }
+  d.row_item0 = c * kMaxRowBlk;
+  d.nrowblk = (sp.ny + kBRB - 1) / kBRB;
+  d.col_item0 = c * kMaxColBlk;
+  d.ncolblk = (d.S + kBT - 1) / kBT;
+  d.max_it = sp.max_it;
+  d.n_steps = sp.n_steps;
+  d.laser_off = al.l_off;
+  d.dx = sp.Lx / sp.nx;
+  d.dy = sp.Ly / sp.ny;
+  d.inv_sqrt_LxLy = 1.0 / sqrt(sp.Lx * sp.Ly);
+  d.Lz = sp.Lz;
+  d.alpha = alpha;
+  d.rho_cp = sp.rho * sp.cp;
+  d.w0 = sqrt(1.0 / sp.Lz);
+  d.w1 = sqrt(2.0 / sp.Lz);
+  d.inv_arc = 1.0 / (alpha * d.rho_cp);
+  d.dt = sp.dt;
+  d.omega = sp.omega;
+  d.tol = sp.tol;
+  d.n_elem = (double)N;
+  d.f1 = 0.82 * sp.dH * sp.Pa / sqrt(2.0 * M_PI * sp.R_v);
+  d.f2 = sp.dH / (sp.R_v * sp.T_boil);
+  d.T_boil = sp.T_boil;
+  d.T_liq = sp.T_liq;
+  d.absorptivity = sp.absorptivity;
+  d.rx = sp.rx;
+  d.ry = sp.ry;
+  d.order = sp.order;
+  d.area_factor = M_PI * pow(2.0, 1.0 - 2.0 / sp.order) * tgamma(2.0 / sp.order) / sp.order;
+  d.cell_integrated = sp.cell_integrated;
+  d.live = 1;
+
+  const int zero = 0;
+  const double zstat[kStatusStride] = {0, 0, 0, 0};
+  hipError_t e = hipSuccess;
+  auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
+  chk(hipMemcpy(p->t1 + d.t1_off, t1.data(), t1.size() * sizeof(double), hipMemcpyHostToDevice));
+  if (sp.n_steps > 0)
+    chk(hipMemcpy(p->laser + 6 * al.l_off, sp.laser, 6 * sizeof(double) * sp.n_steps,
+                  hipMemcpyHostToDevice));
+  chk(hipMemcpy(p->desc + c, &d, sizeof(CuboidDesc), hipMemcpyHostToDevice));
+  chk(hipMemcpy(p->step_idx + c, &zero, sizeof(int), hipMemcpyHostToDevice));
+  chk(hipMemcpy(p->done_at + c, &zero, sizeof(int), hipMemcpyHostToDevice));
+  chk(hipMemcpy(p->status + kStatusStride * c, zstat, sizeof(zstat), hipMemcpyHostToDevice));
+  chk(hipMemsetAsync((char*)p->a + al.a_off * p->elem, 0, al.a_len * p->elem, st));
+  for (int arr = 0; arr < kSurfArrays; ++arr)
+    chk(hipMemsetAsync((char*)p->surf + (arr * p->total_S + al.s_off) * p->elem, 0,
+                       al.s_len * p->elem, st));
+  if (e != hipSuccess) return fail_hip(e, "sgh_batch_add upload");
+  p->hdesc[c] = d;
+  p->halloc[c] = al;
+  p->n_live += 1;
+  p->pending_prime.push_back(c);
+  p->dirty = true;
+  return SGH_OK;
+}
+
+static int rebuild_lists(Plan* p, hipStream_t st) {
+  build_xcd_items(p);
+  hipError_t e = hipSuccess;
+  if ((long long)p->hrow.size() > p->row_cap || (long long)p->hcol.size() > p->col_cap) {
+    set_error("sgh_batch: work list exceeds capacity");
+    return SGH_ERR_STATE;
+  }
+  if (!p->hrow.empty())
+    e = hipMemcpyAsync(p->row_items, p->hrow.data(), p->hrow.size() * sizeof(Item),
+                       hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && !p->hcol.empty())
+    e = hipMemcpyAsync(p->col_items, p->hcol.data(), p->hcol.size() * sizeof(Item),
+                       hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return fail_hip(e, "sgh_batch work lists");
+  drop_graph(p);
+  p->dirty = false;
+  return SGH_OK;
+}
+
 }  // namespace sgh
 
 using namespace sgh;
 
 extern "C" {
 
-int sgh_batch_create(int dtype, int n, const sgh_cuboid_spec* specs, int metrics_rows,
-                     void** plan_out, void* stream) {
-  SGH_REQUIRE(plan_out && specs && n > 0, SGH_ERR_ARG, "sgh_batch_create: bad args");
-  SGH_REQUIRE(dtype == SGH_F64 || dtype == SGH_F32, SGH_ERR_ARG, "sgh_batch_create: dtype");
-  SGH_REQUIRE(metrics_rows >= 0, SGH_ERR_ARG, "sgh_batch_create: metrics_rows");
+int sgh_batch_pool_create(int dtype, int max_cuboids, long long state_capacity,
+                          long long surface_capacity, long long laser_capacity,
+                          int max_nz, int metrics_rows, void** plan_out) {
+  SGH_REQUIRE(plan_out && max_cuboids > 0 && state_capacity > 0 && surface_capacity > 0 &&
+                  laser_capacity > 0 && max_nz > 0 && max_nz <= 4096 && metrics_rows >= 0,
+              SGH_ERR_ARG, "sgh_batch_pool_create: bad args");
+  SGH_REQUIRE(dtype == SGH_F64 || dtype == SGH_F32, SGH_ERR_ARG, "sgh_batch_pool_create: dtype");
   *plan_out = nullptr;
-  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   Plan* p = new Plan();
   p->dtype = dtype;
-  p->n = n;
+  p->n = max_cuboids;
   p->elem = dtype == SGH_F64 ? 8 : 4;
   p->metrics_rows = metrics_rows;
-
-  // matrices per distinct mode count
-  std::vector<int> sizes;
-  std::vector<long long> mat_off;
-  std::vector<double> mats;
-  auto mat_for = [&](int m) -> long long {
-    for (size_t i = 0; i < sizes.size(); ++i)
-      if (sizes[i] == m) return mat_off[i];
-    const long long off = (long long)mats.size();
-    std::vector<double> B = unit_basis(m);
-    mats.insert(mats.end(), B.begin(), B.end());
-    for (int r = 0; r < m; ++r)
-      for (int c = 0; c < m; ++c) mats.push_back(B[(size_t)c * m + r]);
-    sizes.push_back(m);
-    mat_off.push_back(off);
-    return off;
-  };
-
-  std::vector<double> t1, laser;
-  long long a_off = 0, s_off = 0;
-  int n_row_slots = 0, n_col_slots = 0;
-  for (int c = 0; c < n; ++c) {
-    const sgh_cuboid_spec& sp = specs[c];
-    if (sp.nx < 2 || sp.ny < 1 || sp.nz < 1 || sp.nx > kBMaxN || sp.ny > kBMaxN ||
-        sp.Lx <= 0 || sp.Ly <= 0 || sp.Lz <= 0 || sp.k <= 0 || sp.rho <= 0 || sp.cp <= 0 ||
-        sp.dt <= 0 || sp.rx <= 0 || sp.ry <= 0 || sp.R_v <= 0 || sp.T_boil <= 0 ||
-        sp.n_steps < 0 || sp.max_it < 0 || (sp.n_steps > 0 && sp.laser == nullptr) ||
-        (sp.cell_integrated && sp.order != 2.0)) {
-      free_plan(p);
-      set_error("sgh_batch_create: invalid spec for cuboid %d", c);
-      return SGH_ERR_ARG;
-    }
-    CuboidDesc d{};
-    d.nx = sp.nx;
-    d.ny = sp.ny;
-    d.nz = sp.nz;
-    d.S = sp.nx * sp.ny;
-    d.a_off = a_off;
-    d.s_off = (int)s_off;
-    const long long N = (long long)d.S * d.nz;
-    a_off += (N + 63) / 64 * 64;
-    s_off += (d.S + 63) / 64 * 64;
-    d.bx_off = (int)mat_for(sp.nx);
-    d.bxT_off = d.bx_off + sp.nx * sp.nx;
-    d.by_off = (int)mat_for(sp.ny);
-    d.byT_off = d.by_off + sp.ny * sp.ny;
-    d.t1_off = (int)t1.size();
-    const double alpha = sp.k / (sp.rho * sp.cp);
-    const double Ls[3] = {sp.Lx, sp.Ly, sp.Lz};
-    const int ns[3] = {sp.nx, sp.ny, sp.nz};
-    for (int ax = 0; ax < 3; ++ax) {
-      std::vector<double> e(ns[ax]), k2(ns[ax]);
-      for (int m = 0; m < ns[ax]; ++m) {
-        const double kk = M_PI * m / Ls[ax];
-        k2[m] = kk * kk;
-        e[m] = exp(-alpha * k2[m] * sp.dt);
-      }
-      t1.insert(t1.end(), e.begin(), e.end());
-      t1.insert(t1.end(), k2.begin(), k2.end());
-    }
-    d.row_item0 = n_row_slots;
-    d.nrowblk = (sp.ny + kBRB - 1) / kBRB;
-    n_row_slots += d.nrowblk;
-    d.col_item0 = n_col_slots;
-    d.ncolblk = (d.S + kBT - 1) / kBT;
-    n_col_slots += d.ncolblk;
-    d.max_it = sp.max_it;
-    d.n_steps = sp.n_steps;
-    d.laser_off = (long long)laser.size() / 6;
-    if (sp.n_steps > 0) laser.insert(laser.end(), sp.laser, sp.laser + 6LL * sp.n_steps);
-    d.dx = sp.Lx / sp.nx;
-    d.dy = sp.Ly / sp.ny;
-    d.inv_sqrt_LxLy = 1.0 / sqrt(sp.Lx * sp.Ly);
-    d.Lz = sp.Lz;
-    d.alpha = alpha;
-    d.rho_cp = sp.rho * sp.cp;
-    d.w0 = sqrt(1.0 / sp.Lz);
-    d.w1 = sqrt(2.0 / sp.Lz);
-    d.inv_arc = 1.0 / (alpha * d.rho_cp);
-    d.dt = sp.dt;
-    d.omega = sp.omega;
-    d.tol = sp.tol;
-    d.n_elem = (double)N;
-    d.f1 = 0.82 * sp.dH * sp.Pa / sqrt(2.0 * M_PI * sp.R_v);
-    d.f2 = sp.dH / (sp.R_v * sp.T_boil);
-    d.T_boil = sp.T_boil;
-    d.T_liq = sp.T_liq;
-    d.absorptivity = sp.absorptivity;
-    d.rx = sp.rx;
-    d.ry = sp.ry;
-    d.order = sp.order;
-    const double nn = sp.order;
-    d.area_factor = M_PI * pow(2.0, 1.0 - 2.0 / nn) * tgamma(2.0 / nn) / nn;
-    d.cell_integrated = sp.cell_integrated;
-    p->hdesc.push_back(d);
-    p->max_it = sp.max_it > p->max_it ? sp.max_it : p->max_it;
-    p->nx_max = sp.nx > p->nx_max ? sp.nx : p->nx_max;
-    p->ny_max = sp.ny > p->ny_max ? sp.ny : p->ny_max;
-  }
-  p->total_N = a_off;
-  p->n_row_slots = n_row_slots;
-  p->n_col_slots = n_col_slots;
-  build_xcd_items(p);
-  p->total_S = s_off;
+  p->hdesc.assign(max_cuboids, CuboidDesc{});
+  p->halloc.assign(max_cuboids, SlotAlloc{});
+  p->arena_a.init(state_capacity);
+  p->arena_s.init(surface_capacity);
+  p->arena_l.init(laser_capacity);
+  p->total_N = state_capacity;
+  p->total_S = surface_capacity;
+  p->t1_stride = 2 * (kBMaxN + kBMaxN + max_nz);
+  p->mats_cap = 2LL * 16 * kBMaxN * kBMaxN;
   {
-    // LDS of a surface workgroup: Ls[16][ldl] + Ys[16][ldy] + a staging buffer of `bud`
-    // elements that holds X (or R) whole when it fits, else in chunks.
-    const int ldl = dtype == SGH_F64 ? pad_a<double>((p->ny_max + 3) & ~3) : pad_a<float>((p->ny_max + 3) & ~3);
-    const int ldy = dtype == SGH_F64 ? pad_a<double>((p->nx_max + 3) & ~3) : pad_a<float>((p->nx_max + 3) & ~3);
+    // LDS of a surface workgroup: Ls[kBRB][ldl] + Ys[kBRB][ldy] + a staging buffer of
+    // `bud` elements that holds X (or R) whole when it fits, else in chunks.
+    const int n4 = (kBMaxN + 3) & ~3;
+    const int ld = dtype == SGH_F64 ? pad_a<double>(n4) : pad_a<float>(n4);
     const long long elems = (long long)kSurfLds / (long long)p->elem;
-    p->bud = (int)(elems - (long long)kBRB * ldl - (long long)kBRB * ldy);
+    p->bud = (int)(elems - 2LL * kBRB * ld);
     p->shm = (size_t)kSurfLds;
-    const int nx4 = (p->nx_max + 3) & ~3;
-    const long long need1 = 4LL * (dtype == SGH_F64 ? pad_b<double>(p->nx_max) : pad_b<float>(p->nx_max));
-    const long long need2 = (long long)nx4 * (dtype == SGH_F64 ? pad_b<double>(16) : pad_b<float>(16));
+    const long long need1 = 4LL * (dtype == SGH_F64 ? pad_b<double>(kBMaxN) : pad_b<float>(kBMaxN));
+    const long long need2 = (long long)n4 * (dtype == SGH_F64 ? pad_b<double>(16) : pad_b<float>(16));
     if (p->bud < need1 || p->bud < need2) {
       free_plan(p);
-      set_error("sgh_batch_create: surface too wide for the LDS staging buffer");
+      set_error("sgh_batch_pool_create: LDS staging buffer too small");
       return SGH_ERR_ARG;
     }
   }
-
+  p->row_cap = (long long)max_cuboids * kMaxRowBlk + 8LL * kMaxRowBlk * kXcds;
+  p->col_cap = (long long)max_cuboids * kMaxColBlk + 8LL * kMaxColBlk * kXcds;
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
   const size_t E = p->elem;
   chk(dtype == SGH_F64 ? allow_large_lds<double>(p->shm) : allow_large_lds<float>(p->shm));
-  chk(dmalloc(&p->a, p->total_N * E));
-  chk(dmalloc(&p->surf, kSurfArrays * p->total_S * E));
-  chk(dmalloc(&p->mats, mats.size() * E));
-  chk(dmalloc(&p->t1, t1.size() * sizeof(double)));
-  chk(dmalloc(&p->laser, laser.size() * sizeof(double)));
-  chk(dmalloc(&p->desc, n * sizeof(CuboidDesc)));
-  chk(dmalloc(&p->row_items, p->hrow.size() * sizeof(Item)));
-  chk(dmalloc(&p->col_items, p->hcol.size() * sizeof(Item)));
-  chk(dmalloc(&p->row_slots, 3 * (size_t)p->n_row_slots * sizeof(double)));
-  chk(dmalloc(&p->col_slots, (size_t)p->n_col_slots * sizeof(double)));
-  chk(dmalloc(&p->aux_slots, 2 * (size_t)p->n_row_slots * sizeof(double)));
-  chk(dmalloc(&p->done_at, n * sizeof(int)));
-  chk(dmalloc(&p->status, kStatusStride * n * sizeof(double)));
-  chk(dmalloc(&p->step_idx, n * sizeof(int)));
-  chk(dmalloc(&p->metrics, (size_t)metrics_rows * n * kMetricStride * sizeof(double)));
+  chk(dmalloc(&p->a, state_capacity * E));
+  chk(dmalloc(&p->surf, kSurfArrays * surface_capacity * E));
+  chk(dmalloc(&p->mats, p->mats_cap * E));
+  chk(dmalloc(&p->t1, (size_t)max_cuboids * p->t1_stride * sizeof(double)));
+  chk(dmalloc(&p->laser, (size_t)laser_capacity * 6 * sizeof(double)));
+  chk(dmalloc(&p->desc, (size_t)max_cuboids * sizeof(CuboidDesc)));
+  chk(dmalloc(&p->row_items, (size_t)p->row_cap * sizeof(Item)));
+  chk(dmalloc(&p->col_items, (size_t)p->col_cap * sizeof(Item)));
+  chk(dmalloc(&p->prime_items, (size_t)p->col_cap * sizeof(Item)));
+  chk(dmalloc(&p->row_slots, 3 * (size_t)max_cuboids * kMaxRowBlk * sizeof(double)));
+  chk(dmalloc(&p->col_slots, (size_t)max_cuboids * kMaxColBlk * sizeof(double)));
+  chk(dmalloc(&p->aux_slots, 2 * (size_t)max_cuboids * kMaxRowBlk * sizeof(double)));
+  chk(dmalloc(&p->done_at, (size_t)max_cuboids * sizeof(int)));
+  chk(dmalloc(&p->status, (size_t)kStatusStride * max_cuboids * sizeof(double)));
+  chk(dmalloc(&p->step_idx, (size_t)max_cuboids * sizeof(int)));
+  chk(dmalloc(&p->metrics, (size_t)metrics_rows * max_cuboids * kMetricStride * sizeof(double)));
   if (e != hipSuccess) {
     free_plan(p);
-    set_error("sgh_batch_create: device allocation failed: %s", hipGetErrorString(e));
+    set_error("sgh_batch_pool_create: device allocation failed: %s", hipGetErrorString(e));
     return SGH_ERR_NOMEM;
   }
-  std::vector<unsigned char> mbuf(mats.size() * E);
-  for (size_t i = 0; i < mats.size(); ++i) {
-    if (E == 8) reinterpret_cast<double*>(mbuf.data())[i] = mats[i];
-    else reinterpret_cast<float*>(mbuf.data())[i] = (float)mats[i];
-  }
-  chk(hipMemcpyAsync(p->mats, mbuf.data(), mbuf.size(), hipMemcpyHostToDevice, st));
-  chk(hipMemcpyAsync(p->t1, t1.data(), t1.size() * sizeof(double), hipMemcpyHostToDevice, st));
-  if (!laser.empty())
-    chk(hipMemcpyAsync(p->laser, laser.data(), laser.size() * sizeof(double), hipMemcpyHostToDevice, st));
-  chk(hipMemcpyAsync(p->desc, p->hdesc.data(), n * sizeof(CuboidDesc), hipMemcpyHostToDevice, st));
-  chk(hipMemcpyAsync(p->row_items, p->hrow.data(), p->hrow.size() * sizeof(Item), hipMemcpyHostToDevice, st));
-  chk(hipMemcpyAsync(p->col_items, p->hcol.data(), p->hcol.size() * sizeof(Item), hipMemcpyHostToDevice, st));
-  chk(hipMemsetAsync(p->surf, 0, kSurfArrays * p->total_S * E, st));
-  chk(hipMemsetAsync(p->a, 0, p->total_N * E, st));
-  chk(hipMemsetAsync(p->step_idx, 0, n * sizeof(int), st));
-  chk(hipMemsetAsync(p->status, 0, kStatusStride * n * sizeof(double), st));
-  chk(hipMemsetAsync(p->done_at, 0, n * sizeof(int), st));
-  chk(hipStreamSynchronize(st));
+  chk(hipMemcpy(p->desc, p->hdesc.data(), (size_t)max_cuboids * sizeof(CuboidDesc),
+                hipMemcpyHostToDevice));
   if (e != hipSuccess) {
     free_plan(p);
-    return fail_hip(e, "sgh_batch_create upload");
+    return fail_hip(e, "sgh_batch_pool_create init");
+  }
+  *plan_out = p;
+  return SGH_OK;
+}
+
+int sgh_batch_add(void* plan, const sgh_cuboid_spec* spec, int* slot_out, void* stream) {
+  Plan* p = static_cast<Plan*>(plan);
+  SGH_REQUIRE(p && spec && slot_out, SGH_ERR_ARG, "sgh_batch_add: bad args");
+  SGH_REQUIRE(spec_ok(*spec), SGH_ERR_ARG, "sgh_batch_add: invalid cuboid spec");
+  int c = -1;
+  for (int i = 0; i < p->n; ++i)
+    if (!p->hdesc[i].live) { c = i; break; }
+  SGH_REQUIRE(c >= 0, SGH_ERR_NOMEM, "sgh_batch_add: all %d slots in use", p->n);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  hipError_t e = hipStreamSynchronize(st);  // the slot's previous occupant is finished
+  if (e != hipSuccess) return fail_hip(e, "sgh_batch_add");
+  const int rc = add_cuboid(p, *spec, c, st);
+  if (rc) return rc;
+  *slot_out = c;
+  return SGH_OK;
+}
+
+int sgh_batch_remove(void* plan, int c, void* stream) {
+  Plan* p = static_cast<Plan*>(plan);
+  SGH_REQUIRE(p && c >= 0 && c < p->n && p->hdesc[c].live, SGH_ERR_ARG,
+              "sgh_batch_remove: bad slot");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  hipError_t e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return fail_hip(e, "sgh_batch_remove");
+  const SlotAlloc& al = p->halloc[c];
+  p->arena_a.release(al.a_off, al.a_len);
+  p->arena_s.release(al.s_off, al.s_len);
+  p->arena_l.release(al.l_off, al.l_len);
+  p->hdesc[c] = CuboidDesc{};
+  e = hipMemcpy(p->desc + c, &p->hdesc[c], sizeof(CuboidDesc), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return fail_hip(e, "sgh_batch_remove");
+  p->pending_prime.erase(std::remove(p->pending_prime.begin(), p->pending_prime.end(), c),
+                         p->pending_prime.end());
+  p->n_live -= 1;
+  p->dirty = true;
+  return SGH_OK;
+}
+
+int sgh_batch_create(int dtype, int n, const sgh_cuboid_spec* specs, int metrics_rows,
+                     void** plan_out, void* stream) {
+  SGH_REQUIRE(plan_out && specs && n > 0, SGH_ERR_ARG, "sgh_batch_create: bad args");
+  long long cap_a = 0, cap_s = 0, cap_l = 0;
+  int max_nz = 1;
+  for (int c = 0; c < n; ++c) {
+    SGH_REQUIRE(spec_ok(specs[c]), SGH_ERR_ARG, "sgh_batch_create: invalid spec for cuboid %d", c);
+    cap_a += (spec_state_elems(specs[c]) + 63) / 64 * 64;
+    cap_s += ((long long)specs[c].nx * specs[c].ny + 63) / 64 * 64;
+    cap_l += std::max(1, specs[c].n_steps);
+    max_nz = std::max(max_nz, specs[c].nz);
+  }
+  void* pl = nullptr;
+  int rc = sgh_batch_pool_create(dtype, n, cap_a, cap_s, cap_l, max_nz, metrics_rows, &pl);
+  if (rc) return rc;
+  Plan* p = static_cast<Plan*>(pl);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  for (int c = 0; c < n; ++c) {
+    rc = add_cuboid(p, specs[c], c, st);
+    if (rc) {
+      free_plan(p);
+      return rc;
+    }
+  }
+  hipError_t e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    free_plan(p);
+    return fail_hip(e, "sgh_batch_create");
   }
   *plan_out = p;
   return SGH_OK;
@@ -1025,7 +1237,8 @@ int sgh_batch_destroy(void* plan) {
 
 int sgh_batch_state_layout(void* plan, int c, long long* a_offset, int* nx, int* ny, int* nz) {
   Plan* p = static_cast<Plan*>(plan);
-  SGH_REQUIRE(p && c >= 0 && c < p->n, SGH_ERR_ARG, "sgh_batch_state_layout: bad cuboid");
+  SGH_REQUIRE(p && c >= 0 && c < p->n && p->hdesc[c].live, SGH_ERR_ARG,
+              "sgh_batch_state_layout: bad cuboid");
   const CuboidDesc& d = p->hdesc[c];
   if (a_offset) *a_offset = d.a_off;
   if (nx) *nx = d.nx;
@@ -1036,18 +1249,21 @@ int sgh_batch_state_layout(void* plan, int c, long long* a_offset, int* nx, int*
 
 int sgh_batch_set_state(void* plan, int c, const void* src, void* stream) {
   Plan* p = static_cast<Plan*>(plan);
-  SGH_REQUIRE(p && src && c >= 0 && c < p->n, SGH_ERR_ARG, "sgh_batch_set_state: bad args");
+  SGH_REQUIRE(p && src && c >= 0 && c < p->n && p->hdesc[c].live, SGH_ERR_ARG,
+              "sgh_batch_set_state: bad args");
   const CuboidDesc& d = p->hdesc[c];
   hipError_t e = hipMemcpyAsync((char*)p->a + d.a_off * p->elem, src, (size_t)d.S * d.nz * p->elem,
                                 hipMemcpyDeviceToDevice, reinterpret_cast<hipStream_t>(stream));
   if (e != hipSuccess) return fail_hip(e, "sgh_batch_set_state");
-  p->primed = false;
+  if (std::find(p->pending_prime.begin(), p->pending_prime.end(), c) == p->pending_prime.end())
+    p->pending_prime.push_back(c);
   return SGH_OK;
 }
 
 int sgh_batch_get_state(void* plan, int c, void* dst, void* stream) {
   Plan* p = static_cast<Plan*>(plan);
-  SGH_REQUIRE(p && dst && c >= 0 && c < p->n, SGH_ERR_ARG, "sgh_batch_get_state: bad args");
+  SGH_REQUIRE(p && dst && c >= 0 && c < p->n && p->hdesc[c].live, SGH_ERR_ARG,
+              "sgh_batch_get_state: bad args");
   const CuboidDesc& d = p->hdesc[c];
   hipError_t e = hipMemcpyAsync(dst, (char*)p->a + d.a_off * p->elem, (size_t)d.S * d.nz * p->elem,
                                 hipMemcpyDeviceToDevice, reinterpret_cast<hipStream_t>(stream));
@@ -1060,19 +1276,24 @@ int sgh_batch_run(void* plan, int n_steps, int use_graph, void* stream) {
   SGH_REQUIRE(p && n_steps >= 0, SGH_ERR_ARG, "sgh_batch_run: bad args");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   int rc = SGH_OK;
-  if (!p->primed) {
-    rc = p->dtype == SGH_F64 ? prime<double>(p, st) : prime<float>(p, st);
+  if (!p->pending_prime.empty()) {
+    std::vector<Item> items;
+    for (int c : p->pending_prime)
+      for (int b = 0; b < p->hdesc[c].ncolblk; ++b) items.push_back({c, b});
+    rc = p->dtype == SGH_F64 ? prime_new<double>(p, items, st) : prime_new<float>(p, items, st);
     if (rc) return rc;
-    p->primed = true;
+    hipError_t e = hipStreamSynchronize(st);  // prime_items is reused by the next prime
+    if (e != hipSuccess) return fail_hip(e, "sgh_batch_run prime");
+    p->pending_prime.clear();
   }
+  if (p->dirty) {
+    rc = rebuild_lists(p, st);
+    if (rc) return rc;
+  }
+  if (p->hrow.empty()) return SGH_OK;
   if (p->timing) use_graph = 0;
   if (use_graph) {
-    if (p->graph_exec && p->graph_stream != st) {
-      hipGraphExecDestroy(p->graph_exec);
-      hipGraphDestroy(p->graph);
-      p->graph_exec = nullptr;
-      p->graph = nullptr;
-    }
+    if (p->graph_exec && p->graph_stream != st) drop_graph(p);
     if (!p->graph_exec) {
       hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
       if (e != hipSuccess) return fail_hip(e, "sgh_batch_run capture");
@@ -1105,15 +1326,26 @@ int sgh_batch_run(void* plan, int n_steps, int use_graph, void* stream) {
   return SGH_OK;
 }
 
-int sgh_batch_read_metrics(void* plan, double* host_out, int rows, void* stream) {
+int sgh_batch_read_metrics(void* plan, int c, double* host_out, int rows, void* stream) {
   Plan* p = static_cast<Plan*>(plan);
-  SGH_REQUIRE(p && host_out && rows >= 0 && rows <= p->metrics_rows, SGH_ERR_ARG,
-              "sgh_batch_read_metrics: bad args");
+  SGH_REQUIRE(p && host_out && c >= 0 && c < p->n && rows >= 0 && rows <= p->metrics_rows,
+              SGH_ERR_ARG, "sgh_batch_read_metrics: bad args");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  hipError_t e = hipMemcpyAsync(host_out, p->metrics, (size_t)rows * p->n * kMetricStride * sizeof(double),
-                                hipMemcpyDeviceToHost, st);
+  hipError_t e = hipMemcpyAsync(host_out, p->metrics + (size_t)c * p->metrics_rows * kMetricStride,
+                                (size_t)rows * kMetricStride * sizeof(double), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return fail_hip(e, "sgh_batch_read_metrics");
+  return SGH_OK;
+}
+
+int sgh_batch_steps_done(void* plan, int* host_out, void* stream) {
+  Plan* p = static_cast<Plan*>(plan);
+  SGH_REQUIRE(p && host_out, SGH_ERR_ARG, "sgh_batch_steps_done: bad args");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  hipError_t e = hipMemcpyAsync(host_out, p->step_idx, (size_t)p->n * sizeof(int),
+                                hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return fail_hip(e, "sgh_batch_steps_done");
   return SGH_OK;
 }
 
@@ -1143,6 +1375,7 @@ int sgh_batch_timing(void* plan, double* out4) {
 
 #ifdef SGH_BLOCK_TIMESTAMPS
 int sgh_debug_block_timestamps(unsigned long long* host_out, int n) {
+  (void)n;
   return hipMemcpyFromSymbol(host_out, HIP_SYMBOL(g_block_ts), (size_t)4 * 8192 * 6 * 8) == hipSuccess ? 0 : -2;
 }
 #endif
@@ -1150,7 +1383,10 @@ int sgh_debug_block_timestamps(unsigned long long* host_out, int n) {
 int sgh_batch_info(void* plan, long long* out6) {
   Plan* p = static_cast<Plan*>(plan);
   SGH_REQUIRE(p && out6, SGH_ERR_ARG, "sgh_batch_info: bad args");
-  out6[0] = p->total_N;
+  long long used_a = 0;
+  for (int c = 0; c < p->n; ++c)
+    if (p->hdesc[c].live) used_a += p->halloc[c].a_len;
+  out6[0] = used_a;
   out6[1] = p->total_S;
   out6[2] = (long long)p->hrow.size();
   out6[3] = (long long)p->hcol.size();

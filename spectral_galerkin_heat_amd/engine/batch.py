@@ -24,7 +24,8 @@ from .. import _lib
 from .._lib import CuboidSpec, call
 from ..core.laser import build_laser_profile
 
-__all__ = ["BatchEngine", "check_batchable", "laser_table", "smoke_check"]
+__all__ = ["BatchEngine", "BatchPool", "PoolFull", "check_batchable", "footprint_bytes",
+           "laser_table", "smoke_check"]
 
 METRIC_KEYS = ("P_laser", "T_surface_max", "n_evap_iter", "picard_converged", "picard_rel_err")
 
@@ -61,79 +62,106 @@ def laser_table(ctx, n_steps=None):
     return rows
 
 
+def cuboid_spec(ctx, n_steps, precision_code, omega=None, tol=None, max_it=None):
+    """``sgh_cuboid_spec`` and its laser table (the table must outlive the spec's use)."""
+    check_batchable(ctx)
+    num, geom, mat, lp = ctx.num, ctx.geom, ctx.mat, ctx.laser
+    prof = build_laser_profile(lp.profile, lp.super_gaussian_order,
+                               cell_integrated=lp.cell_integrated)
+    tab = np.ascontiguousarray(laser_table(ctx, n_steps))
+    sp = CuboidSpec()
+    sp.nx, sp.ny, sp.nz = int(geom.n.x), int(geom.n.y), int(geom.n.z)
+    sp.max_it = int(max_it if max_it is not None else
+                    (num.max_picard_iter if num.max_picard_iter is not None else 30))
+    sp.n_steps = int(n_steps)
+    sp.cell_integrated = int(bool(prof.cell_integrated))
+    sp.Lx, sp.Ly, sp.Lz = (float(v) for v in geom.size)
+    sp.k, sp.rho, sp.cp = float(mat.k), float(mat.rho), float(mat.Cp)
+    sp.Pa, sp.T_boil, sp.dH = float(mat.Pa), float(mat.T_boil), float(mat.DeltaH_LV)
+    sp.R_v, sp.T_liq = float(mat.R_v), float(mat.T_liquidus)
+    sp.absorptivity, sp.rx, sp.ry = float(lp.absorptivity), float(lp.r_x), float(lp.r_y)
+    sp.order = float(prof.order)
+    sp.dt = float(num.dt)
+    sp.omega = float(omega if omega is not None else
+                     (num.picard_omega if num.picard_omega is not None else 0.1))
+    sp.tol = float(tol if tol is not None else
+                   (num.picard_tol if num.picard_tol is not None else 1e-4))
+    if precision_code == _lib.F32:
+        sp.tol = float(np.float32(sp.tol))  # the reference compares against dtype(tol)
+    sp.laser = tab.ctypes.data_as(C.POINTER(C.c_double))
+    return sp, tab
+
+
+def initial_modes(ctx, shape, tdtype):
+    """a = 0 except a[0,0,0] = T0·√(LxLyLz) (sgh/solvers/spectral.py:176-180)."""
+    a = torch.zeros(shape, dtype=tdtype, device="cuda")
+    g = ctx.geom
+    a[0, 0, 0] = float(ctx.mat.T0) * math.sqrt(float(g.size.x) * float(g.size.y) * float(g.size.z))
+    return a
+
+
+def footprint_bytes(ctx, n_steps, precision="mixed"):
+    """Device bytes a cuboid takes in a pool: state, 10 surface fields, laser table."""
+    w = 8 if precision == "fp64" else 4
+    nx, ny, nz = (int(v) for v in ctx.geom.n)
+    return w * (nx * ny * nz + 10 * nx * ny) + 48 * max(1, int(n_steps))
+
+
 class BatchEngine:
+    """A batch of cuboids advanced together; fixed membership unless built as a pool."""
+
     def __init__(self, contexts, precision: str = "mixed", n_steps=None, metrics_rows=None,
                  omega=None, tol=None, max_it=None):
+        self._setup(precision)
+        self.contexts = list(contexts)
+        self.n = len(self.contexts)
+        if self.n == 0:
+            raise ValueError("empty batch")
+        steps = [int(c.num.n_steps if n_steps is None else n_steps) for c in self.contexts]
+        self.n_steps = steps
+        self.metrics_rows = int(max(steps) if metrics_rows is None else metrics_rows)
+        specs = (CuboidSpec * self.n)()
+        tables = []
+        self.shapes = []
+        for i, ctx in enumerate(self.contexts):
+            sp, tab = cuboid_spec(ctx, steps[i], self.code, omega, tol, max_it)
+            tables.append(tab)
+            specs[i] = sp
+            self.shapes.append((sp.nz, sp.ny, sp.nx))
+        plan = C.c_void_p()
+        call("sgh_batch_create", self.code, self.n, C.cast(specs, C.c_void_p), self.metrics_rows,
+             C.byref(plan), self.stream.cuda_stream)
+        self._plan = plan
+        self.mode_counts = [int(np.prod(s)) for s in self.shapes]
+        self.steps_done = 0
+        self.reset_initial_condition()
+
+    def _setup(self, precision):
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         if not torch.cuda.is_available():
             raise RuntimeError("BatchEngine needs the MI355X (torch.cuda is unavailable)")
         _lib.lib.load()
-        self.contexts = list(contexts)
-        self.n = len(self.contexts)
-        if self.n == 0:
-            raise ValueError("empty batch")
         self.precision = precision
         self.code = _lib.F64 if precision == "fp64" else _lib.F32
         self.tdtype = torch.float64 if precision == "fp64" else torch.float32
+        # graph capture needs a non-default stream; default priority
         self.stream = torch.cuda.Stream()
-        steps = [int(c.num.n_steps if n_steps is None else n_steps) for c in self.contexts]
-        self.n_steps = steps
-        self.metrics_rows = int(max(steps) if metrics_rows is None else metrics_rows)
-        self._tables = []
-        specs = (CuboidSpec * self.n)()
-        self.shapes = []
-        for i, ctx in enumerate(self.contexts):
-            check_batchable(ctx)
-            num, geom, mat, lp = ctx.num, ctx.geom, ctx.mat, ctx.laser
-            prof = build_laser_profile(lp.profile, lp.super_gaussian_order,
-                                       cell_integrated=lp.cell_integrated)
-            tab = np.ascontiguousarray(laser_table(ctx, steps[i]))
-            self._tables.append(tab)
-            sp = specs[i]
-            sp.nx, sp.ny, sp.nz = int(geom.n.x), int(geom.n.y), int(geom.n.z)
-            self.shapes.append((sp.nz, sp.ny, sp.nx))
-            sp.max_it = int(max_it if max_it is not None else
-                            (num.max_picard_iter if num.max_picard_iter is not None else 30))
-            sp.n_steps = steps[i]
-            sp.cell_integrated = int(bool(prof.cell_integrated))
-            sp.Lx, sp.Ly, sp.Lz = (float(v) for v in geom.size)
-            sp.k, sp.rho, sp.cp = float(mat.k), float(mat.rho), float(mat.Cp)
-            sp.Pa, sp.T_boil, sp.dH = float(mat.Pa), float(mat.T_boil), float(mat.DeltaH_LV)
-            sp.R_v, sp.T_liq = float(mat.R_v), float(mat.T_liquidus)
-            sp.absorptivity, sp.rx, sp.ry = float(lp.absorptivity), float(lp.r_x), float(lp.r_y)
-            sp.order = float(prof.order)
-            sp.dt = float(num.dt)
-            sp.omega = float(omega if omega is not None else
-                             (num.picard_omega if num.picard_omega is not None else 0.1))
-            sp.tol = float(tol if tol is not None else
-                           (num.picard_tol if num.picard_tol is not None else 1e-4))
-            if self.code == _lib.F32:
-                # the reference compares against dtype(tol)
-                sp.tol = float(np.float32(sp.tol))
-            sp.laser = tab.ctypes.data_as(C.POINTER(C.c_double))
-        self._specs = specs
-        plan = C.c_void_p()
-        call("sgh_batch_create", self.code, self.n, C.cast(specs, C.c_void_p), self.metrics_rows,
-             C.byref(plan), self.stream.cuda_stream)
-        self._plan = plan
+        self._plan = C.c_void_p()
+
+    @property
+    def info(self):
         info = (C.c_longlong * 6)()
         call("sgh_batch_info", self._plan, C.cast(info, C.c_void_p))
-        self.info = {"volume_elems": info[0], "surface_elems": info[1], "surface_items": info[2],
-                     "volume_items": info[3], "max_it": info[4], "lds_bytes": info[5]}
-        self.mode_counts = [int(np.prod(s)) for s in self.shapes]
-        self.steps_done = 0
-        self.reset_initial_condition()
+        return {"live_state_elems": info[0], "surface_capacity": info[1],
+                "surface_items": info[2], "volume_items": info[3], "max_it": info[4],
+                "lds_bytes": info[5]}
 
     # -- state ------------------------------------------------------------------------
     def reset_initial_condition(self):
-        """a = 0 except a[0,0,0] = T0·√(LxLyLz) (sgh/solvers/spectral.py:176-180)."""
         with torch.cuda.stream(self.stream):
             for i, ctx in enumerate(self.contexts):
-                a = torch.zeros(self.shapes[i], dtype=self.tdtype, device="cuda")
-                g = ctx.geom
-                a[0, 0, 0] = float(ctx.mat.T0) * math.sqrt(float(g.size.x) * float(g.size.y) * float(g.size.z))
-                self.set_state(i, a)
+                self.set_state(i, initial_modes(ctx, self.shapes[i], self.tdtype))
         self.stream.synchronize()
 
     def set_state(self, i, a):
@@ -161,13 +189,20 @@ class BatchEngine:
     def synchronize(self):
         self.stream.synchronize()
 
+    def cuboid_metrics(self, i, rows):
+        out = np.zeros((int(rows), 5), dtype=np.float64)
+        if rows:
+            call("sgh_batch_read_metrics", self._plan, int(i), out.ctypes.data, int(rows),
+                 self.stream.cuda_stream)
+        return out
+
     def metrics(self, rows=None):
         """Per-step, per-cuboid metrics array [rows, n, 5] (P_laser, T_surface_max,
         n_evap_iter, picard_converged, picard_rel_err)."""
         rows = min(self.steps_done, self.metrics_rows) if rows is None else int(rows)
         out = np.zeros((rows, self.n, 5), dtype=np.float64)
-        if rows:
-            call("sgh_batch_read_metrics", self._plan, out.ctypes.data, rows, self.stream.cuda_stream)
+        for i in range(self.n):
+            out[:, i] = self.cuboid_metrics(i, rows)
         return out
 
     def set_timing(self, enable: bool):
@@ -190,6 +225,58 @@ class BatchEngine:
             self.close()
         except Exception:
             pass
+
+
+class BatchPool(BatchEngine):
+    """Continuous batching: cuboids are added to free slots and removed when done.
+
+    Capacities are in elements (state, surface) and laser-table rows; ``add`` raises
+    ``PoolFull`` when the pool has no room for a request.
+    """
+
+    def __init__(self, precision="mixed", max_cuboids=128, state_capacity=1 << 25,
+                 surface_capacity=1 << 21, laser_capacity=1 << 18, max_nz=64, metrics_rows=2048):
+        self._setup(precision)
+        self.metrics_rows = int(metrics_rows)
+        self.max_cuboids = int(max_cuboids)
+        plan = C.c_void_p()
+        call("sgh_batch_pool_create", self.code, self.max_cuboids, int(state_capacity),
+             int(surface_capacity), int(laser_capacity), int(max_nz), self.metrics_rows,
+             C.byref(plan))
+        self._plan = plan
+        self.contexts, self.shapes, self.n_steps = {}, {}, {}
+        self.steps_done = 0
+
+    def add(self, ctx, n_steps, initial_state=None):
+        sp, tab = cuboid_spec(ctx, n_steps, self.code)
+        slot = C.c_int(-1)
+        rc = _lib.lib.sgh_batch_add(self._plan, C.byref(sp), C.byref(slot), self.stream.cuda_stream)
+        if rc == -4:
+            raise PoolFull(_lib.last_error())
+        if rc < 0:
+            raise _lib.SGHError(f"sgh_batch_add failed ({rc}): {_lib.last_error()}")
+        i = slot.value
+        self.contexts[i] = ctx
+        self.shapes[i] = (sp.nz, sp.ny, sp.nx)
+        self.n_steps[i] = int(n_steps)
+        a = initial_modes(ctx, self.shapes[i], self.tdtype) if initial_state is None else initial_state
+        self.set_state(i, a)
+        return i
+
+    def remove(self, i):
+        call("sgh_batch_remove", self._plan, int(i), self.stream.cuda_stream)
+        self.contexts.pop(i, None)
+        self.shapes.pop(i, None)
+        self.n_steps.pop(i, None)
+
+    def steps(self):
+        out = (C.c_int * self.max_cuboids)()
+        call("sgh_batch_steps_done", self._plan, C.cast(out, C.c_void_p), self.stream.cuda_stream)
+        return list(out)
+
+
+class PoolFull(RuntimeError):
+    """The pool has no free slot or no contiguous room for the request."""
 
 
 def smoke_check():

@@ -1,0 +1,189 @@
+"""Request loop: continuous batching of scan simulations on one MI355X.
+
+Requests (a ``SimulationContext`` with its laser path and a step count) arrive over
+time. The server keeps every admitted simulation's modal state resident in HBM inside a
+``BatchPool`` and advances all of them together, a few steps per tick. Between ticks it
+
+* retires simulations whose last step has run (final state and per-step metrics are
+  read back; completion latency = completion time − arrival time),
+* admits queued requests in arrival order while the pool has a free slot and room in
+  its packed arrays (admission by memory footprint; a request that does not fit waits),
+
+so requests that arrive while others are on the GPU join the running batch and their
+work overlaps on the device. A simulation's arithmetic depends only on its own data
+(fixed per-cuboid work decomposition and reduction order), so its result is the same
+alone and in any mix. Every ``check_every``-th request is re-solved alone in FP64 and
+its relative L2 against the served (mixed-precision) result is recorded.
+"""
+
+from __future__ import annotations
+
+import heapq
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from .batch import BatchEngine, BatchPool, PoolFull, footprint_bytes
+
+__all__ = ["Request", "ScanServer", "poisson_arrivals"]
+
+
+@dataclass
+class Request:
+    rid: int
+    ctx: object
+    n_steps: int
+    arrival: float                 # seconds after the server clock started
+    slot: int = -1
+    start_step: int = 0            # pool step counter value at admission
+    admitted_at: float = float("nan")
+    completed_at: float = float("nan")
+    result: object = None          # final modal state (torch, device)
+    metrics: object = None         # [n_steps, 5] numpy
+    fp64_rel_l2: float = float("nan")
+    footprint: int = 0
+
+    @property
+    def latency(self) -> float:
+        return self.completed_at - self.arrival
+
+    @property
+    def modes(self) -> int:
+        return int(np.prod(tuple(self.ctx.geom.n)))
+
+
+def poisson_arrivals(rate_hz: float, n: int, rng) -> np.ndarray:
+    """Arrival times of a Poisson process of the given rate (first at t = 0)."""
+    gaps = rng.exponential(1.0 / rate_hz, size=n)
+    gaps[0] = 0.0
+    return np.cumsum(gaps)
+
+
+@dataclass
+class ServeReport:
+    requests: list = field(default_factory=list)
+    wall_s: float = 0.0
+    mode_steps: int = 0
+    device_steps: int = 0
+
+    @property
+    def latencies(self):
+        return np.array([r.latency for r in self.requests])
+
+    def summary(self):
+        lat = self.latencies
+        return {
+            "requests": len(self.requests),
+            "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,
+            "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None,
+            "mean_latency_s": float(lat.mean()) if len(lat) else None,
+            "aggregate_mode_steps_per_s": self.mode_steps / self.wall_s if self.wall_s else None,
+            "wall_s": self.wall_s,
+            "device_steps": self.device_steps,
+            "fp64_checks": [r.fp64_rel_l2 for r in self.requests if not np.isnan(r.fp64_rel_l2)],
+        }
+
+
+class ScanServer:
+    def __init__(self, precision="mixed", max_cuboids=128, hbm_budget_bytes=8 << 30,
+                 max_nz=64, max_steps=2048, steps_per_tick=8, check_every=20,
+                 keep_results=True):
+        word = 8 if precision == "fp64" else 4
+        # split the budget between the packed state and the surface fields
+        state_cap = int(hbm_budget_bytes * 0.85) // word
+        surf_cap = int(hbm_budget_bytes * 0.15) // (10 * word)
+        self.pool = BatchPool(precision=precision, max_cuboids=max_cuboids,
+                              state_capacity=state_cap, surface_capacity=surf_cap,
+                              laser_capacity=max_cuboids * max_steps, max_nz=max_nz,
+                              metrics_rows=max_steps)
+        self.precision = precision
+        self.max_steps = max_steps
+        self.steps_per_tick = int(steps_per_tick)
+        self.check_every = int(check_every)
+        self.keep_results = keep_results
+        self.device_steps = 0
+
+    def _admit(self, req: Request, now: float) -> bool:
+        try:
+            slot = self.pool.add(req.ctx, req.n_steps)
+        except PoolFull:
+            return False
+        req.slot = slot
+        req.admitted_at = now
+        req.start_step = self.device_steps
+        req.footprint = footprint_bytes(req.ctx, req.n_steps, self.precision)
+        return True
+
+    def serve(self, requests, clock=time.perf_counter, sleep=time.sleep) -> ServeReport:
+        """Serve ``requests`` (sorted by arrival) in real time; returns the report."""
+        queue = sorted(requests, key=lambda r: r.arrival)
+        pending = list(queue)
+        waiting: list = []
+        active: dict = {}             # slot -> request
+        done: list = []
+        for r in pending:
+            if r.n_steps > self.max_steps:
+                raise ValueError(f"request {r.rid}: {r.n_steps} steps > max_steps {self.max_steps}")
+        t0 = clock()
+        finish_heap: list = []        # (device step at which it completes, rid)
+        by_rid = {r.rid: r for r in queue}
+        while pending or waiting or active:
+            now = clock() - t0
+            while pending and pending[0].arrival <= now:
+                waiting.append(pending.pop(0))
+            admitted = []
+            for r in list(waiting):
+                if not self._admit(r, now):
+                    break                    # FIFO: later requests wait behind this one
+                waiting.remove(r)
+                active[r.slot] = r
+                heapq.heappush(finish_heap, (r.start_step + r.n_steps, r.rid))
+                admitted.append(r)
+            if not active:
+                if pending:
+                    sleep(max(0.0, pending[0].arrival - (clock() - t0)))
+                continue
+            # run until the next completion, at most steps_per_tick steps
+            next_finish = finish_heap[0][0]
+            k = max(1, min(self.steps_per_tick, next_finish - self.device_steps))
+            self.pool.run(k)
+            self.device_steps += k
+            if self.device_steps >= next_finish or pending:
+                self.pool.synchronize()
+            now = clock() - t0
+            while finish_heap and finish_heap[0][0] <= self.device_steps:
+                _, rid = heapq.heappop(finish_heap)
+                r = by_rid[rid]
+                self.pool.synchronize()
+                r.completed_at = clock() - t0
+                if self.keep_results:
+                    r.result = self.pool.state(r.slot).clone()
+                r.metrics = self.pool.cuboid_metrics(r.slot, r.n_steps)
+                self.pool.remove(r.slot)
+                del active[r.slot]
+                done.append(r)
+        wall = clock() - t0
+        rep = ServeReport(requests=sorted(done, key=lambda r: r.rid), wall_s=wall,
+                          mode_steps=sum(r.modes * r.n_steps for r in done),
+                          device_steps=self.device_steps)
+        return rep
+
+    def fp64_checks(self, report: ServeReport):
+        """Re-solve every ``check_every``-th request alone in FP64 (the engine pinned to
+        the FP64 oracle at 1e-9) and record the relative L2 of the served result."""
+        for r in report.requests:
+            if r.rid % self.check_every != 0 or r.result is None:
+                continue
+            ref = BatchEngine([r.ctx], precision="fp64", n_steps=r.n_steps)
+            ref.run(r.n_steps)
+            ref.synchronize()
+            b = ref.state(0)
+            a = r.result.double()
+            r.fp64_rel_l2 = float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+            ref.close()
+        return [r.fp64_rel_l2 for r in report.requests if not np.isnan(r.fp64_rel_l2)]
+
+    def close(self):
+        self.pool.close()
