@@ -28,6 +28,13 @@ extern "C" {
 #endif
 
 enum { SGH_F64 = 0, SGH_F32 = 1 };
+
+/* Arithmetic of a 3-D axis contraction (sgh_axis_contract):
+ *   SGH_XFORM_MFMA   matrix cores in the data type (f32 16x16x4 or f64 16x16x4);
+ *   SGH_XFORM_BF16X3 split-BF16: x = hi + lo, three bf16 16x16x32 products
+ *                    hi·hi + hi·lo + lo·hi with FP32 accumulation (FP32 data only);
+ *   SGH_XFORM_VALU   LDS-tiled vector-ALU GEMM in the data type. */
+enum { SGH_XFORM_MFMA = 0, SGH_XFORM_BF16X3 = 1, SGH_XFORM_VALU = 2 };
 enum {
   SGH_OK = 0,
   SGH_ERR_ARG = -1,   /* bad shape, null pointer, unsupported dtype           */
@@ -105,10 +112,11 @@ int sgh_sep2d(int dtype, int batch, int ny, int nx, const void* L, const void* R
               const void* X, int nparts, double scale, void* out, void* stream);
 
 /* out[o,k,i] = scale · Σ_j M[k,j]·in[o,j,i] for an [outer, n, inner] view; M is n×n
- * row-major.  Three calls (x, y, z) make the 3-D DCT of project_volume /
- * reconstruct_volume (spectral_ops.py:83-102). in != out. */
-int sgh_axis_contract(int dtype, long long outer, int n, long long inner, const void* M,
-                      const void* in, double scale, void* out, void* stream);
+ * row-major; variant = SGH_XFORM_*.  Three calls (x, y, z) make the 3-D DCT of
+ * project_volume / reconstruct_volume (spectral_ops.py:83-102) and of DCT_II / IDCT_II
+ * on volumes (spectral_cpu_kernels.py:374-389). in != out. */
+int sgh_axis_contract(int dtype, int variant, long long outer, int n, long long inner,
+                      const void* M, const void* in, double scale, void* out, void* stream);
 
 /* ---- surface forcing -----------------------------------------------------
  * Laser flux on cell centres x_i = (i+½)dx, y_j = (j+½)dy: I0·exp(-2ρⁿ) (order n), or

@@ -78,6 +78,9 @@ class NumParams:
     picard_tol: Optional[float] = None
     picard_omega: Optional[float] = None
     dtype: Any = np.float32
+    # arithmetic of the 3-D cosine transforms on the MI355X backend: "mfma" (matrix cores
+    # in dtype), "bf16x3" (split-BF16 products, float32 only) or "valu" (vector ALU)
+    transform: str = "mfma"
 
 
 @dataclass
@@ -144,7 +147,7 @@ class FineMeshParams:
 _SECTION_KEYS = {
     "simulation": {
         "name", "backend", "dtype", "dt", "duration", "update_interval",
-        "max_picard_iter", "picard_tol", "picard_omega",
+        "max_picard_iter", "picard_tol", "picard_omega", "transform",
     },
     "domain": {"size", "mesh"},
     "material": {
@@ -224,6 +227,7 @@ class SimulationContext:
             picard_omega=(float(sim_cfg["picard_omega"])
                           if sim_cfg.get("picard_omega") is not None else None),
             dtype=real_t,
+            transform=str(sim_cfg.get("transform", "mfma")).lower(),
         )
         geom = GeomParams(size=Vec3(Lx, Ly, Lz), n=Vec3(int(nx), int(ny), int(nz)))
 

@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "sgh_common.h"
+#include "sgh_mfma.h"
 
 static_assert(sizeof(sgh_cuboid_spec) == 176, "sgh_cuboid_spec layout (mirrored in _lib.py)");
 
@@ -305,33 +306,6 @@ __device__ bool picard_active(const BatchPtrs<F>& P, const CuboidDesc& d, int c,
 }
 
 // ---------------------------------------------------------------------------------
-// MFMA tiles. Lane layouts (checked on gfx950): for both instructions A[i][k] sits in
-// lane i + 16k and B[k][j] in lane j + 16k; the 16×16 accumulator holds D[i][j] in lane
-// j + 16(i/4), register i%4 for FP32, and in lane j + 16(i%4), register i/4 for FP64.
-// ---------------------------------------------------------------------------------
-typedef float f4v __attribute__((ext_vector_type(4)));
-typedef double d4v __attribute__((ext_vector_type(4)));
-
-template <typename F>
-struct Mfma;
-template <>
-struct Mfma<float> {
-  using V = f4v;
-  static __device__ __forceinline__ V mma(float a, float b, V c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
-};
-template <>
-struct Mfma<double> {
-  using V = d4v;
-  static __device__ __forceinline__ V mma(double a, double b, V c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  static __device__ __forceinline__ int row(int lane, int v) { return 4 * v + (lane >> 4); }
-};
-
-// ---------------------------------------------------------------------------------
 // surface transforms (one kernel, four roles), 16 output rows per item:
 //   MODE 0 (warm start): G = dx·dy·Bᵧ q_net Bₓᵀ / √(LxLy)
 //   MODE 1 (iteration, forward half): D = S_raw − G, norm slots, G += ω D
@@ -341,22 +315,6 @@ struct Mfma<double> {
 // element is read by exactly one wave of the item); Y goes through LDS; stage 2,
 // out = Y·R, streams R the same way. Column tiles of 16 are dealt to the 4 waves.
 // ---------------------------------------------------------------------------------
-// LDS leading dimensions (in elements) chosen for conflict-free MFMA operand reads: the
-// A operand (16 rows × 4 k) wants a row stride ≡ 2 words (mod 32), the B operand
-// (4 k-rows × 16 columns) a row stride ≡ 16 words (mod 32); a double is 2 words.
-template <typename F>
-__host__ __device__ constexpr int pad_to_mod(int n, int target_words) {
-  const int wpe = (int)(sizeof(F) / 4);
-  const int words = n * wpe;
-  const int add = ((target_words - words % 32) % 32 + 32) % 32;
-  return n + (add + wpe - 1) / wpe;
-}
-template <typename F>
-__host__ __device__ constexpr int pad_a(int n) { return pad_to_mod<F>(n, 2 * (int)(sizeof(F) / 4)); }
-template <typename F>
-__host__ __device__ constexpr int pad_b(int n) { return pad_to_mod<F>(n, 16); }
-
-
 // Copy a rows × cols block (source row stride src_ld, starting at src) into LDS
 // (row stride dst_ld), zero-filling rows >= valid_rows and columns >= valid_cols.
 // Each thread keeps kStageAhead global loads in flight; indices advance incrementally

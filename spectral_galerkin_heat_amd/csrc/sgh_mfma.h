@@ -1,0 +1,66 @@
+// MFMA building blocks for gfx950 shared by the transform kernels.
+//
+// Lane layouts (checked on gfx950 with a one-wave probe):
+//   v_mfma_f32_16x16x4_f32 / v_mfma_f64_16x16x4_f64: A[i][k] in lane i + 16k,
+//     B[k][j] in lane j + 16k (one element per lane, k = 0..3).
+//   v_mfma_f32_16x16x32_bf16: A[i][k] in lane i + 16(k/8), element k%8;
+//     B[k][j] in lane j + 16(k/8), element k%8 (eight bf16 per lane, k = 0..31).
+//   Accumulator D[i][j] (16×16): FP32 results in lane j + 16(i/4), register i%4;
+//     FP64 results in lane j + 16(i%4), register i/4.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace sgh {
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef double d4v __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+
+template <typename F>
+struct Mfma;
+template <>
+struct Mfma<float> {
+  using V = f4v;
+  static __device__ __forceinline__ V mma(float a, float b, V c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  // output row i of accumulator register v in this lane
+  static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+template <>
+struct Mfma<double> {
+  using V = d4v;
+  static __device__ __forceinline__ V mma(double a, double b, V c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int v) { return 4 * v + (lane >> 4); }
+};
+
+__device__ __forceinline__ f4v mma_bf16(bf8v a, bf8v b, f4v c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// Split x = hi + lo with hi = bf16(x) (round to nearest even) and lo = bf16(x − hi):
+// hi·hi' + hi·lo' + lo·hi' reproduces a product to about 2^-16 relative.
+__device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
+  hi = (__bf16)x;
+  lo = (__bf16)(x - (float)hi);
+}
+
+// LDS leading dimensions (in elements) for conflict-free MFMA operand reads: an A
+// operand (16 rows × 4 k, row-major) wants a row stride ≡ 2 words (mod 32), a B operand
+// (4 k-rows × 16 columns) a row stride ≡ 16 words (mod 32); a double is 2 words.
+template <typename F>
+__host__ __device__ constexpr int pad_to_mod(int n, int target_words) {
+  const int wpe = (int)(sizeof(F) / 4);
+  const int words = n * wpe;
+  const int add = ((target_words - words % 32) % 32 + 32) % 32;
+  return n + (add + wpe - 1) / wpe;
+}
+template <typename F>
+__host__ __device__ constexpr int pad_a(int n) { return pad_to_mod<F>(n, 2 * (int)(sizeof(F) / 4)); }
+template <typename F>
+__host__ __device__ constexpr int pad_b(int n) { return pad_to_mod<F>(n, 16); }
+
+}  // namespace sgh

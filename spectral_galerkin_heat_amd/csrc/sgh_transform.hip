@@ -6,9 +6,15 @@
 // a batched small GEMM with no transposes:
 //   sep2d        : out = scale · L · X · R for a surface, one workgroup per 8-row block,
 //                  both products staged through LDS in one launch;
-//   axis_contract: one axis of a 3-D volume, an LDS-tiled strided GEMM
-//                  (x axis: Out[R, n] = In[R, n]·Mᵀ; y/z axes: Out_o = M·In_o).
+//   axis_contract: one axis of a 3-D volume as a strided batched GEMM on the matrix
+//                  cores (x axis: Out[R, n] = In[R, n]·Mᵀ; y/z axes: Out_o = M·In_o),
+//                  64×64 output tiles, 32-deep K slices staged in LDS with the next
+//                  slice prefetched into registers. Variants: MFMA in the data type
+//                  (f32 16x16x4 or f64 16x16x4), split-BF16 (three bf16 16x16x32
+//                  products hi·hi + hi·lo + lo·hi, FP32 accumulation, FP32 data), and
+//                  the LDS-tiled vector-ALU GEMM.
 #include "sgh_common.h"
+#include "sgh_mfma.h"
 
 namespace sgh {
 
@@ -173,9 +179,180 @@ k_gemm_strided(int M, int N, int K, const F* __restrict__ A, long long sAb, long
   }
 }
 
+
+// ---------------------------------------------------------------------------------
+// Strided batched GEMM on the matrix cores:
+//   C[b][r][c] = scale · Σ_k A[b][r][k] · B[b][k][c]
+// A and C are row-major with unit stride along k and c; B is column-fast (B_CF,
+// unit stride along c: the y/z axes, where B is the volume) or k-fast (the x axis,
+// where B = Mᵀ). A 256-thread workgroup computes a 64×64 tile: wave w owns the 32×32
+// quadrant (w/2, w%2) as 2×2 MFMA tiles of 16×16. K advances in slices of 32: the
+// slice is staged in LDS while the next one is already loading into registers.
+// ---------------------------------------------------------------------------------
+constexpr int kGM = 64, kGN = 64, kGK = 32;
+constexpr int kGL = kGM * kGK / kT;   // elements of A (and of B) per thread per slice
+constexpr int kBfLd = kGK + 8;        // bf16 LDS row: 80 bytes, 16-byte aligned reads
+
+template <typename F, int VAR, bool B_CF>
+__global__ void __launch_bounds__(kT)
+k_gemm_mfma(int M, int N, int K, const F* __restrict__ A, long long sAb, long long sAr,
+            const F* __restrict__ B, long long sBb, long long sBk, long long sBc,
+            F* __restrict__ C, long long sCb, long long sCr, F scale) {
+  static_assert(VAR == 0 || (VAR == 1 && sizeof(F) == 4), "split-BF16 takes FP32 data");
+  using Acc = typename Mfma<F>::V;
+  constexpr int ldA = pad_a<F>(kGK), ldB = pad_b<F>(kGN);
+  constexpr int kSmemF = VAR == 0 ? (kGM * ldA + kGK * ldB) * (int)sizeof(F) : 16;
+  constexpr int kSmemB = VAR == 1 ? 4 * kGM * kBfLd * 2 : 16;
+  __shared__ __align__(16) unsigned char sm_f[kSmemF];
+  __shared__ __align__(16) unsigned char sm_b[kSmemB];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int r0 = blockIdx.y * kGM, c0 = blockIdx.x * kGN;
+  const long long bz = blockIdx.z;
+  const F* Ab = A + bz * sAb;
+  const F* Bb = B + bz * sBb;
+
+  // this thread's staging coordinates (slice-relative): A (r, k), B (k, c)
+  const int ar = tid / kGK, ak = tid % kGK;
+  const int bk = B_CF ? tid / kGN : tid % kGK;
+  const int bc = B_CF ? tid % kGN : tid / kGK;
+  constexpr int a_step = kT / kGK;                 // rows of A per q
+  constexpr int b_step = B_CF ? kT / kGN : kT / kGK;  // k rows (B_CF) or columns per q
+
+  F ra[kGL], rb[kGL];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < kGL; ++q) {
+      const int r = r0 + ar + q * a_step, k = k0 + ak;
+      ra[q] = (r < M && k < K) ? Ab[(long long)r * sAr + k] : F(0);
+      const int kb = k0 + (B_CF ? bk + q * b_step : bk);
+      const int cb = c0 + (B_CF ? bc : bc + q * b_step);
+      rb[q] = (kb < K && cb < N) ? Bb[(long long)kb * sBk + (long long)cb * sBc] : F(0);
+    }
+  };
+
+  Acc acc[2][2];
+#pragma unroll
+  for (int m = 0; m < 2; ++m)
+#pragma unroll
+    for (int t = 0; t < 2; ++t) acc[m][t] = Acc{0, 0, 0, 0};
+
+  load(0);
+  for (int k0 = 0; k0 < K; k0 += kGK) {
+    if constexpr (VAR == 0) {
+      F* As = reinterpret_cast<F*>(sm_f);
+      F* Bs = As + kGM * ldA;
+#pragma unroll
+      for (int q = 0; q < kGL; ++q) {
+        As[(ar + q * a_step) * ldA + ak] = ra[q];
+        if (B_CF) Bs[(bk + q * b_step) * ldB + bc] = rb[q];
+        else Bs[bk * ldB + bc + q * b_step] = rb[q];
+      }
+    } else {
+      __bf16* Ah = reinterpret_cast<__bf16*>(sm_b);
+      __bf16* Al = Ah + kGM * kBfLd;
+      __bf16* Bh = Al + kGM * kBfLd;   // column-major: [c][k]
+      __bf16* Bl = Bh + kGN * kBfLd;
+#pragma unroll
+      for (int q = 0; q < kGL; ++q) {
+        const int ia = (ar + q * a_step) * kBfLd + ak;
+        split_bf16((float)ra[q], Ah[ia], Al[ia]);
+        const int ib = B_CF ? bc * kBfLd + bk + q * b_step : (bc + q * b_step) * kBfLd + bk;
+        split_bf16((float)rb[q], Bh[ib], Bl[ib]);
+      }
+    }
+    __syncthreads();
+    if (k0 + kGK < K) load(k0 + kGK);
+    if constexpr (VAR == 0) {
+      const F* As = reinterpret_cast<const F*>(sm_f);
+      const F* Bs = As + kGM * ldA;
+#pragma unroll
+      for (int kk = 0; kk < kGK; kk += 4) {
+        F av[2], bv[2];
+#pragma unroll
+        for (int m = 0; m < 2; ++m) av[m] = As[(wr * 32 + 16 * m + li) * ldA + kk + lk];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) bv[t] = Bs[(kk + lk) * ldB + wc * 32 + 16 * t + li];
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+          for (int t = 0; t < 2; ++t) acc[m][t] = Mfma<F>::mma(av[m], bv[t], acc[m][t]);
+      }
+    } else {
+      const __bf16* Ah = reinterpret_cast<const __bf16*>(sm_b);
+      const __bf16* Al = Ah + kGM * kBfLd;
+      const __bf16* Bh = Al + kGM * kBfLd;
+      const __bf16* Bl = Bh + kGN * kBfLd;
+      bf8v ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+      for (int m = 0; m < 2; ++m) {
+        const int o = (wr * 32 + 16 * m + li) * kBfLd + 8 * lk;
+        ah[m] = *reinterpret_cast<const bf8v*>(Ah + o);
+        al[m] = *reinterpret_cast<const bf8v*>(Al + o);
+      }
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int o = (wc * 32 + 16 * t + li) * kBfLd + 8 * lk;
+        bh[t] = *reinterpret_cast<const bf8v*>(Bh + o);
+        bl[t] = *reinterpret_cast<const bf8v*>(Bl + o);
+      }
+#pragma unroll
+      for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          f4v a4 = reinterpret_cast<f4v&>(acc[m][t]);
+          a4 = mma_bf16(al[m], bh[t], a4);
+          a4 = mma_bf16(ah[m], bl[t], a4);
+          a4 = mma_bf16(ah[m], bh[t], a4);
+          acc[m][t] = reinterpret_cast<Acc&>(a4);
+        }
+    }
+    __syncthreads();
+  }
+
+  F* Cb = C + bz * sCb;
+#pragma unroll
+  for (int m = 0; m < 2; ++m)
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int col = c0 + wc * 32 + 16 * t + li;
+      if (col >= N) continue;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = r0 + wr * 32 + 16 * m + Mfma<F>::row(lane, v);
+        if (row < M) Cb[(long long)row * sCr + col] = scale * acc[m][t][v];
+      }
+    }
+}
+
 }  // namespace sgh
 
 using namespace sgh;
+
+namespace {
+
+template <typename F, int VAR>
+void launch_axis_mfma(long long outer, int n, long long inner, const F* M, const F* in,
+                      F scale, F* out, hipStream_t st) {
+  if (inner == 1) {
+    // Out[o, k] = Σ_j In[o, j]·M[k, j]: A = In (outer × n), B[j][k] = M[k][j] (k-fast)
+    dim3 grid(cdiv(n, kGN), cdiv(outer, kGM), 1);
+    hipLaunchKernelGGL((k_gemm_mfma<F, VAR, false>), grid, dim3(kT), 0, st, (int)outer, n, n,
+                       in, 0LL, (long long)n, M, 0LL, 1LL, (long long)n, out, 0LL,
+                       (long long)n, scale);
+  } else {
+    // Out_o[k, i] = Σ_j M[k, j]·In_o[j, i]: A = M, B = In_o (column-fast), batched over o
+    dim3 grid(cdiv(inner, kGN), cdiv(n, kGM), (unsigned)outer);
+    hipLaunchKernelGGL((k_gemm_mfma<F, VAR, true>), grid, dim3(kT), 0, st, n, (int)inner, n,
+                       M, 0LL, (long long)n, in, (long long)n * inner, inner, 1LL, out,
+                       (long long)n * inner, inner, scale);
+  }
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -196,29 +373,44 @@ int sgh_sep2d(int dtype, int batch, int ny, int nx, const void* L, const void* R
   return SGH_OK;
 }
 
-int sgh_axis_contract(int dtype, long long outer, int n, long long inner, const void* M,
-                      const void* in, double scale, void* out, void* stream) {
+int sgh_axis_contract(int dtype, int variant, long long outer, int n, long long inner,
+                      const void* M, const void* in, double scale, void* out, void* stream) {
   SGH_REQUIRE(outer > 0 && n > 0 && inner > 0 && M && in && out && in != out, SGH_ERR_ARG,
               "sgh_axis_contract: bad args");
+  SGH_REQUIRE(variant == SGH_XFORM_MFMA || variant == SGH_XFORM_BF16X3 ||
+                  variant == SGH_XFORM_VALU,
+              SGH_ERR_ARG, "sgh_axis_contract: unknown variant %d", variant);
+  SGH_REQUIRE(!(variant == SGH_XFORM_BF16X3 && dtype != SGH_F32), SGH_ERR_ARG,
+              "sgh_axis_contract: split-BF16 takes FP32 data");
+  if (inner == 1)
+    SGH_REQUIRE(outer <= (1LL << 31) - 1 && cdiv(outer, kGM) <= 65535u, SGH_ERR_ARG,
+                "sgh_axis_contract: outer=%lld too large for an x-axis contraction", outer);
+  else
+    SGH_REQUIRE(outer <= 65535 && inner <= (1LL << 31) - 1 && cdiv(n, kGM) <= 65535u,
+                SGH_ERR_ARG, "sgh_axis_contract: outer/inner too large");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  SGH_DISPATCH(dtype, {
-    if (inner == 1) {
-      // Out[o, k] = Σ_j In[o, j]·M[k, j]
-      SGH_REQUIRE(outer <= (1LL << 31) - 1, SGH_ERR_ARG, "sgh_axis_contract: outer too large");
-      dim3 grid(cdiv(n, kTN), cdiv(outer, kTM), 1);
-      hipLaunchKernelGGL(k_gemm_strided<F>, grid, dim3(kT), 0, st, (int)outer, n, n,
-                         (const F*)in, 0LL, (long long)n, 1LL, (const F*)M, 0LL, 1LL,
-                         (long long)n, (F*)out, 0LL, (long long)n, 1LL, (F)scale);
-    } else {
-      // Out_o[k, i] = Σ_j M[k, j]·In_o[j, i]
-      SGH_REQUIRE(outer <= 65535 && inner <= (1LL << 31) - 1, SGH_ERR_ARG,
-                  "sgh_axis_contract: outer/inner too large");
-      dim3 grid(cdiv(inner, kTN), cdiv(n, kTM), (unsigned)outer);
-      hipLaunchKernelGGL(k_gemm_strided<F>, grid, dim3(kT), 0, st, n, (int)inner, n,
-                         (const F*)M, 0LL, (long long)n, 1LL, (const F*)in, (long long)n * inner,
-                         inner, 1LL, (F*)out, (long long)n * inner, inner, 1LL, (F)scale);
-    }
-  });
+  if (variant == SGH_XFORM_BF16X3) {
+    launch_axis_mfma<float, 1>(outer, n, inner, (const float*)M, (const float*)in, (float)scale,
+                               (float*)out, st);
+  } else if (variant == SGH_XFORM_MFMA) {
+    SGH_DISPATCH(dtype, launch_axis_mfma<F, 0>(outer, n, inner, (const F*)M, (const F*)in,
+                                               (F)scale, (F*)out, st));
+  } else {
+    SGH_DISPATCH(dtype, {
+      if (inner == 1) {
+        dim3 grid(cdiv(n, kTN), cdiv(outer, kTM), 1);
+        hipLaunchKernelGGL(k_gemm_strided<F>, grid, dim3(kT), 0, st, (int)outer, n, n,
+                           (const F*)in, 0LL, (long long)n, 1LL, (const F*)M, 0LL, 1LL,
+                           (long long)n, (F*)out, 0LL, (long long)n, 1LL, (F)scale);
+      } else {
+        dim3 grid(cdiv(inner, kTN), cdiv(n, kTM), (unsigned)outer);
+        hipLaunchKernelGGL(k_gemm_strided<F>, grid, dim3(kT), 0, st, n, (int)inner, n,
+                           (const F*)M, 0LL, (long long)n, 1LL, (const F*)in,
+                           (long long)n * inner, inner, 1LL, (F*)out, (long long)n * inner,
+                           inner, 1LL, (F)scale);
+      }
+    });
+  }
   SGH_CHECK_LAUNCH("sgh_axis_contract");
   return SGH_OK;
 }

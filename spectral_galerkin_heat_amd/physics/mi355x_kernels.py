@@ -85,16 +85,34 @@ def _dct_mats(n, tdtype):
     return m
 
 
-def _apply_axes(x, mats, scale=1.0):
-    """Contract every axis of a 3-D tensor with its matrix (mats in x, y, z order)."""
+TRANSFORM_VARIANTS = {"mfma": 0, "bf16x3": 1, "valu": 2}
+
+
+def transform_code(name, dtype):
+    """C-ABI variant code of a 3-D transform arithmetic name (SGH_XFORM_*)."""
+    key = str(name).lower()
+    if key not in TRANSFORM_VARIANTS:
+        raise ValueError(f"unknown transform {name!r}; choose one of {sorted(TRANSFORM_VARIANTS)}")
+    if key == "bf16x3" and _tdtype(dtype) != torch.float32:
+        raise ValueError("the split-BF16 transform runs on float32 data (simulation.dtype float32)")
+    return TRANSFORM_VARIANTS[key]
+
+
+def _apply_axes(x, mats, scale=1.0, xform=0):
+    """Contract every axis of a 3-D tensor with its matrix (mats in x, y, z order).
+
+    Three MFMA GEMM launches (``sgh_axis_contract``): x as Out[zy, k] = In[zy, j]·M[k, j],
+    y batched over z planes, z as one GEMM over all (y, x) columns.
+    """
     nz, ny, nx = x.shape
     code = dtype_code(x.dtype)
     st = stream_ptr()
     w1 = torch.empty_like(x)
     w2 = torch.empty_like(x)
-    call("sgh_axis_contract", code, nz * ny, nx, 1, ptr(mats[0]), ptr(x), 1.0, ptr(w1), st)
-    call("sgh_axis_contract", code, nz, ny, nx, ptr(mats[1]), ptr(w1), 1.0, ptr(w2), st)
-    call("sgh_axis_contract", code, 1, nz, ny * nx, ptr(mats[2]), ptr(w2), float(scale), ptr(w1), st)
+    call("sgh_axis_contract", code, xform, nz * ny, nx, 1, ptr(mats[0]), ptr(x), 1.0, ptr(w1), st)
+    call("sgh_axis_contract", code, xform, nz, ny, nx, ptr(mats[1]), ptr(w1), 1.0, ptr(w2), st)
+    call("sgh_axis_contract", code, xform, 1, nz, ny * nx, ptr(mats[2]), ptr(w2), float(scale),
+         ptr(w1), st)
     return w1
 
 
@@ -108,7 +126,9 @@ def _sep2d(X, L, R, scale=1.0, nparts=1, out=None):
 
 
 def DCT_II(q):
-    """Ortho DCT-II over every axis (spectral_cpu_kernels.py:374-381), on MFMA-free LDS GEMMs."""
+    """Ortho DCT-II over every axis (spectral_cpu_kernels.py:374-381).
+
+    2-D: one LDS GEMM-pair launch; 3-D: three MFMA axis contractions in the data type."""
     q = q.contiguous()
     td = q.dtype
     if q.dim() == 2:
@@ -122,7 +142,9 @@ def DCT_II(q):
 
 
 def IDCT_II(a):
-    """Ortho DCT-III over every axis, the inverse of DCT_II (spectral_cpu_kernels.py:383-389)."""
+    """Ortho DCT-III over every axis, the inverse of DCT_II (spectral_cpu_kernels.py:383-389).
+
+    Same launch structure as ``DCT_II``."""
     a = a.contiguous()
     td = a.dtype
     if a.dim() == 2:
@@ -232,6 +254,7 @@ class SpectralSolverState:
         self.fine_mesh = None
         self.buffers = SolverBuffers(num, self.dtype)
         self.K, self.KK = precompute_K_KK(phys, num, geom, self.dtype)
+        self.xform = transform_code(getattr(num, "transform", "mfma"), self.dtype)
         self.a = None
         self.hooks = None
 
@@ -308,13 +331,13 @@ def compute_evaporation_flux(T_surface, q_out, P0, T_boil, DeltaH_LV, R_v, T_liq
 def reconstruct_volume(a, SsState):
     """T = Σ a·B_z B_y B_x on the cell-centred grid = IDCT(a)/√dV (spectral_ops.py:83-91)."""
     g = SsState.grid
-    return _apply_axes(a.contiguous(), (g.BxT, g.ByT, g.BzT))
+    return _apply_axes(a.contiguous(), (g.BxT, g.ByT, g.BzT), xform=SsState.xform)
 
 
 def project_volume(field, SsState):
     """modes = dV·Σ f·B_z B_y B_x = √dV·DCT(f) (spectral_ops.py:94-102)."""
     g = SsState.grid
-    return _apply_axes(field.contiguous(), (g.Bx, g.By, g.Bz), scale=g.dV)
+    return _apply_axes(field.contiguous(), (g.Bx, g.By, g.Bz), scale=g.dV, xform=SsState.xform)
 
 
 def _latent(Q_buffer, T, T_prev, phys, num):

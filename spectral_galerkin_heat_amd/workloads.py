@@ -38,11 +38,13 @@ def make_context(cfg: dict, laser_path) -> SimulationContext:
 
 
 def _cfg(size, mesh, dt, n_steps, mat, radius, absorptivity, power, tol, maxit,
-         dtype="float64", omega=None):
+         dtype="float64", omega=None, transform=None):
     sim = {"backend": "mi355x", "dtype": dtype, "dt": dt, "duration": dt * n_steps,
            "picard_tol": tol, "max_picard_iter": maxit}
     if omega is not None:
         sim["picard_omega"] = omega
+    if transform is not None:
+        sim["transform"] = transform
     return {
         "simulation": sim,
         "domain": {"size": list(size), "mesh": list(mesh)},
@@ -63,13 +65,14 @@ def single_track_constant(mesh=(64, 32, 16), n_steps=320, dtype="float64"):
     return cfg, SingleTrackLaser(0.2e-3, 1.8e-3, 0.5e-3, 0.5, 200.0)
 
 
-def single_track_tdep(mesh=(128, 64, 32), n_steps=640, dtype="float64"):
+def single_track_tdep(mesh=(128, 64, 32), n_steps=640, dtype="float64", transform=None):
     """The same track with latent heat, top/bottom convection and T-dependent k, cp.
 
     L_f = 2.7e5 J/kg over 1658–1723 K (grid-mode latent source), h = 10 W/m²K to
     293 K on z = 0 and z = Lz, k(T) = min(14.6 + 0.0127(T − 293), 30),
     cp(T) = min(450 + 0.28(T − 293), 750) with the caps written through ``fabs``,
     reference k̄ = 22.3, c̄ = 600 (midpoints of their ranges); Δt = 5e-6 s, tol 1e-6.
+    ``transform`` selects the 3-D transform arithmetic ("mfma", "bf16x3", "valu").
     """
     mat = copy.deepcopy(MATERIAL_316L)
     k_lin, cp_lin = "(14.6 + 0.0127*(T - 293.0))", "(450.0 + 0.28*(T - 293.0))"
@@ -82,7 +85,7 @@ def single_track_tdep(mesh=(128, 64, 32), n_steps=640, dtype="float64"):
         "Cp": {"solid": cp_expr, "liquid": cp_expr, "reference": 600.0},
     })
     cfg = _cfg((2.0e-3, 1.0e-3, 0.5e-3), mesh, 5e-6, n_steps, mat,
-               50e-6, 0.35, 200.0, 1e-6, 20, dtype)
+               50e-6, 0.35, 200.0, 1e-6, 20, dtype, transform=transform)
     return cfg, SingleTrackLaser(0.2e-3, 1.8e-3, 0.5e-3, 0.5, 200.0)
 
 

@@ -1,0 +1,88 @@
+"""GPU checks of the 3-D axis contractions (``sgh_axis_contract``) in every arithmetic.
+
+Each variant contracts one axis of a seeded [outer, n, inner] volume with an n×n matrix
+and is compared with the FP64 einsum of the same operands. Tolerances (relative L2):
+MFMA f64 and vector f64 1e-13; MFMA f32 and vector f32 2e-6; split-BF16 (three bf16
+products, FP32 accumulation) 3e-5. Shapes include n and inner that are not multiples of
+the 64×64×32 tile, a batch of planes, and outer large enough for several row tiles.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from oracle import sgh_oracle as o  # noqa: E402
+from spectral_galerkin_heat_amd import _lib  # noqa: E402
+
+DEV = "cuda"
+MFMA, BF16X3, VALU = 0, 1, 2
+TOL = {(MFMA, torch.float64): 1e-13, (VALU, torch.float64): 1e-13,
+       (MFMA, torch.float32): 2e-6, (VALU, torch.float32): 2e-6,
+       (BF16X3, torch.float32): 3e-5}
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def contract(variant, dtype, M, X, scale=1.0):
+    outer, n, inner = X.shape
+    Mt = torch.as_tensor(M).to(DEV, dtype).contiguous()
+    Xt = torch.as_tensor(X).to(DEV, dtype).contiguous()
+    out = torch.empty_like(Xt)
+    _lib.call("sgh_axis_contract", _lib.dtype_code(dtype), variant, outer, n, inner, _lib.ptr(Mt),
+              _lib.ptr(Xt), float(scale), _lib.ptr(out), _lib.stream_ptr())
+    torch.cuda.synchronize()
+    return out.cpu().double().numpy()
+
+
+@pytest.mark.parametrize("variant,dtype", sorted(TOL, key=str))
+@pytest.mark.parametrize("shape", [(5, 7, 1), (3, 9, 11), (1, 24, 4096), (16, 48, 96),
+                                   (130, 70, 1), (2, 130, 70), (64, 128, 1), (1, 32, 8192)])
+def test_axis_contract_matches_fp64_einsum(variant, dtype, shape):
+    rng = np.random.default_rng(sum(shape))
+    outer, n, inner = shape
+    M = rng.standard_normal((n, n))
+    X = rng.standard_normal(shape)
+    X[..., 0, :] += 300.0   # a dominant mean mode, as in a temperature field
+    ref = 0.5 * np.einsum("kj,oji->oki", M, X)
+    got = contract(variant, dtype, M, X, scale=0.5)
+    assert rel(got, ref) < TOL[(variant, dtype)]
+
+
+@pytest.mark.parametrize("shape", [(16, 32, 64), (32, 64, 128), (24, 48, 96)])
+def test_dct3d_every_variant_matches_scipy(shape):
+    from spectral_galerkin_heat_amd.physics import mi355x_kernels as K
+    rng = np.random.default_rng(3)
+    f = 293.0 + 50.0 * rng.standard_normal(shape)
+    ref = o.dct2(f)
+    for (variant, dtype), tol in TOL.items():
+        x = torch.as_tensor(f).to(DEV, dtype)
+        mats = [K._dct_mats(n, dtype)[0] for n in (shape[2], shape[1], shape[0])]
+        got = K._apply_axes(x, mats, xform=variant).cpu().double().numpy()
+        assert rel(got, ref) < 3 * tol, (variant, dtype)
+
+
+def test_bf16x3_rejects_fp64_data():
+    M = np.eye(8)
+    X = np.ones((2, 8, 3))
+    with pytest.raises(_lib.SGHError, match="split-BF16"):
+        contract(BF16X3, torch.float64, M, X)
+
+
+def test_solver_state_selects_transform():
+    from spectral_galerkin_heat_amd.core.parameters import GeomParams, MaterialParams, NumParams
+    from spectral_galerkin_heat_amd.core.vector import Vec3
+    from spectral_galerkin_heat_amd.physics import mi355x_kernels as K
+    geom = GeomParams(size=Vec3(1e-3, 1e-3, 5e-4), n=Vec3(16, 16, 8))
+    phys = MaterialParams(rho=7900.0, k=20.0, Cp=500.0)
+    num = NumParams(dt=1e-6, nx=16, ny=16, nz=8, dtype=np.float32, transform="bf16x3")
+    S = K.SpectralSolverState(phys, geom, num, None)
+    assert S.xform == BF16X3
+    a = torch.zeros((8, 16, 16), device=DEV)
+    a[0, 0, 0] = 293.0 * np.sqrt(1e-3 * 1e-3 * 5e-4)
+    T = K.reconstruct_volume(a, S)
+    np.testing.assert_allclose(T.cpu().numpy(), 293.0, rtol=2e-5)

@@ -177,3 +177,14 @@ def test_gcode_path(tmp_path):
     assert s.is_on and float(s.x) == pytest.approx(5e-3, rel=1e-6) and float(s.power) == 200.0
     assert float(s.v[0]) == pytest.approx(0.01, rel=1e-3)
     assert not p.get_state(2.0, 0.01).is_on
+
+
+def test_melt_pool_extent_counts_cells_from_the_top():
+    import torch
+
+    from spectral_galerkin_heat_amd.physics.meltpool import melt_pool_extent
+    T = torch.full((8, 10, 12), 300.0, dtype=torch.float64)
+    T[5:8, 3:6, 2:9] = 2000.0
+    T[6, 4, 10] = 1723.0          # on the threshold counts as melted
+    assert melt_pool_extent(T, (1.0, 2.0, 3.0)) == (9 * 1.0, 3 * 2.0, 3 * 3.0)
+    assert melt_pool_extent(torch.full((4, 4, 4), 300.0), (1, 1, 1)) == (0.0, 0.0, 0.0)
