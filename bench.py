@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default surface-transform variant instead of autotuning")
     return ap.parse_args()
 
 
@@ -97,8 +99,16 @@ def main():
     N_total = sum(int(np.prod(tuple(c.geom.n))) for c in ctxs)
     log(f"workload: 64 cuboids, sum N = {N_total}, warmup {W}, timed {K}")
 
+    tuning = {}
+    if not args.no_tune:
+        from spectral_galerkin_heat_amd.engine.autotune import autotune
+        for prec in ("mixed", "fp64"):
+            tuning[prec] = autotune(ctxs, prec)
+            log(f"autotune {prec}: {tuning[prec].summary()}")
+
     def timed_engine(precision):
-        eng = BatchEngine(ctxs, precision=precision)
+        tv = tuning.get(precision)
+        eng = BatchEngine(ctxs, precision=precision, variant=tv.variant if tv else None)
         eng.run(W)
         eng.synchronize()
         torch.cuda.synchronize()
@@ -270,6 +280,7 @@ def main():
             "kernel_share_of_step": {"surface_iterations": round(share_surf, 3),
                                      "volume_pass": round(share_vol, 3)},
             "parallelism": "single GPU",
+            "autotune": {p: t.summary() for p, t in tuning.items()},
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -224,6 +224,23 @@ int sgh_batch_timing(void* plan, double* out4);
  *         max Picard cap, LDS bytes per surface workgroup}. */
 int sgh_batch_info(void* plan, long long* out6);
 
+/* ---- autotuning knobs of the batched surface transform (no reference counterpart;
+ * the reference has no batched engine).  A variant is (rows per work item, threads per
+ * workgroup) plus the dynamic LDS bytes per workgroup (4 KiB to 159 KiB); the default is
+ * (16, 512, 150 KiB).  sgh_batch_variants writes up to `capacity` (rows, threads) pairs
+ * and returns how many exist.  sgh_batch_set_variant synchronises `stream`; the work
+ * lists and the captured step graph are rebuilt by the next sgh_batch_run.  Results
+ * are deterministic for a given variant; different variants sum per-item partials in
+ * different groupings (last-bit differences). */
+int sgh_batch_variants(int* out_pairs, int capacity);
+int sgh_batch_set_variant(void* plan, int rows, int threads, int lds_bytes, void* stream);
+/* out3 = {rows, threads, LDS bytes}. */
+int sgh_batch_get_variant(void* plan, int* out3);
+/* Step counters, Picard status and surface fields of every live cuboid back to zero
+ * (synchronises `stream`); the modal states are kept: set them with
+ * sgh_batch_set_state before the next run to restart from an initial condition. */
+int sgh_batch_rewind(void* plan, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

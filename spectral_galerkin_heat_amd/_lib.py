@@ -58,6 +58,10 @@ SIGNATURES = {
     "sgh_batch_set_timing": [vp, i32],
     "sgh_batch_timing": [vp, vp],
     "sgh_batch_info": [vp, vp],
+    "sgh_batch_variants": [vp, i32],
+    "sgh_batch_set_variant": [vp, i32, i32, i32, vp],
+    "sgh_batch_get_variant": [vp, vp],
+    "sgh_batch_rewind": [vp, vp],
 }
 
 

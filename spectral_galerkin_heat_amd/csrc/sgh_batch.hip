@@ -33,16 +33,14 @@ static_assert(sizeof(sgh_cuboid_spec) == 176, "sgh_cuboid_spec layout (mirrored 
 namespace sgh {
 
 constexpr int kBT = 256;        // threads per block
-constexpr int kBRB = 16;        // surface rows per transform item (one MFMA row tile)
-constexpr int kMT = kBRB / 16;   // MFMA row tiles per item
+constexpr int kRowsMin = 16;    // fewest surface rows per transform item (one MFMA row tile)
 constexpr int kBMaxN = 256;     // widest surface axis
 constexpr int kStatusStride = 4;  // n_it, converged, rel, pad
 constexpr int kMetricStride = 5;  // P_laser, T_max, n_it, converged, rel
 constexpr int kSurfArrays = 10;
-constexpr int kST = 512;         // threads of a surface-transform workgroup
 constexpr int kStageAhead = 8;   // global loads in flight per thread while staging into LDS
-constexpr int kTilesPerWave = 2; // 16-column tiles per wave: 16 tiles (nx = 256) over 8 waves
-constexpr int kSurfLds = 150 * 1024;  // LDS bytes per surface workgroup (gfx950 allows 160 KiB)
+constexpr int kSurfLds = 150 * 1024;  // default LDS bytes per surface workgroup
+constexpr int kLdsMax = 159 * 1024;   // dynamic LDS cap: gfx950's 160 KiB less the static part
 constexpr int kXcds = 8;          // MI355X: workgroup b runs on XCD b mod 8
 
 struct CuboidDesc {
@@ -98,6 +96,7 @@ struct BatchPtrs {
   double* metrics;         // [step][cuboid][5]
   int metrics_rows;        // capacity in steps
   int n_cuboids;
+  int rows;                // surface rows per transform item (16 or 32)
 };
 
 __device__ __forceinline__ double dexp_(double x) { return exp(x); }
@@ -229,8 +228,8 @@ __global__ void __launch_bounds__(kBT) k_b_prep(BatchPtrs<F> P) {
     const double x0 = L[0], y0 = L[1], power = L[2], on = L[3], vx = L[4], vy = L[5];
     const double I0 = on != 0.0 ? d.absorptivity * power / (d.area_factor * d.rx * d.ry) : 0.0;
     const double sy = vy * d.dt / d.dy, sx = vx * d.dt / d.dx;
-    const int r0 = it.blk * kBRB;
-    const int rows = min(kBRB, d.ny - r0);
+    const int r0 = it.blk * P.rows;
+    const int rows = min(P.rows, d.ny - r0);
     const F* qo = P.q_evap_old + d.s_off;
     for (int e = threadIdx.x; e < rows * d.nx; e += kBT) {
       const int j = r0 + e / d.nx, i = e % d.nx;
@@ -357,14 +356,18 @@ __device__ unsigned long long g_block_ts[4 * 8192 * 6];
 #define SGH_TS(slot) do {} while (0)
 #endif
 
-template <typename F, int MODE>
-__global__ void __launch_bounds__(kST) k_b_surface(BatchPtrs<F> P, int k, int bud) {
+template <typename F, int MODE, int RB, int NT>
+__global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud) {
   SGH_TS(0);
   using M = Mfma<F>;
   using V = typename M::V;
-  constexpr int NW = kST / kWave;
+  constexpr int kBRB = RB;                      // rows of this item
+  constexpr int kMT = RB / 16;                  // MFMA row tiles
+  constexpr int kST = NT;                       // threads
+  constexpr int NW = NT / kWave;                // waves
+  constexpr int kTilesPerWave = (kBMaxN / 16 + NW - 1) / NW;  // 16-column tiles per wave
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  __shared__ double sh[kST / kWave];
+  __shared__ double sh[NT / kWave];
   __shared__ int s_active;
   const Item it = P.row_items[blockIdx.x];
   if (it.c < 0) return;
@@ -595,7 +598,7 @@ __global__ void __launch_bounds__(kBT) k_b_metrics(BatchPtrs<F> P) {
 // removed between steps (continuous batching); work lists and the captured step graph
 // are rebuilt when membership changes.
 // ---------------------------------------------------------------------------------
-constexpr int kMaxRowBlk = kBMaxN / kBRB;          // row blocks per cuboid (ny <= 256)
+constexpr int kMaxRowBlk = kBMaxN / kRowsMin;      // row blocks per cuboid (ny <= 256)
 constexpr int kMaxColBlk = kBMaxN * kBMaxN / kBT;  // column blocks per cuboid (S <= 65536)
 constexpr int kMaxSizes = 32;                      // distinct mode counts in the matrix table
 
@@ -653,6 +656,8 @@ struct Plan {
   int t1_stride = 0;
   size_t shm = 0;
   int bud = 0;
+  int rows = 16;       // surface rows per transform item
+  int threads = 512;   // threads per surface workgroup
   long long total_N = 0, total_S = 0;
   // device allocations
   void* a = nullptr;
@@ -716,19 +721,73 @@ BatchPtrs<F> ptrs(Plan* p) {
   P.metrics = p->metrics;
   P.metrics_rows = p->metrics_rows;
   P.n_cuboids = p->n;
+  P.rows = p->rows;
   return P;
+}
+
+// Surface-transform variants: (rows per item, threads per workgroup). Each instantiates
+// the four modes of k_b_surface; the autotuner picks one per batch.
+struct SurfVariant {
+  int rows, threads;
+};
+// (32, 1024) is left out: its forward mode exceeds the 128 VGPRs of a 1024-thread group.
+constexpr SurfVariant kSurfVariants[] = {{16, 256}, {16, 512}, {16, 1024}, {32, 512}};
+constexpr int kNumSurfVariants = sizeof(kSurfVariants) / sizeof(kSurfVariants[0]);
+
+template <typename F>
+using SurfFn = void (*)(BatchPtrs<F>, int, int);
+
+template <typename F, int RB, int NT>
+static void surf_fns(SurfFn<F>* f) {
+  f[0] = &k_b_surface<F, 0, RB, NT>;
+  f[1] = &k_b_surface<F, 1, RB, NT>;
+  f[2] = &k_b_surface<F, 2, RB, NT>;
+  f[3] = &k_b_surface<F, 3, RB, NT>;
+}
+
+template <typename F>
+static bool surf_lookup(int rows, int threads, SurfFn<F>* f) {
+  if (rows == 16 && threads == 256) surf_fns<F, 16, 256>(f);
+  else if (rows == 16 && threads == 512) surf_fns<F, 16, 512>(f);
+  else if (rows == 16 && threads == 1024) surf_fns<F, 16, 1024>(f);
+  else if (rows == 32 && threads == 512) surf_fns<F, 32, 512>(f);
+  else return false;
+  return true;
+}
+
+template <typename F>
+static hipError_t launch_surface(const SurfFn<F>* f, int mode, unsigned nrow, const Plan* p,
+                                 BatchPtrs<F>& P, int k, hipStream_t st) {
+  int bud = p->bud;
+  void* args[] = {&P, &k, &bud};
+  return hipLaunchKernel(reinterpret_cast<const void*>(f[mode]), dim3(nrow), dim3(p->threads),
+                         args, p->shm, st);
+}
+
+// LDS of a surface workgroup: Ls[rows][ldl] + Ys[rows][ldy] + a staging buffer of `bud`
+// elements that holds X (or R) whole when it fits, else in chunks of at least 4 rows (X)
+// or 16 columns (R).
+static int surface_budget(int dtype, int rows, long long lds_bytes) {
+  const int n4 = (kBMaxN + 3) & ~3;
+  const int ld = dtype == SGH_F64 ? pad_a<double>(n4) : pad_a<float>(n4);
+  const long long elems = lds_bytes / (dtype == SGH_F64 ? 8 : 4);
+  const long long bud = elems - 2LL * rows * ld;
+  const long long need1 = 4LL * (dtype == SGH_F64 ? pad_b<double>(kBMaxN) : pad_b<float>(kBMaxN));
+  const long long need2 = (long long)n4 * (dtype == SGH_F64 ? pad_b<double>(16) : pad_b<float>(16));
+  return (bud < need1 || bud < need2) ? -1 : (int)bud;
 }
 
 template <typename F>
 static hipError_t allow_large_lds(size_t bytes) {
   hipError_t e = hipSuccess;
-  const void* fns[] = {reinterpret_cast<const void*>(&k_b_surface<F, 0>),
-                       reinterpret_cast<const void*>(&k_b_surface<F, 1>),
-                       reinterpret_cast<const void*>(&k_b_surface<F, 2>),
-                       reinterpret_cast<const void*>(&k_b_surface<F, 3>)};
-  for (const void* f : fns) {
-    hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e == hipSuccess) e = r;
+  for (const SurfVariant& v : kSurfVariants) {
+    SurfFn<F> f[4];
+    surf_lookup<F>(v.rows, v.threads, f);
+    for (int m = 0; m < 4; ++m) {
+      hipError_t r = hipFuncSetAttribute(reinterpret_cast<const void*>(f[m]),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+      if (e == hipSuccess) e = r;
+    }
   }
   return e;
 }
@@ -739,16 +798,24 @@ int enqueue_step(Plan* p, hipStream_t st) {
   const unsigned nrow = (unsigned)p->hrow.size(), ncol = (unsigned)p->hcol.size();
   if (nrow == 0) return SGH_OK;
   const bool tm = p->timing;
+  SurfFn<F> f[4];
+  if (!surf_lookup<F>(p->rows, p->threads, f)) {
+    set_error("sgh_batch: no surface variant rows=%d threads=%d", p->rows, p->threads);
+    return SGH_ERR_STATE;
+  }
+  hipError_t e = hipSuccess;
+  auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
   hipLaunchKernelGGL(k_b_prep<F>, dim3(nrow), dim3(kBT), 0, st, P);
-  hipLaunchKernelGGL((k_b_surface<F, 0>), dim3(nrow), dim3(kST), p->shm, st, P, 0, p->bud);
+  chk(launch_surface<F>(f, 0, nrow, p, P, 0, st));
   if (tm) hipEventRecord(p->ev[0], st);
   for (int k = 0; k < p->max_it; ++k) {
-    hipLaunchKernelGGL((k_b_surface<F, 2>), dim3(nrow), dim3(kST), p->shm, st, P, k, p->bud);
-    hipLaunchKernelGGL((k_b_surface<F, 1>), dim3(nrow), dim3(kST), p->shm, st, P, k, p->bud);
+    chk(launch_surface<F>(f, 2, nrow, p, P, k, st));
+    chk(launch_surface<F>(f, 1, nrow, p, P, k, st));
   }
   if (tm) hipEventRecord(p->ev[1], st);
-  hipLaunchKernelGGL((k_b_surface<F, 3>), dim3(nrow), dim3(kST), p->shm, st, P, p->max_it, p->bud);
+  chk(launch_surface<F>(f, 3, nrow, p, P, p->max_it, st));
   if (tm) hipEventRecord(p->ev[2], st);
+  if (e != hipSuccess) return fail_hip(e, "sgh_batch surface launch");
   hipLaunchKernelGGL(k_b_volume<F>, dim3(ncol), dim3(kBT), 0, st, P, 1);
   if (tm) hipEventRecord(p->ev[3], st);
   hipLaunchKernelGGL(k_b_metrics<F>, dim3(1), dim3(kBT), 0, st, P);
@@ -958,7 +1025,7 @@ static int add_cuboid(Plan* p, const sgh_cuboid_spec& sp, int c, hipStream_t st)
     t1.insert(t1.end(), k2.begin(), k2.end());
   }
   d.row_item0 = c * kMaxRowBlk;
-  d.nrowblk = (sp.ny + kBRB - 1) / kBRB;
+  d.nrowblk = (sp.ny + p->rows - 1) / p->rows;
   d.col_item0 = c * kMaxColBlk;
   d.ncolblk = (d.S + kBT - 1) / kBT;
   d.max_it = sp.max_it;
@@ -1062,28 +1129,19 @@ int sgh_batch_pool_create(int dtype, int max_cuboids, long long state_capacity,
   p->total_S = surface_capacity;
   p->t1_stride = 2 * (kBMaxN + kBMaxN + max_nz);
   p->mats_cap = 2LL * 16 * kBMaxN * kBMaxN;
-  {
-    // LDS of a surface workgroup: Ls[kBRB][ldl] + Ys[kBRB][ldy] + a staging buffer of
-    // `bud` elements that holds X (or R) whole when it fits, else in chunks.
-    const int n4 = (kBMaxN + 3) & ~3;
-    const int ld = dtype == SGH_F64 ? pad_a<double>(n4) : pad_a<float>(n4);
-    const long long elems = (long long)kSurfLds / (long long)p->elem;
-    p->bud = (int)(elems - 2LL * kBRB * ld);
-    p->shm = (size_t)kSurfLds;
-    const long long need1 = 4LL * (dtype == SGH_F64 ? pad_b<double>(kBMaxN) : pad_b<float>(kBMaxN));
-    const long long need2 = (long long)n4 * (dtype == SGH_F64 ? pad_b<double>(16) : pad_b<float>(16));
-    if (p->bud < need1 || p->bud < need2) {
-      free_plan(p);
-      set_error("sgh_batch_pool_create: LDS staging buffer too small");
-      return SGH_ERR_ARG;
-    }
+  p->bud = surface_budget(dtype, p->rows, kSurfLds);
+  p->shm = (size_t)kSurfLds;
+  if (p->bud < 0) {
+    free_plan(p);
+    set_error("sgh_batch_pool_create: LDS staging buffer too small");
+    return SGH_ERR_ARG;
   }
   p->row_cap = (long long)max_cuboids * kMaxRowBlk + 8LL * kMaxRowBlk * kXcds;
   p->col_cap = (long long)max_cuboids * kMaxColBlk + 8LL * kMaxColBlk * kXcds;
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
   const size_t E = p->elem;
-  chk(dtype == SGH_F64 ? allow_large_lds<double>(p->shm) : allow_large_lds<float>(p->shm));
+  chk(dtype == SGH_F64 ? allow_large_lds<double>(kLdsMax) : allow_large_lds<float>(kLdsMax));
   chk(dmalloc(&p->a, state_capacity * E));
   chk(dmalloc(&p->surf, kSurfArrays * surface_capacity * E));
   chk(dmalloc(&p->mats, p->mats_cap * E));
@@ -1337,6 +1395,80 @@ int sgh_debug_block_timestamps(unsigned long long* host_out, int n) {
   return hipMemcpyFromSymbol(host_out, HIP_SYMBOL(g_block_ts), (size_t)4 * 8192 * 6 * 8) == hipSuccess ? 0 : -2;
 }
 #endif
+
+int sgh_batch_variants(int* out_pairs, int capacity) {
+  if (out_pairs)
+    for (int i = 0; i < kNumSurfVariants && i < capacity; ++i) {
+      out_pairs[2 * i] = kSurfVariants[i].rows;
+      out_pairs[2 * i + 1] = kSurfVariants[i].threads;
+    }
+  return kNumSurfVariants;
+}
+
+int sgh_batch_set_variant(void* plan, int rows, int threads, int lds_bytes, void* stream) {
+  Plan* p = static_cast<Plan*>(plan);
+  SGH_REQUIRE(p, SGH_ERR_ARG, "sgh_batch_set_variant: null plan");
+  bool known = false;
+  for (const SurfVariant& v : kSurfVariants) known |= v.rows == rows && v.threads == threads;
+  SGH_REQUIRE(known, SGH_ERR_ARG, "sgh_batch_set_variant: no variant rows=%d threads=%d", rows,
+              threads);
+  SGH_REQUIRE(lds_bytes >= 4096 && lds_bytes <= kLdsMax, SGH_ERR_ARG,
+              "sgh_batch_set_variant: lds_bytes=%d outside [4096, %d]", lds_bytes, kLdsMax);
+  const int bud = surface_budget(p->dtype, rows, lds_bytes);
+  SGH_REQUIRE(bud >= 0, SGH_ERR_ARG,
+              "sgh_batch_set_variant: %d LDS bytes cannot stage a 256-wide operand", lds_bytes);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  hipError_t e = hipStreamSynchronize(st);  // no step of the old variant is in flight
+  if (e != hipSuccess) return fail_hip(e, "sgh_batch_set_variant");
+  p->rows = rows;
+  p->threads = threads;
+  p->bud = bud;
+  p->shm = (size_t)lds_bytes;
+  for (int c = 0; c < p->n; ++c) {
+    CuboidDesc& d = p->hdesc[c];
+    if (!d.live) continue;
+    d.nrowblk = (d.ny + rows - 1) / rows;
+    e = hipMemcpy(p->desc + c, &d, sizeof(CuboidDesc), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail_hip(e, "sgh_batch_set_variant");
+  }
+  p->dirty = true;  // work lists and the step graph are rebuilt by the next run
+  return SGH_OK;
+}
+
+int sgh_batch_rewind(void* plan, void* stream) {
+  Plan* p = static_cast<Plan*>(plan);
+  SGH_REQUIRE(p, SGH_ERR_ARG, "sgh_batch_rewind: null plan");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  hipError_t e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return fail_hip(e, "sgh_batch_rewind");
+  const int zero = 0;
+  const double zstat[kStatusStride] = {0, 0, 0, 0};
+  auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
+  for (int c = 0; c < p->n; ++c) {
+    if (!p->hdesc[c].live) continue;
+    const SlotAlloc& al = p->halloc[c];
+    chk(hipMemcpy(p->step_idx + c, &zero, sizeof(int), hipMemcpyHostToDevice));
+    chk(hipMemcpy(p->done_at + c, &zero, sizeof(int), hipMemcpyHostToDevice));
+    chk(hipMemcpy(p->status + kStatusStride * c, zstat, sizeof(zstat), hipMemcpyHostToDevice));
+    for (int arr = 0; arr < kSurfArrays; ++arr)
+      chk(hipMemsetAsync((char*)p->surf + (arr * p->total_S + al.s_off) * p->elem, 0,
+                         al.s_len * p->elem, st));
+    if (std::find(p->pending_prime.begin(), p->pending_prime.end(), c) == p->pending_prime.end())
+      p->pending_prime.push_back(c);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return fail_hip(e, "sgh_batch_rewind");
+  return SGH_OK;
+}
+
+int sgh_batch_get_variant(void* plan, int* out3) {
+  Plan* p = static_cast<Plan*>(plan);
+  SGH_REQUIRE(p && out3, SGH_ERR_ARG, "sgh_batch_get_variant: bad args");
+  out3[0] = p->rows;
+  out3[1] = p->threads;
+  out3[2] = (int)p->shm;
+  return SGH_OK;
+}
 
 int sgh_batch_info(void* plan, long long* out6) {
   Plan* p = static_cast<Plan*>(plan);

@@ -111,7 +111,10 @@ class BatchEngine:
     """A batch of cuboids advanced together; fixed membership unless built as a pool."""
 
     def __init__(self, contexts, precision: str = "mixed", n_steps=None, metrics_rows=None,
-                 omega=None, tol=None, max_it=None):
+                 omega=None, tol=None, max_it=None, variant=None):
+        """``variant``: None keeps the default surface-transform variant, ``"auto"`` runs
+        the autotuner on a scratch copy of this batch, or a ``(rows, threads, lds_bytes)``
+        tuple / dict sets one explicitly."""
         self._setup(precision)
         self.contexts = list(contexts)
         self.n = len(self.contexts)
@@ -134,6 +137,15 @@ class BatchEngine:
         self._plan = plan
         self.mode_counts = [int(np.prod(s)) for s in self.shapes]
         self.steps_done = 0
+        self.tuning = None
+        if isinstance(variant, str) and variant == "auto":
+            from .autotune import autotune
+            self.tuning = autotune(self.contexts, precision)
+            variant = self.tuning.variant
+        if variant is not None:
+            if isinstance(variant, dict):
+                variant = (variant["rows"], variant["threads"], variant["lds_bytes"])
+            self.set_variant(*variant)
         self.reset_initial_condition()
 
     def _setup(self, precision):
@@ -156,6 +168,25 @@ class BatchEngine:
         return {"live_state_elems": info[0], "surface_capacity": info[1],
                 "surface_items": info[2], "volume_items": info[3], "max_it": info[4],
                 "lds_bytes": info[5]}
+
+    # -- surface-transform variant -----------------------------------------------------
+    def set_variant(self, rows, threads, lds_bytes=150 * 1024):
+        """Rows per transform item, threads per workgroup and LDS bytes per workgroup of
+        the batched surface transforms (takes effect at the next ``run``)."""
+        call("sgh_batch_set_variant", self._plan, int(rows), int(threads), int(lds_bytes),
+             self.stream.cuda_stream)
+
+    @property
+    def variant(self):
+        out = (C.c_int * 3)()
+        call("sgh_batch_get_variant", self._plan, C.cast(out, C.c_void_p))
+        return {"rows": out[0], "threads": out[1], "lds_bytes": out[2]}
+
+    def rewind(self):
+        """Restart every cuboid from its initial condition at step 0."""
+        call("sgh_batch_rewind", self._plan, self.stream.cuda_stream)
+        self.steps_done = 0
+        self.reset_initial_condition()
 
     # -- state ------------------------------------------------------------------------
     def reset_initial_condition(self):
@@ -262,6 +293,9 @@ class BatchPool(BatchEngine):
         a = initial_modes(ctx, self.shapes[i], self.tdtype) if initial_state is None else initial_state
         self.set_state(i, a)
         return i
+
+    def rewind(self):
+        raise NotImplementedError("a pool's members start when they are added")
 
     def remove(self, i):
         call("sgh_batch_remove", self._plan, int(i), self.stream.cuda_stream)

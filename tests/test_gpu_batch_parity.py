@@ -117,3 +117,68 @@ def test_graph_replay_equals_eager_launches():
             assert torch.equal(g.state(i), e.state(i))
     g.close()
     e.close()
+
+
+@pytest.mark.parametrize("lds", [150 * 1024, 159 * 1024, 72 * 1024])
+def test_every_surface_variant_matches_oracle(lds):
+    """Each (rows, threads) variant and LDS budget keeps the engines on the oracle. Small
+    budgets stage the operands in chunks; a budget that cannot hold a 16-column chunk of a
+    256-row operand next to the item's rows is refused (some FP64 and 72 KiB cases)."""
+    from spectral_galerkin_heat_amd import _lib
+    from spectral_galerkin_heat_amd.engine.autotune import surface_variants
+    cases = batch_uneven_cuboids(n=3, seed=21, n_steps=5)
+    refs = [_oracle_run(cfg, lp, 5) for cfg, lp in cases]
+    for precision in ("fp64", "mixed"):
+        eng = _engine(cases, precision)
+        for rows, threads in surface_variants():
+            try:
+                eng.set_variant(rows, threads, lds)
+            except _lib.SGHError:
+                assert precision == "fp64" or lds < 100 * 1024
+                continue
+            eng.rewind()
+            eng.run(5)
+            eng.synchronize()
+            met = eng.metrics()
+            for i, (ref, its, _, _) in enumerate(refs):
+                a = eng.state(i).double().cpu().numpy()
+                tol = 1e-9 if precision == "fp64" else 1e-4
+                assert modal_rel_l2(a, ref.a) < tol, (precision, rows, threads, lds, i)
+                if precision == "fp64":
+                    assert met[:, i, 2].astype(int).tolist() == its
+        eng.close()
+
+
+def test_autotuned_engine_runs_the_chosen_variant():
+    from spectral_galerkin_heat_amd.engine.autotune import autotune
+    cases = batch_uneven_cuboids(n=8, seed=3, n_steps=6)
+    ctxs = [make_context(c, lp) for c, lp in cases]
+    res = autotune(ctxs, "mixed", steps=2, reps=1)
+    assert len(res.table) >= 4
+    assert res.ms_per_step == min(r["ms_per_step"] for r in res.table)
+    from spectral_galerkin_heat_amd.engine.batch import BatchEngine
+    eng = BatchEngine(ctxs, precision="mixed", variant=res.variant)
+    v = eng.variant
+    assert (v["rows"], v["threads"], v["lds_bytes"]) == res.variant
+    eng.run(6)
+    eng.synchronize()
+    ref = BatchEngine(ctxs, precision="fp64")
+    ref.run(6)
+    ref.synchronize()
+    for i in range(len(ctxs)):
+        assert modal_rel_l2(eng.state(i).double().cpu().numpy(), ref.state(i).cpu().numpy()) < 1e-4
+    eng.close()
+    ref.close()
+
+
+def test_alone_equals_in_mix_for_a_tuned_variant():
+    cases = batch_uneven_cuboids(n=4, seed=17, n_steps=6)
+    mix = _engine(cases, "mixed", variant=(32, 512, 150 * 1024))
+    mix.run(6)
+    mix.synchronize()
+    alone = _engine(cases[2:3], "mixed", variant=(32, 512, 150 * 1024))
+    alone.run(6)
+    alone.synchronize()
+    assert torch.equal(mix.state(2), alone.state(0))
+    mix.close()
+    alone.close()
