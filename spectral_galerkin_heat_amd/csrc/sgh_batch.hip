@@ -35,10 +35,10 @@ namespace sgh {
 constexpr int kBT = 256;        // threads per block
 constexpr int kRowsMin = 16;    // fewest surface rows per transform item (one MFMA row tile)
 constexpr int kBMaxN = 256;     // widest surface axis
-constexpr int kStatusStride = 4;  // n_it, converged, rel, pad
+constexpr int kStatusStride = 4;  // n_it, converged, rel, AA of the current step
 constexpr int kMetricStride = 5;  // P_laser, T_max, n_it, converged, rel
 constexpr int kSurfArrays = 10;
-constexpr int kStageAhead = 8;   // global loads in flight per thread while staging into LDS
+constexpr int kStageAhead = 4;   // global loads in flight per thread while staging into LDS
 constexpr int kSurfLds = 150 * 1024;  // default LDS bytes per surface workgroup
 constexpr int kLdsMax = 159 * 1024;   // dynamic LDS cap: gfx950's 160 KiB less the static part
 constexpr int kXcds = 8;          // MI355X: workgroup b runs on XCD b mod 8
@@ -267,26 +267,39 @@ __global__ void __launch_bounds__(kBT) k_b_prep(BatchPtrs<F> P) {
   }
   const double t = block_sum(psum, sh);
   if (threadIdx.x == 0) P.aux_slots[2 * (d.row_item0 + it.blk)] = t;
+  if (it.blk == 0) {
+    // AA = Σ aE² of this step (per-column-item partials from the last volume pass),
+    // used by every Picard decision of the step
+    double aa = 0.0;
+    for (int b = threadIdx.x; b < d.ncolblk; b += kBT) aa += P.col_slots[d.col_item0 + b];
+    aa = block_sum(aa, sh);
+    if (threadIdx.x == 0) P.status[kStatusStride * it.c + 3] = aa;
+  }
 }
 
-// Picard decision for cuboid c before iteration k (k = 0: always run).
-// Returns true if the cuboid is still iterating.
+// Picard decision for cuboid c before iteration k (k = 0: always run), evaluated by the
+// 64 lanes of wave 0 together: lane b loads row item b's norm partials and the wave sums
+// them in a fixed butterfly order; AA = Σ aE² is constant over the step and was summed
+// once by k_b_prep into status[3]. Returns true (on every lane) while the cuboid iterates.
 template <typename F>
 __device__ bool picard_active(const BatchPtrs<F>& P, const CuboidDesc& d, int c, int k,
                               bool record) {
+  const int lane = threadIdx.x & (kWave - 1);
   if (P.done_at[c] >= 0) return false;
   bool done = false, conv = false;
   double rel = __builtin_nan("");
   if (k > 0) {
-    double rr = 0.0, gu = 0.0, g2w = 0.0, aa = 0.0;
-    for (int b = 0; b < d.nrowblk; ++b) {
-      const double* sl = P.row_slots + 3 * (d.row_item0 + b);
-      rr += sl[0];
-      gu += sl[1];
-      g2w += sl[2];
+    double rr = 0.0, gu = 0.0, g2w = 0.0;
+    if (lane < d.nrowblk) {
+      const double* sl = P.row_slots + 3 * (d.row_item0 + lane);
+      rr = sl[0];
+      gu = sl[1];
+      g2w = sl[2];
     }
-    for (int b = 0; b < d.ncolblk; ++b) aa += P.col_slots[d.col_item0 + b];
-    aa += 2.0 * gu + g2w;
+    rr = wave_sum(rr);
+    gu = wave_sum(gu);
+    g2w = wave_sum(g2w);
+    const double aa = P.status[kStatusStride * c + 3] + 2.0 * gu + g2w;
     const double rms_diff = sqrt(rr / d.n_elem);
     const double rms_old = sqrt(aa / d.n_elem);
     rel = rms_diff / (rms_old > 1e-9 ? rms_old : 1e-9);
@@ -294,7 +307,7 @@ __device__ bool picard_active(const BatchPtrs<F>& P, const CuboidDesc& d, int c,
     done = conv;
   }
   if (k >= d.max_it) done = true;
-  if (done && record) {
+  if (done && record && lane == 0) {
     P.done_at[c] = k;
     double* st = P.status + kStatusStride * c;
     st[0] = k;
@@ -316,13 +329,12 @@ __device__ bool picard_active(const BatchPtrs<F>& P, const CuboidDesc& d, int c,
 // ---------------------------------------------------------------------------------
 // Copy a rows × cols block (source row stride src_ld, starting at src) into LDS
 // (row stride dst_ld), zero-filling rows >= valid_rows and columns >= valid_cols.
-// Each thread keeps kStageAhead global loads in flight; indices advance incrementally
+// Each thread keeps A global loads in flight; indices advance incrementally
 // (no per-element division).
-template <typename F, int NT>
+template <typename F, int NT, int A>
 __device__ __forceinline__ void stage_block(F* __restrict__ dst, int dst_ld, const F* __restrict__ src,
                                             int src_ld, int rows, int cols, int valid_rows,
                                             int valid_cols) {
-  constexpr int A = kStageAhead;
   const int total = rows * cols;
   const int step_r = NT / cols, step_c = NT - (NT / cols) * cols;
   int r = threadIdx.x / cols, c = threadIdx.x - (threadIdx.x / cols) * cols;
@@ -345,6 +357,42 @@ __device__ __forceinline__ void stage_block(F* __restrict__ dst, int dst_ld, con
   }
 }
 
+// Element offset XORed into the columns of odd rows of an unpadded LDS operand so that
+// rows kk and kk+1 of an MFMA B operand fall in different halves of the 32 banks; 0
+// when the row length already staggers them (or cannot be swizzled within a row).
+template <typename F>
+__device__ __forceinline__ int swz_of(int ld) {
+  constexpr int half = sizeof(F) == 4 ? 16 : 8;  // 16 words = half the banks
+  return ld % (2 * half) == 0 ? half : 0;
+}
+
+// LDS DMA (gfx950 global_load_lds_dwordx4) of `rows` contiguous rows of `ld` elements
+// from global `src` into LDS `dst`, the columns of odd rows XOR-swizzled by `swz`
+// elements: every 16-byte unit of LDS receives the 16-byte unit of its unswizzled
+// position. Each wave copies 1 KiB per instruction (lane l -> dst unit base + l). The
+// caller waits (s_waitcnt + barrier) before reading. Needs 16-byte aligned src and dst,
+// ld·sizeof(F) a multiple of 16 and swz a multiple of 16 / sizeof(F).
+__device__ __forceinline__ void dma16(const void* src, void* lds_base) {
+  __builtin_amdgcn_global_load_lds(src, lds_base, 16, 0, 0);
+}
+
+template <typename F, int NT>
+__device__ __forceinline__ void dma_rows(F* dst, const F* src, int rows, int ld, int swz) {
+  constexpr int E = 16 / (int)sizeof(F);
+  const int units = rows * ld / E;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  for (int u0 = wave * kWave; u0 < units; u0 += NT) {
+    const int u = u0 + lane;
+    if (u < units) {
+      const int q = u * E;
+      const int r = q / ld;
+      const int cs = q - r * ld;
+      const int c = (r & 1) ? (cs ^ swz) : cs;
+      dma16(src + (long long)r * ld + c, dst + (long long)u0 * E);
+    }
+  }
+}
+
 #ifdef SGH_BLOCK_TIMESTAMPS
 __device__ unsigned long long g_block_ts[4 * 8192 * 6];
 #define SGH_TS(slot)                                                                  \
@@ -357,7 +405,7 @@ __device__ unsigned long long g_block_ts[4 * 8192 * 6];
 #endif
 
 template <typename F, int MODE, int RB, int NT>
-__global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud) {
+__global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud, int lds_elems) {
   SGH_TS(0);
   using M = Mfma<F>;
   using V = typename M::V;
@@ -366,6 +414,7 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
   constexpr int kST = NT;                       // threads
   constexpr int NW = NT / kWave;                // waves
   constexpr int kTilesPerWave = (kBMaxN / 16 + NW - 1) / NW;  // 16-column tiles per wave
+  constexpr int kA = kStageAhead;   // loads in flight per lane while staging
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double sh[NT / kWave];
   __shared__ int s_active;
@@ -373,7 +422,7 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
   if (it.c < 0) return;
   const CuboidDesc d = P.desc[it.c];
   const int tid = threadIdx.x;
-  if (tid == 0) {
+  if (tid < kWave) {  // wave 0 decides; the decision needs every lane of the wave
     bool act;
     if (MODE == 0) act = P.step_idx[it.c] < d.n_steps;
     else if (MODE == 1) act = P.done_at[it.c] < 0;
@@ -382,7 +431,7 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
       picard_active<F>(P, d, it.c, k, it.blk == 0);
       act = P.step_idx[it.c] < d.n_steps;
     }
-    s_active = act ? 1 : 0;
+    if (tid == 0) s_active = act ? 1 : 0;
   }
   __syncthreads();
   if (!s_active) return;
@@ -395,7 +444,7 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
   const int li = lane & 15, lk = lane >> 4;
   const int ntile = (nx + 15) >> 4;
   const int ny4 = (ny + 3) & ~3, nx4 = (nx + 3) & ~3;
-  // LDS layout: Ls[kBRB][ldl] | Ys[kBRB][ldy] | Bb[bud]
+  // LDS layout: Ls[kBRB][ldl] | Ys[kBRB][ldy] | Bb[...]
   const int ldl = pad_a<F>(ny4), ldy = pad_a<F>(nx4);
   F* Ls = reinterpret_cast<F*>(smem_raw);
   F* Ys = Ls + kBRB * ldl;
@@ -404,22 +453,45 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
   const F* Rm = P.mats + (MODE == 0 || MODE == 1 ? d.bxT_off : d.bx_off);
   const F* Xsrc = (MODE <= 1 ? P.q_net : P.Xinv) + so;
 
-  // L rows of this item (zero beyond ny and beyond the last row)
-  stage_block<F, kST>(Ls, ldl, Lm + (long long)r0 * ny, ny, kBRB, ny4, rows, ny);
+  // Joint staging: when X (ny rows) and R (nx rows) fit in LDS together, both are
+  // copied by LDS DMA at once, unpadded with odd rows XOR-swizzled, so the launch has a
+  // single staging round. Otherwise X and then R are staged through registers, in chunks
+  // if need be.
+  const int room = lds_elems - kBRB * (ldl + ldy);
+  const bool joint = (nx * (int)sizeof(F)) % 16 == 0 &&
+                     ((reinterpret_cast<uintptr_t>(Xsrc) | reinterpret_cast<uintptr_t>(Rm)) & 15) == 0 &&
+                     (ny4 + nx4) * nx <= room;
+  const int swz = joint ? swz_of<F>(nx) : 0;
+  F* Xs = Bb;
+  F* Rs = Bb + ny4 * nx;
 
-  // stage 1: Y = L[rows, :] · X, X staged in chunks of kr rows (usually all of it)
-  const int ldb1 = pad_b<F>(nx);
-  const int kr = min(ny4, (bud / ldb1) & ~3);
-  if (kr < 4) return;  // the host guarantees bud >= 4·ldb1; never taken
   V acc[kMT][kTilesPerWave];
 #pragma unroll
   for (int m = 0; m < kMT; ++m)
 #pragma unroll
     for (int t = 0; t < kTilesPerWave; ++t) acc[m][t] = V{0, 0, 0, 0};
-  for (int kc0 = 0; kc0 < ny4; kc0 += kr) {
-    const int kn = min(kr, ny4 - kc0);
-    if (kc0 > 0) __syncthreads();
-    stage_block<F, kST>(Bb, ldb1, Xsrc + (long long)kc0 * nx, nx, kn, nx, ny - kc0, nx);
+
+  if (joint) {
+    // L rows into registers first, then the DMA; one wait covers all of it
+    constexpr int kLr = (kBRB * kBMaxN + kST - 1) / kST;
+    F lr[kLr];
+    const int ltot = kBRB * ny4;
+#pragma unroll
+    for (int q = 0; q < kLr; ++q) {
+      const int e = tid + q * kST;
+      const int r = e / ny4, c = e - (e / ny4) * ny4;
+      lr[q] = (e < ltot && r < rows && c < ny) ? Lm[(long long)(r0 + r) * ny + c] : F(0);
+    }
+    dma_rows<F, kST>(Xs, Xsrc, ny, nx, swz);
+    dma_rows<F, kST>(Rs, Rm, nx, nx, swz);
+    for (int e = tid; e < (ny4 - ny) * nx; e += kST) Xs[ny * nx + e] = F(0);
+    for (int e = tid; e < (nx4 - nx) * nx; e += kST) Rs[nx * nx + e] = F(0);
+#pragma unroll
+    for (int q = 0; q < kLr; ++q) {
+      const int e = tid + q * kST;
+      if (e < ltot) Ls[(e / ny4) * ldl + (e - (e / ny4) * ny4)] = lr[q];
+    }
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     SGH_TS(2);
 #pragma unroll
@@ -428,12 +500,41 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
       if (jt >= ntile) break;
       const int col = jt * 16 + li;
       const bool cin = col < nx;
+      const int colx = (lk & 1) ? (col ^ swz) : col;
 #pragma unroll 4
-      for (int kk = 0; kk < kn; kk += 4) {
-        const F bv = cin ? Bb[(kk + lk) * ldb1 + col] : F(0);
+      for (int kk = 0; kk < ny4; kk += 4) {
+        const F bv = cin ? Xs[(kk + lk) * nx + colx] : F(0);
 #pragma unroll
         for (int m = 0; m < kMT; ++m)
-          acc[m][t] = M::mma(Ls[(16 * m + li) * ldl + kc0 + kk + lk], bv, acc[m][t]);
+          acc[m][t] = M::mma(Ls[(16 * m + li) * ldl + kk + lk], bv, acc[m][t]);
+      }
+    }
+  } else {
+    // L rows of this item (zero beyond ny and beyond the last row)
+    stage_block<F, kST, kA>(Ls, ldl, Lm + (long long)r0 * ny, ny, kBRB, ny4, rows, ny);
+    // stage 1: Y = L[rows, :] · X, X staged in chunks of kr rows (usually all of it)
+    const int ldb1 = pad_b<F>(nx);
+    const int kr = min(ny4, (bud / ldb1) & ~3);
+    if (kr < 4) return;  // the host guarantees bud >= 4·ldb1; never taken
+    for (int kc0 = 0; kc0 < ny4; kc0 += kr) {
+      const int kn = min(kr, ny4 - kc0);
+      if (kc0 > 0) __syncthreads();
+      stage_block<F, kST, kA>(Bb, ldb1, Xsrc + (long long)kc0 * nx, nx, kn, nx, ny - kc0, nx);
+      __syncthreads();
+      SGH_TS(2);
+#pragma unroll
+      for (int t = 0; t < kTilesPerWave; ++t) {
+        const int jt = wave + NW * t;
+        if (jt >= ntile) break;
+        const int col = jt * 16 + li;
+        const bool cin = col < nx;
+#pragma unroll 4
+        for (int kk = 0; kk < kn; kk += 4) {
+          const F bv = cin ? Bb[(kk + lk) * ldb1 + col] : F(0);
+#pragma unroll
+          for (int m = 0; m < kMT; ++m)
+            acc[m][t] = M::mma(Ls[(16 * m + li) * ldl + kc0 + kk + lk], bv, acc[m][t]);
+        }
       }
     }
   }
@@ -459,7 +560,7 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
   SGH_TS(3);
   // epilogue operands of this lane's outputs, loaded before stage 2 waits
   constexpr int NE = 4 * kMT * kTilesPerWave;
-  F e0[NE], e1[NE], e2[NE];
+  F e0[NE], e1[NE];
 #pragma unroll
   for (int t = 0; t < kTilesPerWave; ++t) {
     const int jt = wave + NW * t;
@@ -470,7 +571,7 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
       for (int v = 0; v < 4; ++v) {
         const int j = (t * kMT + m) * 4 + v;
         const int row = 16 * m + M::row(lane, v);
-        e0[j] = e1[j] = e2[j] = F(0);
+        e0[j] = e1[j] = F(0);
         if (jt < ntile && col < nx && row < rows) {
           const int s = so + (r0 + row) * nx + col;
           if (MODE == 0) { e0[j] = P.ZE[s]; e1[j] = P.Ztt[s]; }
@@ -479,74 +580,103 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
       }
   }
 
-  // stage 2: out = Y · R, R staged in column chunks of cw (usually all of it)
-  int cw = min(ntile * 16, max(16, ((bud / nx4) / 16) * 16));
-  while (cw > 16 && nx4 * pad_b<F>(cw) > bud) cw -= 16;
-  const int ldb2 = pad_b<F>(cw);
   double v0 = 0.0, v1 = 0.0, v2 = 0.0;
   if (MODE == 3) v0 = -1.0 / 0.0;
   const double fwd_scale = d.dx * d.dy * d.inv_sqrt_LxLy;
-  for (int c0 = 0; c0 < nx; c0 += cw) {
-    const int cn = min(cw, nx - c0);
-    __syncthreads();
-    stage_block<F, kST>(Bb, ldb2, Rm + c0, nx, nx4, cn, nx, cn);
-    __syncthreads();
+  // the role-specific epilogue of one 16-column tile (rows 16m + row(lane, v))
+  auto finish = [&](int t, int col, const V* a_) {
+#pragma unroll
+    for (int m = 0; m < kMT; ++m)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = 16 * m + M::row(lane, v);
+        if (row >= rows) continue;
+        const int s = so + (r0 + row) * nx + col;
+        const int j = (t * kMT + m) * 4 + v;
+        const F val = a_[m][v];
+        if (MODE == 0) {
+          const F g = (F)((double)val * fwd_scale);
+          P.G[s] = g;
+          P.Xinv[s] = e0[j] + g * e1[j];
+        } else if (MODE == 1) {
+          const F Sr = (F)((double)val * fwd_scale);
+          const F g = P.G[s];
+          const F dd = Sr - g;
+          const double w = (double)P.Wtt[s];
+          v0 += (double)dd * (double)dd * w;
+          v1 += (double)g * (double)P.U[s];
+          v2 += (double)g * (double)g * w;
+          const F gn = g + (F)d.omega * dd;
+          P.G[s] = gn;
+          P.Xinv[s] = P.ZE[s] + gn * P.Ztt[s];
+        } else {
+          const F T = (F)((double)val * d.inv_sqrt_LxLy);
+          if (MODE == 2) {
+            F qe = F(0);
+            if (T >= (F)d.T_liq)
+              qe = (F)d.f1 * ((F(1) / sqrt(T)) * exp((F)d.f2 * (F(1) - (F)d.T_boil / T)));
+            P.q_evap[s] = qe;
+            P.q_net[s] = e0[j] - qe;
+          } else {
+            v0 = (double)T > v0 ? (double)T : v0;
+          }
+        }
+      }
+  };
+
+  // stage 2: out = Y · R
+  if (joint) {
+    __syncthreads();  // Y complete; R has been in LDS since the joint staging
     SGH_TS(4);
 #pragma unroll
     for (int t = 0; t < kTilesPerWave; ++t) {
       const int jt = wave + NW * t;
       if (jt >= ntile) break;
       const int col = jt * 16 + li;
-      if (col - li < c0 || col - li >= c0 + cn) continue;  // tile not in this chunk
       const bool cin = col < nx;
+      const int colx = (lk & 1) ? (col ^ swz) : col;
       V a_[kMT];
 #pragma unroll
       for (int m = 0; m < kMT; ++m) a_[m] = V{0, 0, 0, 0};
 #pragma unroll 4
       for (int kk = 0; kk < nx4; kk += 4) {
-        const F bv = cin ? Bb[(kk + lk) * ldb2 + (col - c0)] : F(0);
+        const F bv = cin ? Rs[(kk + lk) * nx + colx] : F(0);
 #pragma unroll
         for (int m = 0; m < kMT; ++m)
           a_[m] = M::mma(Ys[(16 * m + li) * ldy + kk + lk], bv, a_[m]);
       }
-      if (!cin) continue;
+      if (cin) finish(t, col, a_);
+    }
+  } else {
+    // R staged in column chunks of cw (usually all of it)
+    int cw = min(ntile * 16, max(16, ((bud / nx4) / 16) * 16));
+    while (cw > 16 && nx4 * pad_b<F>(cw) > bud) cw -= 16;
+    const int ldb2 = pad_b<F>(cw);
+    for (int c0 = 0; c0 < nx; c0 += cw) {
+      const int cn = min(cw, nx - c0);
+      __syncthreads();
+      stage_block<F, kST, kA>(Bb, ldb2, Rm + c0, nx, nx4, cn, nx, cn);
+      __syncthreads();
+      SGH_TS(4);
 #pragma unroll
-      for (int m = 0; m < kMT; ++m)
+      for (int t = 0; t < kTilesPerWave; ++t) {
+        const int jt = wave + NW * t;
+        if (jt >= ntile) break;
+        const int col = jt * 16 + li;
+        if (col - li < c0 || col - li >= c0 + cn) continue;  // tile not in this chunk
+        const bool cin = col < nx;
+        V a_[kMT];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int row = 16 * m + M::row(lane, v);
-          if (row >= rows) continue;
-          const int s = so + (r0 + row) * nx + col;
-          const int j = (t * kMT + m) * 4 + v;
-          const F val = a_[m][v];
-          if (MODE == 0) {
-            const F g = (F)((double)val * fwd_scale);
-            P.G[s] = g;
-            P.Xinv[s] = e0[j] + g * e1[j];
-          } else if (MODE == 1) {
-            const F Sr = (F)((double)val * fwd_scale);
-            const F g = P.G[s];
-            const F dd = Sr - g;
-            const double w = (double)P.Wtt[s];
-            v0 += (double)dd * (double)dd * w;
-            v1 += (double)g * (double)P.U[s];
-            v2 += (double)g * (double)g * w;
-            const F gn = g + (F)d.omega * dd;
-            P.G[s] = gn;
-            P.Xinv[s] = P.ZE[s] + gn * P.Ztt[s];
-          } else {
-            const F T = (F)((double)val * d.inv_sqrt_LxLy);
-            if (MODE == 2) {
-              F qe = F(0);
-              if (T >= (F)d.T_liq)
-                qe = (F)d.f1 * ((F(1) / sqrt(T)) * exp((F)d.f2 * (F(1) - (F)d.T_boil / T)));
-              P.q_evap[s] = qe;
-              P.q_net[s] = e0[j] - qe;
-            } else {
-              v0 = (double)T > v0 ? (double)T : v0;
-            }
-          }
+        for (int m = 0; m < kMT; ++m) a_[m] = V{0, 0, 0, 0};
+#pragma unroll 4
+        for (int kk = 0; kk < nx4; kk += 4) {
+          const F bv = cin ? Bb[(kk + lk) * ldb2 + (col - c0)] : F(0);
+#pragma unroll
+          for (int m = 0; m < kMT; ++m)
+            a_[m] = M::mma(Ys[(16 * m + li) * ldy + kk + lk], bv, a_[m]);
         }
+        if (cin) finish(t, col, a_);
+      }
     }
   }
   SGH_TS(5);
@@ -601,6 +731,8 @@ __global__ void __launch_bounds__(kBT) k_b_metrics(BatchPtrs<F> P) {
 constexpr int kMaxRowBlk = kBMaxN / kRowsMin;      // row blocks per cuboid (ny <= 256)
 constexpr int kMaxColBlk = kBMaxN * kBMaxN / kBT;  // column blocks per cuboid (S <= 65536)
 constexpr int kMaxSizes = 32;                      // distinct mode counts in the matrix table
+
+static inline long long rup64(long long v) { return (v + 63) / 64 * 64; }
 
 // First-fit allocator over [0, capacity) with coalescing free list.
 struct Arena {
@@ -735,7 +867,7 @@ constexpr SurfVariant kSurfVariants[] = {{16, 256}, {16, 512}, {16, 1024}, {32, 
 constexpr int kNumSurfVariants = sizeof(kSurfVariants) / sizeof(kSurfVariants[0]);
 
 template <typename F>
-using SurfFn = void (*)(BatchPtrs<F>, int, int);
+using SurfFn = void (*)(BatchPtrs<F>, int, int, int);
 
 template <typename F, int RB, int NT>
 static void surf_fns(SurfFn<F>* f) {
@@ -759,7 +891,8 @@ template <typename F>
 static hipError_t launch_surface(const SurfFn<F>* f, int mode, unsigned nrow, const Plan* p,
                                  BatchPtrs<F>& P, int k, hipStream_t st) {
   int bud = p->bud;
-  void* args[] = {&P, &k, &bud};
+  int lds_elems = (int)(p->shm / p->elem);
+  void* args[] = {&P, &k, &bud, &lds_elems};
   return hipLaunchKernel(reinterpret_cast<const void*>(f[mode]), dim3(nrow), dim3(p->threads),
                          args, p->shm, st);
 }
@@ -926,18 +1059,19 @@ static hipError_t dmalloc(T** ptr, size_t bytes) {
 static long long mats_offset(Plan* p, int m, hipStream_t st, int* rc) {
   for (size_t i = 0; i < p->sizes.size(); ++i)
     if (p->sizes[i] == m) return p->size_off[i];
-  const long long need = 2LL * m * m;
+  const long long mm = rup64((long long)m * m);  // B̂ and B̂ᵀ start 256-byte aligned
+  const long long need = 2LL * mm;
   if ((int)p->sizes.size() >= kMaxSizes || p->mats_used + need > p->mats_cap) {
     *rc = SGH_ERR_NOMEM;
     set_error("sgh_batch: matrix table full (%d distinct mode counts)", (int)p->sizes.size());
     return -1;
   }
   std::vector<double> B = unit_basis(m);
-  std::vector<unsigned char> buf((size_t)need * p->elem);
+  std::vector<unsigned char> buf((size_t)need * p->elem, 0);
   for (int r = 0; r < m; ++r)
     for (int c = 0; c < m; ++c) {
       const double v = B[(size_t)r * m + c], vt = B[(size_t)c * m + r];
-      const size_t i0 = (size_t)r * m + c, i1 = (size_t)m * m + i0;
+      const size_t i0 = (size_t)r * m + c, i1 = (size_t)mm + i0;
       if (p->elem == 8) {
         reinterpret_cast<double*>(buf.data())[i0] = v;
         reinterpret_cast<double*>(buf.data())[i1] = vt;
@@ -1006,9 +1140,9 @@ static int add_cuboid(Plan* p, const sgh_cuboid_spec& sp, int c, hipStream_t st)
   d.a_off = al.a_off;
   d.s_off = (int)al.s_off;
   d.bx_off = (int)bx;
-  d.bxT_off = (int)bx + sp.nx * sp.nx;
+  d.bxT_off = (int)(bx + rup64((long long)sp.nx * sp.nx));
   d.by_off = (int)by;
-  d.byT_off = (int)by + sp.ny * sp.ny;
+  d.byT_off = (int)(by + rup64((long long)sp.ny * sp.ny));
   d.t1_off = c * p->t1_stride;
   const double alpha = sp.k / (sp.rho * sp.cp);
   std::vector<double> t1;
@@ -1126,7 +1260,7 @@ int sgh_batch_pool_create(int dtype, int max_cuboids, long long state_capacity,
   p->arena_s.init(surface_capacity);
   p->arena_l.init(laser_capacity);
   p->total_N = state_capacity;
-  p->total_S = surface_capacity;
+  p->total_S = rup64(surface_capacity);  // each packed surface array starts 256-byte aligned
   p->t1_stride = 2 * (kBMaxN + kBMaxN + max_nz);
   p->mats_cap = 2LL * 16 * kBMaxN * kBMaxN;
   p->bud = surface_budget(dtype, p->rows, kSurfLds);
@@ -1143,7 +1277,7 @@ int sgh_batch_pool_create(int dtype, int max_cuboids, long long state_capacity,
   const size_t E = p->elem;
   chk(dtype == SGH_F64 ? allow_large_lds<double>(kLdsMax) : allow_large_lds<float>(kLdsMax));
   chk(dmalloc(&p->a, state_capacity * E));
-  chk(dmalloc(&p->surf, kSurfArrays * surface_capacity * E));
+  chk(dmalloc(&p->surf, kSurfArrays * p->total_S * E));
   chk(dmalloc(&p->mats, p->mats_cap * E));
   chk(dmalloc(&p->t1, (size_t)max_cuboids * p->t1_stride * sizeof(double)));
   chk(dmalloc(&p->laser, (size_t)laser_capacity * 6 * sizeof(double)));
