@@ -285,21 +285,26 @@ template <typename F>
 __device__ bool picard_active(const BatchPtrs<F>& P, const CuboidDesc& d, int c, int k,
                               bool record) {
   const int lane = threadIdx.x & (kWave - 1);
-  if (P.done_at[c] >= 0) return false;
-  bool done = false, conv = false;
-  double rel = __builtin_nan("");
+  // all loads first (independent of each other), so they are in flight together
+  const int done_at = P.done_at[c];
+  double rr = 0.0, gu = 0.0, g2w = 0.0, aa0 = 0.0;
   if (k > 0) {
-    double rr = 0.0, gu = 0.0, g2w = 0.0;
     if (lane < d.nrowblk) {
       const double* sl = P.row_slots + 3 * (d.row_item0 + lane);
       rr = sl[0];
       gu = sl[1];
       g2w = sl[2];
     }
+    aa0 = P.status[kStatusStride * c + 3];
+  }
+  if (done_at >= 0) return false;
+  bool done = false, conv = false;
+  double rel = __builtin_nan("");
+  if (k > 0) {
     rr = wave_sum(rr);
     gu = wave_sum(gu);
     g2w = wave_sum(g2w);
-    const double aa = P.status[kStatusStride * c + 3] + 2.0 * gu + g2w;
+    const double aa = aa0 + 2.0 * gu + g2w;
     const double rms_diff = sqrt(rr / d.n_elem);
     const double rms_old = sqrt(aa / d.n_elem);
     rel = rms_diff / (rms_old > 1e-9 ? rms_old : 1e-9);
@@ -422,20 +427,21 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
   if (it.c < 0) return;
   const CuboidDesc d = P.desc[it.c];
   const int tid = threadIdx.x;
-  if (tid < kWave) {  // wave 0 decides; the decision needs every lane of the wave
-    bool act;
-    if (MODE == 0) act = P.step_idx[it.c] < d.n_steps;
-    else if (MODE == 1) act = P.done_at[it.c] < 0;
-    else if (MODE == 2) act = picard_active<F>(P, d, it.c, k, it.blk == 0);
-    else {
-      picard_active<F>(P, d, it.c, k, it.blk == 0);
-      act = P.step_idx[it.c] < d.n_steps;
+  // wave 0 decides whether the cuboid still iterates (the decision needs every lane of
+  // the wave); the other waves' loads are already in flight meanwhile
+  auto decide = [&]() {
+    if (tid < kWave) {
+      bool act;
+      if (MODE == 0) act = P.step_idx[it.c] < d.n_steps;
+      else if (MODE == 1) act = P.done_at[it.c] < 0;
+      else if (MODE == 2) act = picard_active<F>(P, d, it.c, k, it.blk == 0);
+      else {
+        picard_active<F>(P, d, it.c, k, it.blk == 0);
+        act = P.step_idx[it.c] < d.n_steps;
+      }
+      if (tid == 0) s_active = act ? 1 : 0;
     }
-    if (tid == 0) s_active = act ? 1 : 0;
-  }
-  __syncthreads();
-  if (!s_active) return;
-  SGH_TS(1);
+  };
 
   const int nx = d.nx, ny = d.ny, so = d.s_off;
   const int r0 = it.blk * kBRB;
@@ -465,14 +471,55 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
   F* Xs = Bb;
   F* Rs = Bb + ny4 * nx;
 
+  // epilogue operands of this lane's outputs (MODE 0: ZE, Ztt; MODE 2: q_las; MODE 1,
+  // when registers allow: G, Wtt, U, ZE, Ztt), loaded up front with the staging
+  constexpr int NE = 4 * kMT * kTilesPerWave;
+  constexpr bool kPre1 = MODE == 1 && NE <= 8;
+  constexpr int NP = kPre1 ? NE : 1;
+  F e0[NE], e1[NE], e2[NP], e3[NP], e4[NP];
+#pragma unroll
+  for (int t = 0; t < kTilesPerWave; ++t) {
+    const int jt = wave + NW * t;
+    const int col = jt * 16 + li;
+#pragma unroll
+    for (int m = 0; m < kMT; ++m)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int j = (t * kMT + m) * 4 + v;
+        const int row = 16 * m + M::row(lane, v);
+        e0[j] = e1[j] = F(0);
+        if (kPre1) e2[j % NP] = e3[j % NP] = e4[j % NP] = F(0);
+        if (jt < ntile && col < nx && row < rows) {
+          const int s = so + (r0 + row) * nx + col;
+          if (MODE == 0) { e0[j] = P.ZE[s]; e1[j] = P.Ztt[s]; }
+          if (MODE == 2) { e0[j] = P.q_las[s]; }
+          if (kPre1) {
+            e0[j] = P.G[s];
+            e1[j] = P.Wtt[s];
+            e2[j % NP] = P.U[s];
+            e3[j % NP] = P.ZE[s];
+            e4[j % NP] = P.Ztt[s];
+          }
+        }
+      }
+  }
+
   V acc[kMT][kTilesPerWave];
 #pragma unroll
   for (int m = 0; m < kMT; ++m)
 #pragma unroll
     for (int t = 0; t < kTilesPerWave; ++t) acc[m][t] = V{0, 0, 0, 0};
 
+  if (!joint) {
+    decide();
+    __syncthreads();
+    if (!s_active) return;
+  }
+  SGH_TS(1);
+
   if (joint) {
-    // L rows into registers first, then the DMA; one wait covers all of it
+    // L rows into registers first, then the DMA, then the decision; one wait covers it
+    // all, and an inactive cuboid leaves only after its DMA has landed
     constexpr int kLr = (kBRB * kBMaxN + kST - 1) / kST;
     F lr[kLr];
     const int ltot = kBRB * ny4;
@@ -484,6 +531,7 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
     }
     dma_rows<F, kST>(Xs, Xsrc, ny, nx, swz);
     dma_rows<F, kST>(Rs, Rm, nx, nx, swz);
+    decide();
     for (int e = tid; e < (ny4 - ny) * nx; e += kST) Xs[ny * nx + e] = F(0);
     for (int e = tid; e < (nx4 - nx) * nx; e += kST) Rs[nx * nx + e] = F(0);
 #pragma unroll
@@ -493,6 +541,7 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
     }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
+    if (!s_active) return;
     SGH_TS(2);
 #pragma unroll
     for (int t = 0; t < kTilesPerWave; ++t) {
@@ -558,27 +607,6 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
     }
 
   SGH_TS(3);
-  // epilogue operands of this lane's outputs, loaded before stage 2 waits
-  constexpr int NE = 4 * kMT * kTilesPerWave;
-  F e0[NE], e1[NE];
-#pragma unroll
-  for (int t = 0; t < kTilesPerWave; ++t) {
-    const int jt = wave + NW * t;
-    const int col = jt * 16 + li;
-#pragma unroll
-    for (int m = 0; m < kMT; ++m)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int j = (t * kMT + m) * 4 + v;
-        const int row = 16 * m + M::row(lane, v);
-        e0[j] = e1[j] = F(0);
-        if (jt < ntile && col < nx && row < rows) {
-          const int s = so + (r0 + row) * nx + col;
-          if (MODE == 0) { e0[j] = P.ZE[s]; e1[j] = P.Ztt[s]; }
-          if (MODE == 2) { e0[j] = P.q_las[s]; }
-        }
-      }
-  }
 
   double v0 = 0.0, v1 = 0.0, v2 = 0.0;
   if (MODE == 3) v0 = -1.0 / 0.0;
@@ -600,15 +628,15 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
           P.Xinv[s] = e0[j] + g * e1[j];
         } else if (MODE == 1) {
           const F Sr = (F)((double)val * fwd_scale);
-          const F g = P.G[s];
+          const F g = kPre1 ? e0[j] : P.G[s];
           const F dd = Sr - g;
-          const double w = (double)P.Wtt[s];
+          const double w = (double)(kPre1 ? e1[j] : P.Wtt[s]);
           v0 += (double)dd * (double)dd * w;
-          v1 += (double)g * (double)P.U[s];
+          v1 += (double)g * (double)(kPre1 ? e2[j % NP] : P.U[s]);
           v2 += (double)g * (double)g * w;
           const F gn = g + (F)d.omega * dd;
           P.G[s] = gn;
-          P.Xinv[s] = P.ZE[s] + gn * P.Ztt[s];
+          P.Xinv[s] = (kPre1 ? e3[j % NP] : P.ZE[s]) + gn * (kPre1 ? e4[j % NP] : P.Ztt[s]);
         } else {
           const F T = (F)((double)val * d.inv_sqrt_LxLy);
           if (MODE == 2) {
