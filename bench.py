@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (a known-size "
+                         "kernel that calibrates HBM byte counters under a profiler)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default surface-transform variant instead of autotuning")
     return ap.parse_args()
@@ -91,6 +94,14 @@ def main():
     from spectral_galerkin_heat_amd.workloads import batch_uneven_cuboids, make_context
 
     assert torch.cuda.is_available(), "bench.py needs the MI355X"
+    if args.copy_probe_mib > 0:
+        n = args.copy_probe_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {args.copy_probe_mib} MiB read + {args.copy_probe_mib} MiB written")
+        del src, dst
     W, K = args.warmup, args.steps
     instr = min(K, 40)
     total_steps = W + K + instr

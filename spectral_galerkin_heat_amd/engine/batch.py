@@ -311,25 +311,3 @@ class BatchPool(BatchEngine):
 
 class PoolFull(RuntimeError):
     """The pool has no free slot or no contiguous room for the request."""
-
-
-def smoke_check():
-    """Three uneven cuboids, FP64 engine vs the oracle for 3 steps (rel-L2 ≤ 1e-9)."""
-    from oracle.sgh_oracle import OracleSolver
-    from ..workloads import batch_uneven_cuboids, make_context
-    cases = batch_uneven_cuboids(n=3, seed=11, n_steps=3)
-    ctxs = [make_context(c, lp) for c, lp in cases]
-    eng = BatchEngine(ctxs, precision="fp64")
-    eng.run(3)
-    eng.synchronize()
-    worst = 0.0
-    for i, (c, lp) in enumerate(cases):
-        ref = OracleSolver(make_context(c, lp)).initialize()
-        for n in range(3):
-            ref.step(n * ref.ctx.num.dt, ref.ctx.num.dt)
-        a = eng.state(i).double().cpu().numpy()
-        worst = max(worst, float(np.linalg.norm(a - ref.a) / np.linalg.norm(ref.a)))
-    eng.close()
-    print(f"smoke: batched FP64 engine, 3 cuboids x 3 steps, worst rel-L2 vs oracle = {worst:.3e}",
-          flush=True)
-    assert worst < 1e-9, worst
