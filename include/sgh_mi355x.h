@@ -227,7 +227,7 @@ int sgh_batch_info(void* plan, long long* out6);
 /* ---- autotuning knobs of the batched surface transform (no reference counterpart;
  * the reference has no batched engine).  A variant is (rows per work item, threads per
  * workgroup) plus the dynamic LDS bytes per workgroup (4 KiB to 159 KiB); the default is
- * (16, 512, 150 KiB).  sgh_batch_variants writes up to `capacity` (rows, threads) pairs
+ * (16, 1024, 150 KiB).  sgh_batch_variants writes up to `capacity` (rows, threads) pairs
  * and returns how many exist.  sgh_batch_set_variant synchronises `stream`; the work
  * lists and the captured step graph are rebuilt by the next sgh_batch_run.  Results
  * are deterministic for a given variant; different variants sum per-item partials in

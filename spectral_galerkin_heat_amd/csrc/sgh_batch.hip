@@ -817,7 +817,7 @@ struct Plan {
   size_t shm = 0;
   int bud = 0;
   int rows = 16;       // surface rows per transform item
-  int threads = 512;   // threads per surface workgroup
+  int threads = 1024;  // threads per surface workgroup (the autotuner's usual pick)
   long long total_N = 0, total_S = 0;
   // device allocations
   void* a = nullptr;
