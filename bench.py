@@ -48,6 +48,12 @@ def parse():
     ap.add_argument("--cpu-cuboids", type=int, default=64)
     ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-sweep", default="10,40,160",
+                    help="comma-separated arrival rates (Hz) for the latency/throughput sweep; "
+                         "empty to skip")
+    ap.add_argument("--serve-tight-mib", type=int, default=48,
+                    help="pool size of the last sweep point, small enough that admission "
+                         "by memory footprint has to queue requests")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -258,31 +264,53 @@ def main():
         log(f"cpu oracle: {t_cpu:.1f} s -> {cpu['value']:.3e} mode-steps/s on {cores} cores")
 
     serving = None
+    sweep = None
     if not args.no_serve:
         from spectral_galerkin_heat_amd.engine.serving import Request, ScanServer, poisson_arrivals
         from spectral_galerkin_heat_amd.workloads import serving_request
         rng = np.random.default_rng(1)
-        arr = poisson_arrivals(args.serve_rate, args.serve_requests, rng)
-        reqs = []
-        for i in range(args.serve_requests):
-            cfg, lp = serving_request(rng)
-            ctx = make_context(cfg, lp)
-            reqs.append(Request(rid=i, ctx=ctx, n_steps=int(ctx.num.n_steps), arrival=float(arr[i])))
-        srv = ScanServer(precision="mixed", max_cuboids=128, hbm_budget_bytes=8 << 30,
-                         max_steps=2048, steps_per_tick=8, check_every=20)
-        rep = srv.serve(reqs)
-        checks = srv.fp64_checks(rep)
-        sm = rep.summary()
-        srv.close()
-        big = sum(1 for r in reqs if r.modes >= 256 * 128 * 64)
-        serving = {"arrival_rate_hz": args.serve_rate, "requests": sm["requests"],
-                   "large_256x128x64_requests": big,
+        shapes = [serving_request(rng) for _ in range(args.serve_requests)]
+        big = sum(1 for cfg, _ in shapes if np.prod(cfg["domain"]["mesh"]) >= 256 * 128 * 64)
+
+        def serve(rate, budget, checks):
+            arr = poisson_arrivals(rate, len(shapes), np.random.default_rng(2))
+            reqs = []
+            for i, (cfg, lp) in enumerate(shapes):
+                ctx = make_context(cfg, lp)
+                reqs.append(Request(rid=i, ctx=ctx, n_steps=int(ctx.num.n_steps),
+                                    arrival=float(arr[i])))
+            srv = ScanServer(precision="mixed", max_cuboids=128, hbm_budget_bytes=budget,
+                             max_steps=2048, steps_per_tick=8, check_every=20)
+            rep = srv.serve(reqs)
+            fp64 = srv.fp64_checks(rep) if checks else []
+            srv.close()
+            sm = rep.summary()
+            out = {"arrival_rate_hz": rate, "requests": sm["requests"],
+                   "hbm_budget_mib": budget >> 20,
                    "p50_latency_s": round(sm["p50_latency_s"], 4),
                    "p99_latency_s": round(sm["p99_latency_s"], 4),
                    "aggregate_mode_steps_per_s": round(sm["aggregate_mode_steps_per_s"], 1),
-                   "wall_s": round(sm["wall_s"], 3),
-                   "fp64_recheck_rel_l2": [float(f"{c:.3e}") for c in checks]}
+                   "queue_wait_p50_s": round(sm["queue_wait_p50_s"], 4),
+                   "queue_wait_max_s": round(sm["queue_wait_max_s"], 4),
+                   "peak_active": sm["peak_active"],
+                   "peak_pool_mib": round(sm["peak_pool_bytes"] / 2**20, 1),
+                   "wall_s": round(sm["wall_s"], 3)}
+            if checks:
+                out["fp64_recheck_rel_l2"] = [float(f"{c:.3e}") for c in fp64]
+            return out
+
+        serving = serve(args.serve_rate, 8 << 30, True)
+        serving["large_256x128x64_requests"] = big
         log(f"serving: {serving}")
+        if args.serve_sweep:
+            sweep = []
+            rates = [float(r) for r in args.serve_sweep.split(",")]
+            for rate in rates:
+                sweep.append(serve(rate, 8 << 30, False))
+                log(f"sweep: {sweep[-1]}")
+            # the same load with a small pool: admission by footprint saturates and queues
+            sweep.append(serve(rates[-1], args.serve_tight_mib << 20, False))
+            log(f"sweep (tight pool): {sweep[-1]}")
 
     out = {
         "metric": METRIC,
@@ -317,6 +345,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "serving": serving,
+        "serving_sweep": sweep,
     }
     print(json.dumps(out), flush=True)
 

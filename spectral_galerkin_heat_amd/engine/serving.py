@@ -67,6 +67,8 @@ class ServeReport:
     wall_s: float = 0.0
     mode_steps: int = 0
     device_steps: int = 0
+    peak_active: int = 0          # most requests resident in the pool at once
+    peak_bytes: int = 0           # most pool bytes (state, surface fields, laser rows) in use
 
     @property
     def latencies(self):
@@ -74,7 +76,12 @@ class ServeReport:
 
     def summary(self):
         lat = self.latencies
+        wait = np.array([r.admitted_at - r.arrival for r in self.requests])
         return {
+            "queue_wait_p50_s": float(np.percentile(wait, 50)) if len(wait) else None,
+            "queue_wait_max_s": float(wait.max()) if len(wait) else None,
+            "peak_active": self.peak_active,
+            "peak_pool_bytes": self.peak_bytes,
             "requests": len(self.requests),
             "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,
             "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None,
@@ -129,6 +136,7 @@ class ScanServer:
         t0 = clock()
         finish_heap: list = []        # (device step at which it completes, rid)
         by_rid = {r.rid: r for r in queue}
+        peak_active = peak_bytes = 0
         while pending or waiting or active:
             now = clock() - t0
             while pending and pending[0].arrival <= now:
@@ -141,6 +149,11 @@ class ScanServer:
                 active[r.slot] = r
                 heapq.heappush(finish_heap, (r.start_step + r.n_steps, r.rid))
                 admitted.append(r)
+            peak_active = max(peak_active, len(active))
+            peak_bytes = max(peak_bytes, sum(r.footprint for r in active.values()))
+            if not active and waiting:
+                raise RuntimeError(f"request {waiting[0].rid} does not fit an empty pool "
+                                   "(raise hbm_budget_bytes or max_nz)")
             if not active:
                 if pending:
                     sleep(max(0.0, pending[0].arrival - (clock() - t0)))
@@ -167,7 +180,8 @@ class ScanServer:
         wall = clock() - t0
         rep = ServeReport(requests=sorted(done, key=lambda r: r.rid), wall_s=wall,
                           mode_steps=sum(r.modes * r.n_steps for r in done),
-                          device_steps=self.device_steps)
+                          device_steps=self.device_steps, peak_active=peak_active,
+                          peak_bytes=peak_bytes)
         return rep
 
     def fp64_checks(self, report: ServeReport):

@@ -188,3 +188,21 @@ def test_melt_pool_extent_counts_cells_from_the_top():
     T[6, 4, 10] = 1723.0          # on the threshold counts as melted
     assert melt_pool_extent(T, (1.0, 2.0, 3.0)) == (9 * 1.0, 3 * 2.0, 3 * 3.0)
     assert melt_pool_extent(torch.full((4, 4, 4), 300.0), (1, 1, 1)) == (0.0, 0.0, 0.0)
+
+
+def test_scan_server_rejects_a_request_that_never_fits():
+    """A request larger than the whole pool must raise instead of waiting forever (the
+    pool is a stand-in here: admission is decided on the host)."""
+    from spectral_galerkin_heat_amd.engine.serving import Request, ScanServer
+    from spectral_galerkin_heat_amd.engine.batch import PoolFull
+
+    class NoRoomPool:
+        def add(self, ctx, n_steps):
+            raise PoolFull("no room")
+
+    srv = ScanServer.__new__(ScanServer)
+    srv.pool, srv.precision, srv.max_steps = NoRoomPool(), "mixed", 64
+    srv.steps_per_tick, srv.check_every, srv.keep_results, srv.device_steps = 8, 20, False, 0
+    req = Request(rid=7, ctx=None, n_steps=4, arrival=0.0)
+    with pytest.raises(RuntimeError, match="request 7 does not fit"):
+        srv.serve([req], clock=lambda: 0.0, sleep=lambda s: None)
