@@ -12,6 +12,10 @@ FP64 GPU path (SpectralSolver on the mi355x backend, cuboid by cuboid, one resid
 per Picard iteration), the FP64 batched engine, the relative L2 of the mixed result against
 the FP64 batched result (itself pinned to the FP64 oracle at 1e-9 by the parity tests),
 a roofline entry for the dominant kernel and the FP64 NumPy oracle timed on host cores.
+Before timing, the autotuner picks each engine's surface-transform launch shape. After it,
+a continuous-batching request loop serves Poisson arrivals of scan requests (p50/p99
+completion latency, aggregate mode-steps/s, FP64 re-solves of every 20th request) and a
+sweep of arrival rates ends with a small pool in which admission by footprint queues.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -94,8 +98,9 @@ def volume_bytes_per_launch(ctxs, word):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (see DESIGN.md, "
-                          "'More than one GPU')"}))
+        print(json.dumps({"error": "single-GPU engine: --gpus must be 1. Independent "
+                          "simulations shard by request (one pool per GPU, no collectives); "
+                          "multi-GPU runs are not part of this package"}))
         sys.exit(2)
     import logging
 
