@@ -371,18 +371,19 @@ __device__ __forceinline__ int swz_of(int ld) {
   return ld % (2 * half) == 0 ? half : 0;
 }
 
-// LDS DMA (gfx950 global_load_lds_dwordx4) of `rows` contiguous rows of `ld` elements
-// from global `src` into LDS `dst`, the columns of odd rows XOR-swizzled by `swz`
-// elements: every 16-byte unit of LDS receives the 16-byte unit of its unswizzled
-// position. Each wave copies 1 KiB per instruction (lane l -> dst unit base + l). The
-// caller waits (s_waitcnt + barrier) before reading. Needs 16-byte aligned src and dst,
-// ld·sizeof(F) a multiple of 16 and swz a multiple of 16 / sizeof(F).
+// LDS DMA (gfx950 global_load_lds_dwordx4) of `rows` rows of `ld` elements from global
+// `src` (row stride `src_ld`) into LDS `dst` packed as [rows][ld], the columns of odd rows
+// XOR-swizzled by `swz` elements: every 16-byte unit of LDS receives the 16-byte unit of
+// its unswizzled position. Each wave copies 1 KiB per instruction (lane l -> dst unit
+// base + l). The caller waits (s_waitcnt + barrier) before reading. Needs 16-byte aligned
+// src rows and dst, ld·sizeof(F) a multiple of 16 and swz a multiple of 16 / sizeof(F).
 __device__ __forceinline__ void dma16(const void* src, void* lds_base) {
   __builtin_amdgcn_global_load_lds(src, lds_base, 16, 0, 0);
 }
 
 template <typename F, int NT>
-__device__ __forceinline__ void dma_rows(F* dst, const F* src, int rows, int ld, int swz) {
+__device__ __forceinline__ void dma_rows(F* dst, const F* src, int rows, int ld, int src_ld,
+                                         int swz) {
   constexpr int E = 16 / (int)sizeof(F);
   const int units = rows * ld / E;
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
@@ -393,7 +394,7 @@ __device__ __forceinline__ void dma_rows(F* dst, const F* src, int rows, int ld,
       const int r = q / ld;
       const int cs = q - r * ld;
       const int c = (r & 1) ? (cs ^ swz) : cs;
-      dma16(src + (long long)r * ld + c, dst + (long long)u0 * E);
+      dma16(src + (long long)r * src_ld + c, dst + (long long)u0 * E);
     }
   }
 }
@@ -459,14 +460,21 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
   const F* Rm = P.mats + (MODE == 0 || MODE == 1 ? d.bxT_off : d.bx_off);
   const F* Xsrc = (MODE <= 1 ? P.q_net : P.Xinv) + so;
 
-  // Joint staging: when X (ny rows) and R (nx rows) fit in LDS together, both are
-  // copied by LDS DMA at once, unpadded with odd rows XOR-swizzled, so the launch has a
-  // single staging round. Otherwise X and then R are staged through registers, in chunks
-  // if need be.
+  // Staging of X (the input field, ny rows) and R (the right matrix, nx rows), by path:
+  //  joint    — X and R fit in LDS together: both copied by LDS DMA at once, unpadded
+  //             with odd rows XOR-swizzled, one wait for the whole launch;
+  //  streamed — they do not: X in row chunks, then R in column chunks, each chunk as
+  //             large as LDS allows (fewest DMA rounds; a round's latency is far longer
+  //             than its MFMA work, so double buffering would not hide it);
+  //  legacy   — sizes the DMA cannot take (rows not a multiple of 16 bytes): staged
+  //             through registers into padded LDS rows, in chunks if need be.
   const int room = lds_elems - kBRB * (ldl + ldy);
-  const bool joint = (nx * (int)sizeof(F)) % 16 == 0 &&
-                     ((reinterpret_cast<uintptr_t>(Xsrc) | reinterpret_cast<uintptr_t>(Rm)) & 15) == 0 &&
-                     (ny4 + nx4) * nx <= room;
+  const bool dma_ok = (nx * (int)sizeof(F)) % 16 == 0 &&
+                      ((reinterpret_cast<uintptr_t>(Xsrc) | reinterpret_cast<uintptr_t>(Rm)) & 15) == 0;
+  const bool joint = dma_ok && (ny4 + nx4) * nx <= room;
+  const int skr = min(ny4, (room / nx) & ~3);              // X rows per streamed chunk
+  const int scw = min(ntile * 16, (room / nx4) & ~15);     // R columns per streamed chunk
+  const bool streamed = !joint && dma_ok && skr >= 4 && scw >= 16;
   const int swz = joint ? swz_of<F>(nx) : 0;
   F* Xs = Bb;
   F* Rs = Bb + ny4 * nx;
@@ -509,104 +517,6 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
   for (int m = 0; m < kMT; ++m)
 #pragma unroll
     for (int t = 0; t < kTilesPerWave; ++t) acc[m][t] = V{0, 0, 0, 0};
-
-  if (!joint) {
-    decide();
-    __syncthreads();
-    if (!s_active) return;
-  }
-  SGH_TS(1);
-
-  if (joint) {
-    // L rows into registers first, then the DMA, then the decision; one wait covers it
-    // all, and an inactive cuboid leaves only after its DMA has landed
-    constexpr int kLr = (kBRB * kBMaxN + kST - 1) / kST;
-    F lr[kLr];
-    const int ltot = kBRB * ny4;
-#pragma unroll
-    for (int q = 0; q < kLr; ++q) {
-      const int e = tid + q * kST;
-      const int r = e / ny4, c = e - (e / ny4) * ny4;
-      lr[q] = (e < ltot && r < rows && c < ny) ? Lm[(long long)(r0 + r) * ny + c] : F(0);
-    }
-    dma_rows<F, kST>(Xs, Xsrc, ny, nx, swz);
-    dma_rows<F, kST>(Rs, Rm, nx, nx, swz);
-    decide();
-    for (int e = tid; e < (ny4 - ny) * nx; e += kST) Xs[ny * nx + e] = F(0);
-    for (int e = tid; e < (nx4 - nx) * nx; e += kST) Rs[nx * nx + e] = F(0);
-#pragma unroll
-    for (int q = 0; q < kLr; ++q) {
-      const int e = tid + q * kST;
-      if (e < ltot) Ls[(e / ny4) * ldl + (e - (e / ny4) * ny4)] = lr[q];
-    }
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (!s_active) return;
-    SGH_TS(2);
-#pragma unroll
-    for (int t = 0; t < kTilesPerWave; ++t) {
-      const int jt = wave + NW * t;
-      if (jt >= ntile) break;
-      const int col = jt * 16 + li;
-      const bool cin = col < nx;
-      const int colx = (lk & 1) ? (col ^ swz) : col;
-#pragma unroll 4
-      for (int kk = 0; kk < ny4; kk += 4) {
-        const F bv = cin ? Xs[(kk + lk) * nx + colx] : F(0);
-#pragma unroll
-        for (int m = 0; m < kMT; ++m)
-          acc[m][t] = M::mma(Ls[(16 * m + li) * ldl + kk + lk], bv, acc[m][t]);
-      }
-    }
-  } else {
-    // L rows of this item (zero beyond ny and beyond the last row)
-    stage_block<F, kST, kA>(Ls, ldl, Lm + (long long)r0 * ny, ny, kBRB, ny4, rows, ny);
-    // stage 1: Y = L[rows, :] · X, X staged in chunks of kr rows (usually all of it)
-    const int ldb1 = pad_b<F>(nx);
-    const int kr = min(ny4, (bud / ldb1) & ~3);
-    if (kr < 4) return;  // the host guarantees bud >= 4·ldb1; never taken
-    for (int kc0 = 0; kc0 < ny4; kc0 += kr) {
-      const int kn = min(kr, ny4 - kc0);
-      if (kc0 > 0) __syncthreads();
-      stage_block<F, kST, kA>(Bb, ldb1, Xsrc + (long long)kc0 * nx, nx, kn, nx, ny - kc0, nx);
-      __syncthreads();
-      SGH_TS(2);
-#pragma unroll
-      for (int t = 0; t < kTilesPerWave; ++t) {
-        const int jt = wave + NW * t;
-        if (jt >= ntile) break;
-        const int col = jt * 16 + li;
-        const bool cin = col < nx;
-#pragma unroll 4
-        for (int kk = 0; kk < kn; kk += 4) {
-          const F bv = cin ? Bb[(kk + lk) * ldb1 + col] : F(0);
-#pragma unroll
-          for (int m = 0; m < kMT; ++m)
-            acc[m][t] = M::mma(Ls[(16 * m + li) * ldl + kc0 + kk + lk], bv, acc[m][t]);
-        }
-      }
-    }
-  }
-#pragma unroll
-  for (int t = 0; t < kTilesPerWave; ++t) {
-    const int jt = wave + NW * t;
-    const int col = jt * 16 + li;
-#pragma unroll
-    for (int m = 0; m < kMT; ++m) {
-      if (jt < ntile && col < nx) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) Ys[(16 * m + M::row(lane, v)) * ldy + col] = acc[m][t][v];
-      }
-      acc[m][t] = V{0, 0, 0, 0};
-    }
-  }
-  if (nx4 > nx)  // zero the k-padding columns of Y
-    for (int e = tid; e < kBRB * (nx4 - nx); e += kST) {
-      const int r = e / (nx4 - nx), c = nx + e % (nx4 - nx);
-      Ys[r * ldy + c] = F(0);
-    }
-
-  SGH_TS(3);
 
   double v0 = 0.0, v1 = 0.0, v2 = 0.0;
   if (MODE == 3) v0 = -1.0 / 0.0;
@@ -651,9 +561,93 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
         }
       }
   };
+  // stage-1 result Y to LDS (and zero its k-padding columns), accumulators reset
+  auto write_Y = [&]() {
+#pragma unroll
+    for (int t = 0; t < kTilesPerWave; ++t) {
+      const int jt = wave + NW * t;
+      const int col = jt * 16 + li;
+#pragma unroll
+      for (int m = 0; m < kMT; ++m) {
+        if (jt < ntile && col < nx) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) Ys[(16 * m + M::row(lane, v)) * ldy + col] = acc[m][t][v];
+        }
+        acc[m][t] = V{0, 0, 0, 0};
+      }
+    }
+    if (nx4 > nx)
+      for (int e = tid; e < kBRB * (nx4 - nx); e += kST) {
+        const int r = e / (nx4 - nx), c = nx + e % (nx4 - nx);
+        Ys[r * ldy + c] = F(0);
+      }
+  };
+  // L rows of this item (zero beyond ny and beyond the last row): 16-row items load them
+  // into registers ahead of the DMA and store them after it; 32-row items, short of
+  // registers in their forward mode, stage them after the DMA has been issued
+  constexpr bool kLreg = RB == 16;
+  constexpr int kLr = kLreg ? (kBRB * kBMaxN + kST - 1) / kST : 1;
+  F lr[kLr];
+  const int ltot = kBRB * ny4;
+  auto load_L = [&]() {
+    if (!kLreg) return;
+#pragma unroll
+    for (int q = 0; q < kLr; ++q) {
+      const int e = tid + q * kST;
+      const int r = e / ny4, c = e - (e / ny4) * ny4;
+      lr[q] = (e < ltot && r < rows && c < ny) ? Lm[(long long)(r0 + r) * ny + c] : F(0);
+    }
+  };
+  auto store_L = [&]() {
+    if (!kLreg) {
+      stage_block<F, kST, kA>(Ls, ldl, Lm + (long long)r0 * ny, ny, kBRB, ny4, rows, ny);
+      return;
+    }
+#pragma unroll
+    for (int q = 0; q < kLr; ++q) {
+      const int e = tid + q * kST;
+      if (e < ltot) Ls[(e / ny4) * ldl + (e - (e / ny4) * ny4)] = lr[q];
+    }
+  };
 
-  // stage 2: out = Y · R
+  if (!joint && !streamed) {
+    decide();
+    __syncthreads();
+    if (!s_active) return;
+  }
+  SGH_TS(1);
+
   if (joint) {
+    // L rows, then the DMA, then the decision; one wait covers it all, and an inactive
+    // cuboid leaves only after its DMA has landed
+    load_L();
+    dma_rows<F, kST>(Xs, Xsrc, ny, nx, nx, swz);
+    dma_rows<F, kST>(Rs, Rm, nx, nx, nx, swz);
+    decide();
+    for (int e = tid; e < (ny4 - ny) * nx; e += kST) Xs[ny * nx + e] = F(0);
+    for (int e = tid; e < (nx4 - nx) * nx; e += kST) Rs[nx * nx + e] = F(0);
+    store_L();
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (!s_active) return;
+    SGH_TS(2);
+#pragma unroll
+    for (int t = 0; t < kTilesPerWave; ++t) {
+      const int jt = wave + NW * t;
+      if (jt >= ntile) break;
+      const int col = jt * 16 + li;
+      const bool cin = col < nx;
+      const int colx = (lk & 1) ? (col ^ swz) : col;
+#pragma unroll 4
+      for (int kk = 0; kk < ny4; kk += 4) {
+        const F bv = cin ? Xs[(kk + lk) * nx + colx] : F(0);
+#pragma unroll
+        for (int m = 0; m < kMT; ++m)
+          acc[m][t] = M::mma(Ls[(16 * m + li) * ldl + kk + lk], bv, acc[m][t]);
+      }
+    }
+    write_Y();
+    SGH_TS(3);
     __syncthreads();  // Y complete; R has been in LDS since the joint staging
     SGH_TS(4);
 #pragma unroll
@@ -675,8 +669,107 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
       }
       if (cin) finish(t, col, a_);
     }
+  } else if (streamed) {
+    // jobs 0 .. n1-1: X rows [i·skr, ...); jobs n1 .. n1+n2-1: R columns [i·scw, ...)
+    const int n1 = (ny4 + skr - 1) / skr, n2 = (nx + scw - 1) / scw;
+    auto issue = [&](int j) {
+      if (j < n1) {
+        const int k0 = j * skr, kn = min(skr, ny4 - k0), kv = max(0, min(kn, ny - k0));
+        dma_rows<F, kST>(Bb, Xsrc + (long long)k0 * nx, kv, nx, nx, swz_of<F>(nx));
+        for (int e = tid; e < (kn - kv) * nx; e += kST) Bb[kv * nx + e] = F(0);
+      } else {
+        const int c0 = (j - n1) * scw, cn = min(scw, nx - c0);
+        dma_rows<F, kST>(Bb, Rm + c0, nx, cn, nx, swz_of<F>(cn));
+        for (int e = tid; e < (nx4 - nx) * cn; e += kST) Bb[nx * cn + e] = F(0);
+      }
+    };
+    load_L();
+    issue(0);
+    decide();
+    store_L();
+    for (int j = 0; j < n1 + n2; ++j) {
+      if (j > 0) {
+        __syncthreads();   // every wave is done with job j - 1's buffer
+        issue(j);
+      }
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();     // job j has landed
+      if (j == 0 && !s_active) return;
+      if (j < n1) {
+        const int k0 = j * skr, kn = min(skr, ny4 - k0);
+        const int sw = swz_of<F>(nx);
+#pragma unroll
+        for (int t = 0; t < kTilesPerWave; ++t) {
+          const int jt = wave + NW * t;
+          if (jt >= ntile) break;
+          const int col = jt * 16 + li;
+          const bool cin = col < nx;
+          const int colx = (lk & 1) ? (col ^ sw) : col;
+#pragma unroll 4
+          for (int kk = 0; kk < kn; kk += 4) {
+            const F bv = cin ? Bb[(kk + lk) * nx + colx] : F(0);
+#pragma unroll
+            for (int m = 0; m < kMT; ++m)
+              acc[m][t] = M::mma(Ls[(16 * m + li) * ldl + k0 + kk + lk], bv, acc[m][t]);
+          }
+        }
+        if (j == n1 - 1) write_Y();   // read by the R jobs after the next barriers
+      } else {
+        const int c0 = (j - n1) * scw, cn = min(scw, nx - c0);
+        const int sw = swz_of<F>(cn);
+#pragma unroll
+        for (int t = 0; t < kTilesPerWave; ++t) {
+          const int jt = wave + NW * t;
+          if (jt >= ntile) break;
+          const int col = jt * 16 + li;
+          if (col - li < c0 || col - li >= c0 + cn) continue;  // tile not in this chunk
+          const bool cin = col < nx;
+          const int cl = col - c0;
+          const int colx = (lk & 1) ? (cl ^ sw) : cl;
+          V a_[kMT];
+#pragma unroll
+          for (int m = 0; m < kMT; ++m) a_[m] = V{0, 0, 0, 0};
+#pragma unroll 4
+          for (int kk = 0; kk < nx4; kk += 4) {
+            const F bv = cin ? Bb[(kk + lk) * cn + colx] : F(0);
+#pragma unroll
+            for (int m = 0; m < kMT; ++m)
+              a_[m] = M::mma(Ys[(16 * m + li) * ldy + kk + lk], bv, a_[m]);
+          }
+          if (cin) finish(t, col, a_);
+        }
+      }
+    }
   } else {
-    // R staged in column chunks of cw (usually all of it)
+    stage_block<F, kST, kA>(Ls, ldl, Lm + (long long)r0 * ny, ny, kBRB, ny4, rows, ny);
+    // stage 1: Y = L[rows, :] · X, X staged in chunks of kr rows (usually all of it)
+    const int ldb1 = pad_b<F>(nx);
+    const int kr = min(ny4, (bud / ldb1) & ~3);
+    if (kr < 4) return;  // the host guarantees bud >= 4·ldb1; never taken
+    for (int kc0 = 0; kc0 < ny4; kc0 += kr) {
+      const int kn = min(kr, ny4 - kc0);
+      if (kc0 > 0) __syncthreads();
+      stage_block<F, kST, kA>(Bb, ldb1, Xsrc + (long long)kc0 * nx, nx, kn, nx, ny - kc0, nx);
+      __syncthreads();
+      SGH_TS(2);
+#pragma unroll
+      for (int t = 0; t < kTilesPerWave; ++t) {
+        const int jt = wave + NW * t;
+        if (jt >= ntile) break;
+        const int col = jt * 16 + li;
+        const bool cin = col < nx;
+#pragma unroll 4
+        for (int kk = 0; kk < kn; kk += 4) {
+          const F bv = cin ? Bb[(kk + lk) * ldb1 + col] : F(0);
+#pragma unroll
+          for (int m = 0; m < kMT; ++m)
+            acc[m][t] = M::mma(Ls[(16 * m + li) * ldl + kc0 + kk + lk], bv, acc[m][t]);
+        }
+      }
+    }
+    write_Y();
+    SGH_TS(3);
+    // stage 2: out = Y · R, R staged in column chunks of cw (usually all of it)
     int cw = min(ntile * 16, max(16, ((bud / nx4) / 16) * 16));
     while (cw > 16 && nx4 * pad_b<F>(cw) > bud) cw -= 16;
     const int ldb2 = pad_b<F>(cw);
