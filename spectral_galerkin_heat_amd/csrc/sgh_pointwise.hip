@@ -25,6 +25,7 @@ void set_error(const char* fmt, ...) {
 }
 
 constexpr int kThreads = 256;
+static_assert(kThreads == kProgThreads, "property kernels index the LDS evaluation stack by thread");
 
 // ---------------------------------------------------------------------------------
 // Propagator tables (FP64 math, expm1 near λ → 0)

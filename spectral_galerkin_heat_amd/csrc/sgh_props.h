@@ -36,9 +36,13 @@ enum Op {
   OP_NEG = 7, OP_FN = 8, OP_ATAN2 = 9, OP_SQR = 10, OP_CUBE = 11, OP_RECIP = 12
 };
 
-__device__ __forceinline__ double mfn(int f, double x) {
+// Elementary functions of the whitelist. The cheap ones are inlined into the
+// interpreter; the transcendental ones are out-of-line calls, so the interpreter loop
+// keeps a small register footprint (inlining all of them made every property kernel
+// spill to scratch memory).
+template <typename F>
+__device__ __attribute__((noinline)) F mfn_slow(int f, F x) {
   switch (f) {
-    case 0: return sqrt(x);
     case 1: return exp(x);
     case 2: return log(x);
     case 3: return log2(x);
@@ -55,75 +59,64 @@ __device__ __forceinline__ double mfn(int f, double x) {
     case 14: return asinh(x);
     case 15: return acosh(x);
     case 16: return atanh(x);
-    case 17: return fabs(x);
-    case 18: return ceil(x);
-    case 19: return floor(x);
-    case 20: return trunc(x);
     case 21: return log1p(x);
     default: return expm1(x);
   }
 }
 
-__device__ __forceinline__ float mfn(int f, float x) {
+template <typename F>
+__device__ __attribute__((noinline)) F mpow(F a, F b) { return pow(a, b); }
+template <typename F>
+__device__ __attribute__((noinline)) F matan2(F a, F b) { return atan2(a, b); }
+
+template <typename F>
+__device__ __forceinline__ F mfn(int f, F x) {
   switch (f) {
-    case 0: return sqrtf(x);
-    case 1: return expf(x);
-    case 2: return logf(x);
-    case 3: return log2f(x);
-    case 4: return log10f(x);
-    case 5: return sinf(x);
-    case 6: return cosf(x);
-    case 7: return tanf(x);
-    case 8: return asinf(x);
-    case 9: return acosf(x);
-    case 10: return atanf(x);
-    case 11: return sinhf(x);
-    case 12: return coshf(x);
-    case 13: return tanhf(x);
-    case 14: return asinhf(x);
-    case 15: return acoshf(x);
-    case 16: return atanhf(x);
-    case 17: return fabsf(x);
-    case 18: return ceilf(x);
-    case 19: return floorf(x);
-    case 20: return truncf(x);
-    case 21: return log1pf(x);
-    default: return expm1f(x);
+    case 0: return sqrt(x);
+    case 17: return fabs(x);
+    case 18: return ceil(x);
+    case 19: return floor(x);
+    case 20: return trunc(x);
+    default: return mfn_slow<F>(f, x);
   }
 }
 
-__device__ __forceinline__ double mpow(double a, double b) { return pow(a, b); }
-__device__ __forceinline__ float mpow(float a, float b) { return powf(a, b); }
-__device__ __forceinline__ double matan2(double a, double b) { return atan2(a, b); }
-__device__ __forceinline__ float matan2(float a, float b) { return atan2f(a, b); }
+// Per-thread evaluation stack in LDS (kProgMaxStack slots × kProgThreads lanes), indexed
+// [slot][thread]: a dynamically indexed private array would live in scratch memory.
+constexpr int kProgThreads = 256;
 
-// Evaluate branch `w` of the program at temperature T (in precision F).
+// Evaluate branch `w` of the program at temperature T (in precision F). Kernels that call
+// it run kProgThreads threads per block.
 template <typename F>
 __device__ F prog_eval(const PropProgram* __restrict__ P, int w, F T) {
-  F st[kProgMaxStack];
+  __shared__ F stk[kProgMaxStack * kProgThreads];
+  F* st = stk + threadIdx.x;
   int sp = 0;
   const int n = P->len[w];
   const int32_t* ops = P->ops[w];
+#define SGH_ST(k) st[(k) * kProgThreads]
   for (int i = 0; i < n; ++i) {
     const int op = ops[i];
     switch (op) {
-      case OP_CONST: st[sp++] = (F)P->consts[w][ops[++i]]; break;
-      case OP_T: st[sp++] = T; break;
-      case OP_ADD: --sp; st[sp - 1] = st[sp - 1] + st[sp]; break;
-      case OP_SUB: --sp; st[sp - 1] = st[sp - 1] - st[sp]; break;
-      case OP_MUL: --sp; st[sp - 1] = st[sp - 1] * st[sp]; break;
-      case OP_DIV: --sp; st[sp - 1] = st[sp - 1] / st[sp]; break;
-      case OP_POW: --sp; st[sp - 1] = mpow(st[sp - 1], st[sp]); break;
-      case OP_NEG: st[sp - 1] = -st[sp - 1]; break;
-      case OP_FN: { const int f = ops[++i]; st[sp - 1] = mfn(f, st[sp - 1]); } break;
-      case OP_ATAN2: --sp; st[sp - 1] = matan2(st[sp - 1], st[sp]); break;
-      case OP_SQR: st[sp - 1] = st[sp - 1] * st[sp - 1]; break;
-      case OP_CUBE: st[sp - 1] = st[sp - 1] * st[sp - 1] * st[sp - 1]; break;
-      case OP_RECIP: st[sp - 1] = F(1) / st[sp - 1]; break;
+      case OP_CONST: SGH_ST(sp) = (F)P->consts[w][ops[++i]]; ++sp; break;
+      case OP_T: SGH_ST(sp) = T; ++sp; break;
+      case OP_ADD: --sp; SGH_ST(sp - 1) = SGH_ST(sp - 1) + SGH_ST(sp); break;
+      case OP_SUB: --sp; SGH_ST(sp - 1) = SGH_ST(sp - 1) - SGH_ST(sp); break;
+      case OP_MUL: --sp; SGH_ST(sp - 1) = SGH_ST(sp - 1) * SGH_ST(sp); break;
+      case OP_DIV: --sp; SGH_ST(sp - 1) = SGH_ST(sp - 1) / SGH_ST(sp); break;
+      case OP_POW: --sp; SGH_ST(sp - 1) = mpow<F>(SGH_ST(sp - 1), SGH_ST(sp)); break;
+      case OP_NEG: SGH_ST(sp - 1) = -SGH_ST(sp - 1); break;
+      case OP_FN: { const int f = ops[++i]; SGH_ST(sp - 1) = mfn<F>(f, SGH_ST(sp - 1)); } break;
+      case OP_ATAN2: --sp; SGH_ST(sp - 1) = matan2<F>(SGH_ST(sp - 1), SGH_ST(sp)); break;
+      case OP_SQR: { const F v = SGH_ST(sp - 1); SGH_ST(sp - 1) = v * v; } break;
+      case OP_CUBE: { const F v = SGH_ST(sp - 1); SGH_ST(sp - 1) = v * v * v; } break;
+      case OP_RECIP: SGH_ST(sp - 1) = F(1) / SGH_ST(sp - 1); break;
       default: break;
     }
   }
-  return sp > 0 ? st[0] : F(0);
+  const F r = sp > 0 ? SGH_ST(0) : F(0);
+#undef SGH_ST
+  return r;
 }
 
 // Liquid fraction clamped to [0, 1]; a step at t_sol when the band is degenerate.

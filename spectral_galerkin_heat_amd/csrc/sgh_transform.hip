@@ -4,8 +4,9 @@
 // cuFFT + recombine on GPU) is, per axis, a multiplication by an n×n cosine matrix.
 // On MI355X the matrices are tiny (n <= 256) and stay in L2/LDS, so each transform is
 // a batched small GEMM with no transposes:
-//   sep2d        : out = scale · L · X · R for a surface, one workgroup per 8-row block,
-//                  both products staged through LDS in one launch;
+//   sep2d        : out = scale · L · X · R for a surface on the matrix cores, one
+//                  workgroup per 16-row block, both products streamed through LDS in
+//                  one launch;
 //   axis_contract: one axis of a 3-D volume as a strided batched GEMM on the matrix
 //                  cores (x axis: Out[R, n] = In[R, n]·Mᵀ; y/z axes: Out_o = M·In_o),
 //                  64×64 output tiles, 32-deep K slices staged in LDS with the next
@@ -13,101 +14,129 @@
 //                  (f32 16x16x4 or f64 16x16x4), split-BF16 (three bf16 16x16x32
 //                  products hi·hi + hi·lo + lo·hi, FP32 accumulation, FP32 data), and
 //                  the LDS-tiled vector-ALU GEMM.
+#include <algorithm>
+
 #include "sgh_common.h"
 #include "sgh_mfma.h"
 
 namespace sgh {
 
 constexpr int kT = 256;
-constexpr int kRB = 8;      // output rows per sep2d workgroup
-constexpr int kKC = 16;     // contraction chunk staged in LDS
-constexpr int kMaxNx = 256; // widest surface axis handled by sep2d
-constexpr int kMaxO = kRB * kMaxNx / kT;
+constexpr int kSRows = 16;    // surface rows per sep2d workgroup (one MFMA row tile)
+constexpr int kMaxNx = 256;   // widest surface axis handled by sep2d
+constexpr int kSChunk = 8192; // elements of the LDS buffer that streams X, then R (at most)
+constexpr int kSLds = 160 * 1024;  // gfx950 LDS per workgroup
 
+// out[b] = scale · L · (Σ_p X[b][p]) · R for an ny × nx surface, on the matrix cores.
+// Workgroup (row block, b) owns 16 output rows: stage 1 streams X (summing the nparts
+// partial surfaces on load) through LDS in K-chunks of rows and accumulates
+// Y = L[rows, :] · X; stage 2 streams R the same way and accumulates Y · R. 4 waves,
+// 16-column tiles dealt round-robin (up to 4 per wave for nx = 256).
 template <typename F>
 __global__ void __launch_bounds__(kT)
 k_sep2d(int ny, int nx, const F* __restrict__ L, const F* __restrict__ R,
-        const F* __restrict__ X, int nparts, F scale, F* __restrict__ out) {
+        const F* __restrict__ X, int nparts, F scale, F* __restrict__ out, int chunk) {
+  using M = Mfma<F>;
+  using V = typename M::V;
+  constexpr int NW = kT / 64;
+  constexpr int TPW = (kMaxNx / 16 + NW - 1) / NW;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  F* Bs = reinterpret_cast<F*>(smem_raw);   // [kKC][nx]
-  F* Ys = Bs + kKC * nx;                     // [kRB][nx]
-  F* As = Ys + kRB * nx;                     // [kRB][kKC]
-  const int tid = threadIdx.x;
+  const int ny4 = (ny + 3) & ~3, nx4 = (nx + 3) & ~3;
+  const int ldl = pad_a<F>(ny4), ldy = pad_a<F>(nx4), ldb = pad_b<F>(nx);
+  F* Ls = reinterpret_cast<F*>(smem_raw);   // [16][ldl]
+  F* Ys = Ls + kSRows * ldl;                // [16][ldy]
+  F* Bs = Ys + kSRows * ldy;                // [kr][ldb]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
   const int b = blockIdx.y;
-  const int r0 = blockIdx.x * kRB;
-  const int rows = min(kRB, ny - r0);
+  const int r0 = blockIdx.x * kSRows;
+  const int rows = min(kSRows, ny - r0);
   const long long plane = (long long)ny * nx;
   const F* Xb = X + (long long)b * nparts * plane;
   F* Ob = out + (long long)b * plane;
-  const int nout = rows * nx;
+  const int ntile = (nx + 15) >> 4;
+  const int kr = (chunk / ldb) & ~3;   // K rows per streamed chunk (the host makes it >= 4)
 
-  int ro[kMaxO], co[kMaxO];
-  F acc[kMaxO];
-#pragma unroll
-  for (int j = 0; j < kMaxO; ++j) {
-    const int o = tid + j * kT;
-    ro[j] = o / nx;
-    co[j] = o - ro[j] * nx;
-    acc[j] = F(0);
+  for (int e = tid; e < kSRows * ny4; e += kT) {
+    const int r = e / ny4, c = e - r * ny4;
+    Ls[r * ldl + c] = (r < rows && c < ny) ? L[(long long)(r0 + r) * ny + c] : F(0);
   }
-
-  // Y = L[r0:r0+rows, :] · X
-  for (int k0 = 0; k0 < ny; k0 += kKC) {
-    const int kc = min(kKC, ny - k0);
-    for (int e = tid; e < kc * nx; e += kT) {
+  V acc[TPW];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) acc[t] = V{0, 0, 0, 0};
+  // stage 1: Y = L[rows, :] · Σ_p X_p
+  for (int k0 = 0; k0 < ny4; k0 += kr) {
+    const int kn = min(kr, ny4 - k0);
+    __syncthreads();
+    for (int e = tid; e < kn * nx; e += kT) {
       const int kk = e / nx, c = e - kk * nx;
-      const long long off = (long long)(k0 + kk) * nx + c;
-      F v = Xb[off];
-      for (int q = 1; q < nparts; ++q) v += Xb[q * plane + off];
-      Bs[kk * nx + c] = v;
-    }
-    for (int e = tid; e < rows * kKC; e += kT) {
-      const int r = e / kKC, kk = e - r * kKC;
-      As[r * kKC + kk] = kk < kc ? L[(long long)(r0 + r) * ny + k0 + kk] : F(0);
+      F v = F(0);
+      if (k0 + kk < ny) {
+        const long long off = (long long)(k0 + kk) * nx + c;
+        v = Xb[off];
+        for (int q = 1; q < nparts; ++q) v += Xb[q * plane + off];
+      }
+      Bs[kk * ldb + c] = v;
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < kMaxO; ++j) {
-      if (tid + j * kT < nout) {
-        const F* a = As + ro[j] * kKC;
-        const F* bcol = Bs + co[j];
-        F s = acc[j];
-        for (int kk = 0; kk < kc; ++kk) s += a[kk] * bcol[kk * nx];
-        acc[j] = s;
+    for (int t = 0; t < TPW; ++t) {
+      const int jt = wave + NW * t;
+      if (jt >= ntile) break;
+      const int col = jt * 16 + li;
+      const bool cin = col < nx;
+      for (int kk = 0; kk < kn; kk += 4) {
+        const F bv = cin ? Bs[(kk + lk) * ldb + col] : F(0);
+        acc[t] = M::mma(Ls[li * ldl + k0 + kk + lk], bv, acc[t]);
       }
     }
-    __syncthreads();
   }
 #pragma unroll
-  for (int j = 0; j < kMaxO; ++j) {
-    if (tid + j * kT < nout) Ys[tid + j * kT] = acc[j];
-    acc[j] = F(0);
+  for (int t = 0; t < TPW; ++t) {
+    const int jt = wave + NW * t;
+    const int col = jt * 16 + li;
+    if (jt < ntile && col < nx) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Ys[M::row(lane, v) * ldy + col] = acc[t][v];
+    }
+    acc[t] = V{0, 0, 0, 0};
   }
-  __syncthreads();
-
-  // out = Y · R
-  for (int k0 = 0; k0 < nx; k0 += kKC) {
-    const int kc = min(kKC, nx - k0);
-    for (int e = tid; e < kc * nx; e += kT) {
+  for (int e = tid; e < kSRows * (nx4 - nx); e += kT) {
+    const int r = e / (nx4 - nx), c = nx + e % (nx4 - nx);
+    Ys[r * ldy + c] = F(0);
+  }
+  // stage 2: out = Y · R, R streamed in K-chunks of rows
+  for (int k0 = 0; k0 < nx4; k0 += kr) {
+    const int kn = min(kr, nx4 - k0);
+    __syncthreads();
+    for (int e = tid; e < kn * nx; e += kT) {
       const int kk = e / nx, c = e - kk * nx;
-      Bs[kk * nx + c] = R[(long long)(k0 + kk) * nx + c];
+      Bs[kk * ldb + c] = (k0 + kk < nx) ? R[(long long)(k0 + kk) * nx + c] : F(0);
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < kMaxO; ++j) {
-      if (tid + j * kT < nout) {
-        const F* y = Ys + ro[j] * nx + k0;
-        const F* bcol = Bs + co[j];
-        F s = acc[j];
-        for (int kk = 0; kk < kc; ++kk) s += y[kk] * bcol[kk * nx];
-        acc[j] = s;
+    for (int t = 0; t < TPW; ++t) {
+      const int jt = wave + NW * t;
+      if (jt >= ntile) break;
+      const int col = jt * 16 + li;
+      const bool cin = col < nx;
+      for (int kk = 0; kk < kn; kk += 4) {
+        const F bv = cin ? Bs[(kk + lk) * ldb + col] : F(0);
+        acc[t] = M::mma(Ys[li * ldy + k0 + kk + lk], bv, acc[t]);
       }
     }
-    __syncthreads();
   }
 #pragma unroll
-  for (int j = 0; j < kMaxO; ++j)
-    if (tid + j * kT < nout) Ob[(long long)(r0 + ro[j]) * nx + co[j]] = scale * acc[j];
+  for (int t = 0; t < TPW; ++t) {
+    const int jt = wave + NW * t;
+    const int col = jt * 16 + li;
+    if (jt >= ntile || col >= nx) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = M::row(lane, v);
+      if (row < rows) Ob[(long long)(r0 + row) * nx + col] = scale * acc[t][v];
+    }
+  }
 }
 
 // C[b][r][c] = scale · Σ_k A[b][r][k] · B[b][k][c] with arbitrary strides.
@@ -364,10 +393,22 @@ int sgh_sep2d(int dtype, int batch, int ny, int nx, const void* L, const void* R
   SGH_REQUIRE(X != out, SGH_ERR_ARG, "sgh_sep2d: in-place transform is not supported");
   SGH_REQUIRE(batch <= 65535, SGH_ERR_ARG, "sgh_sep2d: batch too large");
   SGH_DISPATCH(dtype, {
-    const size_t shm = (size_t)(kKC * nx + kRB * nx + kRB * kKC) * sizeof(F);
-    hipLaunchKernelGGL(k_sep2d<F>, dim3(cdiv(ny, kRB), batch), dim3(kT), shm,
+    const int ny4 = (ny + 3) & ~3, nx4 = (nx + 3) & ~3;
+    const long long fixed = (long long)kSRows * (pad_a<F>(ny4) + pad_a<F>(nx4));
+    const long long chunk = std::min<long long>(kSChunk, kSLds / (long long)sizeof(F) - fixed);
+    SGH_REQUIRE(chunk >= 4LL * pad_b<F>(nx), SGH_ERR_ARG,
+                "sgh_sep2d: ny=%d is too large for one workgroup's LDS", ny);
+    const size_t shm = (size_t)(fixed + chunk) * sizeof(F);
+    static bool lds_raised[2] = {false, false};
+    if (!lds_raised[sizeof(F) == 8]) {
+      hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sep2d<F>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, kSLds);
+      if (e != hipSuccess) return fail_hip(e, "sgh_sep2d LDS attribute");
+      lds_raised[sizeof(F) == 8] = true;
+    }
+    hipLaunchKernelGGL(k_sep2d<F>, dim3(cdiv(ny, kSRows), batch), dim3(kT), shm,
                        reinterpret_cast<hipStream_t>(stream), ny, nx, (const F*)L, (const F*)R,
-                       (const F*)X, nparts, (F)scale, (F*)out);
+                       (const F*)X, nparts, (F)scale, (F*)out, (int)chunk);
   });
   SGH_CHECK_LAUNCH("sgh_sep2d");
   return SGH_OK;
