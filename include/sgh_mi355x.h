@@ -79,6 +79,11 @@ int sgh_mul_inplace(int dtype, long long n, void* a, const void* E, void* stream
 int sgh_zcontract(int dtype, int nx, int ny, int nz, const void* a, const void* w,
                   int nchunks, void* parts, void* stream);
 
+/* parts[0][i] = Σ_q parts[q][i] for `nparts` planes of n elements, in place (the
+ * finishing sum of the z-contraction above, so the 2-D transform reads one plane).
+ * Part of the same einsum (sgh/physics/spectral_ops.py:210-231). */
+int sgh_sum_parts(int dtype, long long n, int nparts, void* parts, void* stream);
+
 /* ---- fused Picard iteration ----------------------------------------------
  * a_raw = aK + KK·(wt⊗St + wb⊗Sb + Q);  r = a_raw - a_old;  a_new = a_old + ω·r;
  * partials[2b] = Σ r², partials[2b+1] = Σ a_old² (FP64, one pair per block);

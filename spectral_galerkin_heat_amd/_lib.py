@@ -34,6 +34,7 @@ SIGNATURES = {
     "sgh_reduce_norms": [vp, i32, f64, vp, vp],
     "sgh_reduce": [i32, i64, vp, i32, vp, vp],
     "sgh_sep2d": [i32, i32, i32, i32, vp, vp, vp, i32, f64, vp, vp],
+    "sgh_sum_parts": [i32, i64, i32, vp, vp],
     "sgh_axis_contract": [i32, i32, i64, i32, i64, vp, vp, f64, vp, vp],
     "sgh_laser_flux": [i32, i32, i32, f64, f64, f64, f64, f64, f64, f64, f64, i32, vp, vp],
     "sgh_shift2d": [i32, i32, i32, vp, f64, f64, vp, f64, vp, vp],

@@ -332,36 +332,6 @@ __device__ bool picard_active(const BatchPtrs<F>& P, const CuboidDesc& d, int c,
 // element is read by exactly one wave of the item); Y goes through LDS; stage 2,
 // out = Y·R, streams R the same way. Column tiles of 16 are dealt to the 4 waves.
 // ---------------------------------------------------------------------------------
-// Copy a rows × cols block (source row stride src_ld, starting at src) into LDS
-// (row stride dst_ld), zero-filling rows >= valid_rows and columns >= valid_cols.
-// Each thread keeps A global loads in flight; indices advance incrementally
-// (no per-element division).
-template <typename F, int NT, int A>
-__device__ __forceinline__ void stage_block(F* __restrict__ dst, int dst_ld, const F* __restrict__ src,
-                                            int src_ld, int rows, int cols, int valid_rows,
-                                            int valid_cols) {
-  const int total = rows * cols;
-  const int step_r = NT / cols, step_c = NT - (NT / cols) * cols;
-  int r = threadIdx.x / cols, c = threadIdx.x - (threadIdx.x / cols) * cols;
-  for (int e0 = threadIdx.x; e0 < total; e0 += A * NT) {
-    F tmp[A];
-    int rr[A], cc[A];
-#pragma unroll
-    for (int q = 0; q < A; ++q) {
-      rr[q] = r;
-      cc[q] = c;
-      const bool in = e0 + q * NT < total;
-      tmp[q] = (in && r < valid_rows && c < valid_cols) ? src[(long long)r * src_ld + c] : F(0);
-      r += step_r;
-      c += step_c;
-      if (c >= cols) { c -= cols; ++r; }
-    }
-#pragma unroll
-    for (int q = 0; q < A; ++q)
-      if (e0 + q * NT < total) dst[rr[q] * dst_ld + cc[q]] = tmp[q];
-  }
-}
-
 // Element offset XORed into the columns of odd rows of an unpadded LDS operand so that
 // rows kk and kk+1 of an MFMA B operand fall in different halves of the 32 banks; 0
 // when the row length already staggers them (or cannot be swizzled within a row).

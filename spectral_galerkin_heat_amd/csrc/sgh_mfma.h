@@ -1,4 +1,4 @@
-// MFMA building blocks for gfx950 shared by the transform kernels.
+// MFMA and LDS-staging building blocks for gfx950 shared by the transform kernels.
 //
 // Lane layouts (checked on gfx950 with a one-wave probe):
 //   v_mfma_f32_16x16x4_f32 / v_mfma_f64_16x16x4_f64: A[i][k] in lane i + 16k,
@@ -62,5 +62,35 @@ template <typename F>
 __host__ __device__ constexpr int pad_a(int n) { return pad_to_mod<F>(n, 2 * (int)(sizeof(F) / 4)); }
 template <typename F>
 __host__ __device__ constexpr int pad_b(int n) { return pad_to_mod<F>(n, 16); }
+
+// Copy a rows × cols block (source row stride src_ld, starting at src) into LDS
+// (row stride dst_ld), zero-filling rows >= valid_rows and columns >= valid_cols.
+// Each thread keeps A global loads in flight; indices advance incrementally
+// (no per-element division).
+template <typename F, int NT, int A>
+__device__ __forceinline__ void stage_block(F* __restrict__ dst, int dst_ld, const F* __restrict__ src,
+                                            int src_ld, int rows, int cols, int valid_rows,
+                                            int valid_cols) {
+  const int total = rows * cols;
+  const int step_r = NT / cols, step_c = NT - (NT / cols) * cols;
+  int r = threadIdx.x / cols, c = threadIdx.x - (threadIdx.x / cols) * cols;
+  for (int e0 = threadIdx.x; e0 < total; e0 += A * NT) {
+    F tmp[A];
+    int rr[A], cc[A];
+#pragma unroll
+    for (int q = 0; q < A; ++q) {
+      rr[q] = r;
+      cc[q] = c;
+      const bool in = e0 + q * NT < total;
+      tmp[q] = (in && r < valid_rows && c < valid_cols) ? src[(long long)r * src_ld + c] : F(0);
+      r += step_r;
+      c += step_c;
+      if (c >= cols) { c -= cols; ++r; }
+    }
+#pragma unroll
+    for (int q = 0; q < A; ++q)
+      if (e0 + q * NT < total) dst[rr[q] * dst_ld + cc[q]] = tmp[q];
+  }
+}
 
 }  // namespace sgh

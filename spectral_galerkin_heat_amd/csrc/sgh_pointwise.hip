@@ -108,6 +108,18 @@ k_zcontract(const F* __restrict__ a, const F* __restrict__ w, F* __restrict__ pa
   parts[(long long)blockIdx.y * S + s] = acc;
 }
 
+// parts[0][i] = Σ_q parts[q][i], in place (the other planes are left as they were).
+template <typename F>
+__global__ void __launch_bounds__(kThreads)
+k_sum_parts(F* __restrict__ parts, long long n, int nparts) {
+  for (long long i = blockIdx.x * (long long)kThreads + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kThreads) {
+    F v = parts[i];
+    for (int q = 1; q < nparts; ++q) v += parts[q * n + i];
+    parts[i] = v;
+  }
+}
+
 // One Picard iteration, column-parallel:
 //   a_raw = aK + KK·(wt⊗St + wb⊗Sb + Q); r = a_raw − a_old; a_new = a_old + ω r
 // with FP64 partial sums of r² and a_old² per block and the next iterate's z-contraction.
@@ -464,6 +476,17 @@ int sgh_zcontract(int dtype, int nx, int ny, int nz, const void* a, const void* 
                        STREAM(stream), (const F*)a, (const F*)w, (F*)parts, S, nz, pc);
   });
   SGH_CHECK_LAUNCH("sgh_zcontract");
+  return SGH_OK;
+}
+
+int sgh_sum_parts(int dtype, long long n, int nparts, void* parts, void* stream) {
+  SGH_REQUIRE(n > 0 && nparts >= 1 && parts, SGH_ERR_ARG, "sgh_sum_parts: bad args");
+  if (nparts == 1) return SGH_OK;
+  SGH_DISPATCH(dtype, {
+    hipLaunchKernelGGL(k_sum_parts<F>, dim3(grid_for(n)), dim3(kThreads), 0, STREAM(stream),
+                       (F*)parts, n, nparts);
+  });
+  SGH_CHECK_LAUNCH("sgh_sum_parts");
   return SGH_OK;
 }
 

@@ -31,7 +31,8 @@ constexpr int kSLds = 160 * 1024;  // gfx950 LDS per workgroup
 // Workgroup (row block, b) owns 16 output rows: stage 1 streams X (summing the nparts
 // partial surfaces on load) through LDS in K-chunks of rows and accumulates
 // Y = L[rows, :] · X; stage 2 streams R the same way and accumulates Y · R. 4 waves,
-// 16-column tiles dealt round-robin (up to 4 per wave for nx = 256).
+// 16-column tiles dealt round-robin (up to 4 per wave for nx = 256). Callers sum
+// z-chunk partials first (sgh_sum_parts) so that X streams as one plane.
 template <typename F>
 __global__ void __launch_bounds__(kT)
 k_sep2d(int ny, int nx, const F* __restrict__ L, const F* __restrict__ R,
@@ -68,15 +69,19 @@ k_sep2d(int ny, int nx, const F* __restrict__ L, const F* __restrict__ R,
   for (int k0 = 0; k0 < ny4; k0 += kr) {
     const int kn = min(kr, ny4 - k0);
     __syncthreads();
-    for (int e = tid; e < kn * nx; e += kT) {
-      const int kk = e / nx, c = e - kk * nx;
-      F v = F(0);
-      if (k0 + kk < ny) {
-        const long long off = (long long)(k0 + kk) * nx + c;
-        v = Xb[off];
-        for (int q = 1; q < nparts; ++q) v += Xb[q * plane + off];
+    if (nparts == 1) {
+      stage_block<F, kT, 4>(Bs, ldb, Xb + (long long)k0 * nx, nx, kn, nx, ny - k0, nx);
+    } else {
+      for (int e = tid; e < kn * nx; e += kT) {
+        const int kk = e / nx, c = e - kk * nx;
+        F v = F(0);
+        if (k0 + kk < ny) {
+          const long long off = (long long)(k0 + kk) * nx + c;
+          v = Xb[off];
+          for (int q = 1; q < nparts; ++q) v += Xb[q * plane + off];
+        }
+        Bs[kk * ldb + c] = v;
       }
-      Bs[kk * ldb + c] = v;
     }
     __syncthreads();
 #pragma unroll
@@ -109,10 +114,7 @@ k_sep2d(int ny, int nx, const F* __restrict__ L, const F* __restrict__ R,
   for (int k0 = 0; k0 < nx4; k0 += kr) {
     const int kn = min(kr, nx4 - k0);
     __syncthreads();
-    for (int e = tid; e < kn * nx; e += kT) {
-      const int kk = e / nx, c = e - kk * nx;
-      Bs[kk * ldb + c] = (k0 + kk < nx) ? R[(long long)(k0 + kk) * nx + c] : F(0);
-    }
+    stage_block<F, kT, 4>(Bs, ldb, R + (long long)k0 * nx, nx, kn, nx, nx - k0, nx);
     __syncthreads();
 #pragma unroll
     for (int t = 0; t < TPW; ++t) {

@@ -307,7 +307,8 @@ def _recon_face(a, w, SsState, out=None):
     nz, ny, nx = a.shape
     code = dtype_code(a.dtype)
     call("sgh_zcontract", code, nx, ny, nz, ptr(a), ptr(w), b.nchunks, ptr(b.zc), stream_ptr())
-    return _sep2d(b.zc, g.ByT, g.Bx, 1.0, b.nchunks, out=out)
+    call("sgh_sum_parts", code, nx * ny, b.nchunks, ptr(b.zc), stream_ptr())
+    return _sep2d(b.zc, g.ByT, g.Bx, 1.0, 1, out=out)
 
 
 def reconstruct_surface_temperature(a, SsState):
@@ -502,7 +503,8 @@ def picard_step(solver, t, dt):
     for iter_k in range(solver.max_picard_iter):
         a_old = a_cur
         a_new = b.a_alt if a_old is b.a_temp else b.a_temp
-        _sep2d(b.zc, g.ByT, g.Bx, 1.0, b.nchunks, out=b.T_s)
+        call("sgh_sum_parts", code, Ssz, b.nchunks, ptr(b.zc), st)
+        _sep2d(b.zc, g.ByT, g.Bx, 1.0, 1, out=b.T_s)
         call("sgh_surface_forcing", code, Ssz, ptr(b.T_s), ptr(b.q_las), float(mat.Pa),
              float(mat.T_boil), float(mat.DeltaH_LV), float(mat.R_v), float(mat.T_liquidus),
              h_top, T0, ptr(prog), kbar, ptr(b.q_evap_buffer), ptr(b.q_tmp), st)
