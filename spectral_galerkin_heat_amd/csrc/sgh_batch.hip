@@ -68,6 +68,13 @@ struct Item {
   int c, blk;
 };
 
+// A surface work item with a copy of its cuboid's descriptor, so that a surface
+// workgroup finds everything it needs in one load (no item -> descriptor chain).
+struct ItemDesc {
+  CuboidDesc d;
+  int c, blk;
+};
+
 template <typename F>
 struct BatchPtrs {
   F* a;            // packed modal state
@@ -86,6 +93,7 @@ struct BatchPtrs {
   const double* laser;     // laser rows
   const CuboidDesc* desc;
   const Item* row_items;
+  const ItemDesc* row_descs;  // row_items[i] with its cuboid's descriptor
   const Item* col_items;
   double* row_slots;       // per row item: rr, gu, g2w  (3 doubles)
   double* col_slots;       // per col item: AA
@@ -393,10 +401,12 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
   constexpr int kA = kStageAhead;   // loads in flight per lane while staging
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double sh[NT / kWave];
+  __shared__ double sh3[3][NT / kWave];
   __shared__ int s_active;
-  const Item it = P.row_items[blockIdx.x];
+  const ItemDesc& idd = P.row_descs[blockIdx.x];
+  const Item it{idd.c, idd.blk};
   if (it.c < 0) return;
-  const CuboidDesc d = P.desc[it.c];
+  const CuboidDesc d = idd.d;
   const int tid = threadIdx.x;
   // wave 0 decides whether the cuboid still iterates (the decision needs every lane of
   // the wave); the other waves' loads are already in flight meanwhile
@@ -772,14 +782,21 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
   }
   SGH_TS(5);
   if (MODE == 1) {
-    const double a0 = block_sum(v0, sh);
-    const double a1 = block_sum(v1, sh);
-    const double a2 = block_sum(v2, sh);
-    if (tid == 0) {
-      double* sl = P.row_slots + 3 * (d.row_item0 + it.blk);
-      sl[0] = a0;
-      sl[1] = a1;
-      sl[2] = a2;
+    // the three norm partials in one exchange: wave sums, then wave 0 adds the waves'
+    // partials in a fixed order
+    v0 = wave_sum(v0);
+    v1 = wave_sum(v1);
+    v2 = wave_sum(v2);
+    if (lane == 0) {
+      sh3[0][wave] = v0;
+      sh3[1][wave] = v1;
+      sh3[2][wave] = v2;
+    }
+    __syncthreads();
+    if (tid < 3) {
+      double a = 0.0;
+      for (int w = 0; w < NW; ++w) a += sh3[tid][w];
+      P.row_slots[3 * (d.row_item0 + it.blk) + tid] = a;
     }
   } else if (MODE == 3) {
     const double m = block_max(v0, sh);
@@ -891,6 +908,7 @@ struct Plan {
   double* laser = nullptr;
   CuboidDesc* desc = nullptr;
   Item* row_items = nullptr;
+  ItemDesc* row_descs = nullptr;
   Item* col_items = nullptr;
   Item* prime_items = nullptr;
   long long row_cap = 0, col_cap = 0;
@@ -934,6 +952,7 @@ BatchPtrs<F> ptrs(Plan* p) {
   P.laser = p->laser;
   P.desc = p->desc;
   P.row_items = p->row_items;
+  P.row_descs = p->row_descs;
   P.col_items = p->col_items;
   P.row_slots = p->row_slots;
   P.col_slots = p->col_slots;
@@ -1134,7 +1153,8 @@ static void free_plan(Plan* p) {
   drop_graph(p);
   for (auto& e : p->ev)
     if (e) hipEventDestroy(e);
-  void* bufs[] = {p->a, p->surf, p->mats, p->t1, p->laser, p->desc, p->row_items, p->col_items,
+  void* bufs[] = {p->a, p->surf, p->mats, p->t1, p->laser, p->desc, p->row_items, p->row_descs,
+                  p->col_items,
                   p->prime_items, p->row_slots, p->col_slots, p->aux_slots, p->done_at,
                   p->status, p->step_idx, p->metrics};
   for (void* b : bufs)
@@ -1313,8 +1333,18 @@ static int rebuild_lists(Plan* p, hipStream_t st) {
     set_error("sgh_batch: work list exceeds capacity");
     return SGH_ERR_STATE;
   }
+  std::vector<ItemDesc> hrd(p->hrow.size());
+  for (size_t i = 0; i < p->hrow.size(); ++i) {
+    const Item& it = p->hrow[i];
+    hrd[i].c = it.c;
+    hrd[i].blk = it.blk;
+    hrd[i].d = it.c >= 0 ? p->hdesc[it.c] : CuboidDesc{};
+  }
   if (!p->hrow.empty())
     e = hipMemcpyAsync(p->row_items, p->hrow.data(), p->hrow.size() * sizeof(Item),
+                       hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && !hrd.empty())
+    e = hipMemcpyAsync(p->row_descs, hrd.data(), hrd.size() * sizeof(ItemDesc),
                        hipMemcpyHostToDevice, st);
   if (e == hipSuccess && !p->hcol.empty())
     e = hipMemcpyAsync(p->col_items, p->hcol.data(), p->hcol.size() * sizeof(Item),
@@ -1374,6 +1404,7 @@ int sgh_batch_pool_create(int dtype, int max_cuboids, long long state_capacity,
   chk(dmalloc(&p->laser, (size_t)laser_capacity * 6 * sizeof(double)));
   chk(dmalloc(&p->desc, (size_t)max_cuboids * sizeof(CuboidDesc)));
   chk(dmalloc(&p->row_items, (size_t)p->row_cap * sizeof(Item)));
+  chk(dmalloc(&p->row_descs, (size_t)p->row_cap * sizeof(ItemDesc)));
   chk(dmalloc(&p->col_items, (size_t)p->col_cap * sizeof(Item)));
   chk(dmalloc(&p->prime_items, (size_t)p->col_cap * sizeof(Item)));
   chk(dmalloc(&p->row_slots, 3 * (size_t)max_cuboids * kMaxRowBlk * sizeof(double)));
