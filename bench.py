@@ -134,6 +134,9 @@ def main():
         for prec in ("mixed", "fp64"):
             tuning[prec] = autotune(ctxs, prec)
             log(f"autotune {prec}: {tuning[prec].summary()}")
+            for row in sorted(tuning[prec].table, key=lambda r: r["ms_per_step"]):
+                log(f"    rows {row['rows']:2d} threads {row['threads']:4d} lds {row['lds_bytes'] >> 10:3d} KiB: "
+                    f"{row['ms_per_step']:.4f} ms/step")
 
     def timed_engine(precision):
         tv = tuning.get(precision)
@@ -157,8 +160,11 @@ def main():
     eng_64, t_64 = timed_engine("fp64")
     fp64_rate = N_total * K / t_64
     log(f"fp64 batched: {t_64 * 1e3:.1f} ms -> {fp64_rate:.3e} mode-steps/s")
-    num, den = 0.0, 0.0
+    from spectral_galerkin_heat_amd.physics.meltpool import cell_temperature, melt_pool_extent
+    num, den, num_x, den_x = 0.0, 0.0, 0.0, 0.0
     rel_max = 0.0
+    pool_dev = 0.0   # largest |Δwidth|/dy or |Δdepth|/dz over the cuboids
+    melting = 0
     for i in range(64):
         a = eng_mx.state(i).double()
         b = eng_64.state(i)
@@ -167,8 +173,21 @@ def main():
         num += d2
         den += n2
         rel_max = max(rel_max, (d2 / n2) ** 0.5)
+        # excursion T − T0: the mean mode without T0·√V
+        g = ctxs[i].geom
+        m0 = float(ctxs[i].mat.T0) * float(np.sqrt(float(g.size.x) * float(g.size.y) * float(g.size.z)))
+        num_x += d2
+        den_x += n2 - float(b[0, 0, 0]) ** 2 + (float(b[0, 0, 0]) - m0) ** 2
+        d = tuple(float(v) for v in g.d)
+        pa = melt_pool_extent(cell_temperature(a, g), d)
+        pb = melt_pool_extent(cell_temperature(b, g), d)
+        melting += pb[1] > 0
+        pool_dev = max(pool_dev, abs(pa[1] - pb[1]) / d[1], abs(pa[2] - pb[2]) / d[2])
     rel_batch = (num / den) ** 0.5
-    log(f"mixed vs fp64: batch rel-L2 {rel_batch:.2e}, worst cuboid {rel_max:.2e}")
+    rel_exc = (num_x / den_x) ** 0.5
+    log(f"mixed vs fp64: batch rel-L2 {rel_batch:.2e} (excursion {rel_exc:.2e}), worst cuboid "
+        f"{rel_max:.2e}; melt pools in {melting} cuboids, max width/depth deviation "
+        f"{pool_dev:.2f} cells")
     eng_64.close()
 
     # live per-kernel timing (HIP events on the engine stream) over an instrumented segment
@@ -341,6 +360,9 @@ def main():
             "fp64_batched_mode_steps_per_s": round(fp64_rate, 1),
             "rel_l2_mixed_vs_fp64_batch": float(f"{rel_batch:.3e}"),
             "rel_l2_mixed_vs_fp64_worst_cuboid": float(f"{rel_max:.3e}"),
+            "rel_l2_excursion_mixed_vs_fp64_batch": float(f"{rel_exc:.3e}"),
+            "melt_pool_cuboids": melting,
+            "melt_pool_max_width_depth_deviation_cells": round(pool_dev, 3),
             "volume_pass_GBps": round(vol_gbs, 1),
             "kernel_share_of_step": {"surface_iterations": round(share_surf, 3),
                                      "volume_pass": round(share_vol, 3)},

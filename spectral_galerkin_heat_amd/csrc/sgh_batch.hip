@@ -42,6 +42,7 @@ constexpr int kStageAhead = 4;   // global loads in flight per thread while stag
 constexpr int kSurfLds = 150 * 1024;  // default LDS bytes per surface workgroup
 constexpr int kLdsMax = 159 * 1024;   // dynamic LDS cap: gfx950's 160 KiB less the static part
 constexpr int kXcds = 8;          // MI355X: workgroup b runs on XCD b mod 8
+constexpr int kCusPerXcd = 32;    // MI355X: 256 CUs
 
 struct CuboidDesc {
   int nx, ny, nz, S;
@@ -62,6 +63,7 @@ struct CuboidDesc {
   double f1, f2, T_boil, T_liq;
   double absorptivity, rx, ry, order, area_factor;
   int cell_integrated, live;  // live: the slot holds a cuboid
+  int item_rows, pad_;        // surface rows per transform item (16 or 32)
 };
 
 struct Item {
@@ -104,7 +106,6 @@ struct BatchPtrs {
   double* metrics;         // [step][cuboid][5]
   int metrics_rows;        // capacity in steps
   int n_cuboids;
-  int rows;                // surface rows per transform item (16 or 32)
 };
 
 __device__ __forceinline__ double dexp_(double x) { return exp(x); }
@@ -236,8 +237,8 @@ __global__ void __launch_bounds__(kBT) k_b_prep(BatchPtrs<F> P) {
     const double x0 = L[0], y0 = L[1], power = L[2], on = L[3], vx = L[4], vy = L[5];
     const double I0 = on != 0.0 ? d.absorptivity * power / (d.area_factor * d.rx * d.ry) : 0.0;
     const double sy = vy * d.dt / d.dy, sx = vx * d.dt / d.dx;
-    const int r0 = it.blk * P.rows;
-    const int rows = min(P.rows, d.ny - r0);
+    const int r0 = it.blk * d.item_rows;
+    const int rows = min(d.item_rows, d.ny - r0);
     const F* qo = P.q_evap_old + d.s_off;
     for (int e = threadIdx.x; e < rows * d.nx; e += kBT) {
       const int j = r0 + e / d.nx, i = e % d.nx;
@@ -378,27 +379,49 @@ __device__ __forceinline__ void dma_rows(F* dst, const F* src, int rows, int ld,
 }
 
 #ifdef SGH_BLOCK_TIMESTAMPS
+// Diagnostic build only: per-block wall-clock stamps of the launches with k == 0 (every
+// cuboid active), slots 0 entry, 1 staging issued, 2 staging landed, 3 stage 1 done,
+// 4 stage 2 begins, 5 end.
 __device__ unsigned long long g_block_ts[4 * 8192 * 6];
 #define SGH_TS(slot)                                                                  \
   do {                                                                                \
-    if (threadIdx.x == 0 && blockIdx.x < 8192)                                        \
+    if (threadIdx.x == 0 && blockIdx.x < 8192 && k == 0)                              \
       g_block_ts[(MODE * 8192 + blockIdx.x) * 6 + (slot)] = wall_clock64();           \
   } while (0)
 #else
 #define SGH_TS(slot) do {} while (0)
 #endif
 
+// acc[m] += Σ_{k < kn} A_m[k] · B[k] on the matrix cores for the lane's 16-column tile,
+// k in steps of 4 (kn a multiple of 4). ga(m, k): this lane's A element of row tile m
+// at k + (lane / 16); gb(k): its B element at k + (lane / 16).
+template <typename F, int KMT, int KS, typename GA, typename GB>
+__device__ __forceinline__ void mma_k(typename Mfma<F>::V (&acc)[KMT], int mt, int kn, GA ga,
+                                      GB gb) {
+  static_assert(KS == 1, "one accumulator chain");
+#pragma unroll 4
+  for (int k = 0; k < kn; k += 4) {
+    const F bv = gb(k);
+#pragma unroll
+    for (int m = 0; m < KMT; ++m)
+      if (m < mt) acc[m] = Mfma<F>::mma(ga(m, k), bv, acc[m]);
+  }
+}
+
 template <typename F, int MODE, int RB, int NT>
 __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud, int lds_elems) {
   SGH_TS(0);
   using M = Mfma<F>;
   using V = typename M::V;
-  constexpr int kBRB = RB;                      // rows of this item
+  constexpr int kBRB = RB;                      // most rows of an item (LDS tiles)
   constexpr int kMT = RB / 16;                  // MFMA row tiles
   constexpr int kST = NT;                       // threads
   constexpr int NW = NT / kWave;                // waves
   constexpr int kTilesPerWave = (kBMaxN / 16 + NW - 1) / NW;  // 16-column tiles per wave
   constexpr int kA = kStageAhead;   // loads in flight per lane while staging
+  // one accumulator chain per tile: splitting K over several chains measured slower (the
+  // LDS operand reads, not the MFMA dependency, set the pace)
+  constexpr int kKS = 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double sh[NT / kWave];
   __shared__ double sh3[3][NT / kWave];
@@ -425,17 +448,19 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
   };
 
   const int nx = d.nx, ny = d.ny, so = d.s_off;
-  const int r0 = it.blk * kBRB;
-  const int rows = min(kBRB, ny - r0);
+  const int r0 = it.blk * d.item_rows;               // d.item_rows <= RB
+  const int rows = min(d.item_rows, ny - r0);
+  const int mt = (rows + 15) >> 4;                   // row tiles holding rows
   const int lane = tid & (kWave - 1), wave = tid / kWave;
   const int li = lane & 15, lk = lane >> 4;
   const int ntile = (nx + 15) >> 4;
   const int ny4 = (ny + 3) & ~3, nx4 = (nx + 3) & ~3;
-  // LDS layout: Ls[kBRB][ldl] | Ys[kBRB][ldy] | Bb[...]
+  // LDS layout: Ls[kBRB][ldl], reused as Ys[kBRB][ldy] once stage 1 is done | Bb[...]
   const int ldl = pad_a<F>(ny4), ldy = pad_a<F>(nx4);
+  const int lds_a = kBRB * (ldl > ldy ? ldl : ldy);
   F* Ls = reinterpret_cast<F*>(smem_raw);
-  F* Ys = Ls + kBRB * ldl;
-  F* Bb = Ys + kBRB * ldy;
+  F* Ys = Ls;
+  F* Bb = Ls + lds_a;
   const F* Lm = P.mats + (MODE == 0 || MODE == 1 ? d.by_off : d.byT_off);
   const F* Rm = P.mats + (MODE == 0 || MODE == 1 ? d.bxT_off : d.bx_off);
   const F* Xsrc = (MODE <= 1 ? P.q_net : P.Xinv) + so;
@@ -448,7 +473,7 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
   //             than its MFMA work, so double buffering would not hide it);
   //  legacy   — sizes the DMA cannot take (rows not a multiple of 16 bytes): staged
   //             through registers into padded LDS rows, in chunks if need be.
-  const int room = lds_elems - kBRB * (ldl + ldy);
+  const int room = lds_elems - lds_a;
   const bool dma_ok = (nx * (int)sizeof(F)) % 16 == 0 &&
                       ((reinterpret_cast<uintptr_t>(Xsrc) | reinterpret_cast<uintptr_t>(Rm)) & 15) == 0;
   const bool joint = dma_ok && (ny4 + nx4) * nx <= room;
@@ -462,7 +487,7 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
   // epilogue operands of this lane's outputs (MODE 0: ZE, Ztt; MODE 2: q_las; MODE 1,
   // when registers allow: G, Wtt, U, ZE, Ztt), loaded up front with the staging
   constexpr int NE = 4 * kMT * kTilesPerWave;
-  constexpr bool kPre1 = MODE == 1 && NE <= 8;
+  constexpr bool kPre1 = MODE == 1 && NE <= 8 && RB == 16;
   constexpr int NP = kPre1 ? NE : 1;
   F e0[NE], e1[NE], e2[NP], e3[NP], e4[NP];
 #pragma unroll
@@ -492,18 +517,20 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
       }
   }
 
-  V acc[kMT][kTilesPerWave];
+  V acc[kTilesPerWave][kMT];
 #pragma unroll
   for (int m = 0; m < kMT; ++m)
 #pragma unroll
-    for (int t = 0; t < kTilesPerWave; ++t) acc[m][t] = V{0, 0, 0, 0};
+    for (int t = 0; t < kTilesPerWave; ++t) acc[t][m] = V{0, 0, 0, 0};
 
   double v0 = 0.0, v1 = 0.0, v2 = 0.0;
   if (MODE == 3) v0 = -1.0 / 0.0;
   const double fwd_scale = d.dx * d.dy * d.inv_sqrt_LxLy;
   // the role-specific epilogue of one 16-column tile (rows 16m + row(lane, v))
   auto finish = [&](int t, int col, const V* a_) {
-#pragma unroll
+    // one row tile at a time: with two row tiles the forward mode's five operand loads
+    // per output would otherwise be live together and spill
+#pragma unroll 1
     for (int m = 0; m < kMT; ++m)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
@@ -551,9 +578,9 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
       for (int m = 0; m < kMT; ++m) {
         if (jt < ntile && col < nx) {
 #pragma unroll
-          for (int v = 0; v < 4; ++v) Ys[(16 * m + M::row(lane, v)) * ldy + col] = acc[m][t][v];
+          for (int v = 0; v < 4; ++v) Ys[(16 * m + M::row(lane, v)) * ldy + col] = acc[t][m][v];
         }
-        acc[m][t] = V{0, 0, 0, 0};
+        acc[t][m] = V{0, 0, 0, 0};
       }
     }
     if (nx4 > nx)
@@ -618,14 +645,11 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
       const int col = jt * 16 + li;
       const bool cin = col < nx;
       const int colx = (lk & 1) ? (col ^ swz) : col;
-#pragma unroll 4
-      for (int kk = 0; kk < ny4; kk += 4) {
-        const F bv = cin ? Xs[(kk + lk) * nx + colx] : F(0);
-#pragma unroll
-        for (int m = 0; m < kMT; ++m)
-          acc[m][t] = M::mma(Ls[(16 * m + li) * ldl + kk + lk], bv, acc[m][t]);
-      }
+      mma_k<F, kMT, kKS>(acc[t], mt, ny4,
+                         [&](int m, int k) { return Ls[(16 * m + li) * ldl + k + lk]; },
+                         [&](int k) { return cin ? Xs[(k + lk) * nx + colx] : F(0); });
     }
+    __syncthreads();  // every wave is done with L before Y overwrites it
     write_Y();
     SGH_TS(3);
     __syncthreads();  // Y complete; R has been in LDS since the joint staging
@@ -640,13 +664,9 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
       V a_[kMT];
 #pragma unroll
       for (int m = 0; m < kMT; ++m) a_[m] = V{0, 0, 0, 0};
-#pragma unroll 4
-      for (int kk = 0; kk < nx4; kk += 4) {
-        const F bv = cin ? Rs[(kk + lk) * nx + colx] : F(0);
-#pragma unroll
-        for (int m = 0; m < kMT; ++m)
-          a_[m] = M::mma(Ys[(16 * m + li) * ldy + kk + lk], bv, a_[m]);
-      }
+      mma_k<F, kMT, kKS>(a_, mt, nx4,
+                         [&](int m, int k) { return Ys[(16 * m + li) * ldy + k + lk]; },
+                         [&](int k) { return cin ? Rs[(k + lk) * nx + colx] : F(0); });
       if (cin) finish(t, col, a_);
     }
   } else if (streamed) {
@@ -685,15 +705,14 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
           const int col = jt * 16 + li;
           const bool cin = col < nx;
           const int colx = (lk & 1) ? (col ^ sw) : col;
-#pragma unroll 4
-          for (int kk = 0; kk < kn; kk += 4) {
-            const F bv = cin ? Bb[(kk + lk) * nx + colx] : F(0);
-#pragma unroll
-            for (int m = 0; m < kMT; ++m)
-              acc[m][t] = M::mma(Ls[(16 * m + li) * ldl + k0 + kk + lk], bv, acc[m][t]);
-          }
+          mma_k<F, kMT, kKS>(acc[t], mt, kn,
+                             [&](int m, int k) { return Ls[(16 * m + li) * ldl + k0 + k + lk]; },
+                             [&](int k) { return cin ? Bb[(k + lk) * nx + colx] : F(0); });
         }
-        if (j == n1 - 1) write_Y();   // read by the R jobs after the next barriers
+        if (j == n1 - 1) {   // Y, read by the R jobs after the next barriers, replaces L
+          __syncthreads();
+          write_Y();
+        }
       } else {
         const int c0 = (j - n1) * scw, cn = min(scw, nx - c0);
         const int sw = swz_of<F>(cn);
@@ -709,13 +728,9 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
           V a_[kMT];
 #pragma unroll
           for (int m = 0; m < kMT; ++m) a_[m] = V{0, 0, 0, 0};
-#pragma unroll 4
-          for (int kk = 0; kk < nx4; kk += 4) {
-            const F bv = cin ? Bb[(kk + lk) * cn + colx] : F(0);
-#pragma unroll
-            for (int m = 0; m < kMT; ++m)
-              a_[m] = M::mma(Ys[(16 * m + li) * ldy + kk + lk], bv, a_[m]);
-          }
+          mma_k<F, kMT, kKS>(a_, mt, nx4,
+                             [&](int m, int k) { return Ys[(16 * m + li) * ldy + k + lk]; },
+                             [&](int k) { return cin ? Bb[(k + lk) * cn + colx] : F(0); });
           if (cin) finish(t, col, a_);
         }
       }
@@ -738,15 +753,12 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
         if (jt >= ntile) break;
         const int col = jt * 16 + li;
         const bool cin = col < nx;
-#pragma unroll 4
-        for (int kk = 0; kk < kn; kk += 4) {
-          const F bv = cin ? Bb[(kk + lk) * ldb1 + col] : F(0);
-#pragma unroll
-          for (int m = 0; m < kMT; ++m)
-            acc[m][t] = M::mma(Ls[(16 * m + li) * ldl + kc0 + kk + lk], bv, acc[m][t]);
-        }
+        mma_k<F, kMT, kKS>(acc[t], mt, kn,
+                           [&](int m, int k) { return Ls[(16 * m + li) * ldl + kc0 + k + lk]; },
+                           [&](int k) { return cin ? Bb[(k + lk) * ldb1 + col] : F(0); });
       }
     }
+    __syncthreads();  // every wave is done with L before Y overwrites it
     write_Y();
     SGH_TS(3);
     // stage 2: out = Y · R, R staged in column chunks of cw (usually all of it)
@@ -769,13 +781,9 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
         V a_[kMT];
 #pragma unroll
         for (int m = 0; m < kMT; ++m) a_[m] = V{0, 0, 0, 0};
-#pragma unroll 4
-        for (int kk = 0; kk < nx4; kk += 4) {
-          const F bv = cin ? Bb[(kk + lk) * ldb2 + (col - c0)] : F(0);
-#pragma unroll
-          for (int m = 0; m < kMT; ++m)
-            a_[m] = M::mma(Ys[(16 * m + li) * ldy + kk + lk], bv, a_[m]);
-        }
+        mma_k<F, kMT, kKS>(a_, mt, nx4,
+                           [&](int m, int k) { return Ys[(16 * m + li) * ldy + k + lk]; },
+                           [&](int k) { return cin ? Bb[(k + lk) * ldb2 + (col - c0)] : F(0); });
         if (cin) finish(t, col, a_);
       }
     }
@@ -896,7 +904,7 @@ struct Plan {
   int t1_stride = 0;
   size_t shm = 0;
   int bud = 0;
-  int rows = 16;       // surface rows per transform item
+  int rows = 16;       // surface rows per transform item; 0: per cuboid (16 or 32), see assign_heights
   int threads = 1024;  // threads per surface workgroup (the autotuner's usual pick)
   long long total_N = 0, total_S = 0;
   // device allocations
@@ -963,7 +971,6 @@ BatchPtrs<F> ptrs(Plan* p) {
   P.metrics = p->metrics;
   P.metrics_rows = p->metrics_rows;
   P.n_cuboids = p->n;
-  P.rows = p->rows;
   return P;
 }
 
@@ -972,8 +979,9 @@ BatchPtrs<F> ptrs(Plan* p) {
 struct SurfVariant {
   int rows, threads;
 };
-// (32, 1024) is left out: its forward mode exceeds the 128 VGPRs of a 1024-thread group.
-constexpr SurfVariant kSurfVariants[] = {{16, 256}, {16, 512}, {16, 1024}, {32, 512}};
+// rows = 0: each cuboid's items hold 16 or 32 rows (assign_heights), on the 32-row kernels.
+constexpr SurfVariant kSurfVariants[] = {{16, 256}, {16, 512}, {16, 1024}, {32, 512},
+                                         {32, 1024}, {0, 512}, {0, 1024}};
 constexpr int kNumSurfVariants = sizeof(kSurfVariants) / sizeof(kSurfVariants[0]);
 
 template <typename F>
@@ -989,10 +997,12 @@ static void surf_fns(SurfFn<F>* f) {
 
 template <typename F>
 static bool surf_lookup(int rows, int threads, SurfFn<F>* f) {
+  if (rows == 0) rows = 32;  // per-cuboid heights run on the 32-row kernels
   if (rows == 16 && threads == 256) surf_fns<F, 16, 256>(f);
   else if (rows == 16 && threads == 512) surf_fns<F, 16, 512>(f);
   else if (rows == 16 && threads == 1024) surf_fns<F, 16, 1024>(f);
   else if (rows == 32 && threads == 512) surf_fns<F, 32, 512>(f);
+  else if (rows == 32 && threads == 1024) surf_fns<F, 32, 1024>(f);
   else return false;
   return true;
 }
@@ -1007,14 +1017,17 @@ static hipError_t launch_surface(const SurfFn<F>* f, int mode, unsigned nrow, co
                          args, p->shm, st);
 }
 
-// LDS of a surface workgroup: Ls[rows][ldl] + Ys[rows][ldy] + a staging buffer of `bud`
-// elements that holds X (or R) whole when it fits, else in chunks of at least 4 rows (X)
-// or 16 columns (R).
+// LDS of a surface workgroup: Ls[rows][ldl] (reused for Ys[rows][ldy]) + a staging buffer
+// of `bud` elements that holds X (or R) whole when it fits, else in chunks of at least 4
+// rows (X) or 16 columns (R).
+// Rows of the kernel tiles for a plan row setting (0: per-cuboid heights, up to 32).
+static inline int kernel_rows(int rows) { return rows == 0 ? 32 : rows; }
+
 static int surface_budget(int dtype, int rows, long long lds_bytes) {
   const int n4 = (kBMaxN + 3) & ~3;
   const int ld = dtype == SGH_F64 ? pad_a<double>(n4) : pad_a<float>(n4);
   const long long elems = lds_bytes / (dtype == SGH_F64 ? 8 : 4);
-  const long long bud = elems - 2LL * rows * ld;
+  const long long bud = elems - (long long)rows * ld;   // L and Y share their rows
   const long long need1 = 4LL * (dtype == SGH_F64 ? pad_b<double>(kBMaxN) : pad_b<float>(kBMaxN));
   const long long need2 = (long long)n4 * (dtype == SGH_F64 ? pad_b<double>(16) : pad_b<float>(16));
   return (bud < need1 || bud < need2) ? -1 : (int)bud;
@@ -1107,8 +1120,44 @@ static std::vector<double> unit_basis(int n) {
 // XCD-aware work lists: each live cuboid is pinned to one of the 8 XCDs (greedy balance of
 // its transform work) and its items sit at list positions ≡ xcd (mod 8), so all
 // workgroups of a cuboid share one L2. Short bins are padded with empty items (c = −1).
+// Per-cuboid item heights for rows = 0: start every cuboid at 16 rows and move the
+// cheapest ones (narrowest surfaces: the least MFMA work per row) to 32 rows until the
+// surface items fit one workgroup per CU (kXcds × kCusPerXcd), so a launch runs in a
+// single round. Returns true if any descriptor changed.
+static bool assign_heights(Plan* p) {
+  std::vector<int> live;
+  for (int c = 0; c < p->n; ++c)
+    if (p->hdesc[c].live) live.push_back(c);
+  std::vector<int> h(p->n, 16);
+  long long items = 0;
+  for (int c : live) items += (p->hdesc[c].ny + 15) / 16;
+  std::vector<int> order = live;
+  std::sort(order.begin(), order.end(), [&](int a, int b) {
+    const CuboidDesc &da = p->hdesc[a], &db = p->hdesc[b];
+    return da.nx != db.nx ? da.nx < db.nx : a < b;
+  });
+  const long long target = (long long)kXcds * (kCusPerXcd - 1);  // one CU per XCD of slack for binning
+  for (int c : order) {
+    if (items <= target) break;
+    const int ny = p->hdesc[c].ny;
+    items -= (ny + 15) / 16 - (ny + 31) / 32;
+    h[c] = 32;
+  }
+  bool changed = false;
+  for (int c : live) {
+    CuboidDesc& d = p->hdesc[c];
+    if (d.item_rows != h[c]) {
+      d.item_rows = h[c];
+      d.nrowblk = (d.ny + h[c] - 1) / h[c];
+      changed = true;
+    }
+  }
+  return changed;
+}
+
 static void build_xcd_items(Plan* p) {
   std::vector<double> load(kXcds, 0.0);
+  std::vector<int> count(kXcds, 0);
   std::vector<std::vector<Item>> rb(kXcds), cb(kXcds);
   std::vector<int> order;
   for (int c = 0; c < p->n; ++c)
@@ -1122,8 +1171,14 @@ static void build_xcd_items(Plan* p) {
     return ca != cb_ ? ca > cb_ : a < b;
   });
   for (int c : order) {
-    const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    // least loaded XCD that still has a CU for each of the cuboid's items; else the one
+    // with the fewest items
+    int x = -1;
+    for (int y = 0; y < kXcds; ++y)
+      if (count[y] + p->hdesc[c].nrowblk <= kCusPerXcd && (x < 0 || load[y] < load[x])) x = y;
+    if (x < 0) x = (int)(std::min_element(count.begin(), count.end()) - count.begin());
     load[x] += cost(c);
+    count[x] += p->hdesc[c].nrowblk;
     const CuboidDesc& d = p->hdesc[c];
     for (int b = 0; b < d.nrowblk; ++b) rb[x].push_back({c, b});
     for (int b = 0; b < d.ncolblk; ++b) cb[x].push_back({c, b});
@@ -1270,7 +1325,8 @@ static int add_cuboid(Plan* p, const sgh_cuboid_spec& sp, int c, hipStream_t st)
     t1.insert(t1.end(), k2.begin(), k2.end());
   }
   d.row_item0 = c * kMaxRowBlk;
-  d.nrowblk = (sp.ny + p->rows - 1) / p->rows;
+  d.item_rows = p->rows == 0 ? 16 : p->rows;  // per-cuboid heights are set when lists are built
+  d.nrowblk = (sp.ny + d.item_rows - 1) / d.item_rows;
   d.col_item0 = c * kMaxColBlk;
   d.ncolblk = (d.S + kBT - 1) / kBT;
   d.max_it = sp.max_it;
@@ -1327,8 +1383,14 @@ static int add_cuboid(Plan* p, const sgh_cuboid_spec& sp, int c, hipStream_t st)
 }
 
 static int rebuild_lists(Plan* p, hipStream_t st) {
-  build_xcd_items(p);
   hipError_t e = hipSuccess;
+  if (p->rows == 0 && assign_heights(p)) {
+    for (int c = 0; c < p->n && e == hipSuccess; ++c)
+      if (p->hdesc[c].live)
+        e = hipMemcpy(p->desc + c, &p->hdesc[c], sizeof(CuboidDesc), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail_hip(e, "sgh_batch heights");
+  }
+  build_xcd_items(p);
   if ((long long)p->hrow.size() > p->row_cap || (long long)p->hcol.size() > p->col_cap) {
     set_error("sgh_batch: work list exceeds capacity");
     return SGH_ERR_STATE;
@@ -1384,7 +1446,7 @@ int sgh_batch_pool_create(int dtype, int max_cuboids, long long state_capacity,
   p->total_S = rup64(surface_capacity);  // each packed surface array starts 256-byte aligned
   p->t1_stride = 2 * (kBMaxN + kBMaxN + max_nz);
   p->mats_cap = 2LL * 16 * kBMaxN * kBMaxN;
-  p->bud = surface_budget(dtype, p->rows, kSurfLds);
+  p->bud = surface_budget(dtype, kernel_rows(p->rows), kSurfLds);
   p->shm = (size_t)kSurfLds;
   if (p->bud < 0) {
     free_plan(p);
@@ -1670,7 +1732,7 @@ int sgh_batch_set_variant(void* plan, int rows, int threads, int lds_bytes, void
               threads);
   SGH_REQUIRE(lds_bytes >= 4096 && lds_bytes <= kLdsMax, SGH_ERR_ARG,
               "sgh_batch_set_variant: lds_bytes=%d outside [4096, %d]", lds_bytes, kLdsMax);
-  const int bud = surface_budget(p->dtype, rows, lds_bytes);
+  const int bud = surface_budget(p->dtype, kernel_rows(rows), lds_bytes);
   SGH_REQUIRE(bud >= 0, SGH_ERR_ARG,
               "sgh_batch_set_variant: %d LDS bytes cannot stage a 256-wide operand", lds_bytes);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
@@ -1683,7 +1745,8 @@ int sgh_batch_set_variant(void* plan, int rows, int threads, int lds_bytes, void
   for (int c = 0; c < p->n; ++c) {
     CuboidDesc& d = p->hdesc[c];
     if (!d.live) continue;
-    d.nrowblk = (d.ny + rows - 1) / rows;
+    d.item_rows = rows == 0 ? 16 : rows;
+    d.nrowblk = (d.ny + d.item_rows - 1) / d.item_rows;
     e = hipMemcpy(p->desc + c, &d, sizeof(CuboidDesc), hipMemcpyHostToDevice);
     if (e != hipSuccess) return fail_hip(e, "sgh_batch_set_variant");
   }
