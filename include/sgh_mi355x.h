@@ -123,6 +123,14 @@ int sgh_sep2d(int dtype, int batch, int ny, int nx, const void* L, const void* R
 int sgh_axis_contract(int dtype, int variant, long long outer, int n, long long inner,
                       const void* M, const void* in, double scale, void* out, void* stream);
 
+/* The same with a rectangular n_out × n_in matrix: out[o,k,i] (k < n_out) =
+ * scale · Σ_j M[k,j]·in[o,j,i] (j < n_in).  With the (n+1) × n node bases it evaluates
+ * the cosine series at the n+1 nodes of each axis, the node-centred reconstruction of
+ * reconstruct_temperature_DCT (sgh/physics/spectral_helpers.py:147-207). */
+int sgh_axis_contract_rect(int dtype, int variant, long long outer, int n_in, int n_out,
+                           long long inner, const void* M, const void* in, double scale,
+                           void* out, void* stream);
+
 /* ---- surface forcing -----------------------------------------------------
  * Laser flux on cell centres x_i = (i+½)dx, y_j = (j+½)dy: I0·exp(-2ρⁿ) (order n), or
  * the cell-mean Gaussian when cell_integrated != 0.  Replaces LaserProfile.flux

@@ -1,0 +1,65 @@
+"""Output-side helpers of the MI355X backend (sgh/physics/spectral_helpers.py).
+
+``reconstruct_temperature_DCT`` evaluates the modal state at the node-centred grid
+x_j = j·Lx/nx (j = 0..nx), y, z likewise, the field the reference's HDF5/XDMF writers
+store. The reference pads the modes to n+1, halves the interior ones and runs an
+unnormalised DCT-I (spectral_helpers.py:147-207); here each axis is one MFMA contraction
+with the (n+1) × n node basis N[j, k] = C_k·cos(πkj/n), on the device.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from .._lib import call, dtype_code, ptr, stream_ptr
+
+__all__ = ["node_basis", "reconstruct_temperature_DCT"]
+
+_BASES: dict = {}
+
+
+def node_basis(n: int, L: float) -> np.ndarray:
+    """N[j, k] = C_k·cos(πkj/n) for j = 0..n, k = 0..n-1 (FP64, host)."""
+    j = np.arange(n + 1)[:, None]
+    k = np.arange(n)[None, :]
+    ang = np.pi * ((k * j) % (2 * n)) / n
+    C = np.full(n, math.sqrt(2.0 / L))
+    C[0] = math.sqrt(1.0 / L)
+    return np.cos(ang) * C[None, :]
+
+
+def _basis(n, L, tdtype, device):
+    key = (n, float(L), tdtype, str(device))
+    B = _BASES.get(key)
+    if B is None:
+        B = torch.as_tensor(node_basis(n, L)).to(device=device, dtype=tdtype).contiguous()
+        _BASES[key] = B
+    return B
+
+
+def reconstruct_temperature_DCT(a, SsState):
+    """Node-centred temperature, shape (N_x+1, N_y+1, N_z+1), from modes ``a`` (nz, ny, nx).
+
+    ``SsState`` supplies the domain (``grid.nx/ny/nz`` and the extents through
+    ``grid.dx·nx`` etc.); the result stays on the device in the modes' dtype.
+    """
+    a = a.contiguous()
+    nz, ny, nx = a.shape
+    g = SsState.grid
+    Ls = (g.dx * nx, g.dy * ny, g.dz * nz)
+    code = dtype_code(a.dtype)
+    st = stream_ptr()
+    Bx, By, Bz = (_basis(n, L, a.dtype, a.device) for n, L in zip((nx, ny, nz), Ls))
+    t1 = torch.empty((nz, ny, nx + 1), dtype=a.dtype, device=a.device)
+    call("sgh_axis_contract_rect", code, 0, nz * ny, nx, nx + 1, 1, ptr(Bx), ptr(a), 1.0,
+         ptr(t1), st)
+    t2 = torch.empty((nz, ny + 1, nx + 1), dtype=a.dtype, device=a.device)
+    call("sgh_axis_contract_rect", code, 0, nz, ny, ny + 1, nx + 1, ptr(By), ptr(t1), 1.0,
+         ptr(t2), st)
+    t3 = torch.empty((nz + 1, ny + 1, nx + 1), dtype=a.dtype, device=a.device)
+    call("sgh_axis_contract_rect", code, 0, 1, nz, nz + 1, (ny + 1) * (nx + 1), ptr(Bz),
+         ptr(t2), 1.0, ptr(t3), st)
+    return t3.permute(2, 1, 0).contiguous()

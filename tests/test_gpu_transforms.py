@@ -86,3 +86,42 @@ def test_solver_state_selects_transform():
     a[0, 0, 0] = 293.0 * np.sqrt(1e-3 * 1e-3 * 5e-4)
     T = K.reconstruct_volume(a, S)
     np.testing.assert_allclose(T.cpu().numpy(), 293.0, rtol=2e-5)
+
+
+@pytest.mark.parametrize("shape_in,n_out", [((3, 9, 11), 10), ((2, 17, 1), 18), ((1, 24, 4096), 25)])
+def test_rectangular_axis_contract(shape_in, n_out):
+    rng = np.random.default_rng(7)
+    outer, n_in, inner = shape_in
+    M = rng.standard_normal((n_out, n_in))
+    X = rng.standard_normal(shape_in)
+    Mt = torch.as_tensor(M).to(DEV).contiguous()
+    Xt = torch.as_tensor(X).to(DEV).contiguous()
+    out = torch.empty((outer, n_out, inner), dtype=torch.float64, device=DEV)
+    _lib.call("sgh_axis_contract_rect", _lib.F64, MFMA, outer, n_in, n_out, inner, _lib.ptr(Mt),
+              _lib.ptr(Xt), 2.0, _lib.ptr(out), _lib.stream_ptr())
+    ref = 2.0 * np.einsum("kj,oji->oki", M, X)
+    assert rel(out.cpu().numpy(), ref) < 1e-13
+
+
+@pytest.mark.parametrize("mesh,dtype,tol", [((12, 10, 8), torch.float64, 1e-13),
+                                            ((64, 32, 16), torch.float64, 1e-13),
+                                            ((48, 40, 24), torch.float32, 2e-6)])
+def test_node_reconstruction_matches_oracle_dct1(mesh, dtype, tol):
+    """reconstruct_temperature_DCT on the device vs the oracle's padded DCT-I."""
+    from spectral_galerkin_heat_amd.core.parameters import GeomParams, MaterialParams, NumParams
+    from spectral_galerkin_heat_amd.core.vector import Vec3
+    from spectral_galerkin_heat_amd.physics import mi355x_kernels as K
+    from spectral_galerkin_heat_amd.physics.spectral_helpers import reconstruct_temperature_DCT
+    nx, ny, nz = mesh
+    size = (1.3e-3, 0.9e-3, 0.4e-3)
+    geom = GeomParams(size=Vec3(*size), n=Vec3(*mesh))
+    num = NumParams(dt=1e-6, nx=nx, ny=ny, nz=nz,
+                    dtype=np.float64 if dtype == torch.float64 else np.float32)
+    S = K.SpectralSolverState(MaterialParams(rho=7900.0, k=20.0, Cp=500.0), geom, num, None)
+    rng = np.random.default_rng(11)
+    a = rng.standard_normal((nz, ny, nx))
+    a[0, 0, 0] += 300.0
+    got = reconstruct_temperature_DCT(torch.as_tensor(a).to(DEV, dtype), S)
+    assert tuple(got.shape) == (nx + 1, ny + 1, nz + 1)
+    ref = o.reconstruct_temperature_DCT(a, o.OracleGrid(size, mesh))
+    assert rel(got.cpu().double().numpy(), ref) < tol
