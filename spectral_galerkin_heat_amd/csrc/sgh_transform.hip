@@ -23,16 +23,18 @@ namespace sgh {
 
 constexpr int kT = 256;
 constexpr int kSRows = 16;    // surface rows per sep2d workgroup (one MFMA row tile)
+constexpr int kSCols = 64;    // output columns per sep2d workgroup
 constexpr int kMaxNx = 256;   // widest surface axis handled by sep2d
 constexpr int kSChunk = 8192; // elements of the LDS buffer that streams X, then R (at most)
 constexpr int kSLds = 160 * 1024;  // gfx950 LDS per workgroup
 
 // out[b] = scale · L · (Σ_p X[b][p]) · R for an ny × nx surface, on the matrix cores.
-// Workgroup (row block, b) owns 16 output rows: stage 1 streams X (summing the nparts
-// partial surfaces on load) through LDS in K-chunks of rows and accumulates
-// Y = L[rows, :] · X; stage 2 streams R the same way and accumulates Y · R. 4 waves,
-// 16-column tiles dealt round-robin (up to 4 per wave for nx = 256). Callers sum
-// z-chunk partials first (sgh_sum_parts) so that X streams as one plane.
+// Workgroup (row block, b, column block) owns 16 output rows × kSCols output columns:
+// stage 1 streams X (summing the nparts partial surfaces on load) through LDS in
+// K-chunks of rows and accumulates the full Y = L[rows, :] · X; stage 2 streams only
+// its columns of R and accumulates Y · R[:, cols]. 4 waves, 16-column tiles dealt
+// round-robin. Callers sum z-chunk partials first (sgh_sum_parts) so that X streams as
+// one plane.
 template <typename F>
 __global__ void __launch_bounds__(kT)
 k_sep2d(int ny, int nx, const F* __restrict__ L, const F* __restrict__ R,
@@ -110,33 +112,33 @@ k_sep2d(int ny, int nx, const F* __restrict__ L, const F* __restrict__ R,
     const int r = e / (nx4 - nx), c = nx + e % (nx4 - nx);
     Ys[r * ldy + c] = F(0);
   }
-  // stage 2: out = Y · R, R streamed in K-chunks of rows
-  for (int k0 = 0; k0 < nx4; k0 += kr) {
-    const int kn = min(kr, nx4 - k0);
+  // stage 2: out[:, c0 : c0 + kSCols] = Y · R[:, c0 : ...], R streamed in K-chunks of rows
+  const int c0 = blockIdx.z * kSCols, cn = min(kSCols, nx - c0);
+  const int ldb2 = pad_b<F>(kSCols);
+  const int kr2 = (chunk / ldb2) & ~3;
+  V acc2 = V{0, 0, 0, 0};
+  const int jt2 = wave;                  // one 16-column tile per wave
+  const int col2 = c0 + jt2 * 16 + li;
+  for (int k0 = 0; k0 < nx4; k0 += kr2) {
+    const int kn = min(kr2, nx4 - k0);
     __syncthreads();
-    stage_block<F, kT, 4>(Bs, ldb, R + (long long)k0 * nx, nx, kn, nx, nx - k0, nx);
+    stage_block<F, kT, 4>(Bs, ldb2, R + (long long)k0 * nx + c0, nx, kn, kSCols, nx - k0, cn);
     __syncthreads();
-#pragma unroll
-    for (int t = 0; t < TPW; ++t) {
-      const int jt = wave + NW * t;
-      if (jt >= ntile) break;
-      const int col = jt * 16 + li;
-      const bool cin = col < nx;
+    if (jt2 * 16 < cn) {
+      const bool cin = col2 < nx;
       for (int kk = 0; kk < kn; kk += 4) {
-        const F bv = cin ? Bs[(kk + lk) * ldb + col] : F(0);
-        acc[t] = M::mma(Ys[li * ldy + k0 + kk + lk], bv, acc[t]);
+        const F bv = cin ? Bs[(kk + lk) * ldb2 + jt2 * 16 + li] : F(0);
+        acc2 = M::mma(Ys[li * ldy + k0 + kk + lk], bv, acc2);
       }
     }
   }
-#pragma unroll
-  for (int t = 0; t < TPW; ++t) {
-    const int jt = wave + NW * t;
-    const int col = jt * 16 + li;
-    if (jt >= ntile || col >= nx) continue;
+  {
+    const int col = col2;
+    if (jt2 * 16 >= cn || col >= nx) return;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int row = M::row(lane, v);
-      if (row < rows) Ob[(long long)(r0 + row) * nx + col] = scale * acc[t][v];
+      if (row < rows) Ob[(long long)(r0 + row) * nx + col] = scale * acc2[v];
     }
   }
 }
@@ -408,7 +410,7 @@ int sgh_sep2d(int dtype, int batch, int ny, int nx, const void* L, const void* R
       if (e != hipSuccess) return fail_hip(e, "sgh_sep2d LDS attribute");
       lds_raised[sizeof(F) == 8] = true;
     }
-    hipLaunchKernelGGL(k_sep2d<F>, dim3(cdiv(ny, kSRows), batch), dim3(kT), shm,
+    hipLaunchKernelGGL(k_sep2d<F>, dim3(cdiv(ny, kSRows), batch, cdiv(nx, kSCols)), dim3(kT), shm,
                        reinterpret_cast<hipStream_t>(stream), ny, nx, (const F*)L, (const F*)R,
                        (const F*)X, nparts, (F)scale, (F*)out, (int)chunk);
   });
