@@ -131,6 +131,31 @@ int sgh_axis_contract_rect(int dtype, int variant, long long outer, int n_in, in
                            long long inner, const void* M, const void* in, double scale,
                            void* out, void* stream);
 
+/* The same with M read at row stride ldm >= n_in, so a window of columns of a wider
+ * basis is used in place.  With the laser-following windows of the fine-box bases
+ * (FineMeshState.update, sgh/physics/spectral_state.py:227-235) three calls make
+ * reconstruct_temperature_box (sgh/physics/spectral_ops.py:173-179) and three make
+ * project_box_to_modes (spectral_ops.py:53-74). */
+int sgh_axis_contract_strided(int dtype, int variant, long long outer, int n_in, int n_out,
+                              long long inner, const void* M, long long ldm, const void* in,
+                              double scale, void* out, void* stream);
+
+/* ---- box-mode history ------------------------------------------------------
+ * out[z] = shift(in[z]) for every z slice of an nz × ny × nx box, bilinear,
+ * out[o] = in(o - s); outside the slice: edge value (nearest != 0) or zero.  The
+ * T_prev (nearest) and Q_prev (constant) shifts of shift_latent_heat_history
+ * (sgh/physics/spectral_ops.py:292-306, _shift_xy_3d spectral_cpu_kernels.py:302-322).
+ * in != out. */
+int sgh_shift3d_xy(int dtype, int nz, int ny, int nx, const void* in, double sy, double sx,
+                   int nearest, void* out, void* stream);
+
+/* Peaks of |Q| over a box for the truncation warning of
+ * _check_latent_box_truncation (sgh/solvers/spectral.py:545-586): out[0] = max |Q|,
+ * out[1] = max over the floor Q[0], out[2] = max over the x faces Q[:,:,0|nx-1],
+ * out[3] = max over the y faces Q[:,0|ny-1,:] (FP64, one launch). */
+int sgh_box_face_peaks(int dtype, int nz, int ny, int nx, const void* Q, double* out,
+                       void* stream);
+
 /* ---- surface forcing -----------------------------------------------------
  * Laser flux on cell centres x_i = (i+½)dx, y_j = (j+½)dy: I0·exp(-2ρⁿ) (order n), or
  * the cell-mean Gaussian when cell_integrated != 0.  Replaces LaserProfile.flux

@@ -2,7 +2,8 @@
 
 FP64 NumPy/SciPy restatement of the reference's spectral Galerkin ETD1 + Picard step
 (``SpectralSolver.step``, ``src/spectral_galerkin_heat/solvers/spectral.py:219-543``),
-grid mode (no fine sub-box), with every physics kernel it calls. Only ``tests/``,
+in grid mode and in box mode (laser-following fine sub-box), with every physics kernel
+it calls. Only ``tests/``,
 ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg may use it, as the
 checker the HIP path is compared against.
 
@@ -28,6 +29,15 @@ OracleSolver.step              solvers/spectral.py:219-543                      
                                                                                    linear limit vs Eagar–Tsai closed form
 OracleLinearSolver.step        solvers/spectral_cpu_linear.py:98-162               Eagar–Tsai closed form, superposition
 reconstruct_temperature_DCT    physics/spectral_helpers.py:147-207                 direct modal sum at nodes
+subgrid_indices                physics/spectral_helpers.py:49-83                   reference test vectors (test_spectral_helpers.py:32-54)
+OracleFineMesh                 physics/spectral_state.py:166-239                   brute-force cosines at fine centres
+reconstruct_box / project_box  physics/spectral_ops.py:53-74,173-179               project∘reconstruct = I (full box, r = 1..3);
+                                                                                   r = 1 box = cell-centred volume transforms
+shift_3d_xy                    physics/spectral_cpu_kernels.py:302-322             scipy.ndimage.shift nearest/constant;
+                                                                                   lab-frame agreement with the window slide
+box mode of OracleSolver.step  solvers/spectral.py:286-534, spectral_ops.py:182-306  r = 1 full box reproduces grid mode
+                                                                                   (test_latent_grid_mode.py:58-97); latent sign
+box_face_peaks                 solvers/spectral.py:545-586                         warning raised when the pool reaches the floor
 compute_L2_structured          io_utils/compute_L2_error.py:436-531                closed form
 =============================  ==================================================  =========
 
@@ -126,6 +136,96 @@ def reconstruct_volume(a, g):
 def project_volume(f, g):
     """modes = sqrt(dV)·DCT(f) (spectral_ops.py:94-102)."""
     return dct2(f) * g.sqrt_dV
+
+
+# ----------------------------------------------------------------------------------
+# Box mode: the laser-following fine sub-box
+# ----------------------------------------------------------------------------------
+
+
+def subgrid_indices(pos, d, n_total, n_box):
+    """(start, end, offset) of an n_box window centred on ``pos`` (spectral_helpers.py:49-83).
+
+    The centre cell is Python's round() (half to even) of pos/d clamped to the grid; the
+    window start is clamped to [0, max(0, n_total - n_box)].
+    """
+    centre = round(max(0.0, min(pos / d, n_total - 1)))
+    start = max(0, min(centre - n_box // 2, max(0, n_total - n_box)))
+    return start, start + n_box, max(0, min(centre - start, n_box - 1))
+
+
+class OracleFineMesh:
+    """Fine sub-box bases and history (FineMeshState, spectral_state.py:166-239).
+
+    Fine spacing d/r per axis; x and y cover the whole domain with a sliding window of
+    n_box cells, z covers the top ``Lz_box`` of the domain (cell centres shifted by
+    Lz − Lz_box) and does not slide. B_full[a][k, i] = C_k·cos(π·k·coord_i/L_a).
+    """
+
+    def __init__(self, size, n, refinement, box_size):
+        Lx, Ly, Lz = (float(v) for v in size)
+        nx, ny, nz = (int(v) for v in n)
+        r = int(refinement)
+        self.refinement = r
+        d = (Lx / nx, Ly / ny, Lz / nz)
+        self.d_fine = [dd / r for dd in d]
+        Lbx, Lby, Lbz = (float(v) for v in box_size)
+        self.n_totals = [int(np.ceil(L / df)) for L, df in zip((Lx, Ly, Lbz), self.d_fine)]
+        coords = [(np.arange(m) + 0.5) * df for m, df in zip(self.n_totals, self.d_fine)]
+        coords[2] = (Lz - Lbz) + coords[2]
+        modes, lengths = (nx, ny, nz), (Lx, Ly, Lz)
+        self.B_full = [c_coef(modes[a], lengths[a])[:, None]
+                       * np.cos(np.pi * np.arange(modes[a])[:, None] * coords[a][None, :]
+                                / lengths[a]) for a in range(3)]
+        self.n_box = [min(int(np.ceil(Lb / df)), m)
+                      for Lb, df, m in zip((Lbx, Lby, Lbz), self.d_fine, self.n_totals)]
+        self.start = [0, 0, 0]
+        self.dV_fine = self.d_fine[0] * self.d_fine[1] * self.d_fine[2]
+        self.T_prev = None
+        self.Q_prev = None
+
+    def update(self, x, y):
+        """Slide the x and y windows to the laser position (spectral_state.py:227-235)."""
+        for a, pos in ((0, x), (1, y)):
+            self.start[a] = subgrid_indices(float(pos), self.d_fine[a], self.n_totals[a],
+                                            self.n_box[a])[0]
+
+    def bases(self):
+        """The active window bases (x, y, z), each (modes, n_box)."""
+        return [self.B_full[a][:, self.start[a]:self.start[a] + self.n_box[a]] for a in range(3)]
+
+
+def reconstruct_box(a, fm):
+    """T on the fine box cells: Σ a·B_z B_y B_x (spectral_ops.py:173-179)."""
+    Bx, By, Bz = fm.bases()
+    return np.einsum("ZYX,Zz,Yy,Xx->zyx", a, Bz, By, Bx, optimize=True)
+
+
+def project_box(f, fm):
+    """Modes of a box field: dV_fine·Σ f·B_z B_y B_x (spectral_ops.py:53-74)."""
+    Bx, By, Bz = fm.bases()
+    return np.einsum("zyx,Zz,Yy,Xx->ZYX", f, Bz, By, Bx, optimize=True) * fm.dV_fine
+
+
+def shift_3d_xy(field, sy, sx, nearest):
+    """Order-1 (y, x) shift of every z slice (_shift_xy_3d, spectral_cpu_kernels.py:302-322)."""
+    return np.stack([shift_2d(field[z], sy, sx, nearest=nearest) for z in range(field.shape[0])])
+
+
+def box_face_peaks(Q, fm):
+    """Faces the latent source must not reach, as in ``_check_latent_box_truncation``
+    (spectral.py:545-586): returns the names whose max |Q| exceeds 1e-2 of the box peak.
+    The floor is always checked; x/y window edges only when the window is narrower than
+    the fine grid."""
+    peak = float(np.max(np.abs(Q)))
+    if peak <= 0.0:
+        return []
+    faces = {"floor (depth)": Q[0, :, :]}
+    if fm.n_box[0] < fm.n_totals[0]:
+        faces["x window edge"] = Q[:, :, (0, -1)]
+    if fm.n_box[1] < fm.n_totals[1]:
+        faces["y window edge"] = Q[:, (0, -1), :]
+    return [k for k, f in faces.items() if float(np.max(np.abs(f))) > 1e-2 * peak]
 
 
 # ----------------------------------------------------------------------------------
@@ -369,7 +469,11 @@ class OracleSolver:
         model = getattr(mat, "model", None)
         self.prop_corr = model is not None and not model.is_constant
         self.props = OracleProps.from_model(model) if self.prop_corr else None
-        self.grid_latent = float(getattr(mat, "L_f", 0.0)) > 0.0
+        fine = getattr(ctx, "fine", None)
+        self.fm = (OracleFineMesh(tuple(geom.size), tuple(geom.n), fine.refinement,
+                                  tuple(fine.box_size)) if fine is not None else None)
+        self.grid_latent = self.fm is None and float(getattr(mat, "L_f", 0.0)) > 0.0
+        self.truncation_faces = None
         if self.prop_corr:
             self.kbar = float(mat.k)
             self.abar = float(mat.rho) * float(mat.Cp)
@@ -420,7 +524,21 @@ class OracleSolver:
         self.a *= self.K
         aK = self.a
         a_temp = self.a_temp
+        fm = self.fm
+        if fm is not None:
+            fm.update(ls.x, ls.y)
         a_temp[...] = aK
+        if fm is not None:
+            # spectral_ops.py:182-195 then :292-306 (history follows the moving box)
+            if fm.T_prev is None:
+                fm.T_prev = reconstruct_box(a_temp, fm)
+                if fm.Q_prev is None:
+                    fm.Q_prev = np.zeros(fm.T_prev.shape)
+            sy = -float(vy) * dt / fm.d_fine[1]
+            sx = -float(vx) * dt / fm.d_fine[0]
+            fm.T_prev = shift_3d_xy(fm.T_prev, sy, sx, nearest=True)
+            if fm.Q_prev is not None:
+                fm.Q_prev = shift_3d_xy(fm.Q_prev, sy, sx, nearest=False)
         if self.track_picard_history:
             self.picard_history = []
 
@@ -429,7 +547,9 @@ class OracleSolver:
         h_top, h_bot, T0 = float(mat.h_conv_top), float(mat.h_conv_bottom), float(mat.T0)
 
         update_modes_etd1(aK, KK, Ct, S_top, a_temp)
-        if self.grid_latent and self.Q_latent_modes_prev is not None:
+        if fm is not None and fm.T_prev is not None and fm.Q_prev is not None:
+            add_source_term_modes(a_temp, KK, project_box(fm.Q_prev, fm))
+        elif self.grid_latent and self.Q_latent_modes_prev is not None:
             add_source_term_modes(a_temp, KK, self.Q_latent_modes_prev)
         if h_top > 0:
             Ts0 = reconstruct_surface(a_temp, g)
@@ -451,7 +571,7 @@ class OracleSolver:
             S_bot_raw = g.dct_scale * dct2(q)
 
         iter_k, rel, conv = -1, float("nan"), False
-        Q_modes = None
+        Q_modes = Q_raw = None
         for iter_k in range(self.max_picard_iter):
             a_old = a_temp.copy()
             Ts = reconstruct_surface(a_old, g)
@@ -466,8 +586,11 @@ class OracleSolver:
                 S_top_raw = S_las - g.dct_scale * dct2(self.q_evap_buffer)
                 if h_top > 0:
                     S_top_raw = S_top_raw - g.dct_scale * dct2(q_conv)
-            Q_modes = None
-            if self.grid_latent:
+            Q_modes = Q_raw = None
+            if fm is not None and fm.T_prev is not None:
+                Q_raw = self._latent(reconstruct_box(a_old, fm), fm.T_prev)
+                Q_modes = project_box(Q_raw, fm)
+            elif self.grid_latent:
                 T_full = reconstruct_volume(a_old, g)
                 Q_modes = project_volume(self._latent(T_full, self.T_prev_full), g)
             update_modes_etd1(aK, KK, Ct, S_top_raw, a_temp)
@@ -506,6 +629,17 @@ class OracleSolver:
         self.a[...] = a_temp
         if self.prop_corr or self.grid_latent:
             self.T_prev_full = reconstruct_volume(self.a, g)
+        if fm is not None:
+            # spectral_ops.py:198-207 and spectral.py:524-534
+            fm.T_prev = reconstruct_box(a_temp, fm)
+            if fm.Q_prev is None:
+                fm.Q_prev = np.zeros(fm.T_prev.shape)
+            if Q_raw is not None:
+                fm.Q_prev = Q_raw.copy()
+                if self.truncation_faces is None:
+                    hit = box_face_peaks(Q_raw, fm)
+                    if hit:
+                        self.truncation_faces = hit
         return self, {"T_surface_max": float(T_temp.max()), "P_laser": float(P_laser),
                       "n_evap_iter": iter_k + 1, "picard_converged": conv,
                       "picard_rel_err": rel}

@@ -4,6 +4,7 @@
 // surfaces [n, m]. A "column" is the nz values sharing one surface index s = n·nx + m;
 // kernels that need a z-contraction assign one thread per column (coalesced along x)
 // and split z into chunks across blockIdx.y so small surfaces still fill the GPU.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <string>
@@ -254,6 +255,61 @@ k_shift2d(const F* __restrict__ in, int ny, int nx, double sy, double sx,
             ty * (w0x * in[b * nx + c] + tx * in[b * nx + d]);
     if (!inr) val = F(0);
     out[s] = (base ? base[s] : F(0)) + alpha * val;
+  }
+}
+
+// The same translate on every z slice of a box: 'nearest' clamps the source to the
+// slice (edge value), otherwise outside the slice is zero.  One thread per output
+// element; z is blockIdx.y so each block stays within a slice.
+template <typename F>
+__global__ void __launch_bounds__(kThreads)
+k_shift3d_xy(const F* __restrict__ in, int ny, int nx, double sy, double sx, int nearest,
+             F* __restrict__ out) {
+  const int S = nx * ny;
+  const long long zoff = (long long)blockIdx.y * S;
+  const F* src = in + zoff;
+  for (int s = blockIdx.x * kThreads + threadIdx.x; s < S; s += gridDim.x * kThreads) {
+    const int oy = s / nx, ox = s - oy * nx;
+    const double py = oy - sy, px = ox - sx;
+    const double fy = floor(py), fx = floor(px);
+    const F ty = (F)(py - fy), tx = (F)(px - fx);
+    const int iy0 = (int)fy, ix0 = (int)fx;
+    const bool inr = (py >= 0.0) && (py <= ny - 1) && (px >= 0.0) && (px <= nx - 1);
+    const int a = min(max(iy0, 0), ny - 1), b = min(max(iy0 + 1, 0), ny - 1);
+    const int c = min(max(ix0, 0), nx - 1), d = min(max(ix0 + 1, 0), nx - 1);
+    const F w0y = F(1) - ty, w0x = F(1) - tx;
+    F val = w0y * (w0x * src[a * nx + c] + tx * src[a * nx + d]) +
+            ty * (w0x * src[b * nx + c] + tx * src[b * nx + d]);
+    if (!nearest && !inr) val = F(0);
+    out[zoff + s] = val;
+  }
+}
+
+// max |Q| over the box, its floor (z = 0), its x faces and its y faces; one block.
+template <typename F>
+__global__ void __launch_bounds__(1024)
+k_box_face_peaks(const F* __restrict__ Q, int nz, int ny, int nx, double* __restrict__ out) {
+  __shared__ double sh[1024 / kWave];
+  const long long S = (long long)nx * ny, N = S * nz;
+  double all = 0.0, floor_ = 0.0, xf = 0.0, yf = 0.0;
+  for (long long e = threadIdx.x; e < N; e += 1024) {
+    const double v = fabs((double)Q[e]);
+    const long long s = e % S;
+    const int j = (int)(s / nx), i = (int)(s - (long long)j * nx);
+    all = v > all ? v : all;
+    if (e < S) floor_ = v > floor_ ? v : floor_;
+    if (i == 0 || i == nx - 1) xf = v > xf ? v : xf;
+    if (j == 0 || j == ny - 1) yf = v > yf ? v : yf;
+  }
+  const double r0 = block_max(all, sh);
+  const double r1 = block_max(floor_, sh);
+  const double r2 = block_max(xf, sh);
+  const double r3 = block_max(yf, sh);
+  if (threadIdx.x == 0) {
+    out[0] = r0;
+    out[1] = r1;
+    out[2] = r2;
+    out[3] = r3;
   }
 }
 
@@ -567,6 +623,36 @@ int sgh_shift2d(int dtype, int ny, int nx, const void* in, double sy, double sx,
                        (const F*)in, ny, nx, sy, sx, (const F*)base, (F)alpha, (F*)out);
   });
   SGH_CHECK_LAUNCH("sgh_shift2d");
+  return SGH_OK;
+}
+
+int sgh_shift3d_xy(int dtype, int nz, int ny, int nx, const void* in, double sy, double sx,
+                   int nearest, void* out, void* stream) {
+  SGH_REQUIRE(nz > 0 && ny > 0 && nx > 0 && in && out && in != out, SGH_ERR_ARG,
+              "sgh_shift3d_xy: bad args");
+  SGH_REQUIRE(nz <= 65535 && (long long)nx * ny <= (1LL << 31) - 1, SGH_ERR_ARG,
+              "sgh_shift3d_xy: box too large");
+  SGH_REQUIRE(std::isfinite(sy) && std::isfinite(sx), SGH_ERR_ARG,
+              "sgh_shift3d_xy: non-finite shift");
+  const int S = nx * ny;
+  SGH_DISPATCH(dtype, {
+    hipLaunchKernelGGL(k_shift3d_xy<F>, dim3(std::min(grid_for(S), 1024u), (unsigned)nz),
+                       dim3(kThreads), 0, STREAM(stream), (const F*)in, ny, nx, sy, sx,
+                       nearest, (F*)out);
+  });
+  SGH_CHECK_LAUNCH("sgh_shift3d_xy");
+  return SGH_OK;
+}
+
+int sgh_box_face_peaks(int dtype, int nz, int ny, int nx, const void* Q, double* out,
+                       void* stream) {
+  SGH_REQUIRE(nz > 0 && ny > 0 && nx > 0 && Q && out, SGH_ERR_ARG,
+              "sgh_box_face_peaks: bad args");
+  SGH_DISPATCH(dtype, {
+    hipLaunchKernelGGL(k_box_face_peaks<F>, dim3(1), dim3(1024), 0, STREAM(stream),
+                       (const F*)Q, nz, ny, nx, out);
+  });
+  SGH_CHECK_LAUNCH("sgh_box_face_peaks");
   return SGH_OK;
 }
 

@@ -367,20 +367,21 @@ using namespace sgh;
 
 namespace {
 
+// M is n_out × n_in with row stride ldm (ldm > n_in: a column window of a wider basis).
 template <typename F, int VAR>
 void launch_axis_mfma(long long outer, int n_in, int n_out, long long inner, const F* M,
-                      const F* in, F scale, F* out, hipStream_t st) {
+                      long long ldm, const F* in, F scale, F* out, hipStream_t st) {
   if (inner == 1) {
     // Out[o, k] = Σ_j In[o, j]·M[k, j]: A = In (outer × n_in), B[j][k] = M[k][j] (k-fast)
     dim3 grid(cdiv(n_out, kGN), cdiv(outer, kGM), 1);
     hipLaunchKernelGGL((k_gemm_mfma<F, VAR, false>), grid, dim3(kT), 0, st, (int)outer, n_out,
-                       n_in, in, 0LL, (long long)n_in, M, 0LL, 1LL, (long long)n_in, out, 0LL,
+                       n_in, in, 0LL, (long long)n_in, M, 0LL, 1LL, ldm, out, 0LL,
                        (long long)n_out, scale);
   } else {
     // Out_o[k, i] = Σ_j M[k, j]·In_o[j, i]: A = M, B = In_o (column-fast), batched over o
     dim3 grid(cdiv(inner, kGN), cdiv(n_out, kGM), (unsigned)outer);
     hipLaunchKernelGGL((k_gemm_mfma<F, VAR, true>), grid, dim3(kT), 0, st, n_out, (int)inner,
-                       n_in, M, 0LL, (long long)n_in, in, (long long)n_in * inner, inner, 1LL,
+                       n_in, M, 0LL, ldm, in, (long long)n_in * inner, inner, 1LL,
                        out, (long long)n_out * inner, inner, scale);
   }
 }
@@ -418,11 +419,13 @@ int sgh_sep2d(int dtype, int batch, int ny, int nx, const void* L, const void* R
   return SGH_OK;
 }
 
-int sgh_axis_contract_rect(int dtype, int variant, long long outer, int n_in, int n_out,
-                           long long inner, const void* M, const void* in, double scale,
-                           void* out, void* stream) {
+int sgh_axis_contract_strided(int dtype, int variant, long long outer, int n_in, int n_out,
+                              long long inner, const void* M, long long ldm, const void* in,
+                              double scale, void* out, void* stream) {
   SGH_REQUIRE(outer > 0 && n_in > 0 && n_out > 0 && inner > 0 && M && in && out && in != out,
               SGH_ERR_ARG, "sgh_axis_contract: bad args");
+  SGH_REQUIRE(ldm >= n_in && ldm <= (1LL << 31) - 1, SGH_ERR_ARG,
+              "sgh_axis_contract: ldm=%lld must be in [n_in=%d, 2^31)", ldm, n_in);
   SGH_REQUIRE(variant == SGH_XFORM_MFMA || variant == SGH_XFORM_BF16X3 ||
                   variant == SGH_XFORM_VALU,
               SGH_ERR_ARG, "sgh_axis_contract: unknown variant %d", variant);
@@ -436,10 +439,10 @@ int sgh_axis_contract_rect(int dtype, int variant, long long outer, int n_in, in
                 SGH_ERR_ARG, "sgh_axis_contract: outer/inner too large");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   if (variant == SGH_XFORM_BF16X3) {
-    launch_axis_mfma<float, 1>(outer, n_in, n_out, inner, (const float*)M, (const float*)in,
-                               (float)scale, (float*)out, st);
+    launch_axis_mfma<float, 1>(outer, n_in, n_out, inner, (const float*)M, ldm,
+                               (const float*)in, (float)scale, (float*)out, st);
   } else if (variant == SGH_XFORM_MFMA) {
-    SGH_DISPATCH(dtype, launch_axis_mfma<F, 0>(outer, n_in, n_out, inner, (const F*)M,
+    SGH_DISPATCH(dtype, launch_axis_mfma<F, 0>(outer, n_in, n_out, inner, (const F*)M, ldm,
                                                (const F*)in, (F)scale, (F*)out, st));
   } else {
     SGH_DISPATCH(dtype, {
@@ -447,11 +450,11 @@ int sgh_axis_contract_rect(int dtype, int variant, long long outer, int n_in, in
         dim3 grid(cdiv(n_out, kTN), cdiv(outer, kTM), 1);
         hipLaunchKernelGGL(k_gemm_strided<F>, grid, dim3(kT), 0, st, (int)outer, n_out, n_in,
                            (const F*)in, 0LL, (long long)n_in, 1LL, (const F*)M, 0LL, 1LL,
-                           (long long)n_in, (F*)out, 0LL, (long long)n_out, 1LL, (F)scale);
+                           ldm, (F*)out, 0LL, (long long)n_out, 1LL, (F)scale);
       } else {
         dim3 grid(cdiv(inner, kTN), cdiv(n_out, kTM), (unsigned)outer);
         hipLaunchKernelGGL(k_gemm_strided<F>, grid, dim3(kT), 0, st, n_out, (int)inner, n_in,
-                           (const F*)M, 0LL, (long long)n_in, 1LL, (const F*)in,
+                           (const F*)M, 0LL, ldm, 1LL, (const F*)in,
                            (long long)n_in * inner, inner, 1LL, (F*)out,
                            (long long)n_out * inner, inner, 1LL, (F)scale);
       }
@@ -459,6 +462,13 @@ int sgh_axis_contract_rect(int dtype, int variant, long long outer, int n_in, in
   }
   SGH_CHECK_LAUNCH("sgh_axis_contract");
   return SGH_OK;
+}
+
+int sgh_axis_contract_rect(int dtype, int variant, long long outer, int n_in, int n_out,
+                           long long inner, const void* M, const void* in, double scale,
+                           void* out, void* stream) {
+  return sgh_axis_contract_strided(dtype, variant, outer, n_in, n_out, inner, M, n_in, in,
+                                   scale, out, stream);
 }
 
 int sgh_axis_contract(int dtype, int variant, long long outer, int n, long long inner,

@@ -8,8 +8,10 @@ Every function calls a gfx950 HIP kernel through the C ABI (``_lib``); none fall
 to host arithmetic. ``picard_step`` is the fused per-step driver the ``SpectralSolver``
 uses on this backend.
 
-Grid mode only (no laser-following fine sub-box): box mode is not implemented here and
-raises ``NotImplementedError`` at state construction.
+Both latent-heat modes: grid mode (latent source on the coarse grid, projected by the
+3-D transforms) and box mode (a laser-following fine sub-box whose bases stay resident
+in HBM; each step's window is a column offset read in place by the strided MFMA
+contraction).
 """
 
 from __future__ import annotations
@@ -200,19 +202,142 @@ class SpectralGrid:
         self.coords_rec = None
 
 
-class FineMeshState:
-    """Laser-following fine sub-box (box mode). Not implemented on this backend."""
+def _cosine_at(n_modes, L, coords):
+    """B[k, i] = C_k·cos(π·k·coords_i/L) (FP64, host)."""
+    C = np.full(n_modes, math.sqrt(2.0 / L))
+    C[0] = math.sqrt(1.0 / L)
+    return C[:, None] * np.cos(np.pi * np.arange(n_modes)[:, None] * coords[None, :] / L)
 
-    def __init__(self, *args, **kwargs):
-        raise NotImplementedError(
-            "box mode (fine_mesh) is not implemented in the MI355X backend; omit the "
-            "fine_mesh section to run in grid mode")
+
+class FineMeshState:
+    """Laser-following fine sub-box (spectral_state.py:166-239), resident on the device.
+
+    Fine spacing ``d/refinement`` per axis. x and y: the bases over the whole fine grid
+    (``modes × n_total``, and their transposes) stay in HBM; ``update`` only moves the
+    window start, which the contractions read in place at row stride ``n_total``. z: the
+    top ``box_size.z`` of the domain (cell centres shifted by ``Lz − Lz_box``), fixed.
+    ``T_prev`` / ``Q_prev`` are the box history ``(nz_box, ny_box, nx_box)``.
+    """
+
+    def __init__(self, geom, grid, fine, dtype=np.float32):
+        from .spectral_helpers import _calculate_subgrid_indices
+        self._window = _calculate_subgrid_indices
+        self.dtype = dtype
+        td = _tdtype(dtype)
+        r = int(fine.refinement)
+        if r < 1:
+            raise ValueError("fine_mesh.refinement must be >= 1")
+        self.refinement = r
+        dx, dy, dz = (float(v) for v in geom.d)
+        Lx, Ly, Lz = (float(v) for v in geom.size)
+        Lbx, Lby, Lbz = (float(v) for v in fine.box_size)
+        if not 0.0 < Lbz <= Lz * (1.0 + 1e-12):
+            raise ValueError("fine_mesh.box_size z must lie in (0, Lz]")
+        self.dx_fine, self.dy_fine, self.dz_fine = dx / r, dy / r, dz / r
+        d_fine = (self.dx_fine, self.dy_fine, self.dz_fine)
+        self.n_fine_totals = [int(np.ceil(L / d)) for L, d in zip((Lx, Ly, Lbz), d_fine)]
+        coords = [(np.arange(n) + 0.5) * d for n, d in zip(self.n_fine_totals, d_fine)]
+        self.coords_fine = coords
+        z_global = (Lz - Lbz) + coords[2]
+        nx, ny, nz = (int(v) for v in geom.n)
+        self.nx_box = min(int(np.ceil(Lbx / self.dx_fine)), self.n_fine_totals[0])
+        self.ny_box = min(int(np.ceil(Lby / self.dy_fine)), self.n_fine_totals[1])
+        self.nz_box = min(int(np.ceil(Lbz / self.dz_fine)), self.n_fine_totals[2])
+        Bx = _cosine_at(nx, Lx, coords[0])
+        By = _cosine_at(ny, Ly, coords[1])
+        Bz = _cosine_at(nz, Lz, z_global)[:, :self.nz_box]
+        self.Bx_full, self.By_full, self.Bz = (_dev(b, dtype) for b in (Bx, By, Bz))
+        self.BxT_full, self.ByT_full, self.BzT = (_dev(np.ascontiguousarray(b.T), dtype)
+                                                  for b in (Bx, By, Bz))
+        self.n_modes = (nx, ny, nz)
+        self.shape = (self.nz_box, self.ny_box, self.nx_box)
+        self.dV_fine = self.dx_fine * self.dy_fine * self.dz_fine
+        self.x_start = self.y_start = 0
+        dev = dict(dtype=td, device=_DEVICE)
+        work = max(nz * ny * self.nx_box, nz * self.ny_box * self.nx_box,
+                   self.nz_box * self.ny_box * nx, self.nz_box * ny * nx)
+        self._w = (torch.empty(work, **dev), torch.empty(work, **dev))
+        self._T = torch.empty(self.shape, **dev)        # history, and its shift target
+        self._T_alt = torch.empty(self.shape, **dev)
+        self._Q = torch.empty(self.shape, **dev)
+        self._Q_alt = torch.empty(self.shape, **dev)
+        self.T_box = torch.empty(self.shape, **dev)     # per-iteration reconstruction
+        self.Q_modes = torch.empty((nz, ny, nx), **dev)
+        self.peaks = torch.zeros(4, dtype=torch.float64, device=_DEVICE)
+        self.T_prev = None
+        self.Q_prev = None
+
+    def update(self, laser_state):
+        """Move the x/y windows to the laser position (spectral_state.py:227-235)."""
+        self.x_start = self._window(laser_state.x, self.dx_fine, self.n_fine_totals[0],
+                                    self.nx_box)[0]
+        self.y_start = self._window(laser_state.y, self.dy_fine, self.n_fine_totals[1],
+                                    self.ny_box)[0]
+
+    @property
+    def B_fine(self):
+        """The active window bases (x, y, z) as views, each ``modes × n_box``."""
+        return [self.Bx_full[:, self.x_start:self.x_start + self.nx_box],
+                self.By_full[:, self.y_start:self.y_start + self.ny_box], self.Bz]
+
+    def _col(self, t, col):
+        return t.data_ptr() + col * t.element_size()
+
+
+def _reconstruct_temperature_box(a, SsState, out=None):
+    """Temperature on the fine box cells, Σ a·B_z B_y B_x (spectral_ops.py:173-179).
+
+    Three strided MFMA contractions: x (modes → window cells, row block of Bₓᵀ at the
+    window start), y likewise, then z over all (y, x) columns."""
+    fm = SsState.fine_mesh
+    if fm is None:
+        raise RuntimeError("Fine mesh not initialized.")
+    a = a.contiguous()
+    nz, ny, nx = a.shape
+    nzb, nyb, nxb = fm.shape
+    code, xf, st = dtype_code(a.dtype), SsState.xform, stream_ptr()
+    w1, w2 = fm._w
+    if out is None:
+        out = torch.empty(fm.shape, dtype=a.dtype, device=a.device)
+    call("sgh_axis_contract_strided", code, xf, nz * ny, nx, nxb, 1,
+         fm._col(fm.BxT_full, fm.x_start * nx), nx, ptr(a), 1.0, ptr(w1), st)
+    call("sgh_axis_contract_strided", code, xf, nz, ny, nyb, nxb,
+         fm._col(fm.ByT_full, fm.y_start * ny), ny, ptr(w1), 1.0, ptr(w2), st)
+    call("sgh_axis_contract_strided", code, xf, 1, nz, nzb, nyb * nxb, ptr(fm.BzT), nz,
+         ptr(w2), 1.0, ptr(out), st)
+    return out
+
+
+def project_box_to_modes(field_box, SsState, out=None):
+    """Modes of a box field, dV_fine·Σ f·B_z B_y B_x (spectral_ops.py:53-74).
+
+    x: window columns of Bₓ read in place at row stride n_total; y likewise; z last, with
+    dV_fine folded into its epilogue."""
+    fm = SsState.fine_mesh
+    if fm is None:
+        raise RuntimeError("Fine mesh not initialized.")
+    f = field_box.contiguous()
+    if tuple(f.shape) != fm.shape:
+        raise ValueError(f"box field has shape {tuple(f.shape)}, expected {fm.shape}")
+    nzb, nyb, nxb = fm.shape
+    nx, ny, nz = fm.n_modes
+    code, xf, st = dtype_code(f.dtype), SsState.xform, stream_ptr()
+    w1, w2 = fm._w
+    if out is None:
+        out = torch.empty((nz, ny, nx), dtype=f.dtype, device=f.device)
+    call("sgh_axis_contract_strided", code, xf, nzb * nyb, nxb, nx, 1,
+         fm._col(fm.Bx_full, fm.x_start), fm.n_fine_totals[0], ptr(f), 1.0, ptr(w1), st)
+    call("sgh_axis_contract_strided", code, xf, nzb, nyb, ny, nx,
+         fm._col(fm.By_full, fm.y_start), fm.n_fine_totals[1], ptr(w1), 1.0, ptr(w2), st)
+    call("sgh_axis_contract_strided", code, xf, 1, nzb, nz, ny * nx, ptr(fm.Bz), nzb,
+         ptr(w2), float(fm.dV_fine), ptr(out), st)
+    return out
 
 
 class SolverBuffers:
     """Reusable device buffers (spectral_state.py:242-275) plus the fused-step scratch."""
 
-    def __init__(self, num, dtype):
+    def __init__(self, num, dtype, fine_mesh=None):
         td = _tdtype(dtype)
         nx, ny, nz = num.nx, num.ny, num.nz
         shp, srf = (nz, ny, nx), (ny, nx)
@@ -221,7 +346,9 @@ class SolverBuffers:
         self.a_alt = torch.empty(shp, **dev)
         self.q_evap_old = torch.zeros(srf, **dev)
         self.q_evap_buffer = torch.zeros(srf, **dev)
-        self.Q_latent_buffer = torch.zeros(shp, **dev)
+        # box mode: the latent source lives on the fine box (spectral_state.py:259-266)
+        self.Q_latent_buffer = torch.zeros(fine_mesh.shape if fine_mesh is not None else shp,
+                                           **dev)
         self.n_elements = dtype(nx * ny * nz)
         self.q_las = torch.zeros(srf, **dev)
         self.q_tmp = torch.zeros(srf, **dev)
@@ -245,14 +372,13 @@ class SpectralSolverState:
     """Device state: grid, buffers, modes ``a`` and the E/KK propagators (spectral_state.py:315-365)."""
 
     def __init__(self, phys, geom, num, fine):
-        if fine is not None:
-            FineMeshState()
         self.dtype = getattr(num, "dtype", np.float32)
         self.code = dtype_code(self.dtype)
         self.xp = None
         self.grid = SpectralGrid(geom, self.dtype)
-        self.fine_mesh = None
-        self.buffers = SolverBuffers(num, self.dtype)
+        self.fine_mesh = (FineMeshState(geom, self.grid, fine, self.dtype)
+                          if fine is not None else None)
+        self.buffers = SolverBuffers(num, self.dtype, self.fine_mesh)
         self.K, self.KK = precompute_K_KK(phys, num, geom, self.dtype)
         self.xform = transform_code(getattr(num, "transform", "mfma"), self.dtype)
         self.a = None
@@ -354,28 +480,76 @@ def compute_latent_heat_source_grid(Q_buffer, T_full, T_prev_full, phys, num, Ss
     _latent(Q_buffer, T_full, T_prev_full, phys, num)
 
 
+def _box_latent(Q_buffer, a, SsState, phys, num):
+    fm = SsState.fine_mesh
+    T_box = _reconstruct_temperature_box(a, SsState, out=fm.T_box)
+    _latent(Q_buffer, T_box, fm.T_prev, phys, num)
+
+
 def compute_latent_heat_source(Q_buffer, phys, num, SsState):
-    """Box-mode latent source; with no fine box the source is zero (spectral_ops.py:234-267)."""
-    if SsState.fine_mesh is None:
+    """Box-mode latent source at ``buffers.a_temp`` against ``T_prev``; zero with no box or
+    no history yet (spectral_ops.py:234-267)."""
+    fm = SsState.fine_mesh
+    if fm is None or fm.T_prev is None:
         Q_buffer.zero_()
         return
-    raise NotImplementedError("box mode is not implemented in the MI355X backend")
+    _box_latent(Q_buffer, SsState.buffers.a_temp, SsState, phys, num)
 
 
-def initialize_latent_heat_if_needed(SsState):
-    """No-op in grid mode (spectral_ops.py:182-195)."""
+def initialize_latent_heat_if_needed(SsState, a=None):
+    """First step only: ``T_prev`` = the box temperature of ``a`` (default
+    ``buffers.a_temp``), ``Q_prev`` = 0 (spectral_ops.py:182-195)."""
+    fm = SsState.fine_mesh
+    if fm is None or fm.T_prev is not None:
+        return
+    src = SsState.buffers.a_temp if a is None else a
+    fm.T_prev = _reconstruct_temperature_box(src, SsState, out=fm._T)
+    if fm.Q_prev is None:
+        fm._Q.zero_()
+        fm.Q_prev = fm._Q
 
 
-def update_latent_heat_history(SsState):
-    """No-op in grid mode (spectral_ops.py:198-207)."""
+def update_latent_heat_history(SsState, a=None):
+    """``T_prev`` = the box temperature of the converged modes ``a`` (default
+    ``buffers.a_temp``) (spectral_ops.py:198-207)."""
+    fm = SsState.fine_mesh
+    if fm is None:
+        return
+    src = SsState.buffers.a_temp if a is None else a
+    if fm.T_prev is None:
+        fm.T_prev = fm._T
+    _reconstruct_temperature_box(src, SsState, out=fm.T_prev)
 
 
 def shift_latent_heat_history(SsState, laser_state, num):
-    """No-op in grid mode (spectral_ops.py:292-306)."""
+    """Move the box history with the box: shift by −v·dt/d_fine in (y, x), ``T_prev`` with
+    edge values, ``Q_prev`` with zeros (spectral_ops.py:292-306)."""
+    fm = SsState.fine_mesh
+    if fm is None or fm.T_prev is None:
+        return
+    sy = -(laser_state.v[1] * num.dt) / fm.dy_fine
+    sx = -(laser_state.v[0] * num.dt) / fm.dx_fine
+    nzb, nyb, nxb = fm.shape
+    code, st = dtype_code(fm.T_prev.dtype), stream_ptr()
+    dst = fm._T_alt if fm.T_prev is fm._T else fm._T
+    call("sgh_shift3d_xy", code, nzb, nyb, nxb, ptr(fm.T_prev), float(sy), float(sx), 1,
+         ptr(dst), st)
+    fm.T_prev = dst
+    if fm.Q_prev is not None:
+        dst = fm._Q_alt if fm.Q_prev is fm._Q else fm._Q
+        call("sgh_shift3d_xy", code, nzb, nyb, nxb, ptr(fm.Q_prev), float(sy), float(sx), 0,
+             ptr(dst), st)
+        fm.Q_prev = dst
 
 
-def project_box_to_modes(field_box, SsState):
-    raise RuntimeError("Fine mesh not initialized.")
+def box_face_peaks(Q_box, SsState):
+    """(max |Q|, floor, x faces, y faces) of a box field, one device launch + one read
+    (the test of _check_latent_box_truncation, spectral.py:545-586)."""
+    fm = SsState.fine_mesh
+    nzb, nyb, nxb = fm.shape
+    call("sgh_box_face_peaks", dtype_code(Q_box.dtype), nzb, nyb, nxb, ptr(Q_box.contiguous()),
+         ptr(fm.peaks), stream_ptr())
+    return [float(v) for v in fm.peaks.cpu()]
 
 
 _PROGRAMS: dict = {}
@@ -459,6 +633,12 @@ def picard_step(solver, t, dt):
     call("sgh_reduce", code, Ssz, ptr(b.q_las), 0, ptr(b.scal), st)
 
     call("sgh_mul_inplace", code, N, ptr(S.a), ptr(S.K), st)
+    fm = S.fine_mesh
+    if fm is not None:
+        # box mode: move the window, seed the history on the first step, shift it
+        fm.update(ls)
+        initialize_latent_heat_if_needed(S, S.a)
+        shift_latent_heat_history(S, ls, num)
     if solver.track_picard_history:
         solver.picard_history = []
 
@@ -467,9 +647,13 @@ def picard_step(solver, t, dt):
          float(vx) * dt / g.dx, ptr(b.q_las), -1.0, ptr(b.q_tmp), st)
     _sep2d(b.q_tmp, g.By, g.BxT, dxdy, out=b.S_top)
     a_cur = b.a_temp
-    replay = solver._grid_latent and solver._Q_latent_modes_prev is not None
+    Q_warm = None
+    if fm is not None and fm.T_prev is not None and fm.Q_prev is not None:
+        Q_warm = project_box_to_modes(fm.Q_prev, S, out=fm.Q_modes)
+    elif solver._grid_latent and solver._Q_latent_modes_prev is not None:
+        Q_warm = solver._Q_latent_modes_prev
     call("sgh_etd1_update", code, nx, ny, nz, ptr(S.a), ptr(S.KK), ptr(g.w_top), ptr(b.S_top),
-         None, None, ptr(solver._Q_latent_modes_prev) if replay else None, ptr(a_cur), st)
+         None, None, ptr(Q_warm), ptr(a_cur), st)
     h_top, h_bot, T0 = float(mat.h_conv_top), float(mat.h_conv_bottom), float(mat.T0)
     if h_top > 0:
         _recon_face(a_cur, g.w_top, S, out=b.T_s)
@@ -499,6 +683,7 @@ def picard_step(solver, t, dt):
     tol = float(solver.convergence_tol)
     iter_k, rel, conv = -1, float("nan"), False
     Q_modes = None
+    box_q = False
     a_raw = None
     for iter_k in range(solver.max_picard_iter):
         a_old = a_cur
@@ -510,7 +695,11 @@ def picard_step(solver, t, dt):
              h_top, T0, ptr(prog), kbar, ptr(b.q_evap_buffer), ptr(b.q_tmp), st)
         _sep2d(b.q_tmp, g.By, g.BxT, dxdy, out=b.S_top)
         Q_modes = None
-        if solver._grid_latent:
+        if fm is not None and fm.T_prev is not None:
+            _box_latent(b.Q_latent_buffer, a_old, S, mat, num)
+            Q_modes = project_box_to_modes(b.Q_latent_buffer, S, out=fm.Q_modes)
+            box_q = True
+        elif solver._grid_latent:
             T_full = reconstruct_volume(a_old, S)
             _latent(b.Q_latent_buffer, T_full, solver._T_prev_full, mat, num)
             Q_modes = project_volume(b.Q_latent_buffer, S)
@@ -557,6 +746,14 @@ def picard_step(solver, t, dt):
     S.a.copy_(a_cur)
     if solver._property_correction or solver._grid_latent:
         solver._T_prev_full.copy_(reconstruct_volume(S.a, S))
+    if fm is not None:
+        update_latent_heat_history(S, S.a)
+        if fm.Q_prev is None:
+            fm._Q.zero_()
+            fm.Q_prev = fm._Q
+        if box_q:
+            fm.Q_prev.copy_(b.Q_latent_buffer)
+            _check_box_truncation(solver, S, b.Q_latent_buffer)
     metrics = {
         "T_surface_max": b.scal[1].clone(),
         "P_laser": b.scal[0] * dxdy,
@@ -565,3 +762,30 @@ def picard_step(solver, t, dt):
         "picard_rel_err": rel,
     }
     return S, metrics
+
+
+def _check_box_truncation(solver, S, Q_box):
+    """Warn once per run when the latent source reaches the box floor or a sliding window
+    edge (> 1e-2 of its peak): the pool has outgrown the box (spectral.py:545-586)."""
+    if getattr(solver, "_latent_truncation_warned", False):
+        return
+    fm = S.fine_mesh
+    peak, floor_, xf, yf = box_face_peaks(Q_box, S)
+    if peak <= 0.0:
+        return
+    faces = {"floor (depth)": floor_}
+    if fm.nx_box < fm.n_fine_totals[0]:
+        faces["x window edge"] = xf
+    if fm.ny_box < fm.n_fine_totals[1]:
+        faces["y window edge"] = yf
+    hit = [name for name, v in faces.items() if v > 1e-2 * peak]
+    if not hit:
+        return
+    import logging
+    logging.getLogger("spectral_galerkin_heat_amd.solvers.spectral").warning(
+        "Latent-heat source reaches the fine-mesh box boundary (%s): the melt pool extends "
+        "beyond the refined sub-box, so its latent heat is truncated. Enlarge "
+        "fine_mesh.box_size (currently %.3g x %.3g x %.3g mm) to enclose the pool.",
+        ", ".join(hit), fm.nx_box * fm.dx_fine * 1e3, fm.ny_box * fm.dy_fine * 1e3,
+        fm.nz_box * fm.dz_fine * 1e3)
+    solver._latent_truncation_warned = True

@@ -16,9 +16,22 @@ import torch
 
 from .._lib import call, dtype_code, ptr, stream_ptr
 
-__all__ = ["node_basis", "reconstruct_temperature_DCT"]
+__all__ = ["_calculate_subgrid_indices", "node_basis", "reconstruct_temperature_DCT"]
 
 _BASES: dict = {}
+
+
+def _calculate_subgrid_indices(pos, dx, n_total_fine, n_box):
+    """Window of ``n_box`` fine cells centred on ``pos`` (spectral_helpers.py:49-83).
+
+    Returns ``(start, end, offset)``: the centre cell is ``round(pos/dx)`` (Python's
+    round, half to even) clamped to the fine grid, the start is clamped so the window
+    stays inside the grid when it fits (start 0 otherwise), and ``offset`` is the centre
+    cell's index inside the window, clamped to it.
+    """
+    centre = round(max(0.0, min(float(pos) / float(dx), n_total_fine - 1)))
+    start = max(0, min(centre - n_box // 2, max(0, n_total_fine - n_box)))
+    return start, start + n_box, max(0, min(centre - start, n_box - 1))
 
 
 def node_basis(n: int, L: float) -> np.ndarray:

@@ -3,8 +3,9 @@
 Mirrors ``SpectralSolver`` (sgh/solvers/spectral.py:38-628): the same constructor
 arguments and Picard-parameter precedence (explicit > config > defaults ω = 0.1,
 tol = 1e-4, cap = 30; ``_resolve_picard_params`` :107-123), the same initial condition
-(``a[0,0,0] = T0·√(LxLyLz)``, :176-180), grid-mode latent and property-correction
-switches (:185-195), ``set_state`` (:588-619) and the metrics dictionary of ``step``
+(``a[0,0,0] = T0·√(LxLyLz)``, :176-180), grid-mode / box-mode latent and
+property-correction switches (:185-195), the once-per-run box truncation warning
+(:545-586), ``set_state`` (:588-619) and the metrics dictionary of ``step``
 (:536-543). The step itself runs through ``backend.kernels.picard_step`` -- on the
 MI355X backend, a sequence of fused HIP kernels with one residual read per iteration.
 """
@@ -44,6 +45,7 @@ class SpectralSolver(HeatSolver):
         self._kbar = None
         self._abar = None
         self._picard_cap_warned = False
+        self._latent_truncation_warned = False
         self._Q_latent_modes_prev = None
         self._profile = build_laser_profile("gaussian")
 
@@ -91,6 +93,7 @@ class SpectralSolver(HeatSolver):
         if self._property_correction or self._grid_latent:
             self._T_prev_full = xp.full((num.nz, num.ny, num.nx), dtype(T0), dtype=dtype)
         self._Q_latent_modes_prev = None
+        self._latent_truncation_warned = False
         if self._property_correction and self._grid_latent:
             logger.warning(
                 "Temperature-dependent run in grid mode (no fine_mesh): the sharp latent-heat "

@@ -16,8 +16,9 @@ from .core.laser import RasterLaserPath, SingleTrackLaser
 from .core.parameters import SimulationContext
 
 __all__ = [
-    "MATERIAL_316L", "batch_uneven_cuboids", "make_context", "single_track_constant",
-    "single_track_tdep", "raster_multitrack", "serving_request",
+    "MATERIAL_316L", "MATERIAL_316L_LATENT", "batch_uneven_cuboids", "make_context",
+    "single_track_box", "single_track_constant", "single_track_tdep", "tdep_track_box",
+    "raster_multitrack", "serving_request",
 ]
 
 R_GAS = 8.314462618
@@ -28,6 +29,16 @@ MATERIAL_316L = {
     "L_f": 0.0, "T_solidus": 0.0, "T_liquidus": 0.0,
     "Pa": 101325.0, "T_boil": 3090.0, "DeltaH_LV": 7.45e6, "R_v": R_GAS / M_316L,
     "h_conv_top": 0.0, "h_conv_bottom": 0.0,
+}
+
+
+# Constant 316L at T0 = 293 K with latent heat, bottom convection and the evaporation
+# constants of the reference's example configurations (02_single_track.yaml).
+MATERIAL_316L_LATENT = {
+    "name": "316L", "rho": 7957.5, "k": 13.851, "Cp": 497.89, "T0": 293.0,
+    "L_f": 2.677e5, "T_solidus": 1674.15, "T_liquidus": 1697.15,
+    "Pa": 101325.0, "T_boil": 3090.0, "DeltaH_LV": 7.416e6, "R_v": 150.774,
+    "h_conv_top": 0.0, "h_conv_bottom": 3000.0,
 }
 
 
@@ -134,3 +145,42 @@ def serving_request(rng, dtype="float32"):
     (cfg, lp), = batch_uneven_cuboids(n=1, seed=int(rng.integers(1 << 31)), n_steps=n_steps,
                                       dtype=dtype)
     return cfg, lp
+
+
+def single_track_box(mesh=(80, 48, 120), n_steps=140, dtype="float64", refinement=2,
+                     box_size=(0.6e-3, 0.6e-3, 0.07e-3), size=(0.6e-3, 0.4e-3, 0.15e-3)):
+    """Box-mode single track (the reference's 02_single_track.yaml case).
+
+    0.6×0.4×0.15 mm cuboid, 80×48×120 modes, constant 316L with latent heat and bottom
+    convection (h = 3000 W/m²K); 120 W, r = 60 µm, A = 0.3, 0.4 m/s along x at
+    y = 0.2 mm from x = 0.15 to 0.5 mm; latent heat on a laser-following fine box
+    (refinement 2, 0.6×0.6×0.07 mm, clipped to the domain); Δt = 5e-6 s, 140 steps,
+    Picard ω 0.1, tol 1e-4, cap 30 (the solver defaults).
+    """
+    cfg = _cfg(size, mesh, 5e-6, n_steps, MATERIAL_316L_LATENT, 60e-6, 0.30, 120.0, 1e-4, 30,
+               dtype)
+    cfg["fine_mesh"] = {"refinement": int(refinement), "box_size": list(box_size)}
+    y = 0.5 * size[1]
+    return cfg, SingleTrackLaser(0.25 * size[0], min(0.5e-3, 0.85 * size[0]), y, 0.4, 120.0)
+
+
+def tdep_track_box(mesh=(100, 60, 96), n_steps=50, dtype="float64", refinement=2,
+                   box_size=(0.2e-3, 0.15e-3, 0.04e-3), size=(0.5e-3, 0.3e-3, 0.12e-3)):
+    """Box mode with temperature-dependent properties (the reference's
+    03_nonlinear_tdep.yaml case): k, ρ, cp solid/liquid expressions of T blended by the
+    liquid fraction, latent heat on a fine box (refinement 2, 0.2×0.15×0.04 mm), bottom
+    convection; 24 W, r = 30 µm, A = 0.3, 0.15 m/s from x = 0.2 mm at mid-y;
+    Δt = 4e-6 s, Picard ω 0.31, tol 1e-4, cap 80.
+    """
+    mat = copy.deepcopy(MATERIAL_316L_LATENT)
+    mat.update({
+        "k": {"solid": "9.248 + 0.01571 * T", "liquid": "12.41 + 0.003279 * T",
+              "reference": 24.5075},
+        "rho": {"solid": "8084.2 - 0.42086 * T - 3.8942e-5 * T**2",
+                "liquid": "7432.7 + 0.039338 * T - 1.8007e-4 * T**2",
+                "reference": 6896.2365},
+        "Cp": {"solid": "458.98 + 0.1328 * T", "liquid": "769.86", "reference": 633.8752},
+    })
+    cfg = _cfg(size, mesh, 4e-6, n_steps, mat, 30e-6, 0.30, 24.0, 1e-4, 80, dtype, omega=0.31)
+    cfg["fine_mesh"] = {"refinement": int(refinement), "box_size": list(box_size)}
+    return cfg, SingleTrackLaser(0.4 * size[0], 0.8 * size[0], 0.5 * size[1], 0.15, 24.0)

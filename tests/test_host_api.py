@@ -206,3 +206,19 @@ def test_scan_server_rejects_a_request_that_never_fits():
     req = Request(rid=7, ctx=None, n_steps=4, arrival=0.0)
     with pytest.raises(RuntimeError, match="request 7 does not fit"):
         srv.serve([req], clock=lambda: 0.0, sleep=lambda s: None)
+
+
+@pytest.mark.parametrize("pos,n_box,expected", [(5.0, 4, (3, 7, 2)), (0.0, 4, (0, 4, 0)),
+                                                (100.0, 4, (16, 20, 3)), (5.0, 30, (0, 30, 5))])
+def test_box_window_reference_vectors(pos, n_box, expected):
+    # tests/unit_tests/physics/test_spectral_helpers.py:32-54
+    from spectral_galerkin_heat_amd.physics.spectral_helpers import _calculate_subgrid_indices
+    assert _calculate_subgrid_indices(pos, 1.0, 20, n_box) == expected
+
+
+def test_box_workloads_parse_with_a_fine_mesh():
+    from spectral_galerkin_heat_amd.workloads import make_context, single_track_box, tdep_track_box
+    ctx = make_context(*single_track_box())
+    assert ctx.fine.refinement == 2 and tuple(ctx.geom.n) == (80, 48, 120)
+    ctx = make_context(*tdep_track_box())
+    assert ctx.fine is not None and not ctx.mat.model.is_constant
