@@ -140,6 +140,20 @@ int sgh_axis_contract_strided(int dtype, int variant, long long outer, int n_in,
                               long long inner, const void* M, long long ldm, const void* in,
                               double scale, void* out, void* stream);
 
+/* Box-mode latent source in one pass over the modes: T on the box cells from the modal
+ * state a (nz × ny × nx) by three contractions, z (BzT, nz_box × nz), y (ByT, the window
+ * rows of the transposed fine y basis, row stride ldy) and x (BxT likewise, stride ldx),
+ * with Q = −ρL(f_l(T) − f_l(T_prev))/dt formed in the last contraction's epilogue, so T
+ * itself never reaches HBM.  Replaces compute_latent_heat_source
+ * (sgh/physics/spectral_ops.py:234-267) with its reconstruct_temperature_box (:173-179).
+ * prog: the property program for the blended-ρ form (core/properties.py:375-430), or
+ * NULL.  work: nz_box·ny·nx + nz_box·ny_box·nx elements. */
+int sgh_box_latent_source(int dtype, int variant, int nx, int ny, int nz, const void* a,
+                          int nxb, int nyb, int nzb, const void* BxT, long long ldx,
+                          const void* ByT, long long ldy, const void* BzT, const void* T_prev,
+                          double T_S, double T_L, double rho, double L_f, double dt,
+                          const void* prog, void* work, void* Q, void* stream);
+
 /* ---- box-mode history ------------------------------------------------------
  * out[z] = shift(in[z]) for every z slice of an nz × ny × nx box, bilinear,
  * out[o] = in(o - s); outside the slice: edge value (nearest != 0) or zero.  The
@@ -152,7 +166,8 @@ int sgh_shift3d_xy(int dtype, int nz, int ny, int nx, const void* in, double sy,
 /* Peaks of |Q| over a box for the truncation warning of
  * _check_latent_box_truncation (sgh/solvers/spectral.py:545-586): out[0] = max |Q|,
  * out[1] = max over the floor Q[0], out[2] = max over the x faces Q[:,:,0|nx-1],
- * out[3] = max over the y faces Q[:,0|ny-1,:] (FP64, one launch). */
+ * out[3] = max over the y faces Q[:,0|ny-1,:] (FP64).  out holds 4·(nz + 1) doubles:
+ * one partial row per z slice follows the result. */
 int sgh_box_face_peaks(int dtype, int nz, int ny, int nx, const void* Q, double* out,
                        void* stream);
 

@@ -40,6 +40,8 @@ SIGNATURES = {
     "sgh_axis_contract_strided": [i32, i32, i64, i32, i32, i64, vp, i64, vp, f64, vp, vp],
     "sgh_shift3d_xy": [i32, i32, i32, i32, vp, f64, f64, i32, vp, vp],
     "sgh_box_face_peaks": [i32, i32, i32, i32, vp, vp, vp],
+    "sgh_box_latent_source": [i32, i32, i32, i32, i32, vp, i32, i32, i32, vp, i64, vp, i64, vp,
+                              vp, f64, f64, f64, f64, f64, vp, vp, vp, vp],
     "sgh_laser_flux": [i32, i32, i32, f64, f64, f64, f64, f64, f64, f64, f64, i32, vp, vp],
     "sgh_shift2d": [i32, i32, i32, vp, f64, f64, vp, f64, vp, vp],
     "sgh_surface_forcing": [i32, i64, vp, vp, f64, f64, f64, f64, f64, f64, f64, vp, f64, vp, vp,

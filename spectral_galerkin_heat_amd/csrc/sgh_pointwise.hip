@@ -285,31 +285,46 @@ k_shift3d_xy(const F* __restrict__ in, int ny, int nx, double sy, double sx, int
   }
 }
 
-// max |Q| over the box, its floor (z = 0), its x faces and its y faces; one block.
+// max |Q| over the box, its floor (z = 0), its x faces and its y faces: one block per
+// z slice writes that slice's four maxima to part[4z..4z+3]; k_face_peaks_final folds them.
 template <typename F>
-__global__ void __launch_bounds__(1024)
-k_box_face_peaks(const F* __restrict__ Q, int nz, int ny, int nx, double* __restrict__ out) {
-  __shared__ double sh[1024 / kWave];
-  const long long S = (long long)nx * ny, N = S * nz;
-  double all = 0.0, floor_ = 0.0, xf = 0.0, yf = 0.0;
-  for (long long e = threadIdx.x; e < N; e += 1024) {
-    const double v = fabs((double)Q[e]);
-    const long long s = e % S;
-    const int j = (int)(s / nx), i = (int)(s - (long long)j * nx);
-    all = v > all ? v : all;
-    if (e < S) floor_ = v > floor_ ? v : floor_;
-    if (i == 0 || i == nx - 1) xf = v > xf ? v : xf;
-    if (j == 0 || j == ny - 1) yf = v > yf ? v : yf;
+__global__ void __launch_bounds__(kThreads)
+k_box_face_peaks(const F* __restrict__ Q, int ny, int nx, double* __restrict__ part) {
+  __shared__ double sh[kThreads / kWave];
+  const int z = blockIdx.x;
+  const F* q = Q + (long long)z * nx * ny;
+  double all = 0.0, xf = 0.0, yf = 0.0;
+  for (int j = threadIdx.x / 64; j < ny; j += kThreads / 64) {
+    const bool yface = j == 0 || j == ny - 1;
+    for (int i = threadIdx.x % 64; i < nx; i += 64) {
+      const double v = fabs((double)q[j * nx + i]);
+      all = v > all ? v : all;
+      if (i == 0 || i == nx - 1) xf = v > xf ? v : xf;
+      if (yface) yf = v > yf ? v : yf;
+    }
   }
   const double r0 = block_max(all, sh);
-  const double r1 = block_max(floor_, sh);
   const double r2 = block_max(xf, sh);
   const double r3 = block_max(yf, sh);
   if (threadIdx.x == 0) {
-    out[0] = r0;
-    out[1] = r1;
-    out[2] = r2;
-    out[3] = r3;
+    part[4 * z + 0] = r0;
+    part[4 * z + 1] = z == 0 ? r0 : 0.0;
+    part[4 * z + 2] = r2;
+    part[4 * z + 3] = r3;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_face_peaks_final(const double* __restrict__ part, int nz, double* __restrict__ out) {
+  __shared__ double sh[kThreads / kWave];
+  double m[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int z = threadIdx.x; z < nz; z += kThreads)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) m[c] = part[4 * z + c] > m[c] ? part[4 * z + c] : m[c];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const double r = block_max(m[c], sh);
+    if (threadIdx.x == 0) out[c] = r;
   }
 }
 
@@ -359,32 +374,12 @@ k_convection(const F* __restrict__ T, long long S, F h, F T0, const PropProgram*
 // Volume forcing: latent heat and the property-correction stencil
 // ---------------------------------------------------------------------------------
 template <typename F>
-__device__ __forceinline__ F clamp01(F v) { return v < F(0) ? F(0) : (v > F(1) ? F(1) : v); }
-
-template <typename F>
 __global__ void __launch_bounds__(kThreads)
-k_latent(const F* __restrict__ T, const F* __restrict__ Tp, long long N, F T_S, F inv_band,
-         int degenerate, F rho, F L, F dt, const PropProgram* __restrict__ prog,
+k_latent(const F* __restrict__ T, const F* __restrict__ Tp, long long N, LatentParams<F> lp,
          F* __restrict__ out) {
   for (long long i = blockIdx.x * (long long)kThreads + threadIdx.x; i < N;
-       i += (long long)gridDim.x * kThreads) {
-    const F t = T[i], tp = Tp[i];
-    if (prog) {
-      const F flc = prog_fl<F>(prog, t), flp = prog_fl<F>(prog, tp);
-      const F r = prog_blend<F>(prog, kRhoSolid, t, flc);
-      out[i] = -r * L * (flc - flp) * (F(1) / dt);
-    } else {
-      F fc, fp;
-      if (degenerate) {
-        fc = t >= T_S ? F(1) : F(0);
-        fp = tp >= T_S ? F(1) : F(0);
-      } else {
-        fc = clamp01((t - T_S) * inv_band);
-        fp = clamp01((tp - T_S) * inv_band);
-      }
-      out[i] = (-rho * L / dt) * (fc - fp);
-    }
-  }
+       i += (long long)gridDim.x * kThreads)
+    out[i] = latent_value<F>(lp, T[i], Tp[i]);
 }
 
 // numpy.gradient (edge_order=1) along one axis of stride `st` and length `n` at index c.
@@ -648,10 +643,14 @@ int sgh_box_face_peaks(int dtype, int nz, int ny, int nx, const void* Q, double*
                        void* stream) {
   SGH_REQUIRE(nz > 0 && ny > 0 && nx > 0 && Q && out, SGH_ERR_ARG,
               "sgh_box_face_peaks: bad args");
+  SGH_REQUIRE(nz <= 65535 && (long long)nx * ny <= (1LL << 31) - 1, SGH_ERR_ARG,
+              "sgh_box_face_peaks: box too large");
   SGH_DISPATCH(dtype, {
-    hipLaunchKernelGGL(k_box_face_peaks<F>, dim3(1), dim3(1024), 0, STREAM(stream),
-                       (const F*)Q, nz, ny, nx, out);
+    hipLaunchKernelGGL(k_box_face_peaks<F>, dim3(nz), dim3(kThreads), 0, STREAM(stream),
+                       (const F*)Q, ny, nx, out + 4);
   });
+  hipLaunchKernelGGL(k_face_peaks_final, dim3(1), dim3(kThreads), 0, STREAM(stream), out + 4,
+                     nz, out);
   SGH_CHECK_LAUNCH("sgh_box_face_peaks");
   return SGH_OK;
 }
@@ -689,12 +688,10 @@ int sgh_latent_source(int dtype, long long n, const void* T, const void* T_prev,
                       double T_L, double rho, double L, double dt, const void* prog,
                       void* out, void* stream) {
   SGH_REQUIRE(n > 0 && T && T_prev && out && dt > 0, SGH_ERR_ARG, "sgh_latent_source: bad args");
-  const int degenerate = T_L <= T_S;
-  const double inv_band = degenerate ? 0.0 : 1.0 / (T_L - T_S);
   SGH_DISPATCH(dtype, {
     hipLaunchKernelGGL(k_latent<F>, dim3(grid_for(n)), dim3(kThreads), 0, STREAM(stream),
-                       (const F*)T, (const F*)T_prev, n, (F)T_S, (F)inv_band, degenerate,
-                       (F)rho, (F)L, (F)dt, (const PropProgram*)prog, (F*)out);
+                       (const F*)T, (const F*)T_prev, n,
+                       make_latent<F>(T_S, T_L, rho, L, dt, prog), (F*)out);
   });
   SGH_CHECK_LAUNCH("sgh_latent_source");
   return SGH_OK;

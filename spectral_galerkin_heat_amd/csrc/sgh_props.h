@@ -133,4 +133,49 @@ __device__ __forceinline__ F prog_blend(const PropProgram* __restrict__ P, int s
   return (F(1) - fl) * prog_eval<F>(P, solid_idx, T) + fl * prog_eval<F>(P, solid_idx + 1, T);
 }
 
+// Latent-heat source −ρ·L·(f_l(T) − f_l(T_prev))/dt. With a property program: blended
+// ρ(T) and the program's liquid fraction; otherwise constant ρ and f_l = (T − T_S)·inv_band
+// clamped to [0, 1] (a step at T_S when the band is degenerate).
+template <typename F>
+struct LatentParams {
+  F T_S, inv_band, rho, L, dt;
+  int degenerate;
+  const PropProgram* prog;
+};
+
+template <typename F>
+__device__ __forceinline__ F latent_value(const LatentParams<F>& p, F t, F tp) {
+  if (p.prog) {
+    const F flc = prog_fl<F>(p.prog, t), flp = prog_fl<F>(p.prog, tp);
+    const F r = prog_blend<F>(p.prog, kRhoSolid, t, flc);
+    return -r * p.L * (flc - flp) * (F(1) / p.dt);
+  }
+  F fc, fp;
+  if (p.degenerate) {
+    fc = t >= p.T_S ? F(1) : F(0);
+    fp = tp >= p.T_S ? F(1) : F(0);
+  } else {
+    fc = (t - p.T_S) * p.inv_band;
+    fc = fc < F(0) ? F(0) : (fc > F(1) ? F(1) : fc);
+    fp = (tp - p.T_S) * p.inv_band;
+    fp = fp < F(0) ? F(0) : (fp > F(1) ? F(1) : fp);
+  }
+  return (-p.rho * p.L / p.dt) * (fc - fp);
+}
+
+template <typename F>
+inline LatentParams<F> make_latent(double T_S, double T_L, double rho, double L, double dt,
+                                   const void* prog) {
+  const int degenerate = T_L <= T_S;
+  LatentParams<F> p;
+  p.T_S = (F)T_S;
+  p.inv_band = (F)(degenerate ? 0.0 : 1.0 / (T_L - T_S));
+  p.rho = (F)rho;
+  p.L = (F)L;
+  p.dt = (F)dt;
+  p.degenerate = degenerate;
+  p.prog = (const PropProgram*)prog;
+  return p;
+}
+
 }  // namespace sgh

@@ -18,6 +18,7 @@
 
 #include "sgh_common.h"
 #include "sgh_mfma.h"
+#include "sgh_props.h"
 
 namespace sgh {
 
@@ -226,11 +227,14 @@ constexpr int kGM = 64, kGN = 64, kGK = 32;
 constexpr int kGL = kGM * kGK / kT;   // elements of A (and of B) per thread per slice
 constexpr int kBfLd = kGK + 8;        // bf16 LDS row: 80 bytes, 16-byte aligned reads
 
-template <typename F, int VAR, bool B_CF>
+// LAT: the epilogue turns each output T into the latent source latent_value(T, Tp) with
+// Tp laid out like C, so a reconstruction's last contraction emits the source directly.
+template <typename F, int VAR, bool B_CF, bool LAT = false>
 __global__ void __launch_bounds__(kT)
 k_gemm_mfma(int M, int N, int K, const F* __restrict__ A, long long sAb, long long sAr,
             const F* __restrict__ B, long long sBb, long long sBk, long long sBc,
-            F* __restrict__ C, long long sCb, long long sCr, F scale) {
+            F* __restrict__ C, long long sCb, long long sCr, F scale,
+            const F* __restrict__ Tp = nullptr, LatentParams<F> lp = {}) {
   static_assert(VAR == 0 || (VAR == 1 && sizeof(F) == 4), "split-BF16 takes FP32 data");
   using Acc = typename Mfma<F>::V;
   constexpr int ldA = pad_a<F>(kGK), ldB = pad_b<F>(kGN);
@@ -356,7 +360,12 @@ k_gemm_mfma(int M, int N, int K, const F* __restrict__ A, long long sAb, long lo
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int row = r0 + wr * 32 + 16 * m + Mfma<F>::row(lane, v);
-        if (row < M) Cb[(long long)row * sCr + col] = scale * acc[m][t][v];
+        if (row < M) {
+          const long long o = (long long)row * sCr + col;
+          F val = scale * acc[m][t][v];
+          if constexpr (LAT) val = latent_value<F>(lp, val, Tp[bz * sCb + o]);
+          Cb[o] = val;
+        }
       }
     }
 }
@@ -370,19 +379,25 @@ namespace {
 // M is n_out × n_in with row stride ldm (ldm > n_in: a column window of a wider basis).
 template <typename F, int VAR>
 void launch_axis_mfma(long long outer, int n_in, int n_out, long long inner, const F* M,
-                      long long ldm, const F* in, F scale, F* out, hipStream_t st) {
+                      long long ldm, const F* in, F scale, F* out, hipStream_t st,
+                      const F* Tp = nullptr, LatentParams<F> lp = {}) {
   if (inner == 1) {
     // Out[o, k] = Σ_j In[o, j]·M[k, j]: A = In (outer × n_in), B[j][k] = M[k][j] (k-fast)
     dim3 grid(cdiv(n_out, kGN), cdiv(outer, kGM), 1);
-    hipLaunchKernelGGL((k_gemm_mfma<F, VAR, false>), grid, dim3(kT), 0, st, (int)outer, n_out,
-                       n_in, in, 0LL, (long long)n_in, M, 0LL, 1LL, ldm, out, 0LL,
-                       (long long)n_out, scale);
+    if (Tp)
+      hipLaunchKernelGGL((k_gemm_mfma<F, VAR, false, true>), grid, dim3(kT), 0, st, (int)outer,
+                         n_out, n_in, in, 0LL, (long long)n_in, M, 0LL, 1LL, ldm, out, 0LL,
+                         (long long)n_out, scale, Tp, lp);
+    else
+      hipLaunchKernelGGL((k_gemm_mfma<F, VAR, false>), grid, dim3(kT), 0, st, (int)outer,
+                         n_out, n_in, in, 0LL, (long long)n_in, M, 0LL, 1LL, ldm, out, 0LL,
+                         (long long)n_out, scale, nullptr, LatentParams<F>{});
   } else {
     // Out_o[k, i] = Σ_j M[k, j]·In_o[j, i]: A = M, B = In_o (column-fast), batched over o
     dim3 grid(cdiv(inner, kGN), cdiv(n_out, kGM), (unsigned)outer);
     hipLaunchKernelGGL((k_gemm_mfma<F, VAR, true>), grid, dim3(kT), 0, st, n_out, (int)inner,
                        n_in, M, 0LL, ldm, in, (long long)n_in * inner, inner, 1LL,
-                       out, (long long)n_out * inner, inner, scale);
+                       out, (long long)n_out * inner, inner, scale, nullptr, LatentParams<F>{});
   }
 }
 
@@ -474,6 +489,56 @@ int sgh_axis_contract_rect(int dtype, int variant, long long outer, int n_in, in
 int sgh_axis_contract(int dtype, int variant, long long outer, int n, long long inner,
                       const void* M, const void* in, double scale, void* out, void* stream) {
   return sgh_axis_contract_rect(dtype, variant, outer, n, n, inner, M, in, scale, out, stream);
+}
+
+int sgh_box_latent_source(int dtype, int variant, int nx, int ny, int nz, const void* a,
+                          int nxb, int nyb, int nzb, const void* BxT, long long ldx,
+                          const void* ByT, long long ldy, const void* BzT, const void* T_prev,
+                          double T_S, double T_L, double rho, double L_f, double dt,
+                          const void* prog, void* work, void* Q, void* stream) {
+  SGH_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nxb > 0 && nyb > 0 && nzb > 0 && a && BxT && ByT &&
+                  BzT && T_prev && work && Q && dt > 0,
+              SGH_ERR_ARG, "sgh_box_latent_source: bad args");
+  SGH_REQUIRE(ldx >= nx && ldy >= ny, SGH_ERR_ARG, "sgh_box_latent_source: ldx/ldy too small");
+  SGH_REQUIRE(variant == SGH_XFORM_MFMA || variant == SGH_XFORM_BF16X3 ||
+                  variant == SGH_XFORM_VALU,
+              SGH_ERR_ARG, "sgh_box_latent_source: unknown variant %d", variant);
+  SGH_REQUIRE(!(variant == SGH_XFORM_BF16X3 && dtype != SGH_F32), SGH_ERR_ARG,
+              "sgh_box_latent_source: split-BF16 takes FP32 data");
+  SGH_REQUIRE(nzb <= 65535 && cdiv((long long)nzb * nyb, kGM) <= 65535u, SGH_ERR_ARG,
+              "sgh_box_latent_source: box too large");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  SGH_DISPATCH(dtype, {
+    F* w1 = (F*)work;                          // (nzb, ny, nx)
+    F* w2 = w1 + (long long)nzb * ny * nx;     // (nzb, nyb, nx)
+    // z, then y, on the coarse x axis (the smallest intermediates), then x with the
+    // latent source in the epilogue
+    int rc = sgh_axis_contract_strided(dtype, variant, 1, nz, nzb, (long long)ny * nx, BzT, nz,
+                                       a, 1.0, w1, stream);
+    if (rc != SGH_OK) return rc;
+    rc = sgh_axis_contract_strided(dtype, variant, nzb, ny, nyb, nx, ByT, ldy, w1, 1.0, w2,
+                                   stream);
+    if (rc != SGH_OK) return rc;
+    const LatentParams<F> lp = make_latent<F>(T_S, T_L, rho, L_f, dt, prog);
+    const long long rows = (long long)nzb * nyb;
+    if (variant == SGH_XFORM_VALU) {
+      rc = sgh_axis_contract_strided(dtype, variant, rows, nx, nxb, 1, BxT, ldx, w2, 1.0, Q,
+                                     stream);
+      if (rc != SGH_OK) return rc;
+      rc = sgh_latent_source(dtype, rows * nxb, Q, T_prev, T_S, T_L, rho, L_f, dt, prog, Q,
+                             stream);
+      if (rc != SGH_OK) return rc;
+    } else if (variant == SGH_XFORM_BF16X3) {
+      launch_axis_mfma<float, 1>(rows, nx, nxb, 1, (const float*)BxT, ldx, (const float*)w2,
+                                 1.0f, (float*)Q, st, (const float*)T_prev,
+                                 make_latent<float>(T_S, T_L, rho, L_f, dt, prog));
+    } else {
+      launch_axis_mfma<F, 0>(rows, nx, nxb, 1, (const F*)BxT, ldx, (const F*)w2, (F)1, (F*)Q,
+                             st, (const F*)T_prev, lp);
+    }
+  });
+  SGH_CHECK_LAUNCH("sgh_box_latent_source");
+  return SGH_OK;
 }
 
 }  // extern "C"

@@ -254,16 +254,17 @@ class FineMeshState:
         self.dV_fine = self.dx_fine * self.dy_fine * self.dz_fine
         self.x_start = self.y_start = 0
         dev = dict(dtype=td, device=_DEVICE)
-        work = max(nz * ny * self.nx_box, nz * self.ny_box * self.nx_box,
-                   self.nz_box * self.ny_box * nx, self.nz_box * ny * nx)
-        self._w = (torch.empty(work, **dev), torch.empty(work, **dev))
+        # intermediates: (nz_box, ny, nx) then (nz_box, ny_box, nx) in both directions
+        work = self.nz_box * max(ny, self.ny_box) * nx
+        self._wbuf = torch.empty(2 * work, **dev)
+        self._w = (self._wbuf[:work], self._wbuf[work:])
         self._T = torch.empty(self.shape, **dev)        # history, and its shift target
         self._T_alt = torch.empty(self.shape, **dev)
         self._Q = torch.empty(self.shape, **dev)
         self._Q_alt = torch.empty(self.shape, **dev)
         self.T_box = torch.empty(self.shape, **dev)     # per-iteration reconstruction
         self.Q_modes = torch.empty((nz, ny, nx), **dev)
-        self.peaks = torch.zeros(4, dtype=torch.float64, device=_DEVICE)
+        self.peaks = torch.zeros(4 * (self.nz_box + 1), dtype=torch.float64, device=_DEVICE)
         self.T_prev = None
         self.Q_prev = None
 
@@ -287,8 +288,8 @@ class FineMeshState:
 def _reconstruct_temperature_box(a, SsState, out=None):
     """Temperature on the fine box cells, Σ a·B_z B_y B_x (spectral_ops.py:173-179).
 
-    Three strided MFMA contractions: x (modes → window cells, row block of Bₓᵀ at the
-    window start), y likewise, then z over all (y, x) columns."""
+    Three strided MFMA contractions, smallest intermediates first: z over all (y, x)
+    mode columns, then y (the row block of B_yᵀ at the window start), then x likewise."""
     fm = SsState.fine_mesh
     if fm is None:
         raise RuntimeError("Fine mesh not initialized.")
@@ -299,12 +300,12 @@ def _reconstruct_temperature_box(a, SsState, out=None):
     w1, w2 = fm._w
     if out is None:
         out = torch.empty(fm.shape, dtype=a.dtype, device=a.device)
-    call("sgh_axis_contract_strided", code, xf, nz * ny, nx, nxb, 1,
-         fm._col(fm.BxT_full, fm.x_start * nx), nx, ptr(a), 1.0, ptr(w1), st)
-    call("sgh_axis_contract_strided", code, xf, nz, ny, nyb, nxb,
+    call("sgh_axis_contract_strided", code, xf, 1, nz, nzb, ny * nx, ptr(fm.BzT), nz, ptr(a),
+         1.0, ptr(w1), st)
+    call("sgh_axis_contract_strided", code, xf, nzb, ny, nyb, nx,
          fm._col(fm.ByT_full, fm.y_start * ny), ny, ptr(w1), 1.0, ptr(w2), st)
-    call("sgh_axis_contract_strided", code, xf, 1, nz, nzb, nyb * nxb, ptr(fm.BzT), nz,
-         ptr(w2), 1.0, ptr(out), st)
+    call("sgh_axis_contract_strided", code, xf, nzb * nyb, nx, nxb, 1,
+         fm._col(fm.BxT_full, fm.x_start * nx), nx, ptr(w2), 1.0, ptr(out), st)
     return out
 
 
@@ -481,9 +482,19 @@ def compute_latent_heat_source_grid(Q_buffer, T_full, T_prev_full, phys, num, Ss
 
 
 def _box_latent(Q_buffer, a, SsState, phys, num):
+    """Q on the box at modes ``a`` against ``T_prev``: one fused entry (three contractions,
+    the latent source formed in the last one's epilogue)."""
     fm = SsState.fine_mesh
-    T_box = _reconstruct_temperature_box(a, SsState, out=fm.T_box)
-    _latent(Q_buffer, T_box, fm.T_prev, phys, num)
+    a = a.contiguous()
+    nz, ny, nx = a.shape
+    nzb, nyb, nxb = fm.shape
+    model = getattr(phys, "model", None)
+    prog = _program(model) if (model is not None and not model.is_constant) else None
+    call("sgh_box_latent_source", dtype_code(a.dtype), SsState.xform, nx, ny, nz, ptr(a),
+         nxb, nyb, nzb, fm._col(fm.BxT_full, fm.x_start * nx), nx,
+         fm._col(fm.ByT_full, fm.y_start * ny), ny, ptr(fm.BzT), ptr(fm.T_prev),
+         float(phys.T_solidus), float(phys.T_liquidus), float(phys.rho), float(phys.L_f),
+         float(num.dt), ptr(prog), ptr(fm._wbuf), ptr(Q_buffer), stream_ptr())
 
 
 def compute_latent_heat_source(Q_buffer, phys, num, SsState):
@@ -549,7 +560,7 @@ def box_face_peaks(Q_box, SsState):
     nzb, nyb, nxb = fm.shape
     call("sgh_box_face_peaks", dtype_code(Q_box.dtype), nzb, nyb, nxb, ptr(Q_box.contiguous()),
          ptr(fm.peaks), stream_ptr())
-    return [float(v) for v in fm.peaks.cpu()]
+    return [float(v) for v in fm.peaks[:4].cpu()]
 
 
 _PROGRAMS: dict = {}
