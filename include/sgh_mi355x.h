@@ -294,6 +294,50 @@ int sgh_batch_get_variant(void* plan, int* out3);
  * sgh_batch_set_state before the next run to restart from an initial condition. */
 int sgh_batch_rewind(void* plan, void* stream);
 
+/* ---- one Picard iteration of the plain path in one call ----------------------
+ * Enqueues, in the order of SpectralSolver.step's loop body (sgh/solvers/spectral.py:
+ * 384-497), every kernel of one relaxed-Picard iteration on one cuboid: top-surface
+ * temperature (z-partial sum + 2-D inverse), evaporation / convection / k̄/k(T) flux
+ * forcing, its projection, the latent source (coarse grid: reconstruct, source, project;
+ * fine box: the fused sgh_box_latent_source then the window projection), the property
+ * correction when `rescale`, the fused ETD1 + relaxation + norm pass, the norm
+ * reduction, and the copy of the three norms to norms_host[3·parity ..] (pinned host
+ * memory).  Buffers indexed by `parity` (= iteration & 1) alternate so that the next
+ * iteration can be enqueued before this one's residual is read: a_old = a_buf[parity],
+ * a_new = a_buf[parity ^ 1]; q_evap, Q_latent, Q_modes and norms_host by parity.
+ * Every pointer is a device pointer except norms_host. */
+typedef struct SghPlainIter {
+  int dtype, xform, nx, ny, nz, nchunks, n_pairs;
+  int latent_mode;     /* 0 none, 1 coarse grid, 2 fine box */
+  int rescale;         /* temperature-dependent properties */
+  int have_bottom;     /* lagged bottom-convection source S_bot_raw */
+  int nxb, nyb, nzb;   /* fine box */
+  int ntx, nty;        /* fine-grid cells along x and y (row strides of the x/y bases) */
+  int x_start, y_start;/* box window of this step */
+  int pad0;
+  double dxdy, dV, dV_fine, n_elements, omega;
+  double Pa, T_boil, dH, R_v, T_liq, h_top, T0, kbar, abar;
+  double T_S, T_L, rho, L_f, dt, dx, dy, dz;
+  const void* aK; const void* KK; void* a_buf[2];
+  const void* w_top; const void* w_bot;
+  void* zc; void* T_s; const void* q_las; void* q_evap[2]; void* q_tmp; void* S_top;
+  const void* S_zero; const void* S_bot_raw;
+  const void* sByT; const void* sBx; const void* sBy; const void* sBxT;   /* surface bases */
+  void* Q_latent[2]; void* Q_modes[2]; const void* T_prev_full; const void* prog;
+  const void* vBxT; const void* vByT; const void* vBzT;   /* volume reconstruction */
+  const void* vBx; const void* vBy; const void* vBz;      /* volume projection */
+  void* vol_w1; void* vol_w2; void* vol_T;                /* N elements each */
+  void* a_raw; void* corr_work; void* corr_f; void* C_modes;   /* corr_work: 3N */
+  const void* fBxT; const void* fByT; const void* fBzT;   /* fine box bases */
+  const void* fBx; const void* fBy; const void* fBz;
+  const void* T_prev_box; void* box_work;  /* box_work: 2·nzb·max(ny, nyb)·nx */
+  double* partials; double* norms; double* norms_host;
+} SghPlainIter;
+
+/* sizeof(SghPlainIter), so a binding can check its mirror of the layout. */
+size_t sgh_plain_iter_size(void);
+int sgh_plain_iterate(const SghPlainIter* d, int parity, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

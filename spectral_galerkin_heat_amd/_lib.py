@@ -40,6 +40,8 @@ SIGNATURES = {
     "sgh_axis_contract_strided": [i32, i32, i64, i32, i32, i64, vp, i64, vp, f64, vp, vp],
     "sgh_shift3d_xy": [i32, i32, i32, i32, vp, f64, f64, i32, vp, vp],
     "sgh_box_face_peaks": [i32, i32, i32, i32, vp, vp, vp],
+    "sgh_plain_iter_size": [],
+    "sgh_plain_iterate": [vp, i32, vp],
     "sgh_box_latent_source": [i32, i32, i32, i32, i32, vp, i32, i32, i32, vp, i64, vp, i64, vp,
                               vp, f64, f64, f64, f64, f64, vp, vp, vp, vp],
     "sgh_laser_flux": [i32, i32, i32, f64, f64, f64, f64, f64, f64, f64, f64, i32, vp, vp],
@@ -89,7 +91,31 @@ class CuboidSpec(C.Structure):
     ]
 
 
-_RESTYPES = {"sgh_last_error": C.c_char_p}
+_RESTYPES = {"sgh_last_error": C.c_char_p, "sgh_plain_iter_size": C.c_size_t}
+
+
+class SghPlainIter(C.Structure):
+    """Mirror of ``SghPlainIter`` in include/sgh_mi355x.h (checked against
+    ``sgh_plain_iter_size`` when the library loads)."""
+
+    _fields_ = (
+        [(n, C.c_int) for n in ("dtype", "xform", "nx", "ny", "nz", "nchunks", "n_pairs",
+                                "latent_mode", "rescale", "have_bottom", "nxb", "nyb", "nzb",
+                                "ntx", "nty", "x_start", "y_start", "pad0")]
+        + [(n, C.c_double) for n in ("dxdy", "dV", "dV_fine", "n_elements", "omega", "Pa",
+                                     "T_boil", "dH", "R_v", "T_liq", "h_top", "T0", "kbar",
+                                     "abar", "T_S", "T_L", "rho", "L_f", "dt", "dx", "dy", "dz")]
+        + [("aK", vp), ("KK", vp), ("a_buf", vp * 2), ("w_top", vp), ("w_bot", vp),
+           ("zc", vp), ("T_s", vp), ("q_las", vp), ("q_evap", vp * 2), ("q_tmp", vp),
+           ("S_top", vp), ("S_zero", vp), ("S_bot_raw", vp),
+           ("sByT", vp), ("sBx", vp), ("sBy", vp), ("sBxT", vp),
+           ("Q_latent", vp * 2), ("Q_modes", vp * 2), ("T_prev_full", vp), ("prog", vp),
+           ("vBxT", vp), ("vByT", vp), ("vBzT", vp), ("vBx", vp), ("vBy", vp), ("vBz", vp),
+           ("vol_w1", vp), ("vol_w2", vp), ("vol_T", vp),
+           ("a_raw", vp), ("corr_work", vp), ("corr_f", vp), ("C_modes", vp),
+           ("fBxT", vp), ("fByT", vp), ("fBzT", vp), ("fBx", vp), ("fBy", vp), ("fBz", vp),
+           ("T_prev_box", vp), ("box_work", vp),
+           ("partials", vp), ("norms", vp), ("norms_host", vp)])
 
 
 class SGHError(RuntimeError):
@@ -118,6 +144,10 @@ class _Lib:
                 continue
             fn.argtypes = argtypes
             fn.restype = _RESTYPES.get(name, C.c_int)
+        if hasattr(h, "sgh_plain_iter_size") and h.sgh_plain_iter_size() != C.sizeof(SghPlainIter):
+            raise SGHError("SghPlainIter layout differs between the library "
+                           f"({h.sgh_plain_iter_size()} bytes) and _lib.py "
+                           f"({C.sizeof(SghPlainIter)} bytes): rebuild the library")
         self._h = h
         return h
 

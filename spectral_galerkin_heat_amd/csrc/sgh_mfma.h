@@ -11,6 +11,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "sgh_common.h"
+
 namespace sgh {
 
 typedef float f4v __attribute__((ext_vector_type(4)));
@@ -90,6 +92,43 @@ __device__ __forceinline__ void stage_block(F* __restrict__ dst, int dst_ld, con
 #pragma unroll
     for (int q = 0; q < A; ++q)
       if (e0 + q * NT < total) dst[rr[q] * dst_ld + cc[q]] = tmp[q];
+  }
+}
+
+// Element offset XORed into the columns of odd rows of an unpadded LDS operand so that
+// rows kk and kk+1 of an MFMA B operand fall in different halves of the 32 banks; 0
+// when the row length already staggers them (or cannot be swizzled within a row).
+template <typename F>
+__device__ __forceinline__ int swz_of(int ld) {
+  constexpr int half = sizeof(F) == 4 ? 16 : 8;  // 16 words = half the banks
+  return ld % (2 * half) == 0 ? half : 0;
+}
+
+// LDS DMA (gfx950 global_load_lds_dwordx4) of `rows` rows of `ld` elements from global
+// `src` (row stride `src_ld`) into LDS `dst` packed as [rows][ld], the columns of odd rows
+// XOR-swizzled by `swz` elements: every 16-byte unit of LDS receives the 16-byte unit of
+// its unswizzled position. Each wave copies 1 KiB per instruction (lane l -> dst unit
+// base + l). The caller waits (s_waitcnt + barrier) before reading. Needs 16-byte aligned
+// src rows and dst, ld·sizeof(F) a multiple of 16 and swz a multiple of 16 / sizeof(F).
+__device__ __forceinline__ void dma16(const void* src, void* lds_base) {
+  __builtin_amdgcn_global_load_lds(src, lds_base, 16, 0, 0);
+}
+
+template <typename F, int NT>
+__device__ __forceinline__ void dma_rows(F* dst, const F* src, int rows, int ld, int src_ld,
+                                         int swz) {
+  constexpr int E = 16 / (int)sizeof(F);
+  const int units = rows * ld / E;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  for (int u0 = wave * kWave; u0 < units; u0 += NT) {
+    const int u = u0 + lane;
+    if (u < units) {
+      const int q = u * E;
+      const int r = q / ld;
+      const int cs = q - r * ld;
+      const int c = (r & 1) ? (cs ^ swz) : cs;
+      dma16(src + (long long)r * src_ld + c, dst + (long long)u0 * E);
+    }
   }
 }
 

@@ -144,6 +144,98 @@ k_sep2d(int ny, int nx, const F* __restrict__ L, const F* __restrict__ R,
   }
 }
 
+// Surfaces whose operands fit in LDS (and rows of a multiple of 16 bytes): everything the
+// workgroup reads lands in one round -- its 16 rows of L through registers, all of X and
+// its kSCols columns of R by LDS DMA (gfx950 global_load_lds_dwordx4, odd rows
+// swizzled), one wait -- then both MFMA stages run out of LDS.
+template <typename F>
+__global__ void __launch_bounds__(kT)
+k_sep2d_direct(int ny, int nx, const F* __restrict__ L, const F* __restrict__ R,
+               const F* __restrict__ X, F scale, F* __restrict__ out) {
+  using M = Mfma<F>;
+  using V = typename M::V;
+  constexpr int NW = kT / 64;
+  constexpr int TPW = (kMaxNx / 16 + NW - 1) / NW;
+  constexpr int kLPT = kSRows * kMaxNx / kT;   // L elements per thread
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int ny4 = (ny + 3) & ~3, nx4 = (nx + 3) & ~3;
+  const int ldl = pad_a<F>(ny4), ldy = pad_a<F>(nx4);
+  const int c0 = blockIdx.z * kSCols, cn = min(kSCols, nx - c0);
+  F* Ls = reinterpret_cast<F*>(smem_raw);    // [16][ldl]
+  F* Ys = Ls + kSRows * ldl;                 // [16][ldy]
+  F* Xs = Ys + kSRows * ldy;                 // [ny4][nx]
+  F* Rs = Xs + (long long)ny4 * nx;          // [nx4][cn]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int r0 = blockIdx.x * kSRows;
+  const int rows = min(kSRows, ny - r0);
+  const long long plane = (long long)ny * nx;
+  const F* Xb = X + (long long)blockIdx.y * plane;
+  F* Ob = out + (long long)blockIdx.y * plane;
+  const int ntile = (nx + 15) >> 4;
+  const int swx = swz_of<F>(nx), swr = swz_of<F>(cn);
+  const int ltot = kSRows * ny4;
+
+  F lr[kLPT];
+#pragma unroll
+  for (int q = 0; q < kLPT; ++q) {
+    const int e = tid + q * kT;
+    const int r = e / ny4, c = e - r * ny4;
+    lr[q] = (e < ltot && r < rows && c < ny) ? L[(long long)(r0 + r) * ny + c] : F(0);
+  }
+  dma_rows<F, kT>(Xs, Xb, ny, nx, nx, swx);
+  dma_rows<F, kT>(Rs, R + c0, nx, cn, nx, swr);
+  for (int e = tid; e < (ny4 - ny) * nx; e += kT) Xs[ny * nx + e] = F(0);
+  for (int e = tid; e < (nx4 - nx) * cn; e += kT) Rs[nx * cn + e] = F(0);
+#pragma unroll
+  for (int q = 0; q < kLPT; ++q) {
+    const int e = tid + q * kT;
+    if (e < ltot) Ls[(e / ny4) * ldl + (e - (e / ny4) * ny4)] = lr[q];
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+
+  // stage 1: Y = L[rows, :] · X
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) {
+    const int jt = wave + NW * t;
+    if (jt >= ntile) break;
+    const int col = jt * 16 + li;
+    const bool cin = col < nx;
+    const int colx = (lk & 1) ? (col ^ swx) : col;
+    V acc = V{0, 0, 0, 0};
+    for (int kk = 0; kk < ny4; kk += 4) {
+      const F bv = cin ? Xs[(kk + lk) * nx + colx] : F(0);
+      acc = M::mma(Ls[li * ldl + kk + lk], bv, acc);
+    }
+    if (cin) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Ys[M::row(lane, v) * ldy + col] = acc[v];
+    }
+  }
+  for (int e = tid; e < kSRows * (nx4 - nx); e += kT) {
+    const int r = e / (nx4 - nx), c = nx + e % (nx4 - nx);
+    Ys[r * ldy + c] = F(0);
+  }
+  __syncthreads();
+  // stage 2: out[:, c0 + 16·wave + (0..15)] = Y · R[:, those columns]
+  if (wave * 16 >= cn) return;
+  const int cl = wave * 16 + li;
+  const bool cin = cl < cn;
+  const int clx = (lk & 1) ? (cl ^ swr) : cl;
+  V acc2 = V{0, 0, 0, 0};
+  for (int kk = 0; kk < nx4; kk += 4) {
+    const F bv = cin ? Rs[(kk + lk) * cn + clx] : F(0);
+    acc2 = M::mma(Ys[li * ldy + kk + lk], bv, acc2);
+  }
+  if (!cin) return;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int row = M::row(lane, v);
+    if (row < rows) Ob[(long long)(r0 + row) * nx + c0 + cl] = scale * acc2[v];
+  }
+}
+
 // C[b][r][c] = scale · Σ_k A[b][r][k] · B[b][k][c] with arbitrary strides.
 constexpr int kTM = 64, kTN = 64, kTK = 16;
 
@@ -415,6 +507,23 @@ int sgh_sep2d(int dtype, int batch, int ny, int nx, const void* L, const void* R
   SGH_DISPATCH(dtype, {
     const int ny4 = (ny + 3) & ~3, nx4 = (nx + 3) & ~3;
     const long long fixed = (long long)kSRows * (pad_a<F>(ny4) + pad_a<F>(nx4));
+    const long long direct_elems = fixed + (long long)ny4 * nx + (long long)nx4 * std::min(kSCols, nx);
+    const bool aligned = ((reinterpret_cast<uintptr_t>(X) | reinterpret_cast<uintptr_t>(R)) & 15) == 0;
+    if (nparts == 1 && aligned && (nx * (int)sizeof(F)) % 16 == 0 &&
+        direct_elems * (long long)sizeof(F) <= kSLds) {
+      static bool direct_raised[2] = {false, false};
+      if (!direct_raised[sizeof(F) == 8]) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sep2d_direct<F>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, kSLds);
+        if (e != hipSuccess) return fail_hip(e, "sgh_sep2d LDS attribute");
+        direct_raised[sizeof(F) == 8] = true;
+      }
+      hipLaunchKernelGGL(k_sep2d_direct<F>, dim3(cdiv(ny, kSRows), batch, cdiv(nx, kSCols)),
+                         dim3(kT), (size_t)direct_elems * sizeof(F),
+                         reinterpret_cast<hipStream_t>(stream), ny, nx, (const F*)L, (const F*)R,
+                         (const F*)X, (F)scale, (F*)out);
+      break;   // leaves the dispatch block: the chunked path below is not needed
+    }
     const long long chunk = std::min<long long>(kSChunk, kSLds / (long long)sizeof(F) - fixed);
     SGH_REQUIRE(chunk >= 4LL * pad_b<F>(nx), SGH_ERR_ARG,
                 "sgh_sep2d: ny=%d is too large for one workgroup's LDS", ny);

@@ -618,6 +618,100 @@ def laser_flux(SsState, profile, laser_params, x0, y0, peak, out=None):
 # ---------------------------------------------------------------------------------
 
 
+class _PlainIter:
+    """Descriptor of the native per-iteration entry (``sgh_plain_iterate``) for one solver
+    state: every buffer the iteration touches, built once; the box window, the box
+    history pointer and the laser-dependent switches are refreshed each step."""
+
+    def __init__(self, solver, S):
+        import ctypes
+        g, b, fm = S.grid, S.buffers, S.fine_mesh
+        mat, num = solver.context.mat, solver.context.num
+        td, dev = b.a_temp.dtype, b.a_temp.device
+        nx, ny, nz = g.nx, g.ny, g.nz
+        shp = (nz, ny, nx)
+        d = _lib.SghPlainIter()
+        self.d = d
+        self.ref = ctypes.byref(d)
+        d.dtype, d.xform = S.code, S.xform
+        d.nx, d.ny, d.nz, d.nchunks, d.n_pairs = nx, ny, nz, b.nchunks, b.n_pairs
+        d.dxdy, d.dV, d.n_elements = g.dx * g.dy, g.dV, float(nx * ny * nz)
+        d.Pa, d.T_boil, d.dH = float(mat.Pa), float(mat.T_boil), float(mat.DeltaH_LV)
+        d.R_v, d.T_liq, d.T0 = float(mat.R_v), float(mat.T_liquidus), float(mat.T0)
+        d.T_S, d.T_L, d.rho = float(mat.T_solidus), float(mat.T_liquidus), float(mat.rho)
+        d.L_f, d.dt = float(getattr(mat, "L_f", 0.0)), float(num.dt)
+        d.dx, d.dy, d.dz = g.dx, g.dy, g.dz
+        d.aK, d.KK = ptr(S.a), ptr(S.KK)
+        self.a_buf = (b.a_temp, b.a_alt)
+        d.a_buf[0], d.a_buf[1] = ptr(b.a_temp), ptr(b.a_alt)
+        d.w_top, d.w_bot = ptr(g.w_top), ptr(g.w_bot)
+        d.zc, d.T_s, d.q_las, d.q_tmp = ptr(b.zc), ptr(b.T_s), ptr(b.q_las), ptr(b.q_tmp)
+        self.q_evap = (b.q_evap_buffer, torch.zeros_like(b.q_evap_buffer))
+        d.q_evap[0], d.q_evap[1] = ptr(self.q_evap[0]), ptr(self.q_evap[1])
+        d.S_top, d.S_zero, d.S_bot_raw = ptr(b.S_top), ptr(b.S_zero), ptr(b.S_bot_raw)
+        d.sByT, d.sBx, d.sBy, d.sBxT = ptr(g.ByT), ptr(g.Bx), ptr(g.By), ptr(g.BxT)
+        self.Q_latent = None
+        self.Q_modes = (None, None)
+        if fm is not None:
+            d.latent_mode = 2
+            d.nxb, d.nyb, d.nzb = fm.nx_box, fm.ny_box, fm.nz_box
+            d.ntx, d.nty = fm.n_fine_totals[0], fm.n_fine_totals[1]
+            d.dV_fine = fm.dV_fine
+            d.fBxT, d.fByT, d.fBzT = ptr(fm.BxT_full), ptr(fm.ByT_full), ptr(fm.BzT)
+            d.fBx, d.fBy, d.fBz = ptr(fm.Bx_full), ptr(fm.By_full), ptr(fm.Bz)
+            d.box_work = ptr(fm._wbuf)
+            self.Q_latent = (b.Q_latent_buffer, torch.zeros_like(b.Q_latent_buffer))
+            self.Q_modes = (fm.Q_modes, torch.empty_like(fm.Q_modes))
+        elif solver._grid_latent:
+            d.latent_mode = 1
+            self.Q_latent = (b.Q_latent_buffer, torch.zeros_like(b.Q_latent_buffer))
+            self.Q_modes = (torch.empty(shp, dtype=td, device=dev),
+                            torch.empty(shp, dtype=td, device=dev))
+        if self.Q_latent is not None:
+            d.Q_latent[0], d.Q_latent[1] = ptr(self.Q_latent[0]), ptr(self.Q_latent[1])
+            d.Q_modes[0], d.Q_modes[1] = ptr(self.Q_modes[0]), ptr(self.Q_modes[1])
+        d.T_prev_full = ptr(solver._T_prev_full)
+        d.vBxT, d.vByT, d.vBzT = ptr(g.BxT), ptr(g.ByT), ptr(g.BzT)
+        d.vBx, d.vBy, d.vBz = ptr(g.Bx), ptr(g.By), ptr(g.Bz)
+        self.vol = None
+        if d.latent_mode == 1 or solver._property_correction:
+            self.vol = [torch.empty(shp, dtype=td, device=dev) for _ in range(3)]
+            d.vol_w1, d.vol_w2, d.vol_T = (ptr(v) for v in self.vol)
+        self.corr = None
+        if solver._property_correction:
+            self.corr = (torch.empty(shp, dtype=td, device=dev),
+                         torch.empty((3,) + shp, dtype=td, device=dev),
+                         torch.empty(shp, dtype=td, device=dev),
+                         torch.empty(shp, dtype=td, device=dev))
+            d.a_raw, d.corr_work, d.corr_f, d.C_modes = (ptr(v) for v in self.corr)
+            d.kbar, d.abar = float(solver._kbar), float(solver._abar)
+        d.partials, d.norms = ptr(b.partials), ptr(b.norms)
+        self.norms_host = torch.zeros(6, dtype=torch.float64).pin_memory()
+        d.norms_host = self.norms_host.data_ptr()
+        self.events = (torch.cuda.Event(), torch.cuda.Event())
+
+    def refresh(self, solver, S, prog, kbar, have_bot):
+        d, fm = self.d, S.fine_mesh
+        d.omega = float(solver.mixing_omega)
+        d.h_top = float(solver.context.mat.h_conv_top)
+        d.rescale = int(bool(solver._property_correction))
+        d.prog = ptr(prog)
+        d.kbar = float(kbar)
+        d.have_bottom = int(bool(have_bot))
+        if fm is not None:
+            d.x_start, d.y_start = fm.x_start, fm.y_start
+            d.T_prev_box = ptr(fm.T_prev)
+
+
+def _plain_iter(solver, S, prog, kbar, have_bot):
+    pi = getattr(S, "_plain_iter", None)
+    if pi is None or pi.d.aK != S.a.data_ptr():
+        pi = _PlainIter(solver, S)
+        S._plain_iter = pi
+    pi.refresh(solver, S, prog, kbar, have_bot)
+    return pi
+
+
 def picard_step(solver, t, dt):
     """One ETD1 step with the relaxed Picard loop, kernel for kernel on the device.
 
@@ -690,51 +784,27 @@ def picard_step(solver, t, dt):
         _sep2d(b.q_tmp, g.By, g.BxT, dxdy, out=b.S_bot_raw)
 
     call("sgh_zcontract", code, nx, ny, nz, ptr(a_cur), ptr(g.w_top), b.nchunks, ptr(b.zc), st)
-    omega = float(solver.mixing_omega)
     tol = float(solver.convergence_tol)
     iter_k, rel, conv = -1, float("nan"), False
-    Q_modes = None
-    box_q = False
-    a_raw = None
-    for iter_k in range(solver.max_picard_iter):
-        a_old = a_cur
-        a_new = b.a_alt if a_old is b.a_temp else b.a_temp
-        call("sgh_sum_parts", code, Ssz, b.nchunks, ptr(b.zc), st)
-        _sep2d(b.zc, g.ByT, g.Bx, 1.0, 1, out=b.T_s)
-        call("sgh_surface_forcing", code, Ssz, ptr(b.T_s), ptr(b.q_las), float(mat.Pa),
-             float(mat.T_boil), float(mat.DeltaH_LV), float(mat.R_v), float(mat.T_liquidus),
-             h_top, T0, ptr(prog), kbar, ptr(b.q_evap_buffer), ptr(b.q_tmp), st)
-        _sep2d(b.q_tmp, g.By, g.BxT, dxdy, out=b.S_top)
-        Q_modes = None
-        if fm is not None and fm.T_prev is not None:
-            _box_latent(b.Q_latent_buffer, a_old, S, mat, num)
-            Q_modes = project_box_to_modes(b.Q_latent_buffer, S, out=fm.Q_modes)
-            box_q = True
-        elif solver._grid_latent:
-            T_full = reconstruct_volume(a_old, S)
-            _latent(b.Q_latent_buffer, T_full, solver._T_prev_full, mat, num)
-            Q_modes = project_volume(b.Q_latent_buffer, S)
-        wb = ptr(g.w_bot) if have_bot else None
-        Sb = ptr(b.S_bot_raw) if have_bot else None
-        if rescale:
-            if a_raw is None:
-                a_raw = torch.empty_like(a_cur)
-            call("sgh_etd1_update", code, nx, ny, nz, ptr(S.a), ptr(S.KK), ptr(g.w_top),
-                 ptr(b.S_top), wb, Sb, ptr(Q_modes), ptr(a_raw), st)
-            C = assemble_property_correction(S, a_raw, solver._T_prev_full, num.dt, mat.model,
-                                             solver._kbar, solver._abar)
-            call("sgh_picard_iter_fused", code, nx, ny, nz, ptr(a_raw), ptr(S.KK), ptr(a_old),
-                 ptr(g.w_top), ptr(b.S_zero), None, None, ptr(C), omega, ptr(g.w_top),
-                 b.nchunks, ptr(b.zc), ptr(a_new), ptr(b.partials), st)
-        else:
-            call("sgh_picard_iter_fused", code, nx, ny, nz, ptr(S.a), ptr(S.KK), ptr(a_old),
-                 ptr(g.w_top), ptr(b.S_top), wb, Sb, ptr(Q_modes), omega, ptr(g.w_top),
-                 b.nchunks, ptr(b.zc), ptr(a_new), ptr(b.partials), st)
-        call("sgh_reduce_norms", ptr(b.partials), b.n_pairs, float(N), ptr(b.norms), st)
-        a_cur = a_new
-        b.norms_host.copy_(b.norms, non_blocking=True)
-        torch.cuda.current_stream().synchronize()
-        rms_diff, _, rel = (float(v) for v in b.norms_host)
+    pi = _plain_iter(solver, S, prog, kbar, have_bot)
+    maxit = int(solver.max_picard_iter)
+    stream = torch.cuda.current_stream()
+
+    def enqueue(k):
+        call("sgh_plain_iterate", pi.ref, k & 1, st)
+        pi.events[k & 1].record(stream)
+
+    # The next iteration is enqueued before this one's residual is read, so the device
+    # never waits on the host; when iteration k converges, iteration k + 1 (which wrote
+    # only the other half of every alternating buffer) is discarded.
+    if maxit > 0:
+        enqueue(0)
+    for iter_k in range(maxit):
+        if iter_k + 1 < maxit:
+            enqueue(iter_k + 1)
+        pi.events[iter_k & 1].synchronize()
+        h = pi.norms_host[3 * (iter_k & 1):3 * (iter_k & 1) + 3].tolist()
+        rms_diff, rel = h[0], h[2]
         if solver.track_picard_history:
             rho_k = None
             if iter_k > 0 and solver.picard_history:
@@ -747,12 +817,19 @@ def picard_step(solver, t, dt):
             conv = True
             break
     else:
-        solver._note_cap(t, rel)
+        if maxit > 0:
+            solver._note_cap(t, rel)
+    last = iter_k & 1 if iter_k >= 0 else None
+    if last is not None:
+        a_cur = pi.a_buf[last ^ 1]
+        b.q_evap_buffer = pi.q_evap[last]
+        if pi.Q_latent is not None:
+            b.Q_latent_buffer = pi.Q_latent[last]
 
     T_temp = _recon_face(a_cur, g.w_top, S, out=b.T_s)
     call("sgh_reduce", code, Ssz, ptr(T_temp), 1, ptr(b.scal[1:]), st)
-    if solver._grid_latent:
-        solver._Q_latent_modes_prev = Q_modes
+    if solver._grid_latent and last is not None:
+        solver._Q_latent_modes_prev = pi.Q_modes[last]
     b.q_evap_old.copy_(b.q_evap_buffer)
     S.a.copy_(a_cur)
     if solver._property_correction or solver._grid_latent:
@@ -762,7 +839,7 @@ def picard_step(solver, t, dt):
         if fm.Q_prev is None:
             fm._Q.zero_()
             fm.Q_prev = fm._Q
-        if box_q:
+        if last is not None and fm.T_prev is not None:
             fm.Q_prev.copy_(b.Q_latent_buffer)
             _check_box_truncation(solver, S, b.Q_latent_buffer)
     metrics = {

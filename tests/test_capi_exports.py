@@ -48,3 +48,10 @@ def test_host_only_entry_points(lib):
     assert lib.sgh_property_program_bytes() == PROGRAM_BYTES
     assert lib.sgh_picard_partials_count(64, 32, 16, 8) == 8 * 8
     assert lib.sgh_picard_partials_count(0, 32, 16, 8) < 0
+
+
+def test_plain_iteration_descriptor_layout_matches_the_library():
+    import ctypes
+    from spectral_galerkin_heat_amd import _lib
+    h = _lib.lib.load()
+    assert h.sgh_plain_iter_size() == ctypes.sizeof(_lib.SghPlainIter)
