@@ -61,6 +61,11 @@ def parse():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--box-steps", type=int, default=40,
+                    help="steps of the box-mode single track (80x48x120 modes, fine box "
+                         "160x96x112) per precision; 0 skips it")
+    ap.add_argument("--box-only", action="store_true",
+                    help="run only the box-mode leg and print its JSON (for profiling)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size "
                          "kernel that calibrates HBM byte counters under a profiler)")
@@ -95,6 +100,64 @@ def volume_bytes_per_launch(ctxs, word):
     return tot
 
 
+def box_mode_leg(steps, with_cpu):
+    """Plain device path with a laser-following fine box (the reference's example-02 single
+    track): FP64 and FP32 rates, FP32 vs FP64 after the same steps, the FP64 state vs the
+    oracle after one step, and the oracle's CPU time for that step."""
+    import numpy as np
+    import torch
+    from spectral_galerkin_heat_amd.backends import get_backend
+    from spectral_galerkin_heat_amd.physics.meltpool import melt_pool_extent
+    from spectral_galerkin_heat_amd.physics import mi355x_kernels as K
+    from spectral_galerkin_heat_amd.solvers.spectral import SpectralSolver
+    from spectral_galerkin_heat_amd.workloads import make_context, single_track_box
+    res, states = {}, {}
+    for dtype in ("float64", "float32"):
+        ctx = make_context(*single_track_box(n_steps=steps + 2, dtype=dtype))
+        s = SpectralSolver(get_backend("mi355x"), ctx)
+        s.initialize()
+        dt = ctx.num.dt
+        for n in range(2):   # untimed: first-call set-up
+            s.step(n * dt, dt)
+        its = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for n in range(2, 2 + steps):
+            its.append(s.step(n * dt, dt)[1]["n_evap_iter"])
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        N = int(np.prod(tuple(ctx.geom.n)))
+        T = K.reconstruct_volume(s.state.a, s.state)
+        pool = melt_pool_extent(T, tuple(ctx.geom.d), T_melt=float(ctx.mat.T_liquidus))
+        states[dtype] = s.state.a.double().clone()
+        res[dtype] = {"ms_per_step": round(el / steps * 1e3, 3),
+                      "mode_steps_per_s": round(N * steps / el, 1),
+                      "picard_iterations_mean": round(float(np.mean(its)), 2),
+                      "melt_pool_um": [round(v * 1e6, 2) for v in pool]}
+        log(f"box mode {dtype}: {res[dtype]}")
+    ref = states["float64"]
+    out = {"workload": "box-mode single track, 80x48x120 modes, fine box 160x96x112 "
+                       f"(refinement 2, sliding in x), latent heat + bottom convection, steps 2..{steps + 1} timed",
+           "fp64": res["float64"], "fp32": res["float32"],
+           "rel_l2_fp32_vs_fp64": float(f"{float(torch.linalg.vector_norm(states['float32'] - ref) / torch.linalg.vector_norm(ref)):.3e}")}
+    if with_cpu:
+        from oracle.sgh_oracle import OracleSolver
+        ctx = make_context(*single_track_box(n_steps=1, dtype="float64"))
+        s = SpectralSolver(get_backend("mi355x"), ctx)
+        s.initialize()
+        s.step(0.0, ctx.num.dt)
+        r = OracleSolver(make_context(*single_track_box(n_steps=1))).initialize()
+        t0 = time.perf_counter()
+        r.step(0.0, ctx.num.dt)
+        t_cpu = time.perf_counter() - t0
+        a = s.state.a.double().cpu().numpy()
+        N = int(np.prod(tuple(ctx.geom.n)))
+        out["fp64_vs_oracle_rel_l2_step1"] = float(f"{np.linalg.norm(a - r.a) / np.linalg.norm(r.a):.3e}")
+        out["cpu_oracle_mode_steps_per_s"] = round(N / t_cpu, 1)
+        out["fp64_speedup_vs_cpu_oracle"] = round(res["float64"]["mode_steps_per_s"] / (N / t_cpu), 1)
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -112,6 +175,10 @@ def main():
     from spectral_galerkin_heat_amd.workloads import batch_uneven_cuboids, make_context
 
     assert torch.cuda.is_available(), "bench.py needs the MI355X"
+    if args.box_only:
+        print(json.dumps({"box_mode": box_mode_leg(max(1, args.box_steps), not args.no_cpu)}),
+              flush=True)
+        return
     if args.copy_probe_mib > 0:
         n = args.copy_probe_mib * (1 << 20) // 4
         src = torch.ones(n, dtype=torch.float32, device="cuda")
@@ -268,6 +335,10 @@ def main():
         log(f"plain fp64 path: {t_plain:.2f} s for {P} steps x 64 cuboids -> {plain_rate:.3e}")
         del solvers
 
+    box = None
+    if args.box_steps > 0:
+        box = box_mode_leg(args.box_steps, not args.no_cpu)
+
     cpu = None
     if not args.no_cpu:
         from oracle.sgh_oracle import OracleSolver
@@ -373,6 +444,7 @@ def main():
         "cpu_baseline": cpu,
         "serving": serving,
         "serving_sweep": sweep,
+        "box_mode": box,
     }
     print(json.dumps(out), flush=True)
 
