@@ -125,6 +125,108 @@ class SpectralSolver(HeatSolver):
         scale = float(dtype(math.sqrt(float(geom.d.x * geom.d.y * geom.d.z))))
         self.state.a = (kernels.DCT_II(T) * scale).contiguous()
 
+    # -- full nonlinear checkpoint --------------------------------------------------------
+    CHECKPOINT_VERSION = 1
+
+    def checkpoint(self, step: int = -1) -> dict:
+        """Everything a later step reads besides the configuration, as host arrays.
+
+        ``set_state`` (spectral.py:588-619) restores only the modes; a nonlinear run also
+        carries the previous evaporation flux, the previous full-volume temperature
+        (property correction / grid-mode latent heat), the replayed latent modes (grid
+        mode) and the box history (box mode). Restoring all of them makes a resumed run
+        repeat the uninterrupted one bit for bit.
+        """
+        if self.state is None:
+            raise RuntimeError("Solver must be initialized before calling checkpoint().")
+        S, b = self.state, self.state.buffers
+        host = lambda t: t.detach().cpu().numpy().copy()  # noqa: E731
+        ck = {"version": self.CHECKPOINT_VERSION, "step": int(step),
+              "shape": list(S.a.shape), "dtype": str(S.a.dtype),
+              "a": host(S.a), "q_evap_old": host(b.q_evap_old),
+              "picard_cap_warned": bool(self._picard_cap_warned),
+              "latent_truncation_warned": bool(self._latent_truncation_warned)}
+        if self._T_prev_full is not None:
+            ck["T_prev_full"] = host(self._T_prev_full)
+        if self._Q_latent_modes_prev is not None:
+            ck["Q_latent_modes_prev"] = host(self._Q_latent_modes_prev)
+        fm = S.fine_mesh
+        if fm is not None:
+            ck["box_shape"] = list(fm.shape)
+            if fm.T_prev is not None:
+                ck["box_T_prev"] = host(fm.T_prev)
+            if fm.Q_prev is not None:
+                ck["box_Q_prev"] = host(fm.Q_prev)
+        return ck
+
+    def restore(self, ck: dict) -> int:
+        """Load a ``checkpoint()`` into an initialized solver of the same configuration;
+        returns the step it was taken after (-1 if none was recorded)."""
+        if self.state is None:
+            raise RuntimeError("Solver must be initialized before calling restore().")
+        S, b = self.state, self.state.buffers
+        if int(ck.get("version", -1)) != self.CHECKPOINT_VERSION:
+            raise ValueError(f"checkpoint version {ck.get('version')} is not "
+                             f"{self.CHECKPOINT_VERSION}")
+        if list(ck["shape"]) != list(S.a.shape) or str(ck["dtype"]) != str(S.a.dtype):
+            raise ValueError(f"checkpoint of shape {list(ck['shape'])} / {ck['dtype']} does not "
+                             f"match this solver ({list(S.a.shape)} / {S.a.dtype})")
+
+        def put(dst, arr):
+            dst.copy_(torch.as_tensor(arr, dtype=dst.dtype))
+
+        put(S.a, ck["a"])
+        put(b.q_evap_old, ck["q_evap_old"])
+        if "T_prev_full" in ck:
+            if self._T_prev_full is None:
+                raise ValueError("checkpoint carries T_prev_full; this solver has no "
+                                 "property correction or grid-mode latent heat")
+            put(self._T_prev_full, ck["T_prev_full"])
+        self._Q_latent_modes_prev = None
+        if "Q_latent_modes_prev" in ck:
+            self._Q_latent_modes_prev = torch.as_tensor(ck["Q_latent_modes_prev"]).to(
+                device=S.a.device, dtype=S.a.dtype).contiguous()
+        fm = S.fine_mesh
+        if ("box_shape" in ck) != (fm is not None):
+            raise ValueError("checkpoint and solver disagree on box mode")
+        if fm is not None:
+            if list(ck["box_shape"]) != list(fm.shape):
+                raise ValueError(f"checkpoint box {list(ck['box_shape'])} != {list(fm.shape)}")
+            fm.T_prev = fm.Q_prev = None
+            if "box_T_prev" in ck:
+                put(fm._T, ck["box_T_prev"])
+                fm.T_prev = fm._T
+            if "box_Q_prev" in ck:
+                put(fm._Q, ck["box_Q_prev"])
+                fm.Q_prev = fm._Q
+        self._picard_cap_warned = bool(ck.get("picard_cap_warned", False))
+        self._latent_truncation_warned = bool(ck.get("latent_truncation_warned", False))
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+        return int(ck.get("step", -1))
+
+    def save_checkpoint(self, path, step: int = -1) -> None:
+        """``checkpoint()`` as an uncompressed ``.npz`` (arrays and scalars only, no pickle)."""
+        import numpy as np
+        ck = self.checkpoint(step)
+        flat = {k: (np.asarray(v) if not isinstance(v, str) else np.array(v)) for k, v in ck.items()}
+        np.savez(path, **flat)
+
+    def load_checkpoint(self, path) -> int:
+        import numpy as np
+        with np.load(path, allow_pickle=False) as z:
+            ck = {k: z[k] for k in z.files}
+        for k in ("version", "step"):
+            ck[k] = int(ck[k])
+        for k in ("shape", "box_shape"):
+            if k in ck:
+                ck[k] = [int(v) for v in ck[k]]
+        ck["dtype"] = str(ck["dtype"])
+        for k in ("picard_cap_warned", "latent_truncation_warned"):
+            if k in ck:
+                ck[k] = bool(ck[k])
+        return self.restore(ck)
+
     def finalize(self) -> None:
         """Release device buffers held by the state."""
         self.state = None

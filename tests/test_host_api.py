@@ -222,3 +222,21 @@ def test_box_workloads_parse_with_a_fine_mesh():
     assert ctx.fine.refinement == 2 and tuple(ctx.geom.n) == (80, 48, 120)
     ctx = make_context(*tdep_track_box())
     assert ctx.fine is not None and not ctx.mat.model.is_constant
+
+
+def test_runner_reads_a_reference_style_yaml_with_gcode(tmp_path):
+    from spectral_galerkin_heat_amd.runner import load_context
+    (tmp_path / "paths").mkdir()
+    (tmp_path / "paths" / "t.gcode").write_text("G21\nG90\nG0 X0.2 Y0.15\nM3 S24\nG1 X0.4 Y0.15 F9000\nM5\n")
+    (tmp_path / "c.yaml").write_text(
+        "simulation: {backend: cpu, duration: {value: 2.0e-4, unit: s}, dt: {value: 4.0e-6, unit: s}}\n"
+        "domain: {size: {value: [0.5e-3, 0.3e-3, 0.12e-3], unit: m}, mesh: {value: [20, 12, 16], unit: ''}}\n"
+        "fine_mesh: {refinement: 2, box_size: [0.2e-3, 0.15e-3, 0.04e-3]}\n"
+        "material: {name: x, rho: 7900.0, k: 20.0, Cp: 500.0, T0: 293.0}\n"
+        "laser: {radius: {value: 30.0e-6, unit: m}, absorptivity: 0.3, power_nominal: {value: 24.0, unit: W},\n"
+        "        path: {type: gcode, file: t.gcode}}\n"
+        "io: {}\n")
+    ctx = load_context(tmp_path / "c.yaml", backend="mi355x", n_steps=7)
+    assert ctx.backend == "mi355x" and ctx.num.n_steps == 7 and ctx.fine.refinement == 2
+    ls = ctx.laser_path.get_state(1e-4, 4e-6)
+    assert ls.is_on and abs(ls.x - 0.215e-3) < 1e-6 and abs(ls.v[0] - 0.15) < 1e-3
