@@ -96,6 +96,24 @@ int sgh_picard_iter_fused(int dtype, int nx, int ny, int nz, const void* aK, con
                           const void* Sb, const void* Q, double omega, const void* wz,
                           int nchunks, void* zc_parts, void* a_new, double* partials,
                           void* stream);
+/* The same pass with the norm reduction of sgh_reduce_norms folded in: the last
+ * workgroup to finish (counted on `counter`, a device word that must start at 0 and is
+ * left at 0) sums every pair in sgh_reduce_norms' fixed order and writes norms[3]
+ * (spectral.py:464-468), so one launch replaces two.  If host_out (pinned, coherent
+ * host memory from sgh_host_alloc) is given, it also writes the three norms there and
+ * then host_out[3] = seq, so the host can poll for this iteration's residual instead of
+ * waiting on the stream (the host read of spectral.py:468). */
+int sgh_picard_iter_fused_norms(int dtype, int nx, int ny, int nz, const void* aK,
+                                const void* KK, const void* a_old, const void* wt,
+                                const void* St, const void* wb, const void* Sb, const void* Q,
+                                double omega, const void* wz, int nchunks, void* zc_parts,
+                                void* a_new, double* partials, unsigned int* counter,
+                                double* norms, double* host_out, double seq, void* stream);
+
+/* Pinned, coherent host memory that kernels can write and the host can poll
+ * (hipHostMalloc with hipHostMallocCoherent); no reference counterpart. */
+int sgh_host_alloc(size_t bytes, void** out);
+int sgh_host_free(void* p);
 
 /* out[0] = √(Σr²/N), out[1] = √(Σa²/N), out[2] = out[0]/max(out[1], 1e-9), summed in a
  * fixed order.  Replaces the norm of spectral.py:464-468. */
@@ -300,12 +318,13 @@ int sgh_batch_rewind(void* plan, void* stream);
  * temperature (z-partial sum + 2-D inverse), evaporation / convection / k̄/k(T) flux
  * forcing, its projection, the latent source (coarse grid: reconstruct, source, project;
  * fine box: the fused sgh_box_latent_source then the window projection), the property
- * correction when `rescale`, the fused ETD1 + relaxation + norm pass, the norm
- * reduction, and the copy of the three norms to norms_host[3·parity ..] (pinned host
- * memory).  Buffers indexed by `parity` (= iteration & 1) alternate so that the next
- * iteration can be enqueued before this one's residual is read: a_old = a_buf[parity],
- * a_new = a_buf[parity ^ 1]; q_evap, Q_latent, Q_modes and norms_host by parity.
- * Every pointer is a device pointer except norms_host. */
+ * correction when `rescale`, and the fused ETD1 + relaxation + norm pass whose last
+ * workgroup writes the norms to norms_host[4·parity ..] followed by `seq` (pinned,
+ * coherent host memory from sgh_host_alloc), which the host polls.  Buffers indexed by
+ * `parity` (= iteration & 1) alternate so that the next iteration can be enqueued
+ * before this one's residual is read: a_old = a_buf[parity], a_new = a_buf[parity ^ 1];
+ * q_evap, Q_latent, Q_modes and norms_host by parity.  Every pointer is a device pointer
+ * except norms_host, which both sides address. */
 typedef struct SghPlainIter {
   int dtype, xform, nx, ny, nz, nchunks, n_pairs;
   int latent_mode;     /* 0 none, 1 coarse grid, 2 fine box */
@@ -332,6 +351,8 @@ typedef struct SghPlainIter {
   const void* fBx; const void* fBy; const void* fBz;
   const void* T_prev_box; void* box_work;  /* box_work: 2·nzb·max(ny, nyb)·nx */
   double* partials; double* norms; double* norms_host;
+  unsigned int* counter;   /* device word at 0: the fused pass's last-workgroup count */
+  double seq;              /* written after the norms; the host waits for it */
 } SghPlainIter;
 
 /* sizeof(SghPlainIter), so a binding can check its mirror of the layout. */

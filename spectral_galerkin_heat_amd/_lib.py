@@ -32,6 +32,10 @@ SIGNATURES = {
     "sgh_picard_iter_fused": [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, f64, vp, i32,
                               vp, vp, vp, vp],
     "sgh_reduce_norms": [vp, i32, f64, vp, vp],
+    "sgh_picard_iter_fused_norms": [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, f64, vp,
+                                    i32, vp, vp, vp, vp, vp, vp, f64, vp],
+    "sgh_host_alloc": [C.c_size_t, C.POINTER(vp)],
+    "sgh_host_free": [vp],
     "sgh_reduce": [i32, i64, vp, i32, vp, vp],
     "sgh_sep2d": [i32, i32, i32, i32, vp, vp, vp, i32, f64, vp, vp],
     "sgh_sum_parts": [i32, i64, i32, vp, vp],
@@ -115,7 +119,8 @@ class SghPlainIter(C.Structure):
            ("a_raw", vp), ("corr_work", vp), ("corr_f", vp), ("C_modes", vp),
            ("fBxT", vp), ("fByT", vp), ("fBzT", vp), ("fBx", vp), ("fBy", vp), ("fBz", vp),
            ("T_prev_box", vp), ("box_work", vp),
-           ("partials", vp), ("norms", vp), ("norms_host", vp)])
+           ("partials", vp), ("norms", vp), ("norms_host", vp), ("counter", vp),
+           ("seq", C.c_double)])
 
 
 class SGHError(RuntimeError):

@@ -35,7 +35,7 @@ int sgh_plain_iterate(const SghPlainIter* d, int parity, void* stream) {
   SGH_REQUIRE(d && (parity == 0 || parity == 1), SGH_ERR_ARG, "sgh_plain_iterate: bad args");
   SGH_REQUIRE(d->latent_mode >= 0 && d->latent_mode <= 2, SGH_ERR_ARG,
               "sgh_plain_iterate: latent_mode %d", d->latent_mode);
-  SGH_REQUIRE(d->norms_host && d->norms && d->partials, SGH_ERR_ARG,
+  SGH_REQUIRE(d->norms_host && d->norms && d->partials && d->counter, SGH_ERR_ARG,
               "sgh_plain_iterate: missing norm buffers");
   const int es = d->dtype == SGH_F64 ? 8 : 4;
   const int nx = d->nx, ny = d->ny, nz = d->nz;
@@ -94,19 +94,18 @@ int sgh_plain_iterate(const SghPlainIter* d, int parity, void* stream) {
                                            d->prog, d->kbar, d->abar, d->dt, d->dx, d->dy,
                                            d->dz, d->corr_work, d->corr_f, stream));
     SGH_TRY(volume_transform(d, d->vBx, d->vBy, d->vBz, d->corr_f, d->dV, d->C_modes, stream));
-    SGH_TRY(sgh_picard_iter_fused(d->dtype, nx, ny, nz, d->a_raw, d->KK, a_old, d->w_top,
-                                  d->S_zero, nullptr, nullptr, d->C_modes, d->omega, d->w_top,
-                                  d->nchunks, d->zc, a_new, d->partials, stream));
+    SGH_TRY(sgh_picard_iter_fused_norms(d->dtype, nx, ny, nz, d->a_raw, d->KK, a_old, d->w_top,
+                                        d->S_zero, nullptr, nullptr, d->C_modes, d->omega,
+                                        d->w_top, d->nchunks, d->zc, a_new, d->partials,
+                                        d->counter, d->norms, d->norms_host + 4 * parity,
+                                        d->seq, stream));
   } else {
-    SGH_TRY(sgh_picard_iter_fused(d->dtype, nx, ny, nz, d->aK, d->KK, a_old, d->w_top,
-                                  d->S_top, wb, Sb, Q_modes, d->omega, d->w_top, d->nchunks,
-                                  d->zc, a_new, d->partials, stream));
+    SGH_TRY(sgh_picard_iter_fused_norms(d->dtype, nx, ny, nz, d->aK, d->KK, a_old, d->w_top,
+                                        d->S_top, wb, Sb, Q_modes, d->omega, d->w_top,
+                                        d->nchunks, d->zc, a_new, d->partials, d->counter,
+                                        d->norms, d->norms_host + 4 * parity, d->seq, stream));
   }
-  SGH_TRY(sgh_reduce_norms(d->partials, d->n_pairs, d->n_elements, d->norms, stream));
 #undef SGH_TRY
-  hipError_t e = hipMemcpyAsync(d->norms_host + 3 * parity, d->norms, 3 * sizeof(double),
-                                hipMemcpyDeviceToHost, reinterpret_cast<hipStream_t>(stream));
-  if (e != hipSuccess) return sgh::fail_hip(e, "sgh_plain_iterate: norm copy");
   return SGH_OK;
 }
 

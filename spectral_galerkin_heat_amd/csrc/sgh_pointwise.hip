@@ -130,8 +130,11 @@ k_picard_fused(const F* __restrict__ aK, const F* __restrict__ KK, const F* __re
                const F* __restrict__ wt, const F* __restrict__ St, const F* __restrict__ wb,
                const F* __restrict__ Sb, const F* __restrict__ Q, F omega,
                const F* __restrict__ wz, F* __restrict__ zc_parts, F* __restrict__ a_new,
-               double* __restrict__ partials, int S, int nz, int pc) {
+               double* __restrict__ partials, int S, int nz, int pc,
+               unsigned int* __restrict__ counter, double n_elements,
+               double* __restrict__ norms, double* __restrict__ host_out, double seq) {
   __shared__ double sh[kThreads / kWave];
+  __shared__ int s_last;
   const int s = blockIdx.x * kThreads + threadIdx.x;
   const int p0 = blockIdx.y * pc;
   const int p1 = min(nz, p0 + pc);
@@ -161,6 +164,43 @@ k_picard_fused(const F* __restrict__ aK, const F* __restrict__ KK, const F* __re
     const long long b = (long long)blockIdx.y * gridDim.x + blockIdx.x;
     partials[2 * b] = trr;
     partials[2 * b + 1] = taa;
+  }
+  if (norms == nullptr) return;
+  // the last block to finish sums every block's pair in the fixed order of
+  // k_reduce_norms and resets the counter for the next launch
+  if (threadIdx.x == 0) {
+    __threadfence();   // this block's pair is visible device-wide before it is counted
+    const unsigned int nb = gridDim.x * gridDim.y;
+    s_last = atomicAdd(counter, 1u) == nb - 1;
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();     // every block's pair is visible to this one
+  const int n_pairs = gridDim.x * gridDim.y;
+  double r2 = 0.0, a2 = 0.0;
+  for (int i = threadIdx.x; i < n_pairs; i += kThreads) {
+    r2 += partials[2 * i];
+    a2 += partials[2 * i + 1];
+  }
+  const double tr = block_sum(r2, sh);
+  const double ta = block_sum(a2, sh);
+  if (threadIdx.x == 0) {
+    const double rms_diff = sqrt(tr / n_elements);
+    const double rms_old = sqrt(ta / n_elements);
+    const double rel = rms_diff / (rms_old > 1e-9 ? rms_old : 1e-9);
+    norms[0] = rms_diff;
+    norms[1] = rms_old;
+    norms[2] = rel;
+    *counter = 0u;
+    if (host_out) {
+      // straight to pinned host memory: the three norms, then (after a system-scope
+      // fence) the sequence number the host polls for
+      host_out[0] = rms_diff;
+      host_out[1] = rms_old;
+      host_out[2] = rel;
+      __threadfence_system();
+      host_out[3] = seq;
+    }
   }
 }
 
@@ -444,6 +484,21 @@ extern "C" {
 
 int sgh_version(void) { return 100; }
 
+int sgh_host_alloc(size_t bytes, void** out) {
+  SGH_REQUIRE(bytes > 0 && out, SGH_ERR_ARG, "sgh_host_alloc: bad args");
+  hipError_t e = hipHostMalloc(out, bytes, hipHostMallocCoherent);
+  if (e != hipSuccess) return fail_hip(e, "sgh_host_alloc");
+  return SGH_OK;
+}
+
+int sgh_host_free(void* p) {
+  if (p) {
+    hipError_t e = hipHostFree(p);
+    if (e != hipSuccess) return fail_hip(e, "sgh_host_free");
+  }
+  return SGH_OK;
+}
+
 const char* sgh_last_error(void) { return g_last_error.c_str(); }
 
 int sgh_property_program_bytes(void) { return (int)sizeof(PropProgram); }
@@ -546,13 +601,18 @@ int sgh_picard_partials_count(int nx, int ny, int nz, int nchunks) {
   return (int)cdiv((long long)nx * ny, kThreads) * nchunks;
 }
 
-int sgh_picard_iter_fused(int dtype, int nx, int ny, int nz, const void* aK, const void* KK,
-                          const void* a_old, const void* wt, const void* St, const void* wb,
-                          const void* Sb, const void* Q, double omega, const void* wz,
-                          int nchunks, void* zc_parts, void* a_new, double* partials,
-                          void* stream) {
+int sgh_picard_iter_fused_norms(int dtype, int nx, int ny, int nz, const void* aK,
+                                const void* KK, const void* a_old, const void* wt,
+                                const void* St, const void* wb, const void* Sb, const void* Q,
+                                double omega, const void* wz, int nchunks, void* zc_parts,
+                                void* a_new, double* partials, unsigned int* counter,
+                                double* norms, double* host_out, double seq, void* stream) {
   SGH_REQUIRE(nx > 0 && ny > 0 && nz > 0 && aK && KK && a_old && wt && St && a_new && partials,
               SGH_ERR_ARG, "sgh_picard_iter_fused: bad args");
+  SGH_REQUIRE((counter == nullptr) == (norms == nullptr), SGH_ERR_ARG,
+              "sgh_picard_iter_fused: counter and norms must both be given or both be NULL");
+  SGH_REQUIRE(host_out == nullptr || norms != nullptr, SGH_ERR_ARG,
+              "sgh_picard_iter_fused: host_out needs the norm reduction");
   SGH_REQUIRE(nchunks >= 1 && nchunks <= nz, SGH_ERR_ARG, "sgh_picard_iter_fused: nchunks");
   SGH_REQUIRE((wb == nullptr) == (Sb == nullptr), SGH_ERR_ARG,
               "sgh_picard_iter_fused: wb and Sb must both be given or both be NULL");
@@ -565,10 +625,21 @@ int sgh_picard_iter_fused(int dtype, int nx, int ny, int nz, const void* aK, con
     hipLaunchKernelGGL(k_picard_fused<F>, dim3(cdiv(S, kThreads), nchunks), dim3(kThreads), 0,
                        STREAM(stream), (const F*)aK, (const F*)KK, (const F*)a_old,
                        (const F*)wt, (const F*)St, (const F*)wb, (const F*)Sb, (const F*)Q,
-                       (F)omega, (const F*)wz, (F*)zc_parts, (F*)a_new, partials, S, nz, pc);
+                       (F)omega, (const F*)wz, (F*)zc_parts, (F*)a_new, partials, S, nz, pc,
+                       counter, (double)nx * ny * nz, norms, host_out, seq);
   });
   SGH_CHECK_LAUNCH("sgh_picard_iter_fused");
   return SGH_OK;
+}
+
+int sgh_picard_iter_fused(int dtype, int nx, int ny, int nz, const void* aK, const void* KK,
+                          const void* a_old, const void* wt, const void* St, const void* wb,
+                          const void* Sb, const void* Q, double omega, const void* wz,
+                          int nchunks, void* zc_parts, void* a_new, double* partials,
+                          void* stream) {
+  return sgh_picard_iter_fused_norms(dtype, nx, ny, nz, aK, KK, a_old, wt, St, wb, Sb, Q, omega,
+                                     wz, nchunks, zc_parts, a_new, partials, nullptr, nullptr,
+                                     nullptr, 0.0, stream);
 }
 
 int sgh_reduce_norms(const double* partials, int n_pairs, double n_elements, double* out,

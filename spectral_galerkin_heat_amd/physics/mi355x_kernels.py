@@ -686,9 +686,43 @@ class _PlainIter:
             d.a_raw, d.corr_work, d.corr_f, d.C_modes = (ptr(v) for v in self.corr)
             d.kbar, d.abar = float(solver._kbar), float(solver._abar)
         d.partials, d.norms = ptr(b.partials), ptr(b.norms)
-        self.norms_host = torch.zeros(6, dtype=torch.float64).pin_memory()
-        d.norms_host = self.norms_host.data_ptr()
-        self.events = (torch.cuda.Event(), torch.cuda.Event())
+        self.counter = torch.zeros(1, dtype=torch.int32, device=dev)
+        d.counter = ptr(self.counter)
+        # pinned, coherent host words the fused pass writes: per parity the three norms
+        # and the sequence number of the iteration that wrote them
+        hp = ctypes.c_void_p()
+        call("sgh_host_alloc", 8 * 8, ctypes.byref(hp))
+        self._host_ptr = hp.value
+        self.norms_host = np.ctypeslib.as_array((ctypes.c_double * 8).from_address(hp.value))
+        self.norms_host[:] = 0.0
+        d.norms_host = hp.value
+        self.seq = 0
+        self.seq_of = [0, 0]
+
+    def __del__(self):
+        ptr_ = getattr(self, "_host_ptr", None)
+        if ptr_:
+            try:
+                _lib.lib.sgh_host_free(ptr_)
+            except Exception:  # interpreter shutdown
+                pass
+            self._host_ptr = None
+
+    def wait(self, k, timeout_s=60.0):
+        """The norms of iteration k once its fused pass has written them."""
+        import time as _time
+        p = k & 1
+        want = float(self.seq_of[p])
+        h = self.norms_host
+        if h[4 * p + 3] != want:
+            t0 = _time.perf_counter()
+            while h[4 * p + 3] != want:
+                if _time.perf_counter() - t0 > timeout_s:
+                    torch.cuda.synchronize()   # raises a device error if one occurred
+                    if h[4 * p + 3] != want:
+                        raise _lib.SGHError(f"Picard iteration {k}: no residual after "
+                                            f"{timeout_s} s")
+        return float(h[4 * p]), float(h[4 * p + 1]), float(h[4 * p + 2])
 
     def refresh(self, solver, S, prog, kbar, have_bot):
         d, fm = self.d, S.fine_mesh
@@ -788,11 +822,12 @@ def picard_step(solver, t, dt):
     iter_k, rel, conv = -1, float("nan"), False
     pi = _plain_iter(solver, S, prog, kbar, have_bot)
     maxit = int(solver.max_picard_iter)
-    stream = torch.cuda.current_stream()
 
     def enqueue(k):
+        pi.seq += 1
+        pi.seq_of[k & 1] = pi.seq
+        pi.d.seq = float(pi.seq)
         call("sgh_plain_iterate", pi.ref, k & 1, st)
-        pi.events[k & 1].record(stream)
 
     # The next iteration is enqueued before this one's residual is read, so the device
     # never waits on the host; when iteration k converges, iteration k + 1 (which wrote
@@ -802,9 +837,7 @@ def picard_step(solver, t, dt):
     for iter_k in range(maxit):
         if iter_k + 1 < maxit:
             enqueue(iter_k + 1)
-        pi.events[iter_k & 1].synchronize()
-        h = pi.norms_host[3 * (iter_k & 1):3 * (iter_k & 1) + 3].tolist()
-        rms_diff, rel = h[0], h[2]
+        rms_diff, _, rel = pi.wait(iter_k)
         if solver.track_picard_history:
             rho_k = None
             if iter_k > 0 and solver.picard_history:
