@@ -134,6 +134,18 @@ int sgh_reduce(int dtype, long long n, const void* x, int op, double* out, void*
 int sgh_sep2d(int dtype, int batch, int ny, int nx, const void* L, const void* R,
               const void* X, int nparts, double scale, void* out, void* stream);
 
+/* The inverse surface transform fused with the top-surface forcing: T = scale·L·X·R
+ * (one surface), then q_evap = evaporation(T) and q_net = (q_las − q_evap − h_top(T − T0))
+ * [· k̄/k(T) with a property program] in the same launch, as sgh_sep2d followed by
+ * sgh_surface_forcing (spectral_ops.py:210-219 then spectral.py:392-406).  Surfaces the
+ * one-round kernel cannot hold, and forcing with a property program (whose per-cell
+ * interpreter is better spread over the separate forcing launch), run those two calls,
+ * T landing in T_s (which the fused form leaves untouched). */
+int sgh_sep2d_forcing(int dtype, int ny, int nx, const void* L, const void* R, const void* X,
+                      double scale, const void* q_las, double P0, double T_boil, double dH,
+                      double R_v, double T_liq, double h_top, double T0, const void* prog,
+                      double kbar, void* T_s, void* q_evap, void* q_net, void* stream);
+
 /* out[o,k,i] = scale · Σ_j M[k,j]·in[o,j,i] for an [outer, n, inner] view; M is n×n
  * row-major; variant = SGH_XFORM_*.  Three calls (x, y, z) make the 3-D DCT of
  * project_volume / reconstruct_volume (spectral_ops.py:83-102) and of DCT_II / IDCT_II

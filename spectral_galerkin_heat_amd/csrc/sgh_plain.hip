@@ -50,10 +50,10 @@ int sgh_plain_iterate(const SghPlainIter* d, int parity, void* stream) {
   } while (0)
   // top-surface temperature of a_old: the z-partials left by the previous fused pass
   SGH_TRY(sgh_sum_parts(d->dtype, S, d->nchunks, d->zc, stream));
-  SGH_TRY(sgh_sep2d(d->dtype, 1, ny, nx, d->sByT, d->sBx, d->zc, 1, 1.0, d->T_s, stream));
-  SGH_TRY(sgh_surface_forcing(d->dtype, S, d->T_s, d->q_las, d->Pa, d->T_boil, d->dH, d->R_v,
-                              d->T_liq, d->h_top, d->T0, d->rescale ? d->prog : nullptr,
-                              d->kbar, d->q_evap[parity], d->q_tmp, stream));
+  SGH_TRY(sgh_sep2d_forcing(d->dtype, ny, nx, d->sByT, d->sBx, d->zc, 1.0, d->q_las, d->Pa,
+                            d->T_boil, d->dH, d->R_v, d->T_liq, d->h_top, d->T0,
+                            d->rescale ? d->prog : nullptr, d->kbar, d->T_s,
+                            d->q_evap[parity], d->q_tmp, stream));
   SGH_TRY(sgh_sep2d(d->dtype, 1, ny, nx, d->sBy, d->sBxT, d->q_tmp, 1, d->dxdy, d->S_top,
                     stream));
   const void* Q_modes = nullptr;

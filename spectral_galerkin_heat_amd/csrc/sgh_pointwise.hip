@@ -369,27 +369,14 @@ k_face_peaks_final(const double* __restrict__ part, int nz, double* __restrict__
 }
 
 template <typename F>
-__device__ __forceinline__ F evap_flux(F T, F f1, F f2, F T_boil, F T_liq) {
-  if (T < T_liq) return F(0);
-  return f1 * ((F(1) / dsqrt(T)) * dexp(f2 * (F(1) - T_boil / T)));
-}
-
-template <typename F>
 __global__ void __launch_bounds__(kThreads)
-k_surface_forcing(const F* __restrict__ T, const F* __restrict__ q_las, long long S, F f1,
-                  F f2, F T_boil, F T_liq, F h_top, F T0, const PropProgram* __restrict__ prog,
-                  F kbar, F* __restrict__ q_evap, F* __restrict__ q_net) {
+k_surface_forcing(const F* __restrict__ T, const F* __restrict__ q_las, long long S,
+                  ForcingParams<F> fp, F* __restrict__ q_evap, F* __restrict__ q_net) {
   for (long long s = blockIdx.x * (long long)kThreads + threadIdx.x; s < S;
        s += (long long)gridDim.x * kThreads) {
-    const F t = T[s];
-    const F qe = evap_flux(t, f1, f2, T_boil, T_liq);
+    F qe, qn;
+    surface_forcing_value<F>(fp, T[s], q_las ? q_las[s] : F(0), qe, qn);
     if (q_evap) q_evap[s] = qe;
-    F qn = (q_las ? q_las[s] : F(0)) - qe;
-    if (h_top > F(0)) qn -= h_top * (t - T0);
-    if (prog) {
-      const F fl = prog_fl<F>(prog, t);
-      qn = qn * (kbar / prog_blend<F>(prog, kKSolid, t, fl));
-    }
     q_net[s] = qn;
   }
 }
@@ -732,13 +719,11 @@ int sgh_surface_forcing(int dtype, long long S, const void* T, const void* q_las
                         void* stream) {
   SGH_REQUIRE(S > 0 && T && q_net, SGH_ERR_ARG, "sgh_surface_forcing: bad args");
   SGH_REQUIRE(R_v > 0 && T_boil > 0, SGH_ERR_ARG, "sgh_surface_forcing: R_v and T_boil must be > 0");
-  const double f1 = 0.82 * dH * P0 / sqrt(2.0 * M_PI * R_v);
-  const double f2 = dH / (R_v * T_boil);
   SGH_DISPATCH(dtype, {
     hipLaunchKernelGGL(k_surface_forcing<F>, dim3(grid_for(S)), dim3(kThreads), 0,
-                       STREAM(stream), (const F*)T, (const F*)q_las, S, (F)f1, (F)f2,
-                       (F)T_boil, (F)T_liq, (F)h_top, (F)T0, (const PropProgram*)prog,
-                       (F)kbar, (F*)q_evap, (F*)q_net);
+                       STREAM(stream), (const F*)T, (const F*)q_las, S,
+                       make_forcing<F>(P0, T_boil, dH, R_v, T_liq, h_top, T0, prog, kbar),
+                       (F*)q_evap, (F*)q_net);
   });
   SGH_CHECK_LAUNCH("sgh_surface_forcing");
   return SGH_OK;

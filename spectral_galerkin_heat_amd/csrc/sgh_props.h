@@ -10,6 +10,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 
 namespace sgh {
@@ -161,6 +162,56 @@ __device__ __forceinline__ F latent_value(const LatentParams<F>& p, F t, F tp) {
     fp = fp < F(0) ? F(0) : (fp > F(1) ? F(1) : fp);
   }
   return (-p.rho * p.L / p.dt) * (fc - fp);
+}
+
+// Top-surface forcing of one cell at temperature t: the evaporation flux (0.82·ΔH·P0 /
+// √(2πR_v) · T^-½ · exp(ΔH/(R_v T_b)·(1 − T_b/T)), zero below T_liq) and the net flux
+// q_las − q_evap − h_top(T − T0), rescaled by k̄/k(T) with a property program.
+template <typename F>
+struct ForcingParams {
+  F f1, f2, T_boil, T_liq, h_top, T0, kbar;
+  const PropProgram* prog;
+};
+
+__device__ __forceinline__ double fexp(double x) { return exp(x); }
+__device__ __forceinline__ float fexp(float x) { return expf(x); }
+__device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float fsqrt(float x) { return sqrtf(x); }
+
+template <typename F>
+__device__ __forceinline__ F evap_flux(F T, F f1, F f2, F T_boil, F T_liq) {
+  if (T < T_liq) return F(0);
+  return f1 * ((F(1) / fsqrt(T)) * fexp(f2 * (F(1) - T_boil / T)));
+}
+
+template <typename F>
+__device__ __forceinline__ void surface_forcing_value(const ForcingParams<F>& p, F t, F q_las,
+                                                      F& q_evap, F& q_net) {
+  const F qe = evap_flux(t, p.f1, p.f2, p.T_boil, p.T_liq);
+  q_evap = qe;
+  F qn = q_las - qe;
+  if (p.h_top > F(0)) qn -= p.h_top * (t - p.T0);
+  if (p.prog) {
+    const F fl = prog_fl<F>(p.prog, t);
+    qn = qn * (p.kbar / prog_blend<F>(p.prog, kKSolid, t, fl));
+  }
+  q_net = qn;
+}
+
+template <typename F>
+inline ForcingParams<F> make_forcing(double P0, double T_boil, double dH, double R_v,
+                                     double T_liq, double h_top, double T0, const void* prog,
+                                     double kbar) {
+  ForcingParams<F> p;
+  p.f1 = (F)(0.82 * dH * P0 / sqrt(2.0 * M_PI * R_v));
+  p.f2 = (F)(dH / (R_v * T_boil));
+  p.T_boil = (F)T_boil;
+  p.T_liq = (F)T_liq;
+  p.h_top = (F)h_top;
+  p.T0 = (F)T0;
+  p.kbar = (F)kbar;
+  p.prog = (const PropProgram*)prog;
+  return p;
 }
 
 template <typename F>

@@ -147,11 +147,15 @@ k_sep2d(int ny, int nx, const F* __restrict__ L, const F* __restrict__ R,
 // Surfaces whose operands fit in LDS (and rows of a multiple of 16 bytes): everything the
 // workgroup reads lands in one round -- its 16 rows of L through registers, all of X and
 // its kSCols columns of R by LDS DMA (gfx950 global_load_lds_dwordx4, odd rows
-// swizzled), one wait -- then both MFMA stages run out of LDS.
-template <typename F>
+// swizzled), one wait -- then both MFMA stages run out of LDS. FORCE: the output is a
+// top-surface temperature and the epilogue writes its evaporation and net fluxes
+// (q_evap, q_net = out) instead.
+template <typename F, bool FORCE = false>
 __global__ void __launch_bounds__(kT)
 k_sep2d_direct(int ny, int nx, const F* __restrict__ L, const F* __restrict__ R,
-               const F* __restrict__ X, F scale, F* __restrict__ out) {
+               const F* __restrict__ X, F scale, F* __restrict__ out,
+               ForcingParams<F> fp = {}, const F* __restrict__ q_las = nullptr,
+               F* __restrict__ q_evap = nullptr) {
   using M = Mfma<F>;
   using V = typename M::V;
   constexpr int NW = kT / 64;
@@ -232,7 +236,17 @@ k_sep2d_direct(int ny, int nx, const F* __restrict__ L, const F* __restrict__ R,
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const int row = M::row(lane, v);
-    if (row < rows) Ob[(long long)(r0 + row) * nx + c0 + cl] = scale * acc2[v];
+    if (row < rows) {
+      const long long o = (long long)(r0 + row) * nx + c0 + cl;
+      if constexpr (FORCE) {
+        F qe, qn;
+        surface_forcing_value<F>(fp, scale * acc2[v], q_las ? q_las[o] : F(0), qe, qn);
+        if (q_evap) q_evap[o] = qe;
+        Ob[o] = qn;
+      } else {
+        Ob[o] = scale * acc2[v];
+      }
+    }
   }
 }
 
@@ -541,6 +555,55 @@ int sgh_sep2d(int dtype, int batch, int ny, int nx, const void* L, const void* R
   });
   SGH_CHECK_LAUNCH("sgh_sep2d");
   return SGH_OK;
+}
+
+int sgh_sep2d_forcing(int dtype, int ny, int nx, const void* L, const void* R, const void* X,
+                      double scale, const void* q_las, double P0, double T_boil, double dH,
+                      double R_v, double T_liq, double h_top, double T0, const void* prog,
+                      double kbar, void* T_s, void* q_evap, void* q_net, void* stream) {
+  SGH_REQUIRE(ny > 0 && nx > 0 && L && R && X && q_net && T_s && X != q_net, SGH_ERR_ARG,
+              "sgh_sep2d_forcing: bad args");
+  SGH_REQUIRE(R_v > 0 && T_boil > 0, SGH_ERR_ARG,
+              "sgh_sep2d_forcing: R_v and T_boil must be > 0");
+  bool fused = false;
+  SGH_DISPATCH(dtype, {
+    const int ny4 = (ny + 3) & ~3, nx4 = (nx + 3) & ~3;
+    const long long direct_elems = (long long)kSRows * (pad_a<F>(ny4) + pad_a<F>(nx4)) +
+                                   (long long)ny4 * nx + (long long)nx4 * std::min(kSCols, nx);
+    // the property interpreter's static LDS stack sits beside the dynamic tiles
+    const long long stack = (long long)kProgMaxStack * kProgThreads * (long long)sizeof(F);
+    const bool aligned =
+        ((reinterpret_cast<uintptr_t>(X) | reinterpret_cast<uintptr_t>(R)) & 15) == 0;
+    // with a property program the per-cell interpreter would run in the few workgroups
+    // of the transform; the separate forcing kernel spreads it over the whole surface
+    if (prog == nullptr && nx <= kMaxNx && aligned && (nx * (int)sizeof(F)) % 16 == 0 &&
+        direct_elems * (long long)sizeof(F) + stack <= kSLds) {
+      static bool raised[2] = {false, false};
+      if (!raised[sizeof(F) == 8]) {
+        hipError_t e = hipFuncSetAttribute(
+            reinterpret_cast<const void*>(&k_sep2d_direct<F, true>),
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)(kSLds - stack));
+        if (e != hipSuccess) return fail_hip(e, "sgh_sep2d_forcing LDS attribute");
+        raised[sizeof(F) == 8] = true;
+      }
+      hipLaunchKernelGGL((k_sep2d_direct<F, true>), dim3(cdiv(ny, kSRows), 1, cdiv(nx, kSCols)),
+                         dim3(kT), (size_t)direct_elems * sizeof(F),
+                         reinterpret_cast<hipStream_t>(stream), ny, nx, (const F*)L,
+                         (const F*)R, (const F*)X, (F)scale, (F*)q_net,
+                         make_forcing<F>(P0, T_boil, dH, R_v, T_liq, h_top, T0, prog, kbar),
+                         (const F*)q_las, (F*)q_evap);
+      fused = true;
+    }
+  });
+  if (fused) {
+    SGH_CHECK_LAUNCH("sgh_sep2d_forcing");
+    return SGH_OK;
+  }
+  // surfaces the one-round kernel cannot take: transform into T_s, then the forcing
+  int rc = sgh_sep2d(dtype, 1, ny, nx, L, R, X, 1, scale, T_s, stream);
+  if (rc != SGH_OK) return rc;
+  return sgh_surface_forcing(dtype, (long long)nx * ny, T_s, q_las, P0, T_boil, dH, R_v, T_liq,
+                             h_top, T0, prog, kbar, q_evap, q_net, stream);
 }
 
 int sgh_axis_contract_strided(int dtype, int variant, long long outer, int n_in, int n_out,

@@ -60,3 +60,43 @@ def test_fused_norm_pass_equals_separate_reduction(dtype, shape, nchunks):
         np.testing.assert_array_equal(host[4 * (n - 1):4 * (n - 1) + 3], outs[0][3])
         assert host[4 * (n - 1) + 3] == 10 + n
     _lib.call("sgh_host_free", hp.value)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("ny,nx,tdep", [(48, 80, False), (32, 64, True), (64, 128, True),
+                                       (20, 25, False), (128, 256, False)])
+def test_fused_inverse_transform_and_forcing_equal_the_two_calls(dtype, ny, nx, tdep):
+    from spectral_galerkin_heat_amd.core.properties import MaterialModel
+    from spectral_galerkin_heat_amd.physics import mi355x_kernels as K
+    rng = np.random.default_rng(ny + nx)
+    L = torch.as_tensor(K.dct_ortho_matrix(ny).T.copy()).to(DEV, dtype)
+    R = torch.as_tensor(K.dct_ortho_matrix(nx)).to(DEV, dtype)
+    X = torch.as_tensor(rng.standard_normal((ny, nx)) * 400.0).to(DEV, dtype)
+    X[0, 0] += 2000.0 * np.sqrt(nx * ny)
+    q_las = torch.as_tensor(rng.uniform(0, 1e9, (ny, nx))).to(DEV, dtype)
+    prog = None
+    if tdep:
+        model = MaterialModel.from_config({
+            "k": {"solid": "9.248 + 0.01571 * T", "liquid": "12.41 + 0.003279 * T",
+                  "reference": 24.5},
+            "rho": {"solid": "7900.0", "liquid": "7400.0", "reference": 7650.0},
+            "Cp": {"solid": "458.98 + 0.1328 * T", "liquid": "769.86", "reference": 633.9}},
+            1674.15, 1697.15)
+        prog = K._program(model)
+    code, st = _lib.dtype_code(dtype), _lib.stream_ptr()
+    phys = (101325.0, 3090.0, 7.416e6, 150.774, 1697.15, 10.0, 293.0)
+    T_ref = torch.empty_like(X)
+    qe_ref, qn_ref = torch.empty_like(X), torch.empty_like(X)
+    _lib.call("sgh_sep2d", code, 1, ny, nx, _lib.ptr(L), _lib.ptr(R), _lib.ptr(X), 1, 1.0,
+              _lib.ptr(T_ref), st)
+    _lib.call("sgh_surface_forcing", code, nx * ny, _lib.ptr(T_ref), _lib.ptr(q_las), *phys,
+              _lib.ptr(prog), 24.5 if tdep else 0.0, _lib.ptr(qe_ref), _lib.ptr(qn_ref), st)
+    T_s = torch.zeros_like(X)
+    qe, qn = torch.empty_like(X), torch.empty_like(X)
+    _lib.call("sgh_sep2d_forcing", code, ny, nx, _lib.ptr(L), _lib.ptr(R), _lib.ptr(X),
+              1.0, _lib.ptr(q_las), *phys, _lib.ptr(prog),
+              24.5 if tdep else 0.0, _lib.ptr(T_s), _lib.ptr(qe), _lib.ptr(qn), st)
+    torch.cuda.synchronize()
+    assert float(qe_ref.abs().max()) > 0.0    # some cells are above the liquidus
+    np.testing.assert_array_equal(qe.cpu().numpy(), qe_ref.cpu().numpy())
+    np.testing.assert_array_equal(qn.cpu().numpy(), qn_ref.cpu().numpy())
