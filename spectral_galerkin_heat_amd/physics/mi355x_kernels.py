@@ -703,6 +703,8 @@ class _PlainIter:
         ptr_ = getattr(self, "_host_ptr", None)
         if ptr_:
             try:
+                # a speculative iteration may still be writing these words
+                torch.cuda.synchronize()
                 _lib.lib.sgh_host_free(ptr_)
             except Exception:  # interpreter shutdown
                 pass
