@@ -184,6 +184,14 @@ int sgh_box_latent_source(int dtype, int variant, int nx, int ny, int nz, const 
                           double T_S, double T_L, double rho, double L_f, double dt,
                           const void* prog, void* work, void* Q, void* stream);
 
+/* Point evaluation of the cosine series, after the x contraction U = a ·ₓ Bx (U is
+ * nz × ny × P for P points): out[i] = Σ_p Bz[p][i] Σ_n By[n][i] U[p][n][i] with By
+ * (ny × P) and Bz (nz × P) the bases at the points.  With sgh_axis_contract_rect for the
+ * x step it replaces reconstruct_temperature_volume_at_points
+ * (sgh/physics/spectral_helpers.py:209-255). */
+int sgh_points_yz(int dtype, int nz, int ny, int P, const void* U, const void* By,
+                  const void* Bz, void* out, void* stream);
+
 /* ---- box-mode history ------------------------------------------------------
  * out[z] = shift(in[z]) for every z slice of an nz × ny × nx box, bilinear,
  * out[o] = in(o - s); outside the slice: edge value (nearest != 0) or zero.  The

@@ -29,6 +29,7 @@ OracleSolver.step              solvers/spectral.py:219-543                      
                                                                                    linear limit vs Eagar–Tsai closed form
 OracleLinearSolver.step        solvers/spectral_cpu_linear.py:98-162               Eagar–Tsai closed form, superposition
 reconstruct_temperature_DCT    physics/spectral_helpers.py:147-207                 direct modal sum at nodes
+reconstruct_at_points          physics/spectral_helpers.py:209-255                 = cell-centred volume at centres, = DCT-I at nodes
 subgrid_indices                physics/spectral_helpers.py:49-83                   reference test vectors (test_spectral_helpers.py:32-54)
 OracleFineMesh                 physics/spectral_state.py:166-239                   brute-force cosines at fine centres
 reconstruct_box / project_box  physics/spectral_ops.py:53-74,173-179               project∘reconstruct = I (full box, r = 1..3);
@@ -679,6 +680,18 @@ class OracleLinearSolver:
         update_modes_etd1(self.a, self.KK, g.Cp_top, S, out)
         self.a = out
         return self, {"P_laser": float(q.sum() * g.dx * g.dy)}
+
+
+def reconstruct_at_points(a, g, coords):
+    """T at arbitrary points (N, 3), coordinates clipped to the domain
+    (spectral_helpers.py:209-255): Σ a[p,n,m]·Bz[p,i]·By[n,i]·Bx[m,i]."""
+    c = np.asarray(coords, dtype=np.float64)
+    x = np.clip(c[:, 0], 0.0, g.Lx)
+    y = np.clip(c[:, 1], 0.0, g.Ly)
+    z = np.clip(c[:, 2], 0.0, g.Lz)
+    B = [g.C[a_][:, None] * np.cos(np.pi * np.arange(n)[:, None] * v[None, :] / L)
+         for a_, (n, L, v) in enumerate(((g.nx, g.Lx, x), (g.ny, g.Ly, y), (g.nz, g.Lz, z)))]
+    return np.einsum("pnm,pi,ni,mi->i", a, B[2], B[1], B[0], optimize=True)
 
 
 def reconstruct_temperature_DCT(a, g):

@@ -368,6 +368,24 @@ k_face_peaks_final(const double* __restrict__ part, int nz, double* __restrict__
   }
 }
 
+// out[i] = Σ_p Bz[p][i] · Σ_n By[n][i] · U[p][n][i]: the y and z sums of a point
+// evaluation after the x contraction U = a ·ₓ Bx (one thread per point).
+template <typename F>
+__global__ void __launch_bounds__(kThreads)
+k_points_yz(const F* __restrict__ U, const F* __restrict__ By, const F* __restrict__ Bz,
+            int nz, int ny, int P, F* __restrict__ out) {
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= P) return;
+  F t = F(0);
+  for (int p = 0; p < nz; ++p) {
+    F u = F(0);
+    const F* Up = U + (long long)p * ny * P + i;
+    for (int n = 0; n < ny; ++n) u += By[(long long)n * P + i] * Up[(long long)n * P];
+    t += Bz[(long long)p * P + i] * u;
+  }
+  out[i] = t;
+}
+
 template <typename F>
 __global__ void __launch_bounds__(kThreads)
 k_surface_forcing(const F* __restrict__ T, const F* __restrict__ q_las, long long S,
@@ -676,6 +694,19 @@ int sgh_shift2d(int dtype, int ny, int nx, const void* in, double sy, double sx,
                        (const F*)in, ny, nx, sy, sx, (const F*)base, (F)alpha, (F*)out);
   });
   SGH_CHECK_LAUNCH("sgh_shift2d");
+  return SGH_OK;
+}
+
+int sgh_points_yz(int dtype, int nz, int ny, int P, const void* U, const void* By,
+                  const void* Bz, void* out, void* stream) {
+  SGH_REQUIRE(nz > 0 && ny > 0 && P > 0 && U && By && Bz && out, SGH_ERR_ARG,
+              "sgh_points_yz: bad args");
+  SGH_DISPATCH(dtype, {
+    hipLaunchKernelGGL(k_points_yz<F>, dim3(cdiv(P, kThreads)), dim3(kThreads), 0,
+                       STREAM(stream), (const F*)U, (const F*)By, (const F*)Bz, nz, ny, P,
+                       (F*)out);
+  });
+  SGH_CHECK_LAUNCH("sgh_points_yz");
   return SGH_OK;
 }
 

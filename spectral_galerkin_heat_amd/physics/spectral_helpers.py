@@ -16,7 +16,8 @@ import torch
 
 from .._lib import call, dtype_code, ptr, stream_ptr
 
-__all__ = ["_calculate_subgrid_indices", "node_basis", "reconstruct_temperature_DCT"]
+__all__ = ["_calculate_subgrid_indices", "node_basis", "reconstruct_temperature_DCT",
+           "reconstruct_temperature_volume", "reconstruct_temperature_volume_at_points"]
 
 _BASES: dict = {}
 
@@ -76,3 +77,46 @@ def reconstruct_temperature_DCT(a, SsState):
     call("sgh_axis_contract_rect", code, 0, 1, nz, nz + 1, (ny + 1) * (nx + 1), ptr(Bz),
          ptr(t2), 1.0, ptr(t3), st)
     return t3.permute(2, 1, 0).contiguous()
+
+
+def reconstruct_temperature_volume(a, SsState):
+    """The node-centred field of ``reconstruct_temperature_DCT`` (spectral_helpers.py:108-144
+    evaluates the same series as a direct tensor product; here both are the same three
+    device contractions)."""
+    return reconstruct_temperature_DCT(a, SsState)
+
+
+def reconstruct_temperature_volume_at_points(a, num, geom, SsState, coords):
+    """T at arbitrary points (spectral_helpers.py:209-255).
+
+    ``coords`` is an (N, 3) array of (x, y, z) in metres, clipped to the domain. The bases
+    at the points are built on the host in FP64 (as the reference builds them), the x
+    contraction is one MFMA launch (``sgh_axis_contract_rect``) and the y, z sums one
+    thread per point (``sgh_points_yz``); the result is an (N,) device tensor in the
+    modes' dtype.
+    """
+    coords = np.asarray(coords)
+    if coords.ndim != 2 or coords.shape[1] != 3:
+        raise ValueError("coords must be of shape (N, 3)")
+    a = a.contiguous()
+    nz, ny, nx = a.shape
+    Ls = (float(geom.size.x), float(geom.size.y), float(geom.size.z))
+    P = coords.shape[0]
+    if P == 0:
+        return torch.empty(0, dtype=a.dtype, device=a.device)
+    vals = [np.clip(coords[:, k].astype(np.float64), 0.0, Ls[k]) for k in range(3)]
+
+    def basis(n, L, v):
+        C = np.full(n, math.sqrt(2.0 / L))
+        C[0] = math.sqrt(1.0 / L)
+        return C[:, None] * np.cos(np.pi * np.arange(n)[:, None] * v[None, :] / L)
+
+    dev = lambda m: torch.as_tensor(np.ascontiguousarray(m)).to(a.device, a.dtype)  # noqa: E731
+    BxT = dev(basis(nx, Ls[0], vals[0]).T)          # (P, nx): the x contraction's matrix
+    By, Bz = dev(basis(ny, Ls[1], vals[1])), dev(basis(nz, Ls[2], vals[2]))
+    code, st = dtype_code(a.dtype), stream_ptr()
+    U = torch.empty((nz, ny, P), dtype=a.dtype, device=a.device)
+    call("sgh_axis_contract_rect", code, 0, nz * ny, nx, P, 1, ptr(BxT), ptr(a), 1.0, ptr(U), st)
+    out = torch.empty(P, dtype=a.dtype, device=a.device)
+    call("sgh_points_yz", code, nz, ny, P, ptr(U), ptr(By), ptr(Bz), ptr(out), st)
+    return out

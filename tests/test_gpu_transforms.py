@@ -125,3 +125,31 @@ def test_node_reconstruction_matches_oracle_dct1(mesh, dtype, tol):
     assert tuple(got.shape) == (nx + 1, ny + 1, nz + 1)
     ref = o.reconstruct_temperature_DCT(a, o.OracleGrid(size, mesh))
     assert rel(got.cpu().double().numpy(), ref) < tol
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12), (torch.float32, 2e-6)])
+def test_point_evaluation_matches_oracle(dtype, tol):
+    from spectral_galerkin_heat_amd.core.parameters import GeomParams, MaterialParams, NumParams
+    from spectral_galerkin_heat_amd.core.vector import Vec3
+    from spectral_galerkin_heat_amd.physics import mi355x_kernels as K
+    from spectral_galerkin_heat_amd.physics.spectral_helpers import (
+        reconstruct_temperature_volume_at_points)
+    mesh, size = (24, 16, 12), (1.2e-3, 0.8e-3, 0.3e-3)
+    geom = GeomParams(size=Vec3(*size), n=Vec3(*mesh))
+    num = NumParams(dt=1e-6, nx=mesh[0], ny=mesh[1], nz=mesh[2],
+                    dtype=np.float64 if dtype == torch.float64 else np.float32)
+    S = K.SpectralSolverState(MaterialParams(rho=7900.0, k=20.0, Cp=500.0), geom, num, None)
+    rng = np.random.default_rng(21)
+    a = rng.standard_normal((12, 16, 24))
+    a[0, 0, 0] += 300.0
+    g = o.OracleGrid(size, mesh)
+    pts = rng.uniform(-0.1e-3, 1.3e-3, (257, 3)) * np.array([1.0, 0.7, 0.25])
+    got = reconstruct_temperature_volume_at_points(torch.as_tensor(a).to(DEV, dtype), num, geom,
+                                                   S, pts).cpu().double().numpy()
+    assert rel(got, o.reconstruct_at_points(a, g, pts)) < tol
+    # at the cell centres the series is the cell-centred volume reconstruction
+    zc, yc, xc = np.meshgrid(g.z, g.y, g.x, indexing="ij")
+    centres = np.stack([xc.ravel(), yc.ravel(), zc.ravel()], axis=1)
+    got_c = reconstruct_temperature_volume_at_points(torch.as_tensor(a).to(DEV, dtype), num,
+                                                     geom, S, centres).cpu().double().numpy()
+    assert rel(got_c, o.reconstruct_volume(a, g).ravel()) < tol

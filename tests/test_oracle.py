@@ -336,3 +336,20 @@ def test_linear_step_matches_eagar_tsai():
     pk = np.unravel_index(int(np.argmax(T)), T.shape)
     pr = np.unravel_index(int(np.argmax(T_ref)), T_ref.shape)
     assert max(abs(a - b) for a, b in zip(pk, pr)) <= 2
+
+
+def test_point_evaluation_reduces_to_volume_and_node_reconstructions():
+    size, mesh = (1.1e-3, 0.7e-3, 0.4e-3), (7, 5, 4)
+    g = o.OracleGrid(size, mesh)
+    a = np.random.default_rng(9).standard_normal((4, 5, 7))
+    zc, yc, xc = np.meshgrid(g.z, g.y, g.x, indexing="ij")
+    pts = np.stack([xc.ravel(), yc.ravel(), zc.ravel()], axis=1)
+    ref = o.reconstruct_volume(a, g).ravel()
+    np.testing.assert_allclose(o.reconstruct_at_points(a, g, pts), ref, rtol=0,
+                               atol=1e-13 * np.abs(ref).max())
+    xn, yn, zn = (np.arange(n + 1) * L / n for n, L in zip(mesh, size))
+    X, Y, Z = np.meshgrid(xn, yn, zn, indexing="ij")
+    nodes = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    ref = o.reconstruct_temperature_DCT(a, g).ravel()
+    np.testing.assert_allclose(o.reconstruct_at_points(a, g, nodes), ref, rtol=0,
+                               atol=1e-13 * np.abs(ref).max())
