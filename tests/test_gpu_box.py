@@ -282,3 +282,42 @@ def test_shallow_box_warns_once_like_the_oracle(caplog):
     warned = [r for r in caplog.records if "fine-mesh box boundary" in r.getMessage()]
     assert len(warned) == 1 and "floor (depth)" in warned[0].getMessage()
     assert s._latent_truncation_warned
+
+
+@pytest.mark.parametrize("mesh,r,box,start,vel", [
+    ((21, 15, 17), 3, (0.2e-3, 0.15e-3, 0.04e-3), (0.02e-3, 0.37e-3), (0.0, -0.3)),
+    ((16, 20, 24), 2, (0.45e-3, 0.1e-3, 0.06e-3), (0.58e-3, 0.05e-3), (-0.5, 0.2)),
+    ((25, 13, 30), 1, (0.3e-3, 0.25e-3, 0.15e-3), (0.3e-3, 0.2e-3), (0.4, 0.4)),
+])
+def test_box_odd_sizes_edges_and_diagonal_motion_fp64_matches_oracle(mesh, r, box, start, vel):
+    # odd mode counts (rows not a multiple of 16 bytes: register staging paths), windows
+    # pinned at both domain edges, motion along y and diagonally, refinement 1 to 3
+    cfg = _latent_cfg({"refinement": r, "box_size": list(box)})
+    cfg["domain"]["mesh"] = list(mesh)
+
+    def lp():
+        return ConstantVelocityLaser(start[0], start[1], vel[0], vel[1], 120.0)
+
+    s, ref, its, its_ref, m, mr = _run_pair(cfg, lp, 5)
+    assert its == its_ref
+    assert rel(host(s.state.a), ref.a) < 1e-9
+    assert rel(host(s.state.fine_mesh.T_prev), ref.fm.T_prev) < 1e-9
+    assert [s.state.fine_mesh.x_start, s.state.fine_mesh.y_start] == ref.fm.start[:2]
+
+
+def test_box_fp32_within_1e4_of_the_fp64_oracle():
+    cfg = _latent_cfg({"refinement": 2, "box_size": [0.3e-3, 0.3e-3, 0.05e-3]})
+    cfg["simulation"]["dtype"] = "float32"
+
+    def lp():
+        return ConstantVelocityLaser(0.2e-3, 0.2e-3, 0.4, 0.0, 120.0)
+
+    s, ctx = _solver(cfg, lp())
+    cfg64 = copy.deepcopy(cfg)
+    cfg64["simulation"]["dtype"] = "float64"
+    ref = o.OracleSolver(make_context(cfg64, lp())).initialize()
+    dt = ctx.num.dt
+    for n in range(6):
+        s.step(n * dt, dt)
+        ref.step(n * dt, dt)
+    assert rel(host(s.state.a), ref.a) < 1e-4
