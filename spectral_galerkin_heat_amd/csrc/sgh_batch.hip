@@ -63,8 +63,18 @@ struct CuboidDesc {
   double f1, f2, T_boil, T_liq;
   double absorptivity, rx, ry, order, area_factor;
   int cell_integrated, live;  // live: the slot holds a cuboid
-  int item_rows, pad_;        // surface rows per transform item (16 or 32)
+  int item_rows, ncb;         // surface rows per transform item (16 or 32); column blocks
 };
+
+// Wide surfaces split their transform items by output columns too (ncb blocks of cb_width
+// columns, a multiple of 16): each item still forms all of Y = L[rows,:]·X, but stages and
+// multiplies only its own columns of R, so a single large cuboid spreads over ncb times as
+// many workgroups. Item blk covers row block blk / ncb and column block blk % ncb.
+__host__ __device__ inline int cb_width(const CuboidDesc& d) {
+  const int nb = d.ncb > 1 ? d.ncb : 1;
+  return ((d.nx + nb - 1) / nb + 15) & ~15;
+}
+__host__ __device__ inline int n_items(const CuboidDesc& d) { return d.nrowblk * d.ncb; }
 
 struct Item {
   int c, blk;
@@ -237,11 +247,13 @@ __global__ void __launch_bounds__(kBT) k_b_prep(BatchPtrs<F> P) {
     const double x0 = L[0], y0 = L[1], power = L[2], on = L[3], vx = L[4], vy = L[5];
     const double I0 = on != 0.0 ? d.absorptivity * power / (d.area_factor * d.rx * d.ry) : 0.0;
     const double sy = vy * d.dt / d.dy, sx = vx * d.dt / d.dx;
-    const int r0 = it.blk * d.item_rows;
+    const int r0 = (it.blk / d.ncb) * d.item_rows;
     const int rows = min(d.item_rows, d.ny - r0);
+    const int c_lo = (it.blk % d.ncb) * cb_width(d);
+    const int cw = min(cb_width(d), d.nx - c_lo);
     const F* qo = P.q_evap_old + d.s_off;
-    for (int e = threadIdx.x; e < rows * d.nx; e += kBT) {
-      const int j = r0 + e / d.nx, i = e % d.nx;
+    for (int e = threadIdx.x; e < rows * cw; e += kBT) {
+      const int j = r0 + e / cw, i = c_lo + e % cw;
       const int s = j * d.nx + i;
       const double x = (i + 0.5) * d.dx, y = (j + 0.5) * d.dy;
       double q;
@@ -298,7 +310,7 @@ __device__ bool picard_active(const BatchPtrs<F>& P, const CuboidDesc& d, int c,
   const int done_at = P.done_at[c];
   double rr = 0.0, gu = 0.0, g2w = 0.0, aa0 = 0.0;
   if (k > 0) {
-    if (lane < d.nrowblk) {
+    if (lane < n_items(d)) {
       const double* sl = P.row_slots + 3 * (d.row_item0 + lane);
       rr = sl[0];
       gu = sl[1];
@@ -411,12 +423,17 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
   };
 
   const int nx = d.nx, ny = d.ny, so = d.s_off;
-  const int r0 = it.blk * d.item_rows;               // d.item_rows <= RB
+  const int r0 = (it.blk / d.ncb) * d.item_rows;     // d.item_rows <= RB
   const int rows = min(d.item_rows, ny - r0);
+  // this item's output columns [c_lo, c_hi) and 16-column tiles [jt_lo, jt_hi)
+  const int c_lo = (it.blk % d.ncb) * cb_width(d);
+  const int c_hi = min(nx, c_lo + cb_width(d));
+  const int cn_all = c_hi - c_lo;
   const int mt = (rows + 15) >> 4;                   // row tiles holding rows
   const int lane = tid & (kWave - 1), wave = tid / kWave;
   const int li = lane & 15, lk = lane >> 4;
   const int ntile = (nx + 15) >> 4;
+  const int jt_lo = c_lo >> 4, jt_hi = (c_hi + 15) >> 4;
   const int ny4 = (ny + 3) & ~3, nx4 = (nx + 3) & ~3;
   // LDS layout: Ls[kBRB][ldl], reused as Ys[kBRB][ldy] once stage 1 is done | Bb[...]
   const int ldl = pad_a<F>(ny4), ldy = pad_a<F>(nx4);
@@ -439,13 +456,14 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
   const int room = lds_elems - lds_a;
   const bool dma_ok = (nx * (int)sizeof(F)) % 16 == 0 &&
                       ((reinterpret_cast<uintptr_t>(Xsrc) | reinterpret_cast<uintptr_t>(Rm)) & 15) == 0;
-  const bool joint = dma_ok && (ny4 + nx4) * nx <= room;
-  const int skr = min(ny4, (room / nx) & ~3);              // X rows per streamed chunk
-  const int scw = min(ntile * 16, (room / nx4) & ~15);     // R columns per streamed chunk
+  const bool joint = dma_ok && ny4 * nx + nx4 * cn_all <= room;
+  const int skr = min(ny4, (room / nx) & ~3);                       // X rows per streamed chunk
+  const int scw = min((jt_hi - jt_lo) * 16, (room / nx4) & ~15);    // R columns per chunk
   const bool streamed = !joint && dma_ok && skr >= 4 && scw >= 16;
-  const int swz = joint ? swz_of<F>(nx) : 0;
+  const int swz = joint ? swz_of<F>(nx) : 0;       // X rows
+  const int swr = joint ? swz_of<F>(cn_all) : 0;   // R rows, this item's columns
   F* Xs = Bb;
-  F* Rs = Bb + ny4 * nx;
+  F* Rs = Bb + ny4 * nx;                           // [nx4][cn_all]
 
   // epilogue operands of this lane's outputs (MODE 0: ZE, Ztt; MODE 2: q_las; MODE 1,
   // when registers allow: G, Wtt, U, ZE, Ztt), loaded up front with the staging
@@ -455,7 +473,7 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
   F e0[NE], e1[NE], e2[NP], e3[NP], e4[NP];
 #pragma unroll
   for (int t = 0; t < kTilesPerWave; ++t) {
-    const int jt = wave + NW * t;
+    const int jt = jt_lo + wave + NW * t;
     const int col = jt * 16 + li;
 #pragma unroll
     for (int m = 0; m < kMT; ++m)
@@ -465,7 +483,7 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
         const int row = 16 * m + M::row(lane, v);
         e0[j] = e1[j] = F(0);
         if (kPre1) e2[j % NP] = e3[j % NP] = e4[j % NP] = F(0);
-        if (jt < ntile && col < nx && row < rows) {
+        if (jt < jt_hi && col < c_hi && row < rows) {
           const int s = so + (r0 + row) * nx + col;
           if (MODE == 0) { e0[j] = P.ZE[s]; e1[j] = P.Ztt[s]; }
           if (MODE == 2) { e0[j] = P.q_las[s]; }
@@ -592,10 +610,10 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
     // cuboid leaves only after its DMA has landed
     load_L();
     dma_rows<F, kST>(Xs, Xsrc, ny, nx, nx, swz);
-    dma_rows<F, kST>(Rs, Rm, nx, nx, nx, swz);
+    dma_rows<F, kST>(Rs, Rm + c_lo, nx, cn_all, nx, swr);
     decide();
     for (int e = tid; e < (ny4 - ny) * nx; e += kST) Xs[ny * nx + e] = F(0);
-    for (int e = tid; e < (nx4 - nx) * nx; e += kST) Rs[nx * nx + e] = F(0);
+    for (int e = tid; e < (nx4 - nx) * cn_all; e += kST) Rs[nx * cn_all + e] = F(0);
     store_L();
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
@@ -619,29 +637,30 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
     SGH_TS(4);
 #pragma unroll
     for (int t = 0; t < kTilesPerWave; ++t) {
-      const int jt = wave + NW * t;
-      if (jt >= ntile) break;
+      const int jt = jt_lo + wave + NW * t;
+      if (jt >= jt_hi) break;
       const int col = jt * 16 + li;
-      const bool cin = col < nx;
-      const int colx = (lk & 1) ? (col ^ swz) : col;
+      const bool cin = col < c_hi;
+      const int cl = col - c_lo;
+      const int colx = (lk & 1) ? (cl ^ swr) : cl;
       V a_[kMT];
 #pragma unroll
       for (int m = 0; m < kMT; ++m) a_[m] = V{0, 0, 0, 0};
       mma_k<F, kMT, kKS>(a_, mt, nx4,
                          [&](int m, int k) { return Ys[(16 * m + li) * ldy + k + lk]; },
-                         [&](int k) { return cin ? Rs[(k + lk) * nx + colx] : F(0); });
+                         [&](int k) { return cin ? Rs[(k + lk) * cn_all + colx] : F(0); });
       if (cin) finish(t, col, a_);
     }
   } else if (streamed) {
     // jobs 0 .. n1-1: X rows [i·skr, ...); jobs n1 .. n1+n2-1: R columns [i·scw, ...)
-    const int n1 = (ny4 + skr - 1) / skr, n2 = (nx + scw - 1) / scw;
+    const int n1 = (ny4 + skr - 1) / skr, n2 = (cn_all + scw - 1) / scw;
     auto issue = [&](int j) {
       if (j < n1) {
         const int k0 = j * skr, kn = min(skr, ny4 - k0), kv = max(0, min(kn, ny - k0));
         dma_rows<F, kST>(Bb, Xsrc + (long long)k0 * nx, kv, nx, nx, swz_of<F>(nx));
         for (int e = tid; e < (kn - kv) * nx; e += kST) Bb[kv * nx + e] = F(0);
       } else {
-        const int c0 = (j - n1) * scw, cn = min(scw, nx - c0);
+        const int c0 = c_lo + (j - n1) * scw, cn = min(scw, c_hi - c0);
         dma_rows<F, kST>(Bb, Rm + c0, nx, cn, nx, swz_of<F>(cn));
         for (int e = tid; e < (nx4 - nx) * cn; e += kST) Bb[nx * cn + e] = F(0);
       }
@@ -677,15 +696,15 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
           write_Y();
         }
       } else {
-        const int c0 = (j - n1) * scw, cn = min(scw, nx - c0);
+        const int c0 = c_lo + (j - n1) * scw, cn = min(scw, c_hi - c0);
         const int sw = swz_of<F>(cn);
 #pragma unroll
         for (int t = 0; t < kTilesPerWave; ++t) {
-          const int jt = wave + NW * t;
-          if (jt >= ntile) break;
+          const int jt = jt_lo + wave + NW * t;
+          if (jt >= jt_hi) break;
           const int col = jt * 16 + li;
           if (col - li < c0 || col - li >= c0 + cn) continue;  // tile not in this chunk
-          const bool cin = col < nx;
+          const bool cin = col < c_hi;
           const int cl = col - c0;
           const int colx = (lk & 1) ? (cl ^ sw) : cl;
           V a_[kMT];
@@ -724,23 +743,23 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
     __syncthreads();  // every wave is done with L before Y overwrites it
     write_Y();
     SGH_TS(3);
-    // stage 2: out = Y · R, R staged in column chunks of cw (usually all of it)
-    int cw = min(ntile * 16, max(16, ((bud / nx4) / 16) * 16));
+    // stage 2: out = Y · R, R's columns of this item staged in chunks of cw (usually all)
+    int cw = min((jt_hi - jt_lo) * 16, max(16, ((bud / nx4) / 16) * 16));
     while (cw > 16 && nx4 * pad_b<F>(cw) > bud) cw -= 16;
     const int ldb2 = pad_b<F>(cw);
-    for (int c0 = 0; c0 < nx; c0 += cw) {
-      const int cn = min(cw, nx - c0);
+    for (int c0 = c_lo; c0 < c_hi; c0 += cw) {
+      const int cn = min(cw, c_hi - c0);
       __syncthreads();
       stage_block<F, kST, kA>(Bb, ldb2, Rm + c0, nx, nx4, cn, nx, cn);
       __syncthreads();
       SGH_TS(4);
 #pragma unroll
       for (int t = 0; t < kTilesPerWave; ++t) {
-        const int jt = wave + NW * t;
-        if (jt >= ntile) break;
+        const int jt = jt_lo + wave + NW * t;
+        if (jt >= jt_hi) break;
         const int col = jt * 16 + li;
         if (col - li < c0 || col - li >= c0 + cn) continue;  // tile not in this chunk
-        const bool cin = col < nx;
+        const bool cin = col < c_hi;
         V a_[kMT];
 #pragma unroll
         for (int m = 0; m < kMT; ++m) a_[m] = V{0, 0, 0, 0};
@@ -784,7 +803,7 @@ __global__ void __launch_bounds__(kBT) k_b_metrics(BatchPtrs<F> P) {
     const int step = P.step_idx[c];
     if (step >= d.n_steps) continue;
     double pl = 0.0, tm = -1.0 / 0.0;
-    for (int b = 0; b < d.nrowblk; ++b) {
+    for (int b = 0; b < n_items(d); ++b) {
       pl += P.aux_slots[2 * (d.row_item0 + b)];
       const double t = P.aux_slots[2 * (d.row_item0 + b) + 1];
       tm = t > tm ? t : tm;
@@ -808,6 +827,15 @@ __global__ void __launch_bounds__(kBT) k_b_metrics(BatchPtrs<F> P) {
 // are rebuilt when membership changes.
 // ---------------------------------------------------------------------------------
 constexpr int kMaxRowBlk = kBMaxN / kRowsMin;      // row blocks per cuboid (ny <= 256)
+constexpr int kColBlkW = 64;                         // column-block width of wide surfaces
+constexpr int kMaxItems = 64;                        // transform items per cuboid (one wave
+                                                     // sums their slots in a decision)
+static_assert(kMaxRowBlk * ((kBMaxN + kColBlkW - 1) / kColBlkW) <= kMaxItems, "item slots");
+
+// Column blocks of a cuboid's transform items: surfaces wider than 128 are split into
+// 64-column blocks (a single large cuboid then spreads over more workgroups); narrower
+// ones keep whole rows.
+static int column_blocks(int nx) { return nx > 128 ? (nx + kColBlkW - 1) / kColBlkW : 1; }
 constexpr int kMaxColBlk = kBMaxN * kBMaxN / kBT;  // column blocks per cuboid (S <= 65536)
 constexpr int kMaxSizes = 32;                      // distinct mode counts in the matrix table
 
@@ -1093,7 +1121,7 @@ static bool assign_heights(Plan* p) {
     if (p->hdesc[c].live) live.push_back(c);
   std::vector<int> h(p->n, 16);
   long long items = 0;
-  for (int c : live) items += (p->hdesc[c].ny + 15) / 16;
+  for (int c : live) items += (long long)p->hdesc[c].ncb * ((p->hdesc[c].ny + 15) / 16);
   std::vector<int> order = live;
   std::sort(order.begin(), order.end(), [&](int a, int b) {
     const CuboidDesc &da = p->hdesc[a], &db = p->hdesc[b];
@@ -1103,7 +1131,7 @@ static bool assign_heights(Plan* p) {
   for (int c : order) {
     if (items <= target) break;
     const int ny = p->hdesc[c].ny;
-    items -= (ny + 15) / 16 - (ny + 31) / 32;
+    items -= (long long)p->hdesc[c].ncb * ((ny + 15) / 16 - (ny + 31) / 32);
     h[c] = 32;
   }
   bool changed = false;
@@ -1138,12 +1166,12 @@ static void build_xcd_items(Plan* p) {
     // with the fewest items
     int x = -1;
     for (int y = 0; y < kXcds; ++y)
-      if (count[y] + p->hdesc[c].nrowblk <= kCusPerXcd && (x < 0 || load[y] < load[x])) x = y;
+      if (count[y] + n_items(p->hdesc[c]) <= kCusPerXcd && (x < 0 || load[y] < load[x])) x = y;
     if (x < 0) x = (int)(std::min_element(count.begin(), count.end()) - count.begin());
     load[x] += cost(c);
-    count[x] += p->hdesc[c].nrowblk;
+    count[x] += n_items(p->hdesc[c]);
     const CuboidDesc& d = p->hdesc[c];
-    for (int b = 0; b < d.nrowblk; ++b) rb[x].push_back({c, b});
+    for (int b = 0; b < n_items(d); ++b) rb[x].push_back({c, b});
     for (int b = 0; b < d.ncolblk; ++b) cb[x].push_back({c, b});
   }
   auto interleave = [](std::vector<std::vector<Item>>& bins, std::vector<Item>& out) {
@@ -1287,9 +1315,10 @@ static int add_cuboid(Plan* p, const sgh_cuboid_spec& sp, int c, hipStream_t st)
     t1.insert(t1.end(), e.begin(), e.end());
     t1.insert(t1.end(), k2.begin(), k2.end());
   }
-  d.row_item0 = c * kMaxRowBlk;
+  d.row_item0 = c * kMaxItems;
   d.item_rows = p->rows == 0 ? 16 : p->rows;  // per-cuboid heights are set when lists are built
   d.nrowblk = (sp.ny + d.item_rows - 1) / d.item_rows;
+  d.ncb = column_blocks(sp.nx);
   d.col_item0 = c * kMaxColBlk;
   d.ncolblk = (d.S + kBT - 1) / kBT;
   d.max_it = sp.max_it;
@@ -1416,7 +1445,7 @@ int sgh_batch_pool_create(int dtype, int max_cuboids, long long state_capacity,
     set_error("sgh_batch_pool_create: LDS staging buffer too small");
     return SGH_ERR_ARG;
   }
-  p->row_cap = (long long)max_cuboids * kMaxRowBlk + 8LL * kMaxRowBlk * kXcds;
+  p->row_cap = (long long)max_cuboids * kMaxItems + 8LL * kMaxItems * kXcds;
   p->col_cap = (long long)max_cuboids * kMaxColBlk + 8LL * kMaxColBlk * kXcds;
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
@@ -1432,9 +1461,9 @@ int sgh_batch_pool_create(int dtype, int max_cuboids, long long state_capacity,
   chk(dmalloc(&p->row_descs, (size_t)p->row_cap * sizeof(ItemDesc)));
   chk(dmalloc(&p->col_items, (size_t)p->col_cap * sizeof(Item)));
   chk(dmalloc(&p->prime_items, (size_t)p->col_cap * sizeof(Item)));
-  chk(dmalloc(&p->row_slots, 3 * (size_t)max_cuboids * kMaxRowBlk * sizeof(double)));
+  chk(dmalloc(&p->row_slots, 3 * (size_t)max_cuboids * kMaxItems * sizeof(double)));
   chk(dmalloc(&p->col_slots, (size_t)max_cuboids * kMaxColBlk * sizeof(double)));
-  chk(dmalloc(&p->aux_slots, 2 * (size_t)max_cuboids * kMaxRowBlk * sizeof(double)));
+  chk(dmalloc(&p->aux_slots, 2 * (size_t)max_cuboids * kMaxItems * sizeof(double)));
   chk(dmalloc(&p->done_at, (size_t)max_cuboids * sizeof(int)));
   chk(dmalloc(&p->status, (size_t)kStatusStride * max_cuboids * sizeof(double)));
   chk(dmalloc(&p->step_idx, (size_t)max_cuboids * sizeof(int)));

@@ -18,7 +18,11 @@ from oracle.sgh_oracle import OracleSolver, modal_rel_l2  # noqa: E402
 from spectral_galerkin_heat_amd.core.laser import SingleTrackLaser  # noqa: E402
 from spectral_galerkin_heat_amd.workloads import MATERIAL_316L, _cfg, make_context  # noqa: E402
 
-SHAPES = [(33, 17, 5), (20, 36, 7), (2, 3, 2), (256, 256, 8), (17, 256, 3), (256, 4, 4)]
+SHAPES = [(33, 17, 5), (20, 36, 7), (2, 3, 2), (256, 256, 8), (17, 256, 3), (256, 4, 4),
+          (144, 20, 4), (200, 33, 3), (130, 70, 3)]
+# surfaces wider than 128 are split into column blocks (256: 4 × 64; 144: 3 × 48; 200:
+# 64, 64, 64, 8; 130: 48, 48, 34)
+SPLIT = [(144, 20, 4), (200, 33, 3), (256, 48, 4), (130, 70, 3), (64, 40, 5)]
 
 
 def case(mesh, n_steps, dtype="float64"):
@@ -51,6 +55,30 @@ def test_batch_engine_ragged_and_widest_shapes(precision):
             assert met[:, i, 2].astype(int).tolist() == its, SHAPES[i]
         else:
             assert modal_rel_l2(a, ref.a) < 1e-4, SHAPES[i]
+    eng.close()
+
+
+@pytest.mark.parametrize("variant", [(16, 256, 153600), (16, 1024, 153600), (32, 512, 153600),
+                                     (0, 1024, 162816), (16, 512, 153600)])
+def test_column_split_items_every_variant_fp64(variant):
+    from spectral_galerkin_heat_amd.engine.batch import BatchEngine
+    n = 3
+    cases = [case(m, n) for m in SPLIT]
+    eng = BatchEngine([make_context(c, lp) for c, lp in cases], precision="fp64",
+                      variant=variant)
+    eng.run(n)
+    eng.synchronize()
+    met = eng.metrics()
+    for i, (cfg, lp) in enumerate(cases):
+        ref = OracleSolver(make_context(cfg, lp)).initialize()
+        dt = ref.ctx.num.dt
+        mets = [ref.step(k * dt, dt)[1] for k in range(n)]
+        a = eng.state(i).double().cpu().numpy()
+        assert modal_rel_l2(a, ref.a) < 1e-9, (SPLIT[i], variant)
+        assert met[:, i, 2].astype(int).tolist() == [m["n_evap_iter"] for m in mets]
+        # P_laser and T_surface_max are summed / maxed over every (row, column) item
+        np.testing.assert_allclose(met[:, i, 0], [m["P_laser"] for m in mets], rtol=1e-12)
+        np.testing.assert_allclose(met[:, i, 1], [m["T_surface_max"] for m in mets], rtol=1e-9)
     eng.close()
 
 
