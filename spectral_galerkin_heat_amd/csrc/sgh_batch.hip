@@ -383,7 +383,7 @@ __device__ __forceinline__ void mma_k(typename Mfma<F>::V (&acc)[KMT], int mt, i
   }
 }
 
-template <typename F, int MODE, int RB, int NT>
+template <typename F, int MODE, int RB, int NT, bool CS>
 __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud, int lds_elems) {
   SGH_TS(0);
   using M = Mfma<F>;
@@ -423,11 +423,13 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
   };
 
   const int nx = d.nx, ny = d.ny, so = d.s_off;
-  const int r0 = (it.blk / d.ncb) * d.item_rows;     // d.item_rows <= RB
+  // this item's rows [r0, r0 + rows), output columns [c_lo, c_hi) and 16-column tiles
+  // [jt_lo, jt_hi); CS = false (no cuboid of the batch is split by columns) keeps whole
+  // rows with no per-item index arithmetic
+  const int r0 = (CS ? it.blk / d.ncb : it.blk) * d.item_rows;   // d.item_rows <= RB
   const int rows = min(d.item_rows, ny - r0);
-  // this item's output columns [c_lo, c_hi) and 16-column tiles [jt_lo, jt_hi)
-  const int c_lo = (it.blk % d.ncb) * cb_width(d);
-  const int c_hi = min(nx, c_lo + cb_width(d));
+  const int c_lo = CS ? (it.blk % d.ncb) * cb_width(d) : 0;
+  const int c_hi = CS ? min(nx, c_lo + cb_width(d)) : nx;
   const int cn_all = c_hi - c_lo;
   const int mt = (rows + 15) >> 4;                   // row tiles holding rows
   const int lane = tid & (kWave - 1), wave = tid / kWave;
@@ -921,6 +923,7 @@ struct Plan {
   int metrics_rows = 0;
   std::vector<int> pending_prime;  // slots added since the last run
   bool dirty = true;               // work lists / graph out of date
+  bool colsplit = false;           // a live cuboid's items are split by columns
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   hipStream_t graph_stream = nullptr;
@@ -978,24 +981,32 @@ constexpr int kNumSurfVariants = sizeof(kSurfVariants) / sizeof(kSurfVariants[0]
 template <typename F>
 using SurfFn = void (*)(BatchPtrs<F>, int, int, int);
 
-template <typename F, int RB, int NT>
+template <typename F, int RB, int NT, bool CS>
 static void surf_fns(SurfFn<F>* f) {
-  f[0] = &k_b_surface<F, 0, RB, NT>;
-  f[1] = &k_b_surface<F, 1, RB, NT>;
-  f[2] = &k_b_surface<F, 2, RB, NT>;
-  f[3] = &k_b_surface<F, 3, RB, NT>;
+  f[0] = &k_b_surface<F, 0, RB, NT, CS>;
+  f[1] = &k_b_surface<F, 1, RB, NT, CS>;
+  f[2] = &k_b_surface<F, 2, RB, NT, CS>;
+  f[3] = &k_b_surface<F, 3, RB, NT, CS>;
 }
 
-template <typename F>
-static bool surf_lookup(int rows, int threads, SurfFn<F>* f) {
+template <typename F, bool CS>
+static bool surf_lookup_cs(int rows, int threads, SurfFn<F>* f) {
   if (rows == 0) rows = 32;  // per-cuboid heights run on the 32-row kernels
-  if (rows == 16 && threads == 256) surf_fns<F, 16, 256>(f);
-  else if (rows == 16 && threads == 512) surf_fns<F, 16, 512>(f);
-  else if (rows == 16 && threads == 1024) surf_fns<F, 16, 1024>(f);
-  else if (rows == 32 && threads == 512) surf_fns<F, 32, 512>(f);
-  else if (rows == 32 && threads == 1024) surf_fns<F, 32, 1024>(f);
+  if (rows == 16 && threads == 256) surf_fns<F, 16, 256, CS>(f);
+  else if (rows == 16 && threads == 512) surf_fns<F, 16, 512, CS>(f);
+  else if (rows == 16 && threads == 1024) surf_fns<F, 16, 1024, CS>(f);
+  else if (rows == 32 && threads == 512) surf_fns<F, 32, 512, CS>(f);
+  else if (rows == 32 && threads == 1024) surf_fns<F, 32, 1024, CS>(f);
   else return false;
   return true;
+}
+
+// The kernels of a variant; `colsplit` selects those that handle column-split items
+// (needed as soon as one live cuboid is wider than 128).
+template <typename F>
+static bool surf_lookup(int rows, int threads, bool colsplit, SurfFn<F>* f) {
+  return colsplit ? surf_lookup_cs<F, true>(rows, threads, f)
+                  : surf_lookup_cs<F, false>(rows, threads, f);
 }
 
 template <typename F>
@@ -1028,12 +1039,14 @@ template <typename F>
 static hipError_t allow_large_lds(size_t bytes) {
   hipError_t e = hipSuccess;
   for (const SurfVariant& v : kSurfVariants) {
-    SurfFn<F> f[4];
-    surf_lookup<F>(v.rows, v.threads, f);
-    for (int m = 0; m < 4; ++m) {
-      hipError_t r = hipFuncSetAttribute(reinterpret_cast<const void*>(f[m]),
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-      if (e == hipSuccess) e = r;
+    for (int cs = 0; cs < 2; ++cs) {
+      SurfFn<F> f[4];
+      surf_lookup<F>(v.rows, v.threads, cs == 1, f);
+      for (int m = 0; m < 4; ++m) {
+        hipError_t r = hipFuncSetAttribute(reinterpret_cast<const void*>(f[m]),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        if (e == hipSuccess) e = r;
+      }
     }
   }
   return e;
@@ -1046,7 +1059,7 @@ int enqueue_step(Plan* p, hipStream_t st) {
   if (nrow == 0) return SGH_OK;
   const bool tm = p->timing;
   SurfFn<F> f[4];
-  if (!surf_lookup<F>(p->rows, p->threads, f)) {
+  if (!surf_lookup<F>(p->rows, p->threads, p->colsplit, f)) {
     set_error("sgh_batch: no surface variant rows=%d threads=%d", p->rows, p->threads);
     return SGH_ERR_STATE;
   }
@@ -1383,6 +1396,9 @@ static int rebuild_lists(Plan* p, hipStream_t st) {
     if (e != hipSuccess) return fail_hip(e, "sgh_batch heights");
   }
   build_xcd_items(p);
+  p->colsplit = false;
+  for (int c = 0; c < p->n; ++c)
+    if (p->hdesc[c].live && p->hdesc[c].ncb > 1) p->colsplit = true;
   if ((long long)p->hrow.size() > p->row_cap || (long long)p->hcol.size() > p->col_cap) {
     set_error("sgh_batch: work list exceeds capacity");
     return SGH_ERR_STATE;
