@@ -69,16 +69,23 @@ def c_coef(n, L):
     return C
 
 
+# scipy.fft worker threads of the oracle's transforms: -1 = every core (the reference's
+# setting, spectral_cpu_kernels.py:374-389); the timing of a one-core baseline sets 1.
+FFT_WORKERS = -1
+
+
 def dct2(q):
     """Ortho DCT-II over every axis (spectral_cpu_kernels.py:374-381)."""
     arr = np.ascontiguousarray(q)
-    return scipy.fft.dctn(arr, type=2, norm="ortho", axes=tuple(range(arr.ndim)), workers=-1)
+    return scipy.fft.dctn(arr, type=2, norm="ortho", axes=tuple(range(arr.ndim)),
+                          workers=FFT_WORKERS)
 
 
 def idct2(a):
     """Ortho DCT-III over every axis, the inverse of dct2 (spectral_cpu_kernels.py:383-389)."""
     arr = np.ascontiguousarray(a)
-    return scipy.fft.dctn(arr, type=3, norm="ortho", axes=tuple(range(arr.ndim)), workers=-1)
+    return scipy.fft.dctn(arr, type=3, norm="ortho", axes=tuple(range(arr.ndim)),
+                          workers=FFT_WORKERS)
 
 
 def precompute_K_KK(k, rho, Cp, dt, n_zyx, L_zyx, dtype=np.float64):
@@ -704,7 +711,7 @@ def reconstruct_temperature_DCT(a, g):
     b = a * wz[:, None, None] * wy[None, :, None] * wx[None, None, :]
     padded = np.zeros((nz + 1, ny + 1, nx + 1))
     padded[:nz, :ny, :nx] = b
-    T = scipy.fft.dctn(padded, type=1, norm=None, axes=(0, 1, 2), workers=-1)
+    T = scipy.fft.dctn(padded, type=1, norm=None, axes=(0, 1, 2), workers=FFT_WORKERS)
     return np.ascontiguousarray(T.transpose(2, 1, 0))
 
 
