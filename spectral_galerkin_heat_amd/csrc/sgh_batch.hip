@@ -385,425 +385,22 @@ __device__ __forceinline__ void mma_k(typename Mfma<F>::V (&acc)[KMT], int mt, i
 
 template <typename F, int MODE, int RB, int NT, bool CS>
 __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud, int lds_elems) {
-  SGH_TS(0);
-  using M = Mfma<F>;
-  using V = typename M::V;
-  constexpr int kBRB = RB;                      // most rows of an item (LDS tiles)
-  constexpr int kMT = RB / 16;                  // MFMA row tiles
-  constexpr int kST = NT;                       // threads
-  constexpr int NW = NT / kWave;                // waves
-  constexpr int kTilesPerWave = (kBMaxN / 16 + NW - 1) / NW;  // 16-column tiles per wave
-  constexpr int kA = kStageAhead;   // loads in flight per lane while staging
-  // one accumulator chain per tile: splitting K over several chains measured slower (the
-  // LDS operand reads, not the MFMA dependency, set the pace)
-  constexpr int kKS = 1;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  __shared__ double sh[NT / kWave];
-  __shared__ double sh3[3][NT / kWave];
-  __shared__ int s_active;
-  const ItemDesc& idd = P.row_descs[blockIdx.x];
-  const Item it{idd.c, idd.blk};
-  if (it.c < 0) return;
-  const CuboidDesc d = idd.d;
-  const int tid = threadIdx.x;
-  // wave 0 decides whether the cuboid still iterates (the decision needs every lane of
-  // the wave); the other waves' loads are already in flight meanwhile
-  auto decide = [&]() {
-    if (tid < kWave) {
-      bool act;
-      if (MODE == 0) act = P.step_idx[it.c] < d.n_steps;
-      else if (MODE == 1) act = P.done_at[it.c] < 0;
-      else if (MODE == 2) act = picard_active<F>(P, d, it.c, k, it.blk == 0);
-      else {
-        picard_active<F>(P, d, it.c, k, it.blk == 0);
-        act = P.step_idx[it.c] < d.n_steps;
-      }
-      if (tid == 0) s_active = act ? 1 : 0;
-    }
-  };
+#include "sgh_surface_pass.inc"
+}
 
-  const int nx = d.nx, ny = d.ny, so = d.s_off;
-  // this item's rows [r0, r0 + rows), output columns [c_lo, c_hi) and 16-column tiles
-  // [jt_lo, jt_hi); CS = false (no cuboid of the batch is split by columns) keeps whole
-  // rows with no per-item index arithmetic
-  const int r0 = (CS ? it.blk / d.ncb : it.blk) * d.item_rows;   // d.item_rows <= RB
-  const int rows = min(d.item_rows, ny - r0);
-  const int c_lo = CS ? (it.blk % d.ncb) * cb_width(d) : 0;
-  const int c_hi = CS ? min(nx, c_lo + cb_width(d)) : nx;
-  const int cn_all = c_hi - c_lo;
-  const int mt = (rows + 15) >> 4;                   // row tiles holding rows
-  const int lane = tid & (kWave - 1), wave = tid / kWave;
-  const int li = lane & 15, lk = lane >> 4;
-  const int ntile = (nx + 15) >> 4;
-  const int jt_lo = c_lo >> 4, jt_hi = (c_hi + 15) >> 4;
-  const int ny4 = (ny + 3) & ~3, nx4 = (nx + 3) & ~3;
-  // LDS layout: Ls[kBRB][ldl], reused as Ys[kBRB][ldy] once stage 1 is done | Bb[...]
-  const int ldl = pad_a<F>(ny4), ldy = pad_a<F>(nx4);
-  const int lds_a = kBRB * (ldl > ldy ? ldl : ldy);
-  F* Ls = reinterpret_cast<F*>(smem_raw);
-  F* Ys = Ls;
-  F* Bb = Ls + lds_a;
-  const F* Lm = P.mats + (MODE == 0 || MODE == 1 ? d.by_off : d.byT_off);
-  const F* Rm = P.mats + (MODE == 0 || MODE == 1 ? d.bxT_off : d.bx_off);
-  const F* Xsrc = (MODE <= 1 ? P.q_net : P.Xinv) + so;
-
-  // Staging of X (the input field, ny rows) and R (the right matrix, nx rows), by path:
-  //  joint    — X and R fit in LDS together: both copied by LDS DMA at once, unpadded
-  //             with odd rows XOR-swizzled, one wait for the whole launch;
-  //  streamed — they do not: X in row chunks, then R in column chunks, each chunk as
-  //             large as LDS allows (fewest DMA rounds; a round's latency is far longer
-  //             than its MFMA work, so double buffering would not hide it);
-  //  legacy   — sizes the DMA cannot take (rows not a multiple of 16 bytes): staged
-  //             through registers into padded LDS rows, in chunks if need be.
-  const int room = lds_elems - lds_a;
-  const bool dma_ok = (nx * (int)sizeof(F)) % 16 == 0 &&
-                      ((reinterpret_cast<uintptr_t>(Xsrc) | reinterpret_cast<uintptr_t>(Rm)) & 15) == 0;
-  const bool joint = dma_ok && ny4 * nx + nx4 * cn_all <= room;
-  const int skr = min(ny4, (room / nx) & ~3);                       // X rows per streamed chunk
-  const int scw = min((jt_hi - jt_lo) * 16, (room / nx4) & ~15);    // R columns per chunk
-  const bool streamed = !joint && dma_ok && skr >= 4 && scw >= 16;
-  const int swz = joint ? swz_of<F>(nx) : 0;       // X rows
-  const int swr = joint ? swz_of<F>(cn_all) : 0;   // R rows, this item's columns
-  F* Xs = Bb;
-  F* Rs = Bb + ny4 * nx;                           // [nx4][cn_all]
-
-  // epilogue operands of this lane's outputs (MODE 0: ZE, Ztt; MODE 2: q_las; MODE 1,
-  // when registers allow: G, Wtt, U, ZE, Ztt), loaded up front with the staging
-  constexpr int NE = 4 * kMT * kTilesPerWave;
-  constexpr bool kPre1 = MODE == 1 && NE <= 8 && RB == 16;
-  constexpr int NP = kPre1 ? NE : 1;
-  F e0[NE], e1[NE], e2[NP], e3[NP], e4[NP];
-#pragma unroll
-  for (int t = 0; t < kTilesPerWave; ++t) {
-    const int jt = jt_lo + wave + NW * t;
-    const int col = jt * 16 + li;
-#pragma unroll
-    for (int m = 0; m < kMT; ++m)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int j = (t * kMT + m) * 4 + v;
-        const int row = 16 * m + M::row(lane, v);
-        e0[j] = e1[j] = F(0);
-        if (kPre1) e2[j % NP] = e3[j % NP] = e4[j % NP] = F(0);
-        if (jt < jt_hi && col < c_hi && row < rows) {
-          const int s = so + (r0 + row) * nx + col;
-          if (MODE == 0) { e0[j] = P.ZE[s]; e1[j] = P.Ztt[s]; }
-          if (MODE == 2) { e0[j] = P.q_las[s]; }
-          if (kPre1) {
-            e0[j] = P.G[s];
-            e1[j] = P.Wtt[s];
-            e2[j % NP] = P.U[s];
-            e3[j % NP] = P.ZE[s];
-            e4[j % NP] = P.Ztt[s];
-          }
-        }
-      }
-  }
-
-  V acc[kTilesPerWave][kMT];
-#pragma unroll
-  for (int m = 0; m < kMT; ++m)
-#pragma unroll
-    for (int t = 0; t < kTilesPerWave; ++t) acc[t][m] = V{0, 0, 0, 0};
-
-  double v0 = 0.0, v1 = 0.0, v2 = 0.0;
-  if (MODE == 3) v0 = -1.0 / 0.0;
-  const double fwd_scale = d.dx * d.dy * d.inv_sqrt_LxLy;
-  // the role-specific epilogue of one 16-column tile (rows 16m + row(lane, v))
-  auto finish_tile = [&](int t, int col, const V* a_, int m) {
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = 16 * m + M::row(lane, v);
-        if (row >= rows) continue;
-        const int s = so + (r0 + row) * nx + col;
-        const int j = (t * kMT + m) * 4 + v;
-        const F val = a_[m][v];
-        if (MODE == 0) {
-          const F g = (F)((double)val * fwd_scale);
-          P.G[s] = g;
-          P.Xinv[s] = e0[j] + g * e1[j];
-        } else if (MODE == 1) {
-          const F Sr = (F)((double)val * fwd_scale);
-          const F g = kPre1 ? e0[j] : P.G[s];
-          const F dd = Sr - g;
-          const double w = (double)(kPre1 ? e1[j] : P.Wtt[s]);
-          v0 += (double)dd * (double)dd * w;
-          v1 += (double)g * (double)(kPre1 ? e2[j % NP] : P.U[s]);
-          v2 += (double)g * (double)g * w;
-          const F gn = g + (F)d.omega * dd;
-          P.G[s] = gn;
-          P.Xinv[s] = (kPre1 ? e3[j % NP] : P.ZE[s]) + gn * (kPre1 ? e4[j % NP] : P.Ztt[s]);
-        } else {
-          const F T = (F)((double)val * d.inv_sqrt_LxLy);
-          if (MODE == 2) {
-            F qe = F(0);
-            if (T >= (F)d.T_liq)
-              qe = (F)d.f1 * ((F(1) / sqrt(T)) * exp((F)d.f2 * (F(1) - (F)d.T_boil / T)));
-            P.q_evap[s] = qe;
-            P.q_net[s] = e0[j] - qe;
-          } else {
-            v0 = (double)T > v0 ? (double)T : v0;
-          }
-        }
-      }
-  };
-  auto finish = [&](int t, int col, const V* a_) {
-    if constexpr ((MODE == 1 && !kPre1) || sizeof(F) == 8 || NT <= 512) {
-      // one row tile at a time: with two row tiles the forward mode's five operand loads
-      // per output would otherwise be live together and spill (and the other cases below
-      // measured no faster unrolled)
-#pragma unroll 1
-      for (int m = 0; m < kMT; ++m) finish_tile(t, col, a_, m);
-    } else {
-      // FP32, 1024 threads: the prefetched operands are indexed by the row tile; unrolled,
-      // they stay in registers instead of a stack array (32-row items: 0.660 -> 0.611 ms
-      // per step of the 64-cuboid batch with per-cuboid heights)
-#pragma unroll
-      for (int m = 0; m < kMT; ++m) finish_tile(t, col, a_, m);
-    }
-  };
-  // stage-1 result Y to LDS (and zero its k-padding columns), accumulators reset
-  auto write_Y = [&]() {
-#pragma unroll
-    for (int t = 0; t < kTilesPerWave; ++t) {
-      const int jt = wave + NW * t;
-      const int col = jt * 16 + li;
-#pragma unroll
-      for (int m = 0; m < kMT; ++m) {
-        if (jt < ntile && col < nx) {
-#pragma unroll
-          for (int v = 0; v < 4; ++v) Ys[(16 * m + M::row(lane, v)) * ldy + col] = acc[t][m][v];
-        }
-        acc[t][m] = V{0, 0, 0, 0};
-      }
-    }
-    if (nx4 > nx)
-      for (int e = tid; e < kBRB * (nx4 - nx); e += kST) {
-        const int r = e / (nx4 - nx), c = nx + e % (nx4 - nx);
-        Ys[r * ldy + c] = F(0);
-      }
-  };
-  // L rows of this item (zero beyond ny and beyond the last row): 16-row items load them
-  // into registers ahead of the DMA and store them after it; 32-row items, short of
-  // registers in their forward mode, stage them after the DMA has been issued
-  constexpr bool kLreg = RB == 16;
-  constexpr int kLr = kLreg ? (kBRB * kBMaxN + kST - 1) / kST : 1;
-  F lr[kLr];
-  const int ltot = kBRB * ny4;
-  auto load_L = [&]() {
-    if (!kLreg) return;
-#pragma unroll
-    for (int q = 0; q < kLr; ++q) {
-      const int e = tid + q * kST;
-      const int r = e / ny4, c = e - (e / ny4) * ny4;
-      lr[q] = (e < ltot && r < rows && c < ny) ? Lm[(long long)(r0 + r) * ny + c] : F(0);
-    }
-  };
-  auto store_L = [&]() {
-    if (!kLreg) {
-      stage_block<F, kST, kA>(Ls, ldl, Lm + (long long)r0 * ny, ny, kBRB, ny4, rows, ny);
-      return;
-    }
-#pragma unroll
-    for (int q = 0; q < kLr; ++q) {
-      const int e = tid + q * kST;
-      if (e < ltot) Ls[(e / ny4) * ldl + (e - (e / ny4) * ny4)] = lr[q];
-    }
-  };
-
-  if (!joint && !streamed) {
-    decide();
-    __syncthreads();
-    if (!s_active) return;
-  }
-  SGH_TS(1);
-
-  if (joint) {
-    // L rows, then the DMA, then the decision; one wait covers it all, and an inactive
-    // cuboid leaves only after its DMA has landed
-    load_L();
-    dma_rows<F, kST>(Xs, Xsrc, ny, nx, nx, swz);
-    dma_rows<F, kST>(Rs, Rm + c_lo, nx, cn_all, nx, swr);
-    decide();
-    for (int e = tid; e < (ny4 - ny) * nx; e += kST) Xs[ny * nx + e] = F(0);
-    for (int e = tid; e < (nx4 - nx) * cn_all; e += kST) Rs[nx * cn_all + e] = F(0);
-    store_L();
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (!s_active) return;
-    SGH_TS(2);
-#pragma unroll
-    for (int t = 0; t < kTilesPerWave; ++t) {
-      const int jt = wave + NW * t;
-      if (jt >= ntile) break;
-      const int col = jt * 16 + li;
-      const bool cin = col < nx;
-      const int colx = (lk & 1) ? (col ^ swz) : col;
-      mma_k<F, kMT, kKS>(acc[t], mt, ny4,
-                         [&](int m, int k) { return Ls[(16 * m + li) * ldl + k + lk]; },
-                         [&](int k) { return cin ? Xs[(k + lk) * nx + colx] : F(0); });
-    }
-    __syncthreads();  // every wave is done with L before Y overwrites it
-    write_Y();
-    SGH_TS(3);
-    __syncthreads();  // Y complete; R has been in LDS since the joint staging
-    SGH_TS(4);
-#pragma unroll
-    for (int t = 0; t < kTilesPerWave; ++t) {
-      const int jt = jt_lo + wave + NW * t;
-      if (jt >= jt_hi) break;
-      const int col = jt * 16 + li;
-      const bool cin = col < c_hi;
-      const int cl = col - c_lo;
-      const int colx = (lk & 1) ? (cl ^ swr) : cl;
-      V a_[kMT];
-#pragma unroll
-      for (int m = 0; m < kMT; ++m) a_[m] = V{0, 0, 0, 0};
-      mma_k<F, kMT, kKS>(a_, mt, nx4,
-                         [&](int m, int k) { return Ys[(16 * m + li) * ldy + k + lk]; },
-                         [&](int k) { return cin ? Rs[(k + lk) * cn_all + colx] : F(0); });
-      if (cin) finish(t, col, a_);
-    }
-  } else if (streamed) {
-    // jobs 0 .. n1-1: X rows [i·skr, ...); jobs n1 .. n1+n2-1: R columns [i·scw, ...)
-    const int n1 = (ny4 + skr - 1) / skr, n2 = (cn_all + scw - 1) / scw;
-    auto issue = [&](int j) {
-      if (j < n1) {
-        const int k0 = j * skr, kn = min(skr, ny4 - k0), kv = max(0, min(kn, ny - k0));
-        dma_rows<F, kST>(Bb, Xsrc + (long long)k0 * nx, kv, nx, nx, swz_of<F>(nx));
-        for (int e = tid; e < (kn - kv) * nx; e += kST) Bb[kv * nx + e] = F(0);
-      } else {
-        const int c0 = c_lo + (j - n1) * scw, cn = min(scw, c_hi - c0);
-        dma_rows<F, kST>(Bb, Rm + c0, nx, cn, nx, swz_of<F>(cn));
-        for (int e = tid; e < (nx4 - nx) * cn; e += kST) Bb[nx * cn + e] = F(0);
-      }
-    };
-    load_L();
-    issue(0);
-    decide();
-    store_L();
-    for (int j = 0; j < n1 + n2; ++j) {
-      if (j > 0) {
-        __syncthreads();   // every wave is done with job j - 1's buffer
-        issue(j);
-      }
-      __builtin_amdgcn_s_waitcnt(0);
-      __syncthreads();     // job j has landed
-      if (j == 0 && !s_active) return;
-      if (j < n1) {
-        const int k0 = j * skr, kn = min(skr, ny4 - k0);
-        const int sw = swz_of<F>(nx);
-#pragma unroll
-        for (int t = 0; t < kTilesPerWave; ++t) {
-          const int jt = wave + NW * t;
-          if (jt >= ntile) break;
-          const int col = jt * 16 + li;
-          const bool cin = col < nx;
-          const int colx = (lk & 1) ? (col ^ sw) : col;
-          mma_k<F, kMT, kKS>(acc[t], mt, kn,
-                             [&](int m, int k) { return Ls[(16 * m + li) * ldl + k0 + k + lk]; },
-                             [&](int k) { return cin ? Bb[(k + lk) * nx + colx] : F(0); });
-        }
-        if (j == n1 - 1) {   // Y, read by the R jobs after the next barriers, replaces L
-          __syncthreads();
-          write_Y();
-        }
-      } else {
-        const int c0 = c_lo + (j - n1) * scw, cn = min(scw, c_hi - c0);
-        const int sw = swz_of<F>(cn);
-#pragma unroll
-        for (int t = 0; t < kTilesPerWave; ++t) {
-          const int jt = jt_lo + wave + NW * t;
-          if (jt >= jt_hi) break;
-          const int col = jt * 16 + li;
-          if (col - li < c0 || col - li >= c0 + cn) continue;  // tile not in this chunk
-          const bool cin = col < c_hi;
-          const int cl = col - c0;
-          const int colx = (lk & 1) ? (cl ^ sw) : cl;
-          V a_[kMT];
-#pragma unroll
-          for (int m = 0; m < kMT; ++m) a_[m] = V{0, 0, 0, 0};
-          mma_k<F, kMT, kKS>(a_, mt, nx4,
-                             [&](int m, int k) { return Ys[(16 * m + li) * ldy + k + lk]; },
-                             [&](int k) { return cin ? Bb[(k + lk) * cn + colx] : F(0); });
-          if (cin) finish(t, col, a_);
-        }
-      }
-    }
+// Per-cuboid heights (variants with rows 0): each workgroup runs its item with the body
+// built for the item's own height, 16 or 32 rows (the 32-row body on a 16-row item idles
+// half its row tiles and, under 1024 threads, has no registers to prefetch the forward
+// pass's operands), so that all items fit one round of CUs without slowing the 16-row ones.
+template <typename F, int MODE, int NT, bool CS>
+__global__ void __launch_bounds__(NT) k_b_surface_hts(BatchPtrs<F> P, int k, int bud,
+                                                      int lds_elems) {
+  if (P.row_descs[blockIdx.x].d.item_rows > 16) {
+    constexpr int RB = 32;
+#include "sgh_surface_pass.inc"
   } else {
-    stage_block<F, kST, kA>(Ls, ldl, Lm + (long long)r0 * ny, ny, kBRB, ny4, rows, ny);
-    // stage 1: Y = L[rows, :] · X, X staged in chunks of kr rows (usually all of it)
-    const int ldb1 = pad_b<F>(nx);
-    const int kr = min(ny4, (bud / ldb1) & ~3);
-    if (kr < 4) return;  // the host guarantees bud >= 4·ldb1; never taken
-    for (int kc0 = 0; kc0 < ny4; kc0 += kr) {
-      const int kn = min(kr, ny4 - kc0);
-      if (kc0 > 0) __syncthreads();
-      stage_block<F, kST, kA>(Bb, ldb1, Xsrc + (long long)kc0 * nx, nx, kn, nx, ny - kc0, nx);
-      __syncthreads();
-      SGH_TS(2);
-#pragma unroll
-      for (int t = 0; t < kTilesPerWave; ++t) {
-        const int jt = wave + NW * t;
-        if (jt >= ntile) break;
-        const int col = jt * 16 + li;
-        const bool cin = col < nx;
-        mma_k<F, kMT, kKS>(acc[t], mt, kn,
-                           [&](int m, int k) { return Ls[(16 * m + li) * ldl + kc0 + k + lk]; },
-                           [&](int k) { return cin ? Bb[(k + lk) * ldb1 + col] : F(0); });
-      }
-    }
-    __syncthreads();  // every wave is done with L before Y overwrites it
-    write_Y();
-    SGH_TS(3);
-    // stage 2: out = Y · R, R's columns of this item staged in chunks of cw (usually all)
-    int cw = min((jt_hi - jt_lo) * 16, max(16, ((bud / nx4) / 16) * 16));
-    while (cw > 16 && nx4 * pad_b<F>(cw) > bud) cw -= 16;
-    const int ldb2 = pad_b<F>(cw);
-    for (int c0 = c_lo; c0 < c_hi; c0 += cw) {
-      const int cn = min(cw, c_hi - c0);
-      __syncthreads();
-      stage_block<F, kST, kA>(Bb, ldb2, Rm + c0, nx, nx4, cn, nx, cn);
-      __syncthreads();
-      SGH_TS(4);
-#pragma unroll
-      for (int t = 0; t < kTilesPerWave; ++t) {
-        const int jt = jt_lo + wave + NW * t;
-        if (jt >= jt_hi) break;
-        const int col = jt * 16 + li;
-        if (col - li < c0 || col - li >= c0 + cn) continue;  // tile not in this chunk
-        const bool cin = col < c_hi;
-        V a_[kMT];
-#pragma unroll
-        for (int m = 0; m < kMT; ++m) a_[m] = V{0, 0, 0, 0};
-        mma_k<F, kMT, kKS>(a_, mt, nx4,
-                           [&](int m, int k) { return Ys[(16 * m + li) * ldy + k + lk]; },
-                           [&](int k) { return cin ? Bb[(k + lk) * ldb2 + (col - c0)] : F(0); });
-        if (cin) finish(t, col, a_);
-      }
-    }
-  }
-  SGH_TS(5);
-  if (MODE == 1) {
-    // the three norm partials in one exchange: wave sums, then wave 0 adds the waves'
-    // partials in a fixed order
-    v0 = wave_sum(v0);
-    v1 = wave_sum(v1);
-    v2 = wave_sum(v2);
-    if (lane == 0) {
-      sh3[0][wave] = v0;
-      sh3[1][wave] = v1;
-      sh3[2][wave] = v2;
-    }
-    __syncthreads();
-    if (tid < 3) {
-      double a = 0.0;
-      for (int w = 0; w < NW; ++w) a += sh3[tid][w];
-      P.row_slots[3 * (d.row_item0 + it.blk) + tid] = a;
-    }
-  } else if (MODE == 3) {
-    const double m = block_max(v0, sh);
-    if (tid == 0) P.aux_slots[2 * (d.row_item0 + it.blk) + 1] = m;
+    constexpr int RB = 16;
+#include "sgh_surface_pass.inc"
   }
 }
 
@@ -1000,10 +597,20 @@ static void surf_fns(SurfFn<F>* f) {
   f[3] = &k_b_surface<F, 3, RB, NT, CS>;
 }
 
+template <typename F, int NT, bool CS>
+static void surf_fns_hts(SurfFn<F>* f) {
+  f[0] = &k_b_surface_hts<F, 0, NT, CS>;
+  f[1] = &k_b_surface_hts<F, 1, NT, CS>;
+  f[2] = &k_b_surface_hts<F, 2, NT, CS>;
+  f[3] = &k_b_surface_hts<F, 3, NT, CS>;
+}
+
 template <typename F, bool CS>
 static bool surf_lookup_cs(int rows, int threads, SurfFn<F>* f) {
-  if (rows == 0) rows = 32;  // per-cuboid heights run on the 32-row kernels
-  if (rows == 16 && threads == 256) surf_fns<F, 16, 256, CS>(f);
+  // per-cuboid heights: one kernel holding the 16- and the 32-row bodies
+  if (rows == 0 && threads == 512) surf_fns_hts<F, 512, CS>(f);
+  else if (rows == 0 && threads == 1024) surf_fns_hts<F, 1024, CS>(f);
+  else if (rows == 16 && threads == 256) surf_fns<F, 16, 256, CS>(f);
   else if (rows == 16 && threads == 512) surf_fns<F, 16, 512, CS>(f);
   else if (rows == 16 && threads == 1024) surf_fns<F, 16, 1024, CS>(f);
   else if (rows == 32 && threads == 512) surf_fns<F, 32, 512, CS>(f);
