@@ -160,27 +160,36 @@ k_picard_fused(const F* __restrict__ aK, const F* __restrict__ KK, const F* __re
   }
   const double trr = block_sum(rr, sh);
   const double taa = block_sum(aa, sh);
+  if (norms == nullptr) {
+    if (threadIdx.x == 0) {
+      const long long b = (long long)blockIdx.y * gridDim.x + blockIdx.x;
+      partials[2 * b] = trr;
+      partials[2 * b + 1] = taa;
+    }
+    return;
+  }
+  // The last block to finish sums every block's pair in the fixed order of
+  // k_reduce_norms and resets the counter for the next launch. The pairs and the counter
+  // are accessed with device-scope atomics (coherent past the L1 and across the XCDs'
+  // L2s), and a block's pair has completed (s_waitcnt) before the block is counted, so no
+  // fence is needed: a device-scope fence writes back and invalidates the L2, which cost
+  // more than the rest of the pass at these sizes.
   if (threadIdx.x == 0) {
     const long long b = (long long)blockIdx.y * gridDim.x + blockIdx.x;
-    partials[2 * b] = trr;
-    partials[2 * b + 1] = taa;
-  }
-  if (norms == nullptr) return;
-  // the last block to finish sums every block's pair in the fixed order of
-  // k_reduce_norms and resets the counter for the next launch
-  if (threadIdx.x == 0) {
-    __threadfence();   // this block's pair is visible device-wide before it is counted
+    __hip_atomic_store(partials + 2 * b, trr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(partials + 2 * b + 1, taa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_waitcnt(0);
     const unsigned int nb = gridDim.x * gridDim.y;
-    s_last = atomicAdd(counter, 1u) == nb - 1;
+    s_last = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+             nb - 1;
   }
   __syncthreads();
   if (!s_last) return;
-  __threadfence();     // every block's pair is visible to this one
   const int n_pairs = gridDim.x * gridDim.y;
   double r2 = 0.0, a2 = 0.0;
   for (int i = threadIdx.x; i < n_pairs; i += kThreads) {
-    r2 += partials[2 * i];
-    a2 += partials[2 * i + 1];
+    r2 += __hip_atomic_load(partials + 2 * i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    a2 += __hip_atomic_load(partials + 2 * i + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   const double tr = block_sum(r2, sh);
   const double ta = block_sum(a2, sh);
@@ -191,7 +200,7 @@ k_picard_fused(const F* __restrict__ aK, const F* __restrict__ KK, const F* __re
     norms[0] = rms_diff;
     norms[1] = rms_old;
     norms[2] = rel;
-    *counter = 0u;
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (host_out) {
       // straight to pinned host memory: the three norms, then (after a system-scope
       // fence) the sequence number the host polls for
