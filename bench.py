@@ -259,8 +259,13 @@ def main():
     eng_64, t_64 = timed_engine("fp64")
     fp64_rate = N_total * K / t_64
     log(f"fp64 batched: {t_64 * 1e3:.1f} ms -> {fp64_rate:.3e} mode-steps/s")
+    from types import SimpleNamespace
+
+    from spectral_galerkin_heat_amd.io_utils import compute_L2_structured
     from spectral_galerkin_heat_amd.physics.meltpool import cell_temperature, melt_pool_extent
+    from spectral_galerkin_heat_amd.physics.spectral_helpers import reconstruct_temperature_DCT
     num, den, num_x, den_x = 0.0, 0.0, 0.0, 0.0
+    rel_trap = []   # the reference's metric: trapezoid-weighted rel-L2 on the node fields
     rel_max = 0.0
     pool_dev = 0.0   # largest |Δwidth|/dy or |Δdepth|/dz over the cuboids
     melting = 0
@@ -282,11 +287,17 @@ def main():
         pb = melt_pool_extent(cell_temperature(b, g), d)
         melting += pb[1] > 0
         pool_dev = max(pool_dev, abs(pa[1] - pb[1]) / d[1], abs(pa[2] - pb[2]) / d[2])
+        shim = SimpleNamespace(grid=SimpleNamespace(dx=d[0], dy=d[1], dz=d[2]))
+        Tb = reconstruct_temperature_DCT(b, shim).permute(2, 1, 0)
+        Ta = reconstruct_temperature_DCT(a, shim).permute(2, 1, 0)
+        nz_, ny_, nx_ = b.shape
+        coords = [np.linspace(0.0, n * h, n + 1) for n, h in ((nx_, d[0]), (ny_, d[1]), (nz_, d[2]))]
+        rel_trap.append(compute_L2_structured(Tb, Ta, *coords)["L2_rel"])
     rel_batch = (num / den) ** 0.5
     rel_exc = (num_x / den_x) ** 0.5
     log(f"mixed vs fp64: batch rel-L2 {rel_batch:.2e} (excursion {rel_exc:.2e}), worst cuboid "
         f"{rel_max:.2e}; melt pools in {melting} cuboids, max width/depth deviation "
-        f"{pool_dev:.2f} cells")
+        f"{pool_dev:.2f} cells; node fields, trapezoid weights: worst {max(rel_trap):.2e}")
     eng_64.close()
 
     # live per-kernel timing (HIP events on the engine stream) over an instrumented segment
@@ -493,6 +504,8 @@ def main():
             "rel_l2_mixed_vs_fp64_batch": float(f"{rel_batch:.3e}"),
             "rel_l2_mixed_vs_fp64_worst_cuboid": float(f"{rel_max:.3e}"),
             "rel_l2_excursion_mixed_vs_fp64_batch": float(f"{rel_exc:.3e}"),
+            "rel_l2_mixed_vs_fp64_node_trapezoid_worst_cuboid": float(f"{max(rel_trap):.3e}"),
+            "rel_l2_mixed_vs_fp64_node_trapezoid_mean": float(f"{float(np.mean(rel_trap)):.3e}"),
             "melt_pool_cuboids": melting,
             "melt_pool_max_width_depth_deviation_cells": round(pool_dev, 3),
             "volume_pass_GBps": round(vol_gbs, 1),
