@@ -234,12 +234,12 @@ def main():
             tuning[prec] = autotune(ctxs, prec)
             log(f"autotune {prec}: {tuning[prec].summary()}")
             for row in sorted(tuning[prec].table, key=lambda r: r["ms_per_step"]):
-                log(f"    rows {row['rows']:2d} threads {row['threads']:4d} lds {row['lds_bytes'] >> 10:3d} KiB: "
-                    f"{row['ms_per_step']:.4f} ms/step")
+                log(f"    rows {row['rows']:2d} threads {row['threads']:4d} lds {row['lds_bytes'] >> 10:3d} KiB"
+                    f"{' persistent' if row.get('loop') else ''}: {row['ms_per_step']:.4f} ms/step")
 
     def timed_engine(precision):
         tv = tuning.get(precision)
-        eng = BatchEngine(ctxs, precision=precision, variant=tv.variant if tv else None)
+        eng = BatchEngine(ctxs, precision=precision, variant=tv.summary() if tv else None)
         eng.run(W)
         eng.synchronize()
         torch.cuda.synchronize()

@@ -327,6 +327,17 @@ int sgh_batch_variants(int* out_pairs, int capacity);
 int sgh_batch_set_variant(void* plan, int rows, int threads, int lds_bytes, void* stream);
 /* out3 = {rows, threads, LDS bytes}. */
 int sgh_batch_get_variant(void* plan, int* out3);
+/* Persistent step kernel (no reference counterpart; the reference has no batched engine).
+ * With enable = 1 each step's surface work (warm start, every Picard iteration, step end)
+ * runs as ONE launch in which the work items of a cuboid meet at device-scope barriers
+ * between passes, instead of 2·cap + 2 launches.  It is used while the work list fits one
+ * workgroup per CU and the variant has it (rows 16 with 512 or 1024 threads, LDS ≤ 158 KiB);
+ * otherwise the per-pass launches run.  The iterates are identical to the per-pass form's.
+ * A barrier wait is bounded: if one times out, sgh_batch_steps_done and
+ * sgh_batch_read_metrics return an error.  Synchronises `stream`; the work lists and step
+ * graph are rebuilt by the next run.  out2 = {requested, in use by the last work lists}. */
+int sgh_batch_set_loop(void* plan, int enable, void* stream);
+int sgh_batch_loop_state(void* plan, int* out2);
 /* Step counters, Picard status and surface fields of every live cuboid back to zero
  * (synchronises `stream`); the modal states are kept: set them with
  * sgh_batch_set_state before the next run to restart from an initial condition. */

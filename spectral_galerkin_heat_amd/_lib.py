@@ -78,6 +78,8 @@ SIGNATURES = {
     "sgh_batch_set_variant": [vp, i32, i32, i32, vp],
     "sgh_batch_get_variant": [vp, vp],
     "sgh_batch_rewind": [vp, vp],
+    "sgh_batch_set_loop": [vp, i32, vp],
+    "sgh_batch_loop_state": [vp, vp],
 }
 
 

@@ -41,6 +41,8 @@ constexpr int kSurfArrays = 10;
 constexpr int kStageAhead = 4;   // global loads in flight per thread while staging into LDS
 constexpr int kSurfLds = 150 * 1024;  // default LDS bytes per surface workgroup
 constexpr int kLdsMax = 159 * 1024;   // dynamic LDS cap: gfx950's 160 KiB less the static part
+constexpr int kLoopLdsMax = 158 * 1024;  // the same for the persistent step kernel (four passes'
+                                         // static reduction buffers)
 constexpr int kXcds = 8;          // MI355X: workgroup b runs on XCD b mod 8
 constexpr int kCusPerXcd = 32;    // MI355X: 256 CUs
 
@@ -114,6 +116,8 @@ struct BatchPtrs {
   double* status;          // per cuboid: n_it, converged, rel
   int* step_idx;           // per cuboid: step counter
   double* metrics;         // [step][cuboid][5]
+  unsigned* bar;           // per cuboid: barrier arrivals of the persistent step kernel
+  int* loop_err;           // set when a barrier of the persistent step kernel timed out
   int metrics_rows;        // capacity in steps
   int n_cuboids;
 };
@@ -240,7 +244,10 @@ __global__ void __launch_bounds__(kBT) k_b_prep(BatchPtrs<F> P) {
   const CuboidDesc d = P.desc[it.c];
   const int step = P.step_idx[it.c];
   const bool live = step < d.n_steps;
-  if (it.blk == 0 && threadIdx.x == 0) P.done_at[it.c] = live ? -1 : 0x7fffffff;
+  if (it.blk == 0 && threadIdx.x == 0) {
+    P.done_at[it.c] = live ? -1 : 0x7fffffff;
+    P.bar[it.c] = 0u;
+  }
   double psum = 0.0;
   if (live) {
     const double* L = P.laser + (d.laser_off + step) * 6;
@@ -302,9 +309,9 @@ __global__ void __launch_bounds__(kBT) k_b_prep(BatchPtrs<F> P) {
 // 64 lanes of wave 0 together: lane b loads row item b's norm partials and the wave sums
 // them in a fixed butterfly order; AA = Σ aE² is constant over the step and was summed
 // once by k_b_prep into status[3]. Returns true (on every lane) while the cuboid iterates.
-template <typename F>
-__device__ bool picard_active(const BatchPtrs<F>& P, const CuboidDesc& d, int c, int k,
-                              bool record) {
+template <typename F, bool COH = false>
+__device__ __forceinline__ bool picard_active(const BatchPtrs<F>& P, const CuboidDesc& d, int c,
+                                              int k, bool record) {
   const int lane = threadIdx.x & (kWave - 1);
   // all loads first (independent of each other), so they are in flight together
   const int done_at = P.done_at[c];
@@ -312,9 +319,9 @@ __device__ bool picard_active(const BatchPtrs<F>& P, const CuboidDesc& d, int c,
   if (k > 0) {
     if (lane < n_items(d)) {
       const double* sl = P.row_slots + 3 * (d.row_item0 + lane);
-      rr = sl[0];
-      gu = sl[1];
-      g2w = sl[2];
+      rr = ld_x<double, COH>(sl);
+      gu = ld_x<double, COH>(sl + 1);
+      g2w = ld_x<double, COH>(sl + 2);
     }
     aa0 = P.status[kStatusStride * c + 3];
   }
@@ -385,6 +392,7 @@ __device__ __forceinline__ void mma_k(typename Mfma<F>::V (&acc)[KMT], int mt, i
 
 template <typename F, int MODE, int RB, int NT, bool CS>
 __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud, int lds_elems) {
+  constexpr bool LOOP = false;
 #include "sgh_surface_pass.inc"
 }
 
@@ -395,11 +403,110 @@ __global__ void __launch_bounds__(NT) k_b_surface(BatchPtrs<F> P, int k, int bud
 template <typename F, int MODE, int NT, bool CS>
 __global__ void __launch_bounds__(NT) k_b_surface_hts(BatchPtrs<F> P, int k, int bud,
                                                       int lds_elems) {
+  constexpr bool LOOP = false;
   if (P.row_descs[blockIdx.x].d.item_rows > 16) {
     constexpr int RB = 32;
 #include "sgh_surface_pass.inc"
   } else {
     constexpr int RB = 16;
+#include "sgh_surface_pass.inc"
+  }
+}
+
+// Barrier of the items of one cuboid inside the persistent step kernel. What the items
+// exchange (X fields, norm slots) is written and read device-coherently (sc1 stores and
+// loads, LDS DMA with sc1), so no cache flush is needed: every wave waits until its
+// stores have completed, the workgroup meets, and one thread adds its arrival to the
+// cuboid's counter (a device-scope atomic) and polls it with device-scope loads until
+// `target` arrivals. (Device-scope release/acquire fences instead write back and
+// invalidate the XCD's L2 at every barrier: measured 2.5× slower.) The wait is bounded:
+// past kBarrierTicks of the 100 MHz wall clock (or once another barrier has timed out) it
+// records the failure in *err and goes on, so the kernel always ends; the host reports it.
+constexpr unsigned long long kBarrierTicks = 2000000ULL;   // 20 ms
+__device__ __forceinline__ void cuboid_barrier(unsigned* cnt, unsigned target, int* err) {
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long t0 = wall_clock64();
+    while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+      if (wall_clock64() - t0 > kBarrierTicks) {
+        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+  }
+  __syncthreads();
+}
+
+// Persistent form of one step's surface work (one launch instead of 2·cap + 2): each
+// workgroup runs its item through the warm start, every Picard iteration (inverse pass,
+// cuboid barrier, forward pass, cuboid barrier) and the step-end pass, taking the Picard
+// decisions between iterations from the same slots and in the same order as the per-pass
+// launches, so the iterates are identical. It needs every item resident at once: the host
+// launches it only when the work list fits one workgroup per CU (the interleaved list
+// gives XCD x the workgroups b with b mod 8 = x), and P.bar[c] = 0 at the start of the step
+// (k_b_prep). The passes are the per-pass kernel's body included in place, each in a scope
+// of its own with MODE and k set, so that they read the kernel arguments directly (as
+// calls taking the argument block by reference they made the compiler copy it to scratch
+// memory and reload pointers from there in every pass). Measured no faster than the
+// per-pass launches on MI355X (graph-launched kernel boundaries cost about what a
+// cross-workgroup barrier costs), so it is opt-in (sgh_batch_set_loop) and the
+// autotuner times it beside them.
+template <typename F, int RB, int NT, bool CS>
+__global__ void __launch_bounds__(NT) k_b_loop(BatchPtrs<F> P, int bud, int lds_elems,
+                                               int max_it) {
+  constexpr bool LOOP = true;
+  __shared__ int s_go;
+  const ItemDesc& item = P.row_descs[blockIdx.x];
+  const int cub = item.c, blk0 = item.blk;
+  if (cub < 0) return;
+  const CuboidDesc dc = item.d;
+  if (P.step_idx[cub] >= dc.n_steps) return;   // finished cuboid: every pass would skip
+  const unsigned n_it = (unsigned)n_items(dc);
+  unsigned arrivals = 0;
+  auto barrier = [&]() {
+    arrivals += n_it;
+    cuboid_barrier(P.bar + cub, arrivals, P.loop_err);
+  };
+  {
+    constexpr int MODE = 0;
+    const int k = 0;
+#include "sgh_surface_pass.inc"
+  }
+  barrier();
+  int kit = 0;
+  for (; kit < max_it; ++kit) {
+    if (threadIdx.x < kWave) {
+      const bool act = picard_active<F, true>(P, dc, cub, kit, blk0 == 0);
+      if (threadIdx.x == 0) s_go = act ? 1 : 0;
+    }
+    __syncthreads();
+    const bool go = __builtin_amdgcn_readfirstlane(s_go) != 0;
+    __syncthreads();
+    if (!go) break;
+    {
+      constexpr int MODE = 2;
+      const int k = kit;
+#include "sgh_surface_pass.inc"
+    }
+    barrier();
+    {
+      constexpr int MODE = 1;
+      const int k = kit;
+#include "sgh_surface_pass.inc"
+    }
+    barrier();
+  }
+  // ran to the iteration cap: the decision that the per-pass form takes in its step-end
+  // launch records the final status
+  if (kit == max_it && threadIdx.x < kWave)
+    picard_active<F, true>(P, dc, cub, max_it, blk0 == 0);
+  {
+    constexpr int MODE = 3;
+    const int k = max_it;
 #include "sgh_surface_pass.inc"
   }
 }
@@ -532,6 +639,10 @@ struct Plan {
   std::vector<int> pending_prime;  // slots added since the last run
   bool dirty = true;               // work lists / graph out of date
   bool colsplit = false;           // a live cuboid's items are split by columns
+  bool loop_pref = false;          // run each step's surface work as one persistent launch
+  bool loop_on = false;            // ... and the work list allows it (all items resident)
+  int n_cu = 0;                    // compute units of the device
+  unsigned* bar = nullptr;         // [n] barrier counters + 1 error word
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   hipStream_t graph_stream = nullptr;
@@ -572,6 +683,8 @@ BatchPtrs<F> ptrs(Plan* p) {
   P.step_idx = p->step_idx;
   P.metrics = p->metrics;
   P.metrics_rows = p->metrics_rows;
+  P.bar = p->bar;
+  P.loop_err = reinterpret_cast<int*>(p->bar + p->n);
   P.n_cuboids = p->n;
   return P;
 }
@@ -628,6 +741,19 @@ static bool surf_lookup(int rows, int threads, bool colsplit, SurfFn<F>* f) {
 }
 
 template <typename F>
+using LoopFn = void (*)(BatchPtrs<F>, int, int, int);
+
+// Persistent step kernels of the fixed 16-row variants with 512 or 1024 threads.
+template <typename F>
+static LoopFn<F> loop_lookup(int rows, int threads, bool colsplit) {
+  if (rows == 16 && threads == 512)
+    return colsplit ? &k_b_loop<F, 16, 512, true> : &k_b_loop<F, 16, 512, false>;
+  if (rows == 16 && threads == 1024)
+    return colsplit ? &k_b_loop<F, 16, 1024, true> : &k_b_loop<F, 16, 1024, false>;
+  return nullptr;
+}
+
+template <typename F>
 static hipError_t launch_surface(const SurfFn<F>* f, int mode, unsigned nrow, const Plan* p,
                                  BatchPtrs<F>& P, int k, hipStream_t st) {
   int bud = p->bud;
@@ -658,11 +784,14 @@ static hipError_t allow_large_lds(size_t bytes) {
   hipError_t e = hipSuccess;
   for (const SurfVariant& v : kSurfVariants) {
     for (int cs = 0; cs < 2; ++cs) {
-      SurfFn<F> f[4];
+      SurfFn<F> f[5];
       surf_lookup<F>(v.rows, v.threads, cs == 1, f);
-      for (int m = 0; m < 4; ++m) {
+      f[4] = reinterpret_cast<SurfFn<F>>(loop_lookup<F>(v.rows, v.threads, cs == 1));
+      for (int m = 0; m < 5; ++m) {
+        if (!f[m]) continue;
+        const int cap = m == 4 ? std::min((int)bytes, kLoopLdsMax) : (int)bytes;
         hipError_t r = hipFuncSetAttribute(reinterpret_cast<const void*>(f[m]),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, cap);
         if (e == hipSuccess) e = r;
       }
     }
@@ -684,15 +813,27 @@ int enqueue_step(Plan* p, hipStream_t st) {
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
   hipLaunchKernelGGL(k_b_prep<F>, dim3(nrow), dim3(kBT), 0, st, P);
-  chk(launch_surface<F>(f, 0, nrow, p, P, 0, st));
-  if (tm) hipEventRecord(p->ev[0], st);
-  for (int k = 0; k < p->max_it; ++k) {
-    chk(launch_surface<F>(f, 2, nrow, p, P, k, st));
-    chk(launch_surface<F>(f, 1, nrow, p, P, k, st));
+  if (p->loop_on) {
+    // the whole step's surface work in one launch (timed as a whole)
+    LoopFn<F> lf = loop_lookup<F>(p->rows, p->threads, p->colsplit);
+    if (tm) hipEventRecord(p->ev[0], st);
+    int bud = p->bud, lds_elems = (int)(p->shm / p->elem), max_it = p->max_it;
+    void* args[] = {&P, &bud, &lds_elems, &max_it};
+    chk(hipLaunchKernel(reinterpret_cast<const void*>(lf), dim3(nrow), dim3(p->threads), args,
+                        p->shm, st));
+    if (tm) hipEventRecord(p->ev[1], st);
+    if (tm) hipEventRecord(p->ev[2], st);
+  } else {
+    chk(launch_surface<F>(f, 0, nrow, p, P, 0, st));
+    if (tm) hipEventRecord(p->ev[0], st);
+    for (int k = 0; k < p->max_it; ++k) {
+      chk(launch_surface<F>(f, 2, nrow, p, P, k, st));
+      chk(launch_surface<F>(f, 1, nrow, p, P, k, st));
+    }
+    if (tm) hipEventRecord(p->ev[1], st);
+    chk(launch_surface<F>(f, 3, nrow, p, P, p->max_it, st));
+    if (tm) hipEventRecord(p->ev[2], st);
   }
-  if (tm) hipEventRecord(p->ev[1], st);
-  chk(launch_surface<F>(f, 3, nrow, p, P, p->max_it, st));
-  if (tm) hipEventRecord(p->ev[2], st);
   if (e != hipSuccess) return fail_hip(e, "sgh_batch surface launch");
   hipLaunchKernelGGL(k_b_volume<F>, dim3(ncol), dim3(kBT), 0, st, P, 1);
   if (tm) hipEventRecord(p->ev[3], st);
@@ -705,7 +846,7 @@ int enqueue_step(Plan* p, hipStream_t st) {
     hipEventElapsedTime(&a, p->ev[0], p->ev[1]);
     hipEventElapsedTime(&b, p->ev[2], p->ev[3]);
     p->t_surface_ms += a;
-    p->n_surface += 2LL * p->max_it;
+    p->n_surface += p->loop_on ? 1LL : 2LL * p->max_it;
     p->t_volume_ms += b;
     p->n_volume += 1;
   }
@@ -833,7 +974,7 @@ static void free_plan(Plan* p) {
   void* bufs[] = {p->a, p->surf, p->mats, p->t1, p->laser, p->desc, p->row_items, p->row_descs,
                   p->col_items,
                   p->prime_items, p->row_slots, p->col_slots, p->aux_slots, p->done_at,
-                  p->status, p->step_idx, p->metrics};
+                  p->status, p->step_idx, p->metrics, p->bar};
   for (void* b : bufs)
     if (b) hipFree(b);
   delete p;
@@ -1017,6 +1158,13 @@ static int rebuild_lists(Plan* p, hipStream_t st) {
   p->colsplit = false;
   for (int c = 0; c < p->n; ++c)
     if (p->hdesc[c].live && p->hdesc[c].ncb > 1) p->colsplit = true;
+  // the persistent step kernel needs every workgroup resident: one per CU (its LDS), and
+  // the interleaved list gives XCD x the workgroups b with b mod 8 = x
+  const bool has_loop = p->dtype == SGH_F64
+                            ? loop_lookup<double>(p->rows, p->threads, p->colsplit) != nullptr
+                            : loop_lookup<float>(p->rows, p->threads, p->colsplit) != nullptr;
+  p->loop_on = p->loop_pref && has_loop && (long long)p->hrow.size() <= p->n_cu &&
+               p->shm <= (size_t)kLoopLdsMax;
   if ((long long)p->hrow.size() > p->row_cap || (long long)p->hcol.size() > p->col_cap) {
     set_error("sgh_batch: work list exceeds capacity");
     return SGH_ERR_STATE;
@@ -1102,6 +1250,13 @@ int sgh_batch_pool_create(int dtype, int max_cuboids, long long state_capacity,
   chk(dmalloc(&p->status, (size_t)kStatusStride * max_cuboids * sizeof(double)));
   chk(dmalloc(&p->step_idx, (size_t)max_cuboids * sizeof(int)));
   chk(dmalloc(&p->metrics, (size_t)metrics_rows * max_cuboids * kMetricStride * sizeof(double)));
+  chk(dmalloc(&p->bar, ((size_t)max_cuboids + 1) * sizeof(unsigned)));
+  if (e == hipSuccess) e = hipMemset(p->bar, 0, ((size_t)max_cuboids + 1) * sizeof(unsigned));
+  if (e == hipSuccess) {
+    int dev = 0;
+    e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&p->n_cu, hipDeviceAttributeMultiprocessorCount, dev);
+  }
   if (e != hipSuccess) {
     free_plan(p);
     set_error("sgh_batch_pool_create: device allocation failed: %s", hipGetErrorString(e));
@@ -1286,6 +1441,23 @@ int sgh_batch_run(void* plan, int n_steps, int use_graph, void* stream) {
   return SGH_OK;
 }
 
+// A persistent step kernel whose cuboid barrier timed out (its workgroups were not all
+// resident) leaves an error word; the synchronous readers report it and clear it.
+static int check_loop_error(Plan* p, const char* where) {
+  int err = 0;
+  hipError_t e = hipMemcpy(&err, p->bar + p->n, sizeof(int), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return fail_hip(e, where);
+  if (err) {
+    const int zero = 0;
+    hipMemcpy(p->bar + p->n, &zero, sizeof(int), hipMemcpyHostToDevice);
+    set_error("%s: a cuboid barrier of the persistent step kernel timed out (not every "
+              "workgroup was resident); the steps since the last check are invalid. Turn the "
+              "persistent kernel off with sgh_batch_set_loop(plan, 0, stream)", where);
+    return SGH_ERR_STATE;
+  }
+  return SGH_OK;
+}
+
 int sgh_batch_read_metrics(void* plan, int c, double* host_out, int rows, void* stream) {
   Plan* p = static_cast<Plan*>(plan);
   SGH_REQUIRE(p && host_out && c >= 0 && c < p->n && rows >= 0 && rows <= p->metrics_rows,
@@ -1295,7 +1467,7 @@ int sgh_batch_read_metrics(void* plan, int c, double* host_out, int rows, void* 
                                 (size_t)rows * kMetricStride * sizeof(double), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return fail_hip(e, "sgh_batch_read_metrics");
-  return SGH_OK;
+  return check_loop_error(p, "sgh_batch_read_metrics");
 }
 
 int sgh_batch_steps_done(void* plan, int* host_out, void* stream) {
@@ -1306,6 +1478,26 @@ int sgh_batch_steps_done(void* plan, int* host_out, void* stream) {
                                 hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return fail_hip(e, "sgh_batch_steps_done");
+  return check_loop_error(p, "sgh_batch_steps_done");
+}
+
+int sgh_batch_set_loop(void* plan, int enable, void* stream) {
+  Plan* p = static_cast<Plan*>(plan);
+  SGH_REQUIRE(p, SGH_ERR_ARG, "sgh_batch_set_loop: null plan");
+  hipError_t e = hipStreamSynchronize(reinterpret_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return fail_hip(e, "sgh_batch_set_loop");
+  if (p->loop_pref != (enable != 0)) {
+    p->loop_pref = enable != 0;
+    p->dirty = true;  // work lists and the step graph are rebuilt by the next run
+  }
+  return SGH_OK;
+}
+
+int sgh_batch_loop_state(void* plan, int* out2) {
+  Plan* p = static_cast<Plan*>(plan);
+  SGH_REQUIRE(p && out2, SGH_ERR_ARG, "sgh_batch_loop_state: bad args");
+  out2[0] = p->loop_pref ? 1 : 0;
+  out2[1] = p->loop_on ? 1 : 0;
   return SGH_OK;
 }
 

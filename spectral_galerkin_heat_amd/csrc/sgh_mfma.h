@@ -65,11 +65,26 @@ __host__ __device__ constexpr int pad_a(int n) { return pad_to_mod<F>(n, 2 * (in
 template <typename F>
 __host__ __device__ constexpr int pad_b(int n) { return pad_to_mod<F>(n, 16); }
 
+// Device-coherent access (COH): loads and stores with device scope (relaxed atomics,
+// sc1 on gfx950), which bypass the CU's L1 — for data that other workgroups write or read
+// within the same kernel, ordered by the completion waits of a barrier.
+template <typename F, bool COH>
+__device__ __forceinline__ F ld_x(const F* p) {
+  if constexpr (COH) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else return *p;
+}
+template <typename F, bool COH>
+__device__ __forceinline__ void st_x(F* p, F v) {
+  if constexpr (COH) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else *p = v;
+}
+constexpr int kCoherentAux = 16;   // cache-policy operand of an LDS DMA: sc1 (device scope)
+
 // Copy a rows × cols block (source row stride src_ld, starting at src) into LDS
 // (row stride dst_ld), zero-filling rows >= valid_rows and columns >= valid_cols.
 // Each thread keeps A global loads in flight; indices advance incrementally
-// (no per-element division).
-template <typename F, int NT, int A>
+// (no per-element division). COH: device-coherent loads.
+template <typename F, int NT, int A, bool COH = false>
 __device__ __forceinline__ void stage_block(F* __restrict__ dst, int dst_ld, const F* __restrict__ src,
                                             int src_ld, int rows, int cols, int valid_rows,
                                             int valid_cols) {
@@ -84,7 +99,8 @@ __device__ __forceinline__ void stage_block(F* __restrict__ dst, int dst_ld, con
       rr[q] = r;
       cc[q] = c;
       const bool in = e0 + q * NT < total;
-      tmp[q] = (in && r < valid_rows && c < valid_cols) ? src[(long long)r * src_ld + c] : F(0);
+      tmp[q] = (in && r < valid_rows && c < valid_cols) ? ld_x<F, COH>(src + (long long)r * src_ld + c)
+                                                         : F(0);
       r += step_r;
       c += step_c;
       if (c >= cols) { c -= cols; ++r; }
@@ -110,11 +126,12 @@ __device__ __forceinline__ int swz_of(int ld) {
 // its unswizzled position. Each wave copies 1 KiB per instruction (lane l -> dst unit
 // base + l). The caller waits (s_waitcnt + barrier) before reading. Needs 16-byte aligned
 // src rows and dst, ld·sizeof(F) a multiple of 16 and swz a multiple of 16 / sizeof(F).
+template <int AUX = 0>
 __device__ __forceinline__ void dma16(const void* src, void* lds_base) {
-  __builtin_amdgcn_global_load_lds(src, lds_base, 16, 0, 0);
+  __builtin_amdgcn_global_load_lds(src, lds_base, 16, 0, AUX);
 }
 
-template <typename F, int NT>
+template <typename F, int NT, int AUX = 0>
 __device__ __forceinline__ void dma_rows(F* dst, const F* src, int rows, int ld, int src_ld,
                                          int swz) {
   constexpr int E = 16 / (int)sizeof(F);
@@ -127,7 +144,7 @@ __device__ __forceinline__ void dma_rows(F* dst, const F* src, int rows, int ld,
       const int r = q / ld;
       const int cs = q - r * ld;
       const int c = (r & 1) ? (cs ^ swz) : cs;
-      dma16(src + (long long)r * src_ld + c, dst + (long long)u0 * E);
+      dma16<AUX>(src + (long long)r * src_ld + c, dst + (long long)u0 * E);
     }
   }
 }

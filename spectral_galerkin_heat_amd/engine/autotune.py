@@ -9,9 +9,11 @@ workgroups share a CU). The best choice depends on the batch: the number of work
 against the 256 CUs, the widest surface axis, the data type.
 
 ``autotune`` builds a scratch engine over the same cuboids, and for every variant the
-library offers (``sgh_batch_variants``) and every LDS budget: rewinds to the initial
-condition, captures the step graph, replays ``warmup`` steps, then times ``steps``
-graph-replayed steps with HIP events on the engine's stream (best of ``reps``). Every
+library offers (``sgh_batch_variants``), every LDS budget and, where the variant has one and
+the work list fits one workgroup per CU, the persistent one-launch-per-step kernel as well
+as the per-pass launches: rewinds to the initial condition, captures the step graph,
+replays ``warmup`` steps, then times ``steps`` graph-replayed steps with HIP events on the
+engine's stream (best of ``reps``). Every
 variant runs the identical trajectory, so the work compared is the same. The fastest is
 returned (and stored in a JSON cache keyed by device, precision and the sorted cuboid
 shapes when ``cache_path`` is given).
@@ -57,6 +59,7 @@ class TuneResult:
     ms_per_step: float
     default_ms_per_step: float
     table: list = field(default_factory=list)
+    loop: bool = False
 
     @property
     def variant(self):
@@ -64,7 +67,7 @@ class TuneResult:
 
     def summary(self):
         return {"rows": self.rows, "threads": self.threads, "lds_bytes": self.lds_bytes,
-                "ms_per_step": round(self.ms_per_step, 4),
+                "loop": self.loop, "ms_per_step": round(self.ms_per_step, 4),
                 "default_ms_per_step": round(self.default_ms_per_step, 4),
                 "candidates": len(self.table)}
 
@@ -105,21 +108,26 @@ def autotune(contexts, precision="mixed", steps=4, warmup=1, reps=2, lds_choices
                     eng.set_variant(rows, threads, lds)
                 except _lib.SGHError:
                     continue   # this LDS budget cannot stage a full-width operand
-                eng.rewind()
-                eng.run(warmup, use_graph=True)
-                eng.synchronize()
-                ms = _time_steps(eng, steps, reps)
-                table.append({"rows": rows, "threads": threads, "lds_bytes": lds,
-                              "ms_per_step": ms})
+                for loop in (False, True):
+                    eng.set_loop(loop)
+                    eng.rewind()
+                    eng.run(warmup, use_graph=True)
+                    eng.synchronize()
+                    if loop and not eng.loop_state["active"]:
+                        continue   # no persistent kernel for this variant or work list
+                    ms = _time_steps(eng, steps, reps)
+                    table.append({"rows": rows, "threads": threads, "lds_bytes": lds,
+                                  "loop": loop, "ms_per_step": ms})
+            eng.set_loop(False)
     finally:
         eng.close()
     best = min(table, key=lambda r: r["ms_per_step"])
-    dflt = [r for r in table if (r["rows"], r["threads"], r["lds_bytes"]) ==
-            (default["rows"], default["threads"], default["lds_bytes"])]
+    dflt = [r for r in table if (r["rows"], r["threads"], r["lds_bytes"], r["loop"]) ==
+            (default["rows"], default["threads"], default["lds_bytes"], False)]
     res = TuneResult(rows=best["rows"], threads=best["threads"], lds_bytes=best["lds_bytes"],
                      ms_per_step=best["ms_per_step"],
                      default_ms_per_step=dflt[0]["ms_per_step"] if dflt else float("nan"),
-                     table=table)
+                     table=table, loop=bool(best["loop"]))
     if cache_path:
         cache[key] = asdict(res)
         os.makedirs(os.path.dirname(os.path.abspath(cache_path)), exist_ok=True)

@@ -138,14 +138,18 @@ class BatchEngine:
         self.mode_counts = [int(np.prod(s)) for s in self.shapes]
         self.steps_done = 0
         self.tuning = None
+        loop = False
         if isinstance(variant, str) and variant == "auto":
             from .autotune import autotune
             self.tuning = autotune(self.contexts, precision)
-            variant = self.tuning.variant
+            variant, loop = self.tuning.variant, self.tuning.loop
         if variant is not None:
             if isinstance(variant, dict):
+                loop = bool(variant.get("loop", False))
                 variant = (variant["rows"], variant["threads"], variant["lds_bytes"])
             self.set_variant(*variant)
+        if loop:
+            self.set_loop(True)
         self.reset_initial_condition()
 
     def _setup(self, precision):
@@ -181,6 +185,20 @@ class BatchEngine:
         out = (C.c_int * 3)()
         call("sgh_batch_get_variant", self._plan, C.cast(out, C.c_void_p))
         return {"rows": out[0], "threads": out[1], "lds_bytes": out[2]}
+
+    def set_loop(self, enable: bool):
+        """Run each step's surface work as one persistent launch (the items of a cuboid
+        meet at device-scope barriers) instead of one launch per pass. Used only while the
+        work list fits one workgroup per CU and for the 16-row variants with 512 or 1024
+        threads; otherwise the per-pass launches run. Same iterates either way."""
+        call("sgh_batch_set_loop", self._plan, int(bool(enable)), self.stream.cuda_stream)
+
+    @property
+    def loop_state(self):
+        """{'requested', 'active'}; 'active' reflects the work lists of the last run."""
+        out = (C.c_int * 2)()
+        call("sgh_batch_loop_state", self._plan, C.cast(out, C.c_void_p))
+        return {"requested": bool(out[0]), "active": bool(out[1])}
 
     def rewind(self):
         """Restart every cuboid from its initial condition at step 0."""
