@@ -337,6 +337,9 @@ int sgh_batch_get_variant(void* plan, int* out3);
  * sgh_batch_read_metrics return an error.  Synchronises `stream`; the work lists and step
  * graph are rebuilt by the next run.  out2 = {requested, in use by the last work lists}. */
 int sgh_batch_set_loop(void* plan, int enable, void* stream);
+/* Bound of one barrier wait of the persistent step kernel, in seconds (default 0.02; 0 makes
+ * any wait time out at once, which tests use to exercise the error path). */
+int sgh_batch_set_loop_timeout(void* plan, double seconds, void* stream);
 int sgh_batch_loop_state(void* plan, int* out2);
 /* Step counters, Picard status and surface fields of every live cuboid back to zero
  * (synchronises `stream`); the modal states are kept: set them with

@@ -79,6 +79,7 @@ SIGNATURES = {
     "sgh_batch_get_variant": [vp, vp],
     "sgh_batch_rewind": [vp, vp],
     "sgh_batch_set_loop": [vp, i32, vp],
+    "sgh_batch_set_loop_timeout": [vp, f64, vp],
     "sgh_batch_loop_state": [vp, vp],
 }
 

@@ -420,10 +420,11 @@ __global__ void __launch_bounds__(NT) k_b_surface_hts(BatchPtrs<F> P, int k, int
 // cuboid's counter (a device-scope atomic) and polls it with device-scope loads until
 // `target` arrivals. (Device-scope release/acquire fences instead write back and
 // invalidate the XCD's L2 at every barrier: measured 2.5× slower.) The wait is bounded:
-// past kBarrierTicks of the 100 MHz wall clock (or once another barrier has timed out) it
+// past `ticks` of the 100 MHz wall clock (or once another barrier has timed out) it
 // records the failure in *err and goes on, so the kernel always ends; the host reports it.
-constexpr unsigned long long kBarrierTicks = 2000000ULL;   // 20 ms
-__device__ __forceinline__ void cuboid_barrier(unsigned* cnt, unsigned target, int* err) {
+constexpr unsigned long long kBarrierTicks = 2000000ULL;   // default: 20 ms
+__device__ __forceinline__ void cuboid_barrier(unsigned* cnt, unsigned target, int* err,
+                                               unsigned long long ticks) {
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -431,7 +432,7 @@ __device__ __forceinline__ void cuboid_barrier(unsigned* cnt, unsigned target, i
     const unsigned long long t0 = wall_clock64();
     while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
       if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-      if (wall_clock64() - t0 > kBarrierTicks) {
+      if (wall_clock64() - t0 > ticks) {
         __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         break;
       }
@@ -457,7 +458,7 @@ __device__ __forceinline__ void cuboid_barrier(unsigned* cnt, unsigned target, i
 // autotuner times it beside them.
 template <typename F, int RB, int NT, bool CS>
 __global__ void __launch_bounds__(NT) k_b_loop(BatchPtrs<F> P, int bud, int lds_elems,
-                                               int max_it) {
+                                               int max_it, unsigned long long ticks) {
   constexpr bool LOOP = true;
   __shared__ int s_go;
   const ItemDesc& item = P.row_descs[blockIdx.x];
@@ -469,7 +470,7 @@ __global__ void __launch_bounds__(NT) k_b_loop(BatchPtrs<F> P, int bud, int lds_
   unsigned arrivals = 0;
   auto barrier = [&]() {
     arrivals += n_it;
-    cuboid_barrier(P.bar + cub, arrivals, P.loop_err);
+    cuboid_barrier(P.bar + cub, arrivals, P.loop_err, ticks);
   };
   {
     constexpr int MODE = 0;
@@ -641,6 +642,7 @@ struct Plan {
   bool colsplit = false;           // a live cuboid's items are split by columns
   bool loop_pref = false;          // run each step's surface work as one persistent launch
   bool loop_on = false;            // ... and the work list allows it (all items resident)
+  unsigned long long barrier_ticks = kBarrierTicks;  // bound of a barrier wait (100 MHz)
   int n_cu = 0;                    // compute units of the device
   unsigned* bar = nullptr;         // [n] barrier counters + 1 error word
   hipGraph_t graph = nullptr;
@@ -741,7 +743,7 @@ static bool surf_lookup(int rows, int threads, bool colsplit, SurfFn<F>* f) {
 }
 
 template <typename F>
-using LoopFn = void (*)(BatchPtrs<F>, int, int, int);
+using LoopFn = void (*)(BatchPtrs<F>, int, int, int, unsigned long long);
 
 // Persistent step kernels of the fixed 16-row variants with 512 or 1024 threads.
 template <typename F>
@@ -818,7 +820,8 @@ int enqueue_step(Plan* p, hipStream_t st) {
     LoopFn<F> lf = loop_lookup<F>(p->rows, p->threads, p->colsplit);
     if (tm) hipEventRecord(p->ev[0], st);
     int bud = p->bud, lds_elems = (int)(p->shm / p->elem), max_it = p->max_it;
-    void* args[] = {&P, &bud, &lds_elems, &max_it};
+    unsigned long long ticks = p->barrier_ticks;
+    void* args[] = {&P, &bud, &lds_elems, &max_it, &ticks};
     chk(hipLaunchKernel(reinterpret_cast<const void*>(lf), dim3(nrow), dim3(p->threads), args,
                         p->shm, st));
     if (tm) hipEventRecord(p->ev[1], st);
@@ -1490,6 +1493,17 @@ int sgh_batch_set_loop(void* plan, int enable, void* stream) {
     p->loop_pref = enable != 0;
     p->dirty = true;  // work lists and the step graph are rebuilt by the next run
   }
+  return SGH_OK;
+}
+
+int sgh_batch_set_loop_timeout(void* plan, double seconds, void* stream) {
+  Plan* p = static_cast<Plan*>(plan);
+  SGH_REQUIRE(p && seconds >= 0.0 && seconds <= 60.0, SGH_ERR_ARG,
+              "sgh_batch_set_loop_timeout: seconds must be in [0, 60]");
+  hipError_t e = hipStreamSynchronize(reinterpret_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return fail_hip(e, "sgh_batch_set_loop_timeout");
+  p->barrier_ticks = (unsigned long long)(seconds * 1e8);
+  p->dirty = true;  // the captured step graph holds the old bound
   return SGH_OK;
 }
 

@@ -193,6 +193,11 @@ class BatchEngine:
         threads; otherwise the per-pass launches run. Same iterates either way."""
         call("sgh_batch_set_loop", self._plan, int(bool(enable)), self.stream.cuda_stream)
 
+    def set_loop_timeout(self, seconds: float):
+        """Bound of one barrier wait of the persistent kernel (default 0.02 s). A wait that
+        times out is reported by the next ``steps``/``metrics`` read as an error."""
+        call("sgh_batch_set_loop_timeout", self._plan, float(seconds), self.stream.cuda_stream)
+
     @property
     def loop_state(self):
         """{'requested', 'active'}; 'active' reflects the work lists of the last run."""

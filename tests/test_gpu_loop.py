@@ -119,3 +119,25 @@ def test_variants_without_a_persistent_kernel_fall_back():
     assert lb["requested"] and not lb["active"]
     _same(sa, ma, sb, mb)
     eng.close()
+
+
+def test_barrier_timeout_is_reported_and_the_per_pass_form_recovers():
+    # a zero bound makes the first barrier that has to wait time out: every wait in the
+    # kernel then ends at once, the kernel finishes, and the host reports the failure
+    from spectral_galerkin_heat_amd._lib import SGHError
+    cases = batch_uneven_cuboids(n=6, seed=11, n_steps=4)
+    eng = _engine(cases, "mixed", (16, 512, 153600))
+    sa, ma, _ = _run(eng, 2, False)
+    eng.rewind()
+    eng.set_loop(True)
+    eng.set_loop_timeout(0.0)
+    eng.run(2)
+    eng.synchronize()
+    assert eng.loop_state["active"]
+    with pytest.raises(SGHError, match="timed out"):
+        eng.metrics()
+    # the error word is cleared once reported; the default bound gives the right iterates
+    eng.set_loop_timeout(0.02)
+    sb, mb, _ = _run(eng, 2, True)
+    _same(sa, ma, sb, mb)
+    eng.close()
