@@ -1484,9 +1484,17 @@ int sgh_batch_steps_done(void* plan, int* host_out, void* stream) {
   return check_loop_error(p, "sgh_batch_steps_done");
 }
 
+// The persistent step kernel is switched off: a run of the GPU test suite that included it
+// (through the autotuner's candidates) ended in a GPU memory fault whose cause is not found.
+// Until it is, requests to turn it on are refused and the per-pass launches run.
+constexpr bool kLoopEnabled = false;
+
 int sgh_batch_set_loop(void* plan, int enable, void* stream) {
   Plan* p = static_cast<Plan*>(plan);
   SGH_REQUIRE(p, SGH_ERR_ARG, "sgh_batch_set_loop: null plan");
+  SGH_REQUIRE(kLoopEnabled || enable == 0, SGH_ERR_STATE,
+              "sgh_batch_set_loop: the persistent step kernel is switched off (a GPU memory "
+              "fault in a run that used it is not yet explained); the per-pass launches run");
   hipError_t e = hipStreamSynchronize(reinterpret_cast<hipStream_t>(stream));
   if (e != hipSuccess) return fail_hip(e, "sgh_batch_set_loop");
   if (p->loop_pref != (enable != 0)) {

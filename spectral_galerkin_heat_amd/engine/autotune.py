@@ -108,7 +108,9 @@ def autotune(contexts, precision="mixed", steps=4, warmup=1, reps=2, lds_choices
                     eng.set_variant(rows, threads, lds)
                 except _lib.SGHError:
                     continue   # this LDS budget cannot stage a full-width operand
-                for loop in (False, True):
+                # the persistent step kernel is switched off in the library (set_loop(True)
+                # is refused), so only the per-pass launches are timed
+                for loop in (False,):
                     eng.set_loop(loop)
                     eng.rewind()
                     eng.run(warmup, use_graph=True)

@@ -11,7 +11,9 @@ per-pass launches.
 import numpy as np
 import pytest
 
-pytestmark = pytest.mark.gpu
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skip(reason="the persistent step kernel is switched off: a GPU memory "
+                                      "fault in a test run that used it is not yet explained")]
 
 torch = pytest.importorskip("torch")
 
