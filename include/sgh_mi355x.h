@@ -335,7 +335,9 @@ int sgh_batch_get_variant(void* plan, int* out3);
  * otherwise the per-pass launches run.  The iterates are identical to the per-pass form's.
  * A barrier wait is bounded: if one times out, sgh_batch_steps_done and
  * sgh_batch_read_metrics return an error.  Synchronises `stream`; the work lists and step
- * graph are rebuilt by the next run.  out2 = {requested, in use by the last work lists}. */
+ * graph are rebuilt by the next run.  out2 = {requested, in use by the last work lists}.
+ * Currently switched off: enable = 1 returns SGH_ERR_STATE (a GPU memory fault in a run that
+ * used the kernel is not yet explained). */
 int sgh_batch_set_loop(void* plan, int enable, void* stream);
 /* Bound of one barrier wait of the persistent step kernel, in seconds (default 0.02; 0 makes
  * any wait time out at once, which tests use to exercise the error path). */
