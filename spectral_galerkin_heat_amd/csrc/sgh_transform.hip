@@ -335,13 +335,20 @@ constexpr int kBfLd = kGK + 8;        // bf16 LDS row: 80 bytes, 16-byte aligned
 
 // LAT: the epilogue turns each output T into the latent source latent_value(T, Tp) with
 // Tp laid out like C, so a reconstruction's last contraction emits the source directly.
-template <typename F, int VAR, bool B_CF, bool LAT = false>
+// TM: rows of the output tile, 64 (four waves as 2×2, each 32 rows × 32 columns) or 32
+// (four waves side by side, each 32 rows × 16 columns) — the latter doubles the workgroups
+// of a contraction whose 64-row grid would leave most CUs idle.
+template <typename F, int VAR, bool B_CF, bool LAT = false, int TM = kGM>
 __global__ void __launch_bounds__(kT)
 k_gemm_mfma(int M, int N, int K, const F* __restrict__ A, long long sAb, long long sAr,
             const F* __restrict__ B, long long sBb, long long sBk, long long sBc,
             F* __restrict__ C, long long sCb, long long sCr, F scale,
             const F* __restrict__ Tp = nullptr, LatentParams<F> lp = {}) {
   static_assert(VAR == 0 || (VAR == 1 && sizeof(F) == 4), "split-BF16 takes FP32 data");
+  static_assert(TM == 64 || TM == 32, "output tile rows");
+  constexpr int TT = TM == 64 ? 2 : 1;             // 16-column tiles per wave
+  constexpr int CW = 16 * TT;                      // columns per wave
+  constexpr int kGLA = TM * kGK / kT;              // A elements per thread per slice
   using Acc = typename Mfma<F>::V;
   constexpr int ldA = pad_a<F>(kGK), ldB = pad_b<F>(kGN);
   constexpr int kSmemF = VAR == 0 ? (kGM * ldA + kGK * ldB) * (int)sizeof(F) : 16;
@@ -352,8 +359,8 @@ k_gemm_mfma(int M, int N, int K, const F* __restrict__ A, long long sAb, long lo
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
-  const int wr = wave >> 1, wc = wave & 1;
-  const int r0 = blockIdx.y * kGM, c0 = blockIdx.x * kGN;
+  const int wr = TM == 64 ? wave >> 1 : 0, wc = TM == 64 ? wave & 1 : wave;
+  const int r0 = blockIdx.y * TM, c0 = blockIdx.x * kGN;
   const long long bz = blockIdx.z;
   const F* Ab = A + bz * sAb;
   const F* Bb = B + bz * sBb;
@@ -365,23 +372,26 @@ k_gemm_mfma(int M, int N, int K, const F* __restrict__ A, long long sAb, long lo
   constexpr int a_step = kT / kGK;                 // rows of A per q
   constexpr int b_step = B_CF ? kT / kGN : kT / kGK;  // k rows (B_CF) or columns per q
 
-  F ra[kGL], rb[kGL];
+  F ra[kGLA], rb[kGL];
   auto load = [&](int k0) {
 #pragma unroll
-    for (int q = 0; q < kGL; ++q) {
+    for (int q = 0; q < kGLA; ++q) {
       const int r = r0 + ar + q * a_step, k = k0 + ak;
       ra[q] = (r < M && k < K) ? Ab[(long long)r * sAr + k] : F(0);
+    }
+#pragma unroll
+    for (int q = 0; q < kGL; ++q) {
       const int kb = k0 + (B_CF ? bk + q * b_step : bk);
       const int cb = c0 + (B_CF ? bc : bc + q * b_step);
       rb[q] = (kb < K && cb < N) ? Bb[(long long)kb * sBk + (long long)cb * sBc] : F(0);
     }
   };
 
-  Acc acc[2][2];
+  Acc acc[2][TT];
 #pragma unroll
   for (int m = 0; m < 2; ++m)
 #pragma unroll
-    for (int t = 0; t < 2; ++t) acc[m][t] = Acc{0, 0, 0, 0};
+    for (int t = 0; t < TT; ++t) acc[m][t] = Acc{0, 0, 0, 0};
 
   load(0);
   for (int k0 = 0; k0 < K; k0 += kGK) {
@@ -389,8 +399,9 @@ k_gemm_mfma(int M, int N, int K, const F* __restrict__ A, long long sAb, long lo
       F* As = reinterpret_cast<F*>(sm_f);
       F* Bs = As + kGM * ldA;
 #pragma unroll
+      for (int q = 0; q < kGLA; ++q) As[(ar + q * a_step) * ldA + ak] = ra[q];
+#pragma unroll
       for (int q = 0; q < kGL; ++q) {
-        As[(ar + q * a_step) * ldA + ak] = ra[q];
         if (B_CF) Bs[(bk + q * b_step) * ldB + bc] = rb[q];
         else Bs[bk * ldB + bc + q * b_step] = rb[q];
       }
@@ -400,9 +411,12 @@ k_gemm_mfma(int M, int N, int K, const F* __restrict__ A, long long sAb, long lo
       __bf16* Bh = Al + kGM * kBfLd;   // column-major: [c][k]
       __bf16* Bl = Bh + kGN * kBfLd;
 #pragma unroll
-      for (int q = 0; q < kGL; ++q) {
+      for (int q = 0; q < kGLA; ++q) {
         const int ia = (ar + q * a_step) * kBfLd + ak;
         split_bf16((float)ra[q], Ah[ia], Al[ia]);
+      }
+#pragma unroll
+      for (int q = 0; q < kGL; ++q) {
         const int ib = B_CF ? bc * kBfLd + bk + q * b_step : (bc + q * b_step) * kBfLd + bk;
         split_bf16((float)rb[q], Bh[ib], Bl[ib]);
       }
@@ -414,22 +428,22 @@ k_gemm_mfma(int M, int N, int K, const F* __restrict__ A, long long sAb, long lo
       const F* Bs = As + kGM * ldA;
 #pragma unroll
       for (int kk = 0; kk < kGK; kk += 4) {
-        F av[2], bv[2];
+        F av[2], bv[TT];
 #pragma unroll
         for (int m = 0; m < 2; ++m) av[m] = As[(wr * 32 + 16 * m + li) * ldA + kk + lk];
 #pragma unroll
-        for (int t = 0; t < 2; ++t) bv[t] = Bs[(kk + lk) * ldB + wc * 32 + 16 * t + li];
+        for (int t = 0; t < TT; ++t) bv[t] = Bs[(kk + lk) * ldB + wc * CW + 16 * t + li];
 #pragma unroll
         for (int m = 0; m < 2; ++m)
 #pragma unroll
-          for (int t = 0; t < 2; ++t) acc[m][t] = Mfma<F>::mma(av[m], bv[t], acc[m][t]);
+          for (int t = 0; t < TT; ++t) acc[m][t] = Mfma<F>::mma(av[m], bv[t], acc[m][t]);
       }
     } else {
       const __bf16* Ah = reinterpret_cast<const __bf16*>(sm_b);
       const __bf16* Al = Ah + kGM * kBfLd;
       const __bf16* Bh = Al + kGM * kBfLd;
       const __bf16* Bl = Bh + kGN * kBfLd;
-      bf8v ah[2], al[2], bh[2], bl[2];
+      bf8v ah[2], al[2], bh[TT], bl[TT];
 #pragma unroll
       for (int m = 0; m < 2; ++m) {
         const int o = (wr * 32 + 16 * m + li) * kBfLd + 8 * lk;
@@ -437,15 +451,15 @@ k_gemm_mfma(int M, int N, int K, const F* __restrict__ A, long long sAb, long lo
         al[m] = *reinterpret_cast<const bf8v*>(Al + o);
       }
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        const int o = (wc * 32 + 16 * t + li) * kBfLd + 8 * lk;
+      for (int t = 0; t < TT; ++t) {
+        const int o = (wc * CW + 16 * t + li) * kBfLd + 8 * lk;
         bh[t] = *reinterpret_cast<const bf8v*>(Bh + o);
         bl[t] = *reinterpret_cast<const bf8v*>(Bl + o);
       }
 #pragma unroll
       for (int m = 0; m < 2; ++m)
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
+        for (int t = 0; t < TT; ++t) {
           f4v a4 = reinterpret_cast<f4v&>(acc[m][t]);
           a4 = mma_bf16(al[m], bh[t], a4);
           a4 = mma_bf16(ah[m], bl[t], a4);
@@ -460,8 +474,8 @@ k_gemm_mfma(int M, int N, int K, const F* __restrict__ A, long long sAb, long lo
 #pragma unroll
   for (int m = 0; m < 2; ++m)
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      const int col = c0 + wc * 32 + 16 * t + li;
+    for (int t = 0; t < TT; ++t) {
+      const int col = c0 + wc * CW + 16 * t + li;
       if (col >= N) continue;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
@@ -483,28 +497,44 @@ using namespace sgh;
 namespace {
 
 // M is n_out × n_in with row stride ldm (ldm > n_in: a column window of a wider basis).
+// Output tiles of 64 rows, or of 32 when 64-row tiles would leave the CUs mostly idle (fewer
+// workgroups than kFillWgs) or the output has no more than 32 rows.
+constexpr long long kFillWgs = 256;
+template <typename F, int VAR, int TM>
+void launch_axis_tm(long long outer, int n_in, int n_out, long long inner, const F* M,
+                    long long ldm, const F* in, F scale, F* out, hipStream_t st, const F* Tp,
+                    LatentParams<F> lp) {
+  if (inner == 1) {
+    // Out[o, k] = Σ_j In[o, j]·M[k, j]: A = In (outer × n_in), B[j][k] = M[k][j] (k-fast)
+    dim3 grid(cdiv(n_out, kGN), cdiv(outer, TM), 1);
+    if (Tp)
+      hipLaunchKernelGGL((k_gemm_mfma<F, VAR, false, true, TM>), grid, dim3(kT), 0, st,
+                         (int)outer, n_out, n_in, in, 0LL, (long long)n_in, M, 0LL, 1LL, ldm,
+                         out, 0LL, (long long)n_out, scale, Tp, lp);
+    else
+      hipLaunchKernelGGL((k_gemm_mfma<F, VAR, false, false, TM>), grid, dim3(kT), 0, st,
+                         (int)outer, n_out, n_in, in, 0LL, (long long)n_in, M, 0LL, 1LL, ldm,
+                         out, 0LL, (long long)n_out, scale, nullptr, LatentParams<F>{});
+  } else {
+    // Out_o[k, i] = Σ_j M[k, j]·In_o[j, i]: A = M, B = In_o (column-fast), batched over o
+    dim3 grid(cdiv(inner, kGN), cdiv(n_out, TM), (unsigned)outer);
+    hipLaunchKernelGGL((k_gemm_mfma<F, VAR, true, false, TM>), grid, dim3(kT), 0, st, n_out,
+                       (int)inner, n_in, M, 0LL, ldm, in, (long long)n_in * inner, inner, 1LL,
+                       out, (long long)n_out * inner, inner, scale, nullptr, LatentParams<F>{});
+  }
+}
+
 template <typename F, int VAR>
 void launch_axis_mfma(long long outer, int n_in, int n_out, long long inner, const F* M,
                       long long ldm, const F* in, F scale, F* out, hipStream_t st,
                       const F* Tp = nullptr, LatentParams<F> lp = {}) {
-  if (inner == 1) {
-    // Out[o, k] = Σ_j In[o, j]·M[k, j]: A = In (outer × n_in), B[j][k] = M[k][j] (k-fast)
-    dim3 grid(cdiv(n_out, kGN), cdiv(outer, kGM), 1);
-    if (Tp)
-      hipLaunchKernelGGL((k_gemm_mfma<F, VAR, false, true>), grid, dim3(kT), 0, st, (int)outer,
-                         n_out, n_in, in, 0LL, (long long)n_in, M, 0LL, 1LL, ldm, out, 0LL,
-                         (long long)n_out, scale, Tp, lp);
-    else
-      hipLaunchKernelGGL((k_gemm_mfma<F, VAR, false>), grid, dim3(kT), 0, st, (int)outer,
-                         n_out, n_in, in, 0LL, (long long)n_in, M, 0LL, 1LL, ldm, out, 0LL,
-                         (long long)n_out, scale, nullptr, LatentParams<F>{});
-  } else {
-    // Out_o[k, i] = Σ_j M[k, j]·In_o[j, i]: A = M, B = In_o (column-fast), batched over o
-    dim3 grid(cdiv(inner, kGN), cdiv(n_out, kGM), (unsigned)outer);
-    hipLaunchKernelGGL((k_gemm_mfma<F, VAR, true>), grid, dim3(kT), 0, st, n_out, (int)inner,
-                       n_in, M, 0LL, ldm, in, (long long)n_in * inner, inner, 1LL,
-                       out, (long long)n_out * inner, inner, scale, nullptr, LatentParams<F>{});
-  }
+  const long long rows = inner == 1 ? outer : n_out;
+  const long long wgs = inner == 1 ? cdiv(n_out, kGN) * cdiv(outer, kGM)
+                                   : cdiv(inner, kGN) * cdiv(n_out, kGM) * outer;
+  if (rows <= 32 || wgs < kFillWgs)
+    launch_axis_tm<F, VAR, 32>(outer, n_in, n_out, inner, M, ldm, in, scale, out, st, Tp, lp);
+  else
+    launch_axis_tm<F, VAR, 64>(outer, n_in, n_out, inner, M, ldm, in, scale, out, st, Tp, lp);
 }
 
 }  // namespace
