@@ -329,6 +329,21 @@ k_gemm_strided(int M, int N, int K, const F* __restrict__ A, long long sAb, long
 // quadrant (w/2, w%2) as 2×2 MFMA tiles of 16×16. K advances in slices of 32: the
 // slice is staged in LDS while the next one is already loading into registers.
 // ---------------------------------------------------------------------------------
+// Workgroups are dealt to the 8 XCDs round-robin by linear id, each XCD with its own L2.
+// Renumber them so that XCD x takes one contiguous range of tiles (x-fastest in the grid):
+// the column tiles of one row block (which read the same rows of A) then run on one XCD
+// at about the same time, and all but the first read those rows from its L2.
+__device__ __forceinline__ void xcd_tile(int& bx, int& by, int& bz) {
+  const unsigned gx = gridDim.x, gy = gridDim.y;
+  const unsigned n = gx * gy * gridDim.z;
+  const unsigned lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const unsigned per = n / 8u, rem = n % 8u, x = lin % 8u, i = lin / 8u;
+  const unsigned t = (x < rem ? x * (per + 1u) : rem * (per + 1u) + (x - rem) * per) + i;
+  bx = (int)(t % gx);
+  by = (int)((t / gx) % gy);
+  bz = (int)(t / (gx * gy));
+}
+
 constexpr int kGM = 64, kGN = 64, kGK = 32;
 constexpr int kGL = kGM * kGK / kT;   // elements of A (and of B) per thread per slice
 constexpr int kBfLd = kGK + 8;        // bf16 LDS row: 80 bytes, 16-byte aligned reads
@@ -360,8 +375,10 @@ k_gemm_mfma(int M, int N, int K, const F* __restrict__ A, long long sAb, long lo
   const int lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const int wr = TM == 64 ? wave >> 1 : 0, wc = TM == 64 ? wave & 1 : wave;
-  const int r0 = blockIdx.y * TM, c0 = blockIdx.x * kGN;
-  const long long bz = blockIdx.z;
+  int tbx, tby, tbz;
+  xcd_tile(tbx, tby, tbz);
+  const int r0 = tby * TM, c0 = tbx * kGN;
+  const long long bz = tbz;
   const F* Ab = A + bz * sAb;
   const F* Bb = B + bz * sBb;
 
@@ -498,8 +515,10 @@ namespace {
 
 // M is n_out × n_in with row stride ldm (ldm > n_in: a column window of a wider basis).
 // Output tiles of 64 rows, or of 32 when 64-row tiles would leave the CUs mostly idle (fewer
-// workgroups than kFillWgs) or the output has no more than 32 rows.
-constexpr long long kFillWgs = 256;
+// workgroups than kFillWgs<F>) or the output has no more than 32 rows. FP32 contractions
+// gain from 32-row tiles up to 1024 workgroups (box mode 3.43 -> 3.25 ms/step); FP64 ones do
+// not (4.03 -> 4.04).
+template <typename F> constexpr long long kFillWgs = sizeof(F) == 4 ? 1024 : 256;
 template <typename F, int VAR, int TM>
 void launch_axis_tm(long long outer, int n_in, int n_out, long long inner, const F* M,
                     long long ldm, const F* in, F scale, F* out, hipStream_t st, const F* Tp,
@@ -531,7 +550,7 @@ void launch_axis_mfma(long long outer, int n_in, int n_out, long long inner, con
   const long long rows = inner == 1 ? outer : n_out;
   const long long wgs = inner == 1 ? cdiv(n_out, kGN) * cdiv(outer, kGM)
                                    : cdiv(inner, kGN) * cdiv(n_out, kGM) * outer;
-  if (rows <= 32 || wgs < kFillWgs)
+  if (rows <= 32 || wgs < kFillWgs<F>)
     launch_axis_tm<F, VAR, 32>(outer, n_in, n_out, inner, M, ldm, in, scale, out, st, Tp, lp);
   else
     launch_axis_tm<F, VAR, 64>(outer, n_in, n_out, inner, M, ldm, in, scale, out, st, Tp, lp);
