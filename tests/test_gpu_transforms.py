@@ -53,6 +53,25 @@ def test_axis_contract_matches_fp64_einsum(variant, dtype, shape):
     assert rel(got, ref) < TOL[(variant, dtype)]
 
 
+@pytest.mark.parametrize("shape", [(12000, 160, 1), (22050, 192, 1), (4100, 80, 1),
+                                   (37, 96, 160), (112, 96, 160), (301, 128, 192)])
+def test_axis_contract_tile_shapes_match_fp64_einsum(shape):
+    # sizes on both sides of the 32-row / 64-row output-tile choice (below 256 workgroups of
+    # 64-row tiles in FP64, 1024 in FP32), in both layouts (inner = 1: rows of the volume;
+    # inner > 1: planes of a batch), with tile counts that are and are not multiples of the
+    # 8 XCDs the tiles are dealt over
+    rng = np.random.default_rng(sum(shape))
+    outer, n, inner = shape
+    M = rng.standard_normal((n, n))
+    X = rng.standard_normal(shape)
+    X[..., 0, :] += 300.0
+    ref = 0.5 * np.einsum("kj,oji->oki", M, X, optimize=True)
+    for variant, dtype in ((MFMA, torch.float64), (MFMA, torch.float32),
+                           (BF16X3, torch.float32)):
+        got = contract(variant, dtype, M, X, scale=0.5)
+        assert rel(got, ref) < TOL[(variant, dtype)], (variant, dtype)
+
+
 @pytest.mark.parametrize("shape", [(16, 32, 64), (32, 64, 128), (24, 48, 96)])
 def test_dct3d_every_variant_matches_scipy(shape):
     from spectral_galerkin_heat_amd.physics import mi355x_kernels as K
