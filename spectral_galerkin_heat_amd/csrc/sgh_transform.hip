@@ -548,8 +548,8 @@ void launch_axis_mfma(long long outer, int n_in, int n_out, long long inner, con
                       long long ldm, const F* in, F scale, F* out, hipStream_t st,
                       const F* Tp = nullptr, LatentParams<F> lp = {}) {
   const long long rows = inner == 1 ? outer : n_out;
-  const long long wgs = inner == 1 ? cdiv(n_out, kGN) * cdiv(outer, kGM)
-                                   : cdiv(inner, kGN) * cdiv(n_out, kGM) * outer;
+  const long long wgs = inner == 1 ? (long long)cdiv(n_out, kGN) * cdiv(outer, kGM)
+                                   : (long long)cdiv(inner, kGN) * cdiv(n_out, kGM) * outer;
   if (rows <= 32 || wgs < kFillWgs<F>)
     launch_axis_tm<F, VAR, 32>(outer, n_in, n_out, inner, M, ldm, in, scale, out, st, Tp, lp);
   else
@@ -673,6 +673,11 @@ int sgh_axis_contract_strided(int dtype, int variant, long long outer, int n_in,
   else
     SGH_REQUIRE(outer <= 65535 && inner <= (1LL << 31) - 1 && cdiv(n_out, kGM) <= 65535u,
                 SGH_ERR_ARG, "sgh_axis_contract: outer/inner too large");
+  // the tile renumbering over the XCDs counts workgroups in 32 bits
+  const long long tiles = inner == 1 ? (long long)cdiv(n_out, kGN) * cdiv(outer, kGM)
+                                     : (long long)cdiv(inner, kGN) * cdiv(n_out, kGM) * outer;
+  SGH_REQUIRE(tiles <= (1LL << 31) - 1, SGH_ERR_ARG,
+              "sgh_axis_contract: %lld output tiles exceed 2^31 - 1", tiles);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   if (variant == SGH_XFORM_BF16X3) {
     launch_axis_mfma<float, 1>(outer, n_in, n_out, inner, (const float*)M, ldm,
@@ -726,8 +731,9 @@ int sgh_box_latent_source(int dtype, int variant, int nx, int ny, int nz, const 
               SGH_ERR_ARG, "sgh_box_latent_source: unknown variant %d", variant);
   SGH_REQUIRE(!(variant == SGH_XFORM_BF16X3 && dtype != SGH_F32), SGH_ERR_ARG,
               "sgh_box_latent_source: split-BF16 takes FP32 data");
-  SGH_REQUIRE(nzb <= 65535 && cdiv((long long)nzb * nyb, kGM) <= 65535u, SGH_ERR_ARG,
-              "sgh_box_latent_source: box too large");
+  SGH_REQUIRE(nzb <= 65535 && cdiv((long long)nzb * nyb, kGM) <= 65535u &&
+                  (long long)cdiv(nxb, kGN) * cdiv((long long)nzb * nyb, kGM) <= (1LL << 31) - 1,
+              SGH_ERR_ARG, "sgh_box_latent_source: box too large");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   SGH_DISPATCH(dtype, {
     F* w1 = (F*)work;                          // (nzb, ny, nx)

@@ -55,3 +55,17 @@ def test_plain_iteration_descriptor_layout_matches_the_library():
     from spectral_galerkin_heat_amd import _lib
     h = _lib.lib.load()
     assert h.sgh_plain_iter_size() == ctypes.sizeof(_lib.SghPlainIter)
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"),
+                    reason="argument checks only, run where no GPU could be launched on")
+@pytest.mark.parametrize("outer,n,inner", [(65535, 8, (1 << 31) - 1), (4194240, (1 << 31) - 1, 1)])
+def test_axis_contract_refuses_more_output_tiles_than_2_31(outer, n, inner):
+    # refused on the host before any launch: the tile renumbering counts in 32 bits
+    from spectral_galerkin_heat_amd import _lib
+    h = _lib.lib.load()
+    p = ctypes.c_void_p(256)
+    rc = h.sgh_axis_contract_strided(_lib.F64, 0, outer, 8, n, inner, p, 8, ctypes.c_void_p(512),
+                                     0.5, ctypes.c_void_p(1024), None)
+    assert rc == -1
+    assert b"output tiles" in h.sgh_last_error()
