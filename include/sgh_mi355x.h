@@ -293,9 +293,15 @@ int sgh_batch_create(int dtype, int n, const sgh_cuboid_spec* specs, int metrics
                      void** plan, void* stream);
 /* Continuous batching: place a cuboid in a free slot (state zero, step 0; copies the
  * laser table) or free a slot. Both synchronise the stream. Returns SGH_ERR_NOMEM when
- * the pool has no room (the caller keeps the request queued). */
+ * the pool has no room (the caller keeps the request queued), SGH_ERR_ARG for a spec the
+ * pool can never hold (nz above the pool's max_nz, a surface axis above 256). The cosine
+ * matrices of a mode count are shared by the cuboids that use it and released with the
+ * last of them. */
 int sgh_batch_add(void* plan, const sgh_cuboid_spec* spec, int* slot, void* stream);
 int sgh_batch_remove(void* plan, int slot, void* stream);
+/* Distinct surface mode counts whose matrices the pool holds now: writes up to
+ * `capacity` of them to out and returns how many there are. */
+int sgh_batch_matrix_sizes(void* plan, int* out, int capacity);
 int sgh_batch_destroy(void* plan);
 /* Advance every live cuboid by n_steps steps (one captured graph per step if use_graph). */
 int sgh_batch_run(void* plan, int n_steps, int use_graph, void* stream);

@@ -63,6 +63,7 @@ SIGNATURES = {
     "sgh_batch_create": [i32, i32, vp, i32, C.POINTER(vp), vp],
     "sgh_batch_add": [vp, vp, C.POINTER(i32), vp],
     "sgh_batch_remove": [vp, i32, vp],
+    "sgh_batch_matrix_sizes": [vp, vp, i32],
     "sgh_batch_steps_done": [vp, vp, vp],
     "sgh_batch_destroy": [vp],
     "sgh_batch_run": [vp, i32, i32, vp],

@@ -555,7 +555,6 @@ static_assert(kMaxRowBlk * ((kBMaxN + kColBlkW - 1) / kColBlkW) <= kMaxItems, "i
 // ones keep whole rows.
 static int column_blocks(int nx) { return nx > 128 ? (nx + kColBlkW - 1) / kColBlkW : 1; }
 constexpr int kMaxColBlk = kBMaxN * kBMaxN / kBT;  // column blocks per cuboid (S <= 65536)
-constexpr int kMaxSizes = 32;                      // distinct mode counts in the matrix table
 
 static inline long long rup64(long long v) { return (v + 63) / 64 * 64; }
 
@@ -608,8 +607,13 @@ struct Plan {
   std::vector<SlotAlloc> halloc;
   std::vector<Item> hrow, hcol;
   Arena arena_a, arena_s, arena_l;
+  // matrix table: one (B̂, B̂ᵀ) pair per distinct mode count in use, reference-counted by
+  // the live cuboids' x and y axes and released when the last one leaves
   std::vector<int> sizes;
-  std::vector<int> size_off;
+  std::vector<long long> size_off;
+  std::vector<int> size_refs;
+  Arena arena_m;
+  int max_nz = 0;    // deepest cuboid the 1-D table slots hold (t1_stride)
   int t1_stride = 0;
   size_t shm = 0;
   int bud = 0;
@@ -620,7 +624,7 @@ struct Plan {
   void* a = nullptr;
   void* surf = nullptr;  // kSurfArrays arrays of total_S
   void* mats = nullptr;
-  long long mats_cap = 0, mats_used = 0;
+  long long mats_cap = 0;
   double* t1 = nullptr;
   double* laser = nullptr;
   CuboidDesc* desc = nullptr;
@@ -988,14 +992,21 @@ static hipError_t dmalloc(T** ptr, size_t bytes) {
   return hipMalloc((void**)ptr, bytes < 16 ? 16 : bytes);
 }
 
-static long long mats_offset(Plan* p, int m, hipStream_t st, int* rc) {
+// Offset of the (B̂, B̂ᵀ) pair for mode count m, uploading it if no live cuboid uses m yet;
+// every successful call takes one reference (dropped by mats_release).
+static long long mats_offset(Plan* p, int m, int* rc) {
   for (size_t i = 0; i < p->sizes.size(); ++i)
-    if (p->sizes[i] == m) return p->size_off[i];
+    if (p->sizes[i] == m) {
+      p->size_refs[i] += 1;
+      return p->size_off[i];
+    }
   const long long mm = rup64((long long)m * m);  // B̂ and B̂ᵀ start 256-byte aligned
   const long long need = 2LL * mm;
-  if ((int)p->sizes.size() >= kMaxSizes || p->mats_used + need > p->mats_cap) {
+  const long long off = p->arena_m.alloc(need, 64);
+  if (off < 0) {
     *rc = SGH_ERR_NOMEM;
-    set_error("sgh_batch: matrix table full (%d distinct mode counts)", (int)p->sizes.size());
+    set_error("sgh_batch: matrix table full (%d distinct mode counts in use, %lld elements)",
+              (int)p->sizes.size(), p->mats_cap);
     return -1;
   }
   std::vector<double> B = unit_basis(m);
@@ -1012,18 +1023,30 @@ static long long mats_offset(Plan* p, int m, hipStream_t st, int* rc) {
         reinterpret_cast<float*>(buf.data())[i1] = (float)vt;
       }
     }
-  const long long off = p->mats_used;
   // synchronous: the host buffer goes out of scope
   hipError_t e = hipMemcpy((char*)p->mats + off * p->elem, buf.data(), buf.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
+    p->arena_m.release(off, need);
     *rc = fail_hip(e, "sgh_batch matrix upload");
     return -1;
   }
-  (void)st;
-  p->mats_used += need;
   p->sizes.push_back(m);
   p->size_off.push_back(off);
+  p->size_refs.push_back(1);
   return off;
+}
+
+// Drop one reference to mode count m's matrices; the last one frees their table room.
+static void mats_release(Plan* p, int m) {
+  for (size_t i = 0; i < p->sizes.size(); ++i)
+    if (p->sizes[i] == m) {
+      if (--p->size_refs[i] > 0) return;
+      p->arena_m.release(p->size_off[i], 2LL * rup64((long long)m * m));
+      p->sizes.erase(p->sizes.begin() + i);
+      p->size_off.erase(p->size_off.begin() + i);
+      p->size_refs.erase(p->size_refs.begin() + i);
+      return;
+    }
 }
 
 static bool spec_ok(const sgh_cuboid_spec& sp) {
@@ -1061,9 +1084,10 @@ static int add_cuboid(Plan* p, const sgh_cuboid_spec& sp, int c, hipStream_t st)
     return SGH_ERR_NOMEM;
   }
   int rc = SGH_OK;
-  const long long bx = mats_offset(p, sp.nx, st, &rc);
-  const long long by = rc ? -1 : mats_offset(p, sp.ny, st, &rc);
+  const long long bx = mats_offset(p, sp.nx, &rc);
+  const long long by = rc ? -1 : mats_offset(p, sp.ny, &rc);
   if (rc) {
+    if (bx >= 0) mats_release(p, sp.nx);
     p->arena_a.release(al.a_off, al.a_len);
     p->arena_s.release(al.s_off, al.s_len);
     p->arena_l.release(al.l_off, al.l_len);
@@ -1128,6 +1152,16 @@ static int add_cuboid(Plan* p, const sgh_cuboid_spec& sp, int c, hipStream_t st)
   const double zstat[kStatusStride] = {0, 0, 0, 0};
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
+  if ((long long)t1.size() > p->t1_stride) {   // callers check nz <= max_nz first
+    mats_release(p, sp.nx);
+    mats_release(p, sp.ny);
+    p->arena_a.release(al.a_off, al.a_len);
+    p->arena_s.release(al.s_off, al.s_len);
+    p->arena_l.release(al.l_off, al.l_len);
+    set_error("sgh_batch_add: 1-D tables of %d doubles exceed the slot's %d", (int)t1.size(),
+              p->t1_stride);
+    return SGH_ERR_ARG;
+  }
   chk(hipMemcpy(p->t1 + d.t1_off, t1.data(), t1.size() * sizeof(double), hipMemcpyHostToDevice));
   if (sp.n_steps > 0)
     chk(hipMemcpy(p->laser + 6 * al.l_off, sp.laser, 6 * sizeof(double) * sp.n_steps,
@@ -1221,8 +1255,10 @@ int sgh_batch_pool_create(int dtype, int max_cuboids, long long state_capacity,
   p->arena_l.init(laser_capacity);
   p->total_N = state_capacity;
   p->total_S = rup64(surface_capacity);  // each packed surface array starts 256-byte aligned
+  p->max_nz = max_nz;
   p->t1_stride = 2 * (kBMaxN + kBMaxN + max_nz);
-  p->mats_cap = 2LL * 16 * kBMaxN * kBMaxN;
+  p->mats_cap = 2LL * 16 * kBMaxN * kBMaxN;   // room for 16 distinct 256-wide pairs
+  p->arena_m.init(p->mats_cap);
   p->bud = surface_budget(dtype, kernel_rows(p->rows), kSurfLds);
   p->shm = (size_t)kSurfLds;
   if (p->bud < 0) {
@@ -1279,6 +1315,8 @@ int sgh_batch_add(void* plan, const sgh_cuboid_spec* spec, int* slot_out, void* 
   Plan* p = static_cast<Plan*>(plan);
   SGH_REQUIRE(p && spec && slot_out, SGH_ERR_ARG, "sgh_batch_add: bad args");
   SGH_REQUIRE(spec_ok(*spec), SGH_ERR_ARG, "sgh_batch_add: invalid cuboid spec");
+  SGH_REQUIRE(spec->nz <= p->max_nz, SGH_ERR_ARG,
+              "sgh_batch_add: nz = %d exceeds the pool's max_nz = %d", spec->nz, p->max_nz);
   int c = -1;
   for (int i = 0; i < p->n; ++i)
     if (!p->hdesc[i].live) { c = i; break; }
@@ -1292,6 +1330,14 @@ int sgh_batch_add(void* plan, const sgh_cuboid_spec* spec, int* slot_out, void* 
   return SGH_OK;
 }
 
+int sgh_batch_matrix_sizes(void* plan, int* out, int capacity) {
+  Plan* p = static_cast<Plan*>(plan);
+  SGH_REQUIRE(p && capacity >= 0 && (out || capacity == 0), SGH_ERR_ARG,
+              "sgh_batch_matrix_sizes: bad args");
+  for (int i = 0; i < (int)p->sizes.size() && i < capacity; ++i) out[i] = p->sizes[i];
+  return (int)p->sizes.size();
+}
+
 int sgh_batch_remove(void* plan, int c, void* stream) {
   Plan* p = static_cast<Plan*>(plan);
   SGH_REQUIRE(p && c >= 0 && c < p->n && p->hdesc[c].live, SGH_ERR_ARG,
@@ -1303,6 +1349,8 @@ int sgh_batch_remove(void* plan, int c, void* stream) {
   p->arena_a.release(al.a_off, al.a_len);
   p->arena_s.release(al.s_off, al.s_len);
   p->arena_l.release(al.l_off, al.l_len);
+  mats_release(p, p->hdesc[c].nx);
+  mats_release(p, p->hdesc[c].ny);
   p->hdesc[c] = CuboidDesc{};
   e = hipMemcpy(p->desc + c, &p->hdesc[c], sizeof(CuboidDesc), hipMemcpyHostToDevice);
   if (e != hipSuccess) return fail_hip(e, "sgh_batch_remove");

@@ -293,15 +293,36 @@ class BatchPool(BatchEngine):
         self._setup(precision)
         self.metrics_rows = int(metrics_rows)
         self.max_cuboids = int(max_cuboids)
+        self.max_nz = int(max_nz)
         plan = C.c_void_p()
         call("sgh_batch_pool_create", self.code, self.max_cuboids, int(state_capacity),
-             int(surface_capacity), int(laser_capacity), int(max_nz), self.metrics_rows,
+             int(surface_capacity), int(laser_capacity), self.max_nz, self.metrics_rows,
              C.byref(plan))
         self._plan = plan
         self.contexts, self.shapes, self.n_steps = {}, {}, {}
         self.steps_done = 0
 
+    def check_admissible(self, ctx, n_steps):
+        """Raise ``ValueError`` for a request this pool can never hold, whatever is freed:
+        a cuboid deeper than ``max_nz``, more steps than the metrics rows, or forcing the
+        batched engine does not cover."""
+        check_batchable(ctx)
+        nz = int(ctx.geom.n.z)
+        if nz > self.max_nz:
+            raise ValueError(f"nz = {nz} exceeds the pool's max_nz = {self.max_nz}")
+        if int(n_steps) > self.metrics_rows:
+            raise ValueError(f"{int(n_steps)} steps exceed the pool's {self.metrics_rows} "
+                             "metrics rows")
+
+    def matrix_sizes(self):
+        """Distinct surface mode counts whose cosine matrices the pool holds now."""
+        n = call("sgh_batch_matrix_sizes", self._plan, None, 0)
+        out = (C.c_int * max(1, n))()
+        n = call("sgh_batch_matrix_sizes", self._plan, C.cast(out, C.c_void_p), n)
+        return sorted(out[:n])
+
     def add(self, ctx, n_steps, initial_state=None):
+        self.check_admissible(ctx, n_steps)
         sp, tab = cuboid_spec(ctx, n_steps, self.code)
         slot = C.c_int(-1)
         rc = _lib.lib.sgh_batch_add(self._plan, C.byref(sp), C.byref(slot), self.stream.cuda_stream)

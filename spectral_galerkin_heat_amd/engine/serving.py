@@ -111,11 +111,13 @@ class ScanServer:
         self.check_every = int(check_every)
         self.keep_results = keep_results
         self.device_steps = 0
+        self._last_refusal = ""
 
     def _admit(self, req: Request, now: float) -> bool:
         try:
             slot = self.pool.add(req.ctx, req.n_steps)
-        except PoolFull:
+        except PoolFull as exc:
+            self._last_refusal = str(exc)
             return False
         req.slot = slot
         req.admitted_at = now
@@ -133,6 +135,10 @@ class ScanServer:
         for r in pending:
             if r.n_steps > self.max_steps:
                 raise ValueError(f"request {r.rid}: {r.n_steps} steps > max_steps {self.max_steps}")
+            try:
+                self.pool.check_admissible(r.ctx, r.n_steps)
+            except ValueError as exc:
+                raise ValueError(f"request {r.rid}: {exc}") from None
         t0 = clock()
         finish_heap: list = []        # (device step at which it completes, rid)
         by_rid = {r.rid: r for r in queue}
@@ -153,7 +159,7 @@ class ScanServer:
             peak_bytes = max(peak_bytes, sum(r.footprint for r in active.values()))
             if not active and waiting:
                 raise RuntimeError(f"request {waiting[0].rid} does not fit an empty pool "
-                                   "(raise hbm_budget_bytes or max_nz)")
+                                   f"(raise hbm_budget_bytes): {self._last_refusal}")
             if not active:
                 if pending:
                     sleep(max(0.0, pending[0].arrival - (clock() - t0)))

@@ -18,7 +18,7 @@ from .core.parameters import SimulationContext
 __all__ = [
     "MATERIAL_316L", "MATERIAL_316L_LATENT", "batch_uneven_cuboids", "make_context",
     "single_track_box", "single_track_constant", "single_track_tdep", "tdep_track_box",
-    "raster_multitrack", "serving_request",
+    "raster_multitrack", "serving_request", "serving_stream",
 ]
 
 R_GAS = 8.314462618
@@ -145,6 +145,25 @@ def serving_request(rng, dtype="float32"):
     (cfg, lp), = batch_uneven_cuboids(n=1, seed=int(rng.integers(1 << 31)), n_steps=n_steps,
                                       dtype=dtype)
     return cfg, lp
+
+
+def serving_stream(n, rate_hz, seed, dtype="float32"):
+    """``n`` scan requests drawn by ``serving_request`` from ``default_rng(seed)``, arriving
+    as a Poisson process of ``rate_hz`` (first at t = 0, gaps from ``default_rng(seed + 1)``).
+
+    Returns a list of ``(cfg, laser_path, n_steps, arrival_s)``.
+    """
+    rng = np.random.default_rng(seed)
+    shapes = [serving_request(rng, dtype=dtype) for _ in range(n)]
+    gaps = np.random.default_rng(seed + 1).exponential(1.0 / rate_hz, size=n)
+    gaps[0] = 0.0
+    arrivals = np.cumsum(gaps)
+    out = []
+    for (cfg, lp), t in zip(shapes, arrivals):
+        sim = cfg["simulation"]
+        steps = int(round(sim["duration"] / sim["dt"]))
+        out.append((cfg, lp, steps, float(t)))
+    return out
 
 
 def single_track_box(mesh=(80, 48, 120), n_steps=140, dtype="float64", refinement=2,

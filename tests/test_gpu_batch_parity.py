@@ -182,3 +182,61 @@ def test_alone_equals_in_mix_for_a_tuned_variant():
     assert torch.equal(mix.state(2), alone.state(0))
     mix.close()
     alone.close()
+
+
+def test_c4_sixty_four_cuboids_fp64_matches_oracle():
+    """The benchmark batch itself: 64 uneven cuboids from default_rng(0), 3 steps of the
+    FP64 engine against the oracle per cuboid (≤ 1e-9, identical Picard counts). Its work
+    list holds more surface items than the 256 CUs, so a second round of workgroups runs,
+    and its cuboids are binned over the 8 XCDs."""
+    cases = batch_uneven_cuboids(n=64, seed=0, n_steps=3)
+    eng = _engine(cases, "fp64")
+    info = eng.info
+    assert info["surface_items"] > 256, info
+    eng.run(3)
+    eng.synchronize()
+    met = eng.metrics()
+    worst = 0.0
+    for i, (cfg, lp) in enumerate(cases):
+        ref, its, tmax, pl = _oracle_run(cfg, lp, 3)
+        err = modal_rel_l2(eng.state(i).cpu().numpy(), ref.a)
+        worst = max(worst, err)
+        assert err < 1e-9, (i, err)
+        assert met[:, i, 2].astype(int).tolist() == its, i
+        np.testing.assert_allclose(met[:, i, 1], tmax, rtol=1e-9)
+    print(f"C4 64 cuboids, 3 FP64 steps: worst rel-L2 vs oracle {worst:.2e}")
+    eng.close()
+
+
+def test_c4_mixed_autotuned_matches_fp64_over_1000_steps():
+    """The full C4 run: 1000 steps of the mixed engine with the autotuned launch shape
+    against the FP64 engine (≤ 1e-4 relative L2 per cuboid; melt-pool width and depth of
+    the cell-centred fields within one grid spacing)."""
+    from spectral_galerkin_heat_amd.engine.batch import BatchEngine
+    from spectral_galerkin_heat_amd.physics.meltpool import cell_temperature, melt_pool_extent
+    cases = batch_uneven_cuboids(n=64, seed=0, n_steps=1000)
+    ctxs = [make_context(c, lp) for c, lp in cases]
+    mixed = BatchEngine(ctxs, precision="mixed", variant="auto")
+    fp64 = BatchEngine(ctxs, precision="fp64")
+    mixed.run(1000)
+    fp64.run(1000)
+    mixed.synchronize()
+    fp64.synchronize()
+    worst, melted = 0.0, 0
+    for i, ctx in enumerate(ctxs):
+        a = mixed.state(i).double()
+        b = fp64.state(i)
+        err = modal_rel_l2(a.cpu().numpy(), b.cpu().numpy())
+        worst = max(worst, err)
+        assert err < 1e-4, (i, err)
+        g = ctx.geom
+        d = (g.size.x / g.n.x, g.size.y / g.n.y, g.size.z / g.n.z)
+        ea = melt_pool_extent(cell_temperature(a, g), d)
+        eb = melt_pool_extent(cell_temperature(b, g), d)
+        melted += eb[0] > 0
+        assert abs(ea[1] - eb[1]) <= d[1] * (1 + 1e-9), (i, ea, eb)
+        assert abs(ea[2] - eb[2]) <= d[2] * (1 + 1e-9), (i, ea, eb)
+    print(f"C4 1000 steps, variant {mixed.variant}: worst mixed-vs-FP64 rel-L2 {worst:.2e}, "
+          f"{melted} cuboids melt at the end")
+    mixed.close()
+    fp64.close()

@@ -50,6 +50,51 @@ def test_pool_add_remove_and_capacity():
     pool.close()
 
 
+def test_pool_refuses_a_cuboid_deeper_than_max_nz():
+    # the 1-D table slot of a cuboid holds 2·(512 + max_nz) doubles: a deeper cuboid would
+    # write into the next slot's tables, so both the binding and the C entry point refuse it
+    import ctypes as C
+    from spectral_galerkin_heat_amd import _lib
+    from spectral_galerkin_heat_amd.engine.batch import BatchPool, cuboid_spec
+    from spectral_galerkin_heat_amd.workloads import single_track_constant
+    pool = BatchPool(precision="mixed", max_cuboids=2, state_capacity=1 << 20,
+                     surface_capacity=1 << 14, laser_capacity=64, max_nz=16, metrics_rows=8)
+    ok, lp = single_track_constant(mesh=(32, 16, 16), n_steps=4, dtype="float32")
+    deep, _ = single_track_constant(mesh=(32, 16, 24), n_steps=4, dtype="float32")
+    with pytest.raises(ValueError, match="max_nz"):
+        pool.add(make_context(deep, lp), 4)
+    sp, tab = cuboid_spec(make_context(deep, lp), 4, _lib.F32)
+    slot = C.c_int(-1)
+    rc = _lib.lib.sgh_batch_add(pool._plan, C.byref(sp), C.byref(slot), pool.stream.cuda_stream)
+    assert rc == -1 and "max_nz" in _lib.last_error()
+    i = pool.add(make_context(ok, lp), 4)
+    pool.run(4)
+    pool.synchronize()
+    assert pool.steps()[i] == 4
+    pool.close()
+
+
+def test_pool_releases_matrices_of_departed_mode_counts():
+    # 40 distinct surface widths come and go one after another; each width's cosine
+    # matrices are freed with its last cuboid, so the table never fills
+    from spectral_galerkin_heat_amd.engine.batch import BatchPool
+    from spectral_galerkin_heat_amd.workloads import single_track_constant
+    pool = BatchPool(precision="fp64", max_cuboids=4, state_capacity=1 << 20,
+                     surface_capacity=1 << 14, laser_capacity=256, max_nz=8, metrics_rows=4)
+    keep_cfg, lp = single_track_constant(mesh=(24, 20, 4), n_steps=2)
+    keep = pool.add(make_context(keep_cfg, lp), 2)
+    for nx in range(200, 240):
+        cfg, lp2 = single_track_constant(mesh=(nx, 12, 4), n_steps=2)
+        i = pool.add(make_context(cfg, lp2), 2)
+        assert pool.matrix_sizes() == sorted({12, 20, 24, nx})
+        pool.run(1)
+        pool.remove(i)
+        assert pool.matrix_sizes() == [20, 24]
+    pool.remove(keep)
+    assert pool.matrix_sizes() == []
+    pool.close()
+
+
 def test_served_results_equal_solo_runs():
     from spectral_galerkin_heat_amd.engine.batch import BatchEngine
     from spectral_galerkin_heat_amd.engine.serving import ScanServer
@@ -78,6 +123,41 @@ def test_fp64_recheck_within_1e4():
     srv = ScanServer(precision="mixed", max_cuboids=8, hbm_budget_bytes=1 << 28, max_steps=64,
                      check_every=1)
     rep = srv.serve(reqs)
+    checks = srv.fp64_checks(rep)
+    assert len(checks) == 3 and max(checks) < 1e-4, checks
+    srv.close()
+
+
+def test_c5_poisson_stream_with_rasters():
+    """C5: 60 requests from the serving distribution (3 of them 256×128×64 rasters with
+    1-5 tracks, 200-2000 steps each) arriving as a 40 Hz Poisson stream. Every served
+    result equals a solo run of the same request bit for bit for a sample holding a
+    raster, and every 20th request re-solved in FP64 agrees to 1e-4."""
+    from spectral_galerkin_heat_amd.engine.batch import BatchEngine
+    from spectral_galerkin_heat_amd.engine.serving import Request, ScanServer
+    from spectral_galerkin_heat_amd.workloads import serving_stream
+    stream = serving_stream(60, 40.0, seed=3)
+    reqs = [Request(rid=i, ctx=make_context(cfg, lp), n_steps=n, arrival=t)
+            for i, (cfg, lp, n, t) in enumerate(stream)]
+    rasters = [r.rid for r in reqs if tuple(r.ctx.geom.n) == (256, 128, 64)]
+    assert len(rasters) >= 2
+    srv = ScanServer(precision="mixed", max_cuboids=160, hbm_budget_bytes=8 << 30,
+                     max_steps=2048, steps_per_tick=8, check_every=20)
+    rep = srv.serve(reqs)
+    assert len(rep.requests) == 60
+    assert rep.peak_active >= 10
+    s = rep.summary()
+    print("C5 serving:", {k: s[k] for k in ("p50_latency_s", "p99_latency_s",
+                                            "aggregate_mode_steps_per_s", "peak_active")})
+    by = {r.rid: r for r in rep.requests}
+    for rid in (rasters[0], 7, 33):
+        r = by[rid]
+        solo = BatchEngine([r.ctx], precision="mixed", n_steps=r.n_steps)
+        solo.run(r.n_steps)
+        solo.synchronize()
+        assert torch.equal(solo.state(0), r.result), rid
+        np.testing.assert_array_equal(solo.metrics()[:, 0], r.metrics)
+        solo.close()
     checks = srv.fp64_checks(rep)
     assert len(checks) == 3 and max(checks) < 1e-4, checks
     srv.close()

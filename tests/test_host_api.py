@@ -197,14 +197,50 @@ def test_scan_server_rejects_a_request_that_never_fits():
     from spectral_galerkin_heat_amd.engine.batch import PoolFull
 
     class NoRoomPool:
+        def check_admissible(self, ctx, n_steps):
+            pass
+
         def add(self, ctx, n_steps):
             raise PoolFull("no room")
 
     srv = ScanServer.__new__(ScanServer)
     srv.pool, srv.precision, srv.max_steps = NoRoomPool(), "mixed", 64
     srv.steps_per_tick, srv.check_every, srv.keep_results, srv.device_steps = 8, 20, False, 0
+    srv._last_refusal = ""
     req = Request(rid=7, ctx=None, n_steps=4, arrival=0.0)
-    with pytest.raises(RuntimeError, match="request 7 does not fit"):
+    with pytest.raises(RuntimeError, match="request 7 does not fit.*no room"):
+        srv.serve([req], clock=lambda: 0.0, sleep=lambda s: None)
+
+
+def test_pool_admission_refuses_what_no_free_room_can_hold():
+    # a cuboid deeper than the pool's max_nz (its 1-D table slot), more steps than the
+    # metrics rows, or forcing outside the batched engine's scope is refused before any
+    # device call, whatever the pool holds; the server refuses such a request up front
+    from types import SimpleNamespace
+    from spectral_galerkin_heat_amd.engine.batch import BatchPool
+    from spectral_galerkin_heat_amd.engine.serving import Request, ScanServer
+    from spectral_galerkin_heat_amd.workloads import (make_context, single_track_constant,
+                                                      single_track_tdep)
+    pool = SimpleNamespace(max_nz=16, metrics_rows=8)
+    cfg, lp = single_track_constant(mesh=(32, 16, 16), n_steps=4)
+    BatchPool.check_admissible(pool, make_context(cfg, lp), 8)
+    deep, _ = single_track_constant(mesh=(32, 16, 24), n_steps=4)
+    with pytest.raises(ValueError, match="max_nz"):
+        BatchPool.check_admissible(pool, make_context(deep, lp), 4)
+    with pytest.raises(ValueError, match="metrics rows"):
+        BatchPool.check_admissible(pool, make_context(cfg, lp), 9)
+    tdep, lp2 = single_track_tdep(mesh=(16, 8, 8), n_steps=2)
+    with pytest.raises(ValueError, match="latent heat"):
+        BatchPool.check_admissible(pool, make_context(tdep, lp2), 2)
+
+    class Pool:
+        def check_admissible(self, ctx, n_steps):
+            BatchPool.check_admissible(pool, ctx, n_steps)
+
+    srv = ScanServer.__new__(ScanServer)
+    srv.pool, srv.max_steps = Pool(), 64
+    req = Request(rid=3, ctx=make_context(deep, lp), n_steps=4, arrival=0.0)
+    with pytest.raises(ValueError, match="request 3: nz = 24 exceeds"):
         srv.serve([req], clock=lambda: 0.0, sleep=lambda s: None)
 
 
