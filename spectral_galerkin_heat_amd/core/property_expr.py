@@ -1,227 +1,232 @@
-"""Whitelisted temperature expressions for material properties.
+"""Temperature expressions for material properties, compiled once to a postfix program.
 
-Same accepted language as the reference's ``PropertyExpr``
-(``src/spectral_galerkin_heat/core/property_expr.py:69-147,226-300``): a bare number,
-or an expression of ``T`` using ``+ - * / **``, unary ``+``/``-``, the constants ``pi``
-and ``e`` and a fixed list of math functions. Anything else raises ``ValueError`` at
-construction.
+Accepted language (the interface of the reference's ``PropertyExpr``,
+``src/spectral_galerkin_heat/core/property_expr.py:69-121``): a bare number, or an
+expression of ``T`` built from numeric literals, the names ``T``, ``pi`` and ``e``, the
+binary operators ``+ - * / **``, unary ``+``/``-``, and calls of the math functions listed
+in ``FUNC_IDS`` (``atan2`` with two arguments, the others with one, no keywords).
+Anything else is refused with ``ValueError`` when the expression is built.
 
-The reference generates Numba/CUDA source text per expression. Here an expression is
-compiled once into a small stack program (``to_program``) that the HIP property kernels
-interpret per cell, so one compiled library serves any config without runtime codegen.
+One representation serves the host and the device. The syntax tree is lowered, in one
+pass that also does the checking, into a postfix program: a list of integer words
+(an opcode, followed by an operand word for ``CONST`` and ``FN``) over a pool of
+constants. The HIP property kernels interpret that program per cell
+(``csrc/sgh_props.h``) and ``PropertyExpr.__call__`` interprets the same program on
+NumPy values, so both sides evaluate the same sequence of operations. Powers by the
+literals 2, 0.5, -1 and 1 become square, square root, reciprocal and nothing, which is
+what NumPy's ``power`` does for those exponents.
 """
 
 from __future__ import annotations
 
 import ast
 import math
+import operator
 
 import numpy as np
 
 __all__ = ["PropertyExpr", "FUNC_IDS", "OPCODES"]
 
-# function name -> numpy ufunc name (host evaluation)
-_NUMPY_FUNC_NAME = {
-    "sqrt": "sqrt", "exp": "exp", "log": "log", "log2": "log2", "log10": "log10",
-    "sin": "sin", "cos": "cos", "tan": "tan",
-    "asin": "arcsin", "acos": "arccos", "atan": "arctan", "atan2": "arctan2",
-    "sinh": "sinh", "cosh": "cosh", "tanh": "tanh",
-    "asinh": "arcsinh", "acosh": "arccosh", "atanh": "arctanh",
-    "fabs": "abs", "ceil": "ceil", "floor": "floor", "trunc": "trunc",
-    "log1p": "log1p", "expm1": "expm1",
-}
-_ALLOWED_CONSTANTS = {"pi": math.pi, "e": math.e}
-_ALLOWED_BINOPS = (ast.Add, ast.Sub, ast.Mult, ast.Div, ast.Pow)
-_ALLOWED_UNARYOPS = (ast.UAdd, ast.USub)
-
-# Device stack-program encoding (must match sgh_props.h).
+# Opcodes of the device program (the values are fixed by csrc/sgh_props.h).
 OPCODES = {
     "CONST": 0, "T": 1, "ADD": 2, "SUB": 3, "MUL": 4, "DIV": 5, "POW": 6,
     "NEG": 7, "FN": 8, "ATAN2": 9, "SQR": 10, "CUBE": 11, "RECIP": 12,
 }
-FUNC_IDS = {name: i for i, name in enumerate(
-    ["sqrt", "exp", "log", "log2", "log10", "sin", "cos", "tan", "asin", "acos",
-     "atan", "sinh", "cosh", "tanh", "asinh", "acosh", "atanh", "fabs", "ceil",
-     "floor", "trunc", "log1p", "expm1"])}
+
+# Whitelisted one-argument functions: device function id -> (name, NumPy ufunc).
+_FUNCTIONS = (
+    ("sqrt", np.sqrt), ("exp", np.exp), ("log", np.log), ("log2", np.log2),
+    ("log10", np.log10), ("sin", np.sin), ("cos", np.cos), ("tan", np.tan),
+    ("asin", np.arcsin), ("acos", np.arccos), ("atan", np.arctan), ("sinh", np.sinh),
+    ("cosh", np.cosh), ("tanh", np.tanh), ("asinh", np.arcsinh), ("acosh", np.arccosh),
+    ("atanh", np.arctanh), ("fabs", np.abs), ("ceil", np.ceil), ("floor", np.floor),
+    ("trunc", np.trunc), ("log1p", np.log1p), ("expm1", np.expm1),
+)
+FUNC_IDS = {name: i for i, (name, _) in enumerate(_FUNCTIONS)}
+_NAMED_CONSTANTS = {"pi": math.pi, "e": math.e}
+
+# Device evaluator capacity (csrc/sgh_props.h: kProgMaxOps, kProgMaxConsts, kProgMaxStack).
 MAX_OPS = 64
 MAX_CONSTS = 24
 MAX_STACK = 16
 
-
-def _validate(node):
-    if isinstance(node, ast.Constant):
-        if isinstance(node.value, bool) or not isinstance(node.value, (int, float)):
-            raise ValueError(f"Only numeric literals are allowed, got {node.value!r}.")
-        return
-    if isinstance(node, ast.Name):
-        if node.id != "T" and node.id not in _ALLOWED_CONSTANTS:
-            raise ValueError(
-                f"Unknown name {node.id!r} in property expression "
-                f"(only 'T' and {sorted(_ALLOWED_CONSTANTS)} are allowed).")
-        return
-    if isinstance(node, ast.BinOp):
-        if not isinstance(node.op, _ALLOWED_BINOPS):
-            raise ValueError(f"Operator {type(node.op).__name__} is not allowed.")
-        _validate(node.left)
-        _validate(node.right)
-        return
-    if isinstance(node, ast.UnaryOp):
-        if not isinstance(node.op, _ALLOWED_UNARYOPS):
-            raise ValueError(f"Unary operator {type(node.op).__name__} is not allowed.")
-        _validate(node.operand)
-        return
-    if isinstance(node, ast.Call):
-        if (not isinstance(node.func, ast.Name) or node.func.id not in _NUMPY_FUNC_NAME
-                or node.keywords or not node.args):
-            name = getattr(node.func, "id", "<expr>")
-            raise ValueError(f"Function {name!r} is not allowed in a property expression.")
-        want = 2 if node.func.id == "atan2" else 1
-        if len(node.args) != want:
-            raise ValueError(f"{node.func.id} takes {want} argument(s).")
-        for arg in node.args:
-            _validate(arg)
-        return
-    raise ValueError(f"{type(node).__name__} is not allowed in a property expression.")
+_BINARY_CODES = {ast.Add: OPCODES["ADD"], ast.Sub: OPCODES["SUB"], ast.Mult: OPCODES["MUL"],
+                 ast.Div: OPCODES["DIV"], ast.Pow: OPCODES["POW"]}
+# powers NumPy evaluates by a cheaper exact operation: exponent -> replacement words
+_POWER_SHORTCUTS = {2.0: [OPCODES["SQR"]], 0.5: [OPCODES["FN"], FUNC_IDS["sqrt"]],
+                    -1.0: [OPCODES["RECIP"]], 1.0: []}
 
 
-def _eval(node, T):
-    if isinstance(node, ast.Constant):
-        return node.value
-    if isinstance(node, ast.Name):
-        return T if node.id == "T" else _ALLOWED_CONSTANTS[node.id]
-    if isinstance(node, ast.BinOp):
-        left, right = _eval(node.left, T), _eval(node.right, T)
-        op = type(node.op)
-        if op is ast.Add:
-            return left + right
-        if op is ast.Sub:
-            return left - right
-        if op is ast.Mult:
-            return left * right
-        if op is ast.Div:
-            return left / right
-        return left ** right
-    if isinstance(node, ast.UnaryOp):
-        val = _eval(node.operand, T)
-        return -val if isinstance(node.op, ast.USub) else val
-    fn = getattr(np, _NUMPY_FUNC_NAME[node.func.id])
-    return fn(*(_eval(a, T) for a in node.args))
+class _Lowering:
+    """Checks a syntax tree against the whitelist while emitting its postfix program."""
 
+    def __init__(self, source: str):
+        self.source = source
+        self.words: list[int] = []
+        self.pool: list[float] = []
 
-class _Compiler:
-    def __init__(self):
-        self.ops: list[int] = []
-        self.consts: list[float] = []
-        self.depth = 0
-        self.max_depth = 0
+    def refuse(self, what: str):
+        raise ValueError(f"property expression {self.source!r}: {what} is not allowed")
 
-    def _push(self):
-        self.depth += 1
-        self.max_depth = max(self.max_depth, self.depth)
+    def constant(self, value) -> None:
+        value = float(value)
+        if value not in self.pool:
+            self.pool.append(value)
+        self.words += [OPCODES["CONST"], self.pool.index(value)]
 
-    def const(self, v: float):
-        v = float(v)
-        if v in self.consts:
-            idx = self.consts.index(v)
+    def lower(self, node) -> None:
+        handler = getattr(self, "_" + type(node).__name__, None)
+        if handler is None:
+            self.refuse(f"a {type(node).__name__} node")
+        handler(node)
+
+    def _Constant(self, node):
+        if type(node.value) not in (int, float):
+            self.refuse(f"the literal {node.value!r} (numbers only)")
+        self.constant(node.value)
+
+    def _Name(self, node):
+        if node.id == "T":
+            self.words.append(OPCODES["T"])
+        elif node.id in _NAMED_CONSTANTS:
+            self.constant(_NAMED_CONSTANTS[node.id])
         else:
-            idx = len(self.consts)
-            self.consts.append(v)
-        self.ops += [OPCODES["CONST"], idx]
-        self._push()
+            self.refuse(f"the name {node.id!r} (T, pi and e only)")
 
-    def emit(self, node):
-        if isinstance(node, ast.Constant):
-            self.const(node.value)
-        elif isinstance(node, ast.Name):
-            if node.id == "T":
-                self.ops.append(OPCODES["T"])
-                self._push()
-            else:
-                self.const(_ALLOWED_CONSTANTS[node.id])
-        elif isinstance(node, ast.BinOp):
-            if isinstance(node.op, ast.Pow) and isinstance(node.right, ast.Constant) \
-                    and float(node.right.value) in (2.0, 0.5, -1.0, 1.0):
-                # numpy's scalar-power fast paths: square, sqrt, reciprocal, identity.
-                p = float(node.right.value)
-                self.emit(node.left)
-                if p == 2.0:
-                    self.ops.append(OPCODES["SQR"])
-                elif p == 0.5:
-                    self.ops += [OPCODES["FN"], FUNC_IDS["sqrt"]]
-                elif p == -1.0:
-                    self.ops.append(OPCODES["RECIP"])
-                return
-            self.emit(node.left)
-            self.emit(node.right)
-            op = type(node.op)
-            code = {ast.Add: "ADD", ast.Sub: "SUB", ast.Mult: "MUL",
-                    ast.Div: "DIV", ast.Pow: "POW"}[op]
-            self.ops.append(OPCODES[code])
-            self.depth -= 1
-        elif isinstance(node, ast.UnaryOp):
-            self.emit(node.operand)
-            if isinstance(node.op, ast.USub):
-                self.ops.append(OPCODES["NEG"])
-        else:  # ast.Call
-            for a in node.args:
-                self.emit(a)
-            if node.func.id == "atan2":
-                self.ops.append(OPCODES["ATAN2"])
-                self.depth -= 1
-            else:
-                self.ops += [OPCODES["FN"], FUNC_IDS[node.func.id]]
+    def _UnaryOp(self, node):
+        if not isinstance(node.op, (ast.UAdd, ast.USub)):
+            self.refuse(f"the unary operator {type(node.op).__name__}")
+        self.lower(node.operand)
+        if isinstance(node.op, ast.USub):
+            self.words.append(OPCODES["NEG"])
+
+    def _BinOp(self, node):
+        code = _BINARY_CODES.get(type(node.op))
+        if code is None:
+            self.refuse(f"the operator {type(node.op).__name__}")
+        right = node.right
+        if (code == OPCODES["POW"] and isinstance(right, ast.Constant)
+                and type(right.value) in (int, float) and float(right.value) in _POWER_SHORTCUTS):
+            self.lower(node.left)
+            self.words += _POWER_SHORTCUTS[float(right.value)]
+            return
+        self.lower(node.left)
+        self.lower(right)
+        self.words.append(code)
+
+    def _Call(self, node):
+        name = node.func.id if isinstance(node.func, ast.Name) else None
+        arity = 2 if name == "atan2" else 1
+        if (name is None or (name not in FUNC_IDS and name != "atan2") or node.keywords
+                or len(node.args) != arity):
+            self.refuse(f"the call of {name or 'an expression'!s} with "
+                        f"{len(node.args)} argument(s)")
+        for arg in node.args:
+            self.lower(arg)
+        if name == "atan2":
+            self.words.append(OPCODES["ATAN2"])
+        else:
+            self.words += [OPCODES["FN"], FUNC_IDS[name]]
+
+
+def _stack_depth(words) -> int:
+    """Deepest evaluation stack the postfix program needs."""
+    depth = deepest = 0
+    i = 0
+    while i < len(words):
+        op = words[i]
+        if op in (OPCODES["CONST"], OPCODES["FN"]):
+            i += 1
+        if op in (OPCODES["CONST"], OPCODES["T"]):
+            depth += 1
+        elif op in (OPCODES["ADD"], OPCODES["SUB"], OPCODES["MUL"], OPCODES["DIV"],
+                    OPCODES["POW"], OPCODES["ATAN2"]):
+            depth -= 1
+        deepest = max(deepest, depth)
+        i += 1
+    return deepest
+
+
+_HOST_BINARY = {OPCODES["ADD"]: operator.add, OPCODES["SUB"]: operator.sub,
+                OPCODES["MUL"]: operator.mul, OPCODES["DIV"]: operator.truediv,
+                OPCODES["POW"]: operator.pow, OPCODES["ATAN2"]: np.arctan2}
+_HOST_UNARY = {OPCODES["NEG"]: operator.neg, OPCODES["SQR"]: lambda v: v * v,
+               OPCODES["CUBE"]: lambda v: v * v * v, OPCODES["RECIP"]: lambda v: 1.0 / v}
+
+
+def _interpret(words, pool, T):
+    """Run a postfix program on the host: T is a Python scalar or a NumPy array."""
+    stack = []
+    i = 0
+    while i < len(words):
+        op = words[i]
+        if op == OPCODES["CONST"]:
+            i += 1
+            stack.append(pool[words[i]])
+        elif op == OPCODES["T"]:
+            stack.append(T)
+        elif op == OPCODES["FN"]:
+            i += 1
+            stack[-1] = _FUNCTIONS[words[i]][1](stack[-1])
+        elif op in _HOST_UNARY:
+            stack[-1] = _HOST_UNARY[op](stack[-1])
+        else:
+            rhs = stack.pop()
+            stack[-1] = _HOST_BINARY[op](stack[-1], rhs)
+        i += 1
+    return stack[0]
 
 
 class PropertyExpr:
-    """A validated math expression of ``T`` (or a bare constant)."""
+    """A property branch: a bare number, or a whitelisted expression of ``T``."""
 
-    __slots__ = ("_const", "_tree", "source")
+    __slots__ = ("source", "_value", "_words", "_pool")
 
     def __init__(self, expr):
         self.source = str(expr).strip()
         try:
-            self._const = float(self.source)
-            self._tree = None
-            return
+            self._value = float(self.source)
         except ValueError:
-            pass
+            self._value = None
+        if self._value is not None:
+            self._words, self._pool = [OPCODES["CONST"], 0], [self._value]
+            return
         try:
-            tree = ast.parse(self.source, mode="eval").body
+            tree = ast.parse(self.source, mode="eval")
         except SyntaxError as exc:
-            raise ValueError(f"Invalid property expression {self.source!r}: {exc}") from None
-        _validate(tree)
-        self._const = None
-        self._tree = tree
+            raise ValueError(f"property expression {self.source!r} does not parse: "
+                             f"{exc.msg}") from None
+        low = _Lowering(self.source)
+        low.lower(tree.body)
+        self._words, self._pool = low.words, low.pool
 
     @property
     def is_constant(self) -> bool:
-        return self._const is not None
+        """True for a bare number (the constant-property fast path)."""
+        return self._value is not None
 
     @property
     def constant_value(self) -> float:
-        if self._const is None:
-            raise ValueError(f"{self.source!r} is not a constant expression.")
-        return self._const
+        if self._value is None:
+            raise ValueError(f"property expression {self.source!r} depends on T")
+        return self._value
 
     def __call__(self, T):
-        """Evaluate on the host (Python scalar or NumPy array)."""
-        if self._const is not None:
-            return T * 0.0 + self._const
-        return _eval(self._tree, T)
+        """Value at ``T`` (a Python scalar or a NumPy array); a constant keeps T's shape."""
+        if self._value is not None:
+            return T * 0.0 + self._value
+        return _interpret(self._words, self._pool, T)
 
     def to_program(self) -> tuple[list[int], list[float]]:
-        """Compile to the device stack program ``(ops, consts)``."""
-        c = _Compiler()
-        if self._const is not None:
-            c.const(self._const)
-        else:
-            c.emit(self._tree)
-        if len(c.ops) > MAX_OPS or len(c.consts) > MAX_CONSTS or c.max_depth > MAX_STACK:
+        """The device program ``(words, constants)``; refused when it exceeds the device
+        evaluator's capacity."""
+        depth = _stack_depth(self._words)
+        if len(self._words) > MAX_OPS or len(self._pool) > MAX_CONSTS or depth > MAX_STACK:
             raise ValueError(
-                f"Property expression {self.source!r} is too large for the device "
-                f"evaluator (ops {len(c.ops)}/{MAX_OPS}, constants "
-                f"{len(c.consts)}/{MAX_CONSTS}, stack {c.max_depth}/{MAX_STACK}).")
-        return c.ops, c.consts
+                f"property expression {self.source!r} needs {len(self._words)} program "
+                f"words, {len(self._pool)} constants and a stack of {depth}; the device "
+                f"evaluator holds {MAX_OPS}, {MAX_CONSTS} and {MAX_STACK}")
+        return list(self._words), list(self._pool)
 
     def __repr__(self):
         return f"PropertyExpr({self.source!r})"

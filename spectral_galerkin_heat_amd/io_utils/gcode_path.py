@@ -1,105 +1,151 @@
-"""G-code laser path: G0/G1 moves, M3 S<power> / M5, G20/G21 units.
+"""Laser path read from a G-code file, held as arrays of timed segments.
 
-Restates ``src/spectral_galerkin_heat/io_utils/gcode_path.py:15-119``: positions are
-quantised to float32 after unit scaling, a segment is emitted only for a move made
-while the laser is on, off-moves take no time, and the state interpolates linearly in
-time along the active segment with a finite-difference velocity over ``dt``. Before
-the first segment the beam is off at ``initial_position``; after the last it is off
-at the final point.
+The semantics are those of the reference's ``GCodeLaserPath``
+(``src/spectral_galerkin_heat/io_utils/gcode_path.py:15-119``), pinned by its unit tests
+(``tests/unit_tests/io_utils/test_gcode_path.py:24-58``) and its example paths:
+
+* recognised lines (after stripping; empty lines and lines starting with ``;`` are
+  skipped): ``G21`` (millimetres) and ``G20`` (inches) set the scale to metres; a line
+  starting with ``G0`` or ``G1`` is a move whose ``X``/``Y``/``F`` words (every
+  whitespace-separated word starting with that letter) set the target and the feed, each
+  scaled and rounded to float32; ``M3`` switches the beam on with the power of its ``S``
+  word, ``M5`` switches it off and zeroes the power;
+* a move made while the beam is on becomes a segment lasting ``distance / (feed / 60)``
+  (1 m/s when no feed was given); moves with the beam off take no time, so the clock runs
+  only along lit segments;
+* at time t inside a segment the beam is on at the float32 linear interpolation, with the
+  finite-difference velocity over ``dt`` (zero at the segment start); after the last
+  segment it is off at the path's end, before the first off at ``initial_position``
+  (scaled by the file's unit). Segment times and points are float32; the query time
+  enters the comparisons and the interpolation as given, so a Python float is compared
+  and interpolated in float32 and a float64 scalar in float64.
+
+The file is parsed once into NumPy arrays (start and end points, start and end times,
+power); a query finds its segment by binary search over the end times, which rise with
+the segment index.
 """
 
 from __future__ import annotations
+
+import math
 
 import numpy as np
 
 from ..core.laser import LaserPath, LaserState
 
-__all__ = ["GCodeLaserPath"]
+__all__ = ["GCodeLaserPath", "read_gcode_segments"]
+
+_SCALE_TO_M = {"G21": 1.0 / 1000.0, "G20": 25.4 / 1000.0}
+_F32_ZERO = np.float32(0.0)
+
+
+def _last_value(words, letter, scale=1.0):
+    """Value of the last word that starts with ``letter``, times ``scale`` (None if no word
+    does). Every such word must be a number."""
+    values = [float(w[1:]) * scale for w in words if w.startswith(letter)]
+    return values[-1] if values else None
+
+
+def read_gcode_segments(path):
+    """Parse a G-code file into ``(table, unit_scale)``.
+
+    ``table`` is a dict of float32 arrays: ``p0`` and ``p1`` (segments × 2, start and end
+    point in metres), ``t0`` and ``t1`` (start and end time in seconds) and ``power`` (W).
+    """
+    scale = 1.0
+    xy = [0.0, 0.0]                 # current point (float32 values held as Python floats)
+    prev = None                     # point of the previous move
+    feed = np.float32(0.0)
+    power, lit = 0.0, False
+    clock = 0.0                     # seconds along the lit segments (float64)
+    rows = []
+    with open(path) as fh:
+        for raw in fh:
+            line = raw.strip()
+            if not line or line.startswith(";"):
+                continue
+            code = line[:3]
+            if code in _SCALE_TO_M:
+                scale = _SCALE_TO_M[code]
+                continue
+            words = line.split()
+            if line.startswith(("G0", "G1")):
+                for axis, letter in enumerate("XY"):
+                    v = _last_value(words, letter, scale)
+                    if v is not None:
+                        xy[axis] = float(np.float32(v))
+                v = _last_value(words, "F", scale)
+                if v is not None:
+                    feed = np.float32(v)
+                if prev is not None and lit:
+                    ddx, ddy = xy[0] - prev[0], xy[1] - prev[1]
+                    speed = float(feed) / 60.0 if feed > 0 else 1.0
+                    t_end = clock + math.sqrt(ddx * ddx + ddy * ddy) / speed
+                    rows.append((prev[0], prev[1], xy[0], xy[1], clock, t_end, power))
+                    clock = t_end
+                prev = (xy[0], xy[1])
+            elif line.startswith("M3"):
+                v = _last_value(words, "S")
+                if v is not None:
+                    power = v
+                lit = True
+            elif line.startswith("M5"):
+                lit, power = False, 0.0
+    arr = np.array(rows, dtype=np.float64).reshape(-1, 7).astype(np.float32)
+    table = {"p0": arr[:, 0:2].copy(), "p1": arr[:, 2:4].copy(), "t0": arr[:, 4].copy(),
+             "t1": arr[:, 5].copy(), "power": arr[:, 6].copy()}
+    return table, scale
 
 
 class GCodeLaserPath(LaserPath):
+    """``LaserPath`` of a G-code file (lit moves become timed segments)."""
+
     def __init__(self, gcode_file, initial_position=(0.0, 0.0)):
-        self.segments, self.unit_scale = self._parse_gcode(gcode_file)
-        self.initial_position = (
-            np.float32(initial_position[0]) * np.float32(self.unit_scale),
-            np.float32(initial_position[1]) * np.float32(self.unit_scale),
-        )
-        self.current_segment = 0
+        self._seg, self.unit_scale = read_gcode_segments(gcode_file)
+        s32 = np.float32(self.unit_scale)
+        self.initial_position = (np.float32(initial_position[0]) * s32,
+                                 np.float32(initial_position[1]) * s32)
+
+    @property
+    def n_segments(self) -> int:
+        return int(self._seg["t0"].shape[0])
+
+    @property
+    def segments(self):
+        """Segments as ``((x0, y0), (x1, y1), t0, t1, power, is_on)`` tuples of float32."""
+        s = self._seg
+        return [((s["p0"][i, 0], s["p0"][i, 1]), (s["p1"][i, 0], s["p1"][i, 1]), s["t0"][i],
+                 s["t1"][i], s["power"][i], True) for i in range(self.n_segments)]
 
     @staticmethod
-    def _parse_gcode(filepath):
-        segments = []
-        pos = [0.0, 0.0]
-        power, on, feed = 0.0, False, 0.0
-        last_pos, last_t, scale = None, 0.0, 1.0
-        with open(filepath) as f:
-            for raw in f:
-                line = raw.strip()
-                if not line or line.startswith(";"):
-                    continue
-                if line.startswith("G21"):
-                    scale = 1.0 / 1000.0
-                    continue
-                if line.startswith("G20"):
-                    scale = 25.4 / 1000.0
-                    continue
-                if line.startswith(("G0", "G1")):
-                    x = y = None
-                    for tok in line.split():
-                        if tok.startswith("X"):
-                            x = np.float32(float(tok[1:]) * scale)
-                        elif tok.startswith("Y"):
-                            y = np.float32(float(tok[1:]) * scale)
-                        elif tok.startswith("F"):
-                            feed = np.float32(float(tok[1:]) * scale)
-                    if x is not None:
-                        pos[0] = float(x)
-                    if y is not None:
-                        pos[1] = float(y)
-                    if last_pos is not None and on:
-                        ddx, ddy = pos[0] - last_pos[0], pos[1] - last_pos[1]
-                        dist = float(np.sqrt(ddx * ddx + ddy * ddy))
-                        speed = float(feed) / 60.0 if feed > 0 else 1.0
-                        seg_dt = dist / speed if speed > 0 else 0.0
-                        t1 = float(last_t + seg_dt)
-                        segments.append((
-                            (np.float32(last_pos[0]), np.float32(last_pos[1])),
-                            (np.float32(pos[0]), np.float32(pos[1])),
-                            np.float32(last_t), np.float32(t1), np.float32(power), bool(on)))
-                        last_t = t1
-                    last_pos = list(pos)
-                elif line.startswith("M3"):
-                    for tok in line.split():
-                        if tok.startswith("S"):
-                            power = float(tok[1:])
-                    on = True
-                elif line.startswith("M5"):
-                    on = False
-                    power = 0.0
-        return segments, scale
+    def _off(x, y) -> LaserState:
+        return LaserState(x=np.float32(x), y=np.float32(y), power=_F32_ZERO, is_on=False,
+                          v=(_F32_ZERO, _F32_ZERO))
 
     def get_state(self, time: float, dt: float) -> LaserState:
-        for start, end, t0, t1, power, on in self.segments:
-            if t0 <= time <= t1:
-                frac = (time - t0) / (t1 - t0) if t1 > t0 else 0.0
-                x = np.float32(start[0] + frac * (end[0] - start[0]))
-                y = np.float32(start[1] + frac * (end[1] - start[1]))
-                if time > t0 and dt > 0.0:
-                    pf = (((time - dt) - t0) / (t1 - t0)
-                          if (time - dt) >= t0 and (t1 - t0) > 0 else 0.0)
-                    px = start[0] + pf * (end[0] - start[0])
-                    py = start[1] + pf * (end[1] - start[1])
-                    vx = np.float32((float(x) - float(px)) / float(dt))
-                    vy = np.float32((float(y) - float(py)) / float(dt))
-                else:
-                    vx = vy = np.float32(0.0)
-                return LaserState(x=np.float32(x), y=np.float32(y), power=np.float32(power),
-                                  is_on=on, v=(vx, vy))
-        if self.segments and time > self.segments[-1][3]:
-            end = self.segments[-1][1]
-            return LaserState(x=np.float32(end[0]), y=np.float32(end[1]),
-                              power=np.float32(0.0), is_on=False,
-                              v=(np.float32(0.0), np.float32(0.0)))
-        return LaserState(x=np.float32(self.initial_position[0]),
-                          y=np.float32(self.initial_position[1]), power=np.float32(0.0),
-                          is_on=False, v=(np.float32(0.0), np.float32(0.0)))
+        # ``time`` takes part in every comparison and product as given, so NumPy's scalar
+        # promotion sets the arithmetic: float32 for a Python float, float64 for np.float64
+        s = self._seg
+        n = self.n_segments
+        if n:
+            kind = np.result_type(s["t1"], time)
+            k = int(np.searchsorted(s["t1"].astype(kind), np.asarray(time, dtype=kind),
+                                    side="left"))  # first segment ending at or after time
+        else:
+            k = 0
+        if k < n and s["t0"][k] <= time:
+            t0, t1 = s["t0"][k], s["t1"][k]
+            p0, d = s["p0"][k], s["p1"][k] - s["p0"][k]
+            frac = (time - t0) / (t1 - t0) if t1 > t0 else 0.0
+            here = (p0 + frac * d).astype(np.float32)
+            vel = (_F32_ZERO, _F32_ZERO)
+            if time > t0 and dt > 0.0:
+                back = time - dt
+                f_back = (back - t0) / (t1 - t0) if (back >= t0 and t1 - t0 > 0) else 0.0
+                there = p0 + f_back * d
+                vel = tuple(np.float32((float(here[i]) - float(there[i])) / float(dt))
+                            for i in range(2))
+            return LaserState(x=here[0], y=here[1], power=s["power"][k], is_on=True, v=vel)
+        if n and time > s["t1"][-1]:
+            return self._off(s["p1"][-1, 0], s["p1"][-1, 1])
+        return self._off(*self.initial_position)

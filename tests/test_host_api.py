@@ -167,18 +167,6 @@ def test_batch_rejects_volume_forcing():
         check_batchable(make_context(c, lp))
 
 
-def test_gcode_path(tmp_path):
-    from spectral_galerkin_heat_amd.io_utils.gcode_path import GCodeLaserPath
-    f = tmp_path / "t.gcode"
-    f.write_text("G21\nG0 X0 Y0\nM3 S200\nG1 X10 Y0 F600\nM5\nG0 X0 Y1\n")
-    p = GCodeLaserPath(str(f))
-    assert len(p.segments) == 1
-    s = p.get_state(0.5, 0.01)
-    assert s.is_on and float(s.x) == pytest.approx(5e-3, rel=1e-6) and float(s.power) == 200.0
-    assert float(s.v[0]) == pytest.approx(0.01, rel=1e-3)
-    assert not p.get_state(2.0, 0.01).is_on
-
-
 def test_melt_pool_extent_counts_cells_from_the_top():
     import torch
 
