@@ -110,6 +110,27 @@ int sgh_picard_iter_fused_norms(int dtype, int nx, int ny, int nz, const void* a
                                 void* a_new, double* partials, unsigned int* counter,
                                 double* norms, double* host_out, double seq, void* stream);
 
+/* The same pass for callers that cannot allocate device memory (code under hipGraph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). No counterpart in the
+ * reference. Its scratch (the per-workgroup norm pairs and the last-workgroup counter)
+ * is a fixed array of SGH_NOALLOC_MAX_PAIRS pairs (1 MiB) that the library holds on every
+ * GPU it runs on until the process ends, so the caller passes only inputs and outputs
+ * (norms: 3 device doubles; host_out and seq as above). Refuses
+ * sgh_picard_partials_count(nx, ny, nz, nchunks) > SGH_NOALLOC_MAX_PAIRS with
+ * SGH_ERR_ARG. Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (different streams, no ordering between them) race and must not be issued. The
+ * first call on a device also enqueues a one-thread kernel on `stream` that zeroes the
+ * counter; the pass leaves it at zero. Results are bit-identical to
+ * sgh_picard_iter_fused_norms with caller buffers. */
+#define SGH_NOALLOC_MAX_PAIRS 65536
+int sgh_picard_iter_fused_norms_noalloc(int dtype, int nx, int ny, int nz, const void* aK,
+                                        const void* KK, const void* a_old, const void* wt,
+                                        const void* St, const void* wb, const void* Sb,
+                                        const void* Q, double omega, const void* wz,
+                                        int nchunks, void* zc_parts, void* a_new,
+                                        double* norms, double* host_out, double seq,
+                                        void* stream);
+
 /* Pinned, coherent host memory that kernels can write and the host can poll
  * (hipHostMalloc with hipHostMallocCoherent); no reference counterpart. */
 int sgh_host_alloc(size_t bytes, void** out);

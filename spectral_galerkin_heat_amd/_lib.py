@@ -34,6 +34,8 @@ SIGNATURES = {
     "sgh_reduce_norms": [vp, i32, f64, vp, vp],
     "sgh_picard_iter_fused_norms": [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, f64, vp,
                                     i32, vp, vp, vp, vp, vp, vp, f64, vp],
+    "sgh_picard_iter_fused_norms_noalloc": [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp,
+                                            vp, f64, vp, i32, vp, vp, vp, vp, f64, vp],
     "sgh_host_alloc": [C.c_size_t, C.POINTER(vp)],
     "sgh_points_yz": [i32, i32, i32, i32, vp, vp, vp, vp, vp],
     "sgh_sep2d_forcing": [i32, i32, i32, vp, vp, vp, f64, vp, f64, f64, f64, f64, f64, f64, f64,

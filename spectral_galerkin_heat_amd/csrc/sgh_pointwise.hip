@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <mutex>
 #include <string>
 
 #include "sgh_common.h"
@@ -654,6 +655,77 @@ int sgh_picard_iter_fused(int dtype, int nx, int ny, int nz, const void* aK, con
   return sgh_picard_iter_fused_norms(dtype, nx, ny, nz, aK, KK, a_old, wt, St, wb, Sb, Q, omega,
                                      wz, nchunks, zc_parts, a_new, partials, nullptr, nullptr,
                                      nullptr, 0.0, stream);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------
+// Allocation-free form of the fused Picard pass: its scratch (the per-block norm pairs
+// and the last-workgroup counter) is a fixed device array of this library instead of
+// caller buffers. Every GPU holds its own copy until the process ends; the host reaches
+// it only through the address hipGetSymbolAddress gives on that device.
+// ---------------------------------------------------------------------------------
+namespace sgh {
+
+constexpr int kNoAllocMaxPairs = SGH_NOALLOC_MAX_PAIRS;
+// [0, 2·kNoAllocMaxPairs): norm pairs; then one 32-bit counter word (zero at load, and
+// the pass's last workgroup puts it back to zero)
+static __device__ __align__(256) double scratch[2 * kNoAllocMaxPairs + 32];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+__global__ void k_zero_word(unsigned int* w) { *w = 0u; }
+
+// Device address of the current device's scratch array; on the first call for a device
+// it also enqueues a one-thread kernel on `stream` that zeroes the counter word.
+static int scratch_for_current_device(hipStream_t stream, double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail_hip(e, "sgh_picard_iter_fused_norms_noalloc: hipGetDevice");
+  SGH_REQUIRE(dev >= 0 && dev < kMaxDevices, SGH_ERR_STATE,
+              "sgh_picard_iter_fused_norms_noalloc: device %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    double* ptr = nullptr;
+    e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr)
+      return fail_hip(e != hipSuccess ? e : hipErrorInvalidSymbol,
+                      "sgh_picard_iter_fused_norms_noalloc: hipGetSymbolAddress");
+    hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(1), 0, stream,
+                       reinterpret_cast<unsigned int*>(ptr + 2 * kNoAllocMaxPairs));
+    SGH_CHECK_LAUNCH("sgh_picard_iter_fused_norms_noalloc: counter reset");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return SGH_OK;
+}
+
+}  // namespace sgh
+
+extern "C" {
+
+int sgh_picard_iter_fused_norms_noalloc(int dtype, int nx, int ny, int nz, const void* aK,
+                                        const void* KK, const void* a_old, const void* wt,
+                                        const void* St, const void* wb, const void* Sb,
+                                        const void* Q, double omega, const void* wz,
+                                        int nchunks, void* zc_parts, void* a_new,
+                                        double* norms, double* host_out, double seq,
+                                        void* stream) {
+  SGH_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nchunks >= 1 && nchunks <= nz && norms,
+              SGH_ERR_ARG, "sgh_picard_iter_fused_norms_noalloc: bad args");
+  const long long pairs = (long long)cdiv((long long)nx * ny, kThreads) * nchunks;
+  SGH_REQUIRE(pairs <= kNoAllocMaxPairs, SGH_ERR_ARG,
+              "sgh_picard_iter_fused_norms_noalloc: %lld workgroups (nx·ny/256 · nchunks) "
+              "exceed the fixed scratch of %d norm pairs", pairs, kNoAllocMaxPairs);
+  double* scr = nullptr;
+  const int rc = scratch_for_current_device(STREAM(stream), &scr);
+  if (rc) return rc;
+  return sgh_picard_iter_fused_norms(
+      dtype, nx, ny, nz, aK, KK, a_old, wt, St, wb, Sb, Q, omega, wz, nchunks, zc_parts, a_new,
+      scr, reinterpret_cast<unsigned int*>(scr + 2 * kNoAllocMaxPairs), norms, host_out, seq,
+      stream);
 }
 
 int sgh_reduce_norms(const double* partials, int n_pairs, double n_elements, double* out,

@@ -69,3 +69,19 @@ def test_axis_contract_refuses_more_output_tiles_than_2_31(outer, n, inner):
                                      0.5, ctypes.c_void_p(1024), None)
     assert rc == -1
     assert b"output tiles" in h.sgh_last_error()
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"),
+                    reason="argument checks only, run where no GPU could be launched on")
+def test_noalloc_pass_refuses_more_pairs_than_its_fixed_scratch():
+    # refused on the host before the device is touched: 256×256 surface (256 workgroups per
+    # z chunk) × 257 chunks exceeds the 65536 norm pairs of the library's fixed array
+    from spectral_galerkin_heat_amd import _lib
+    h = _lib.lib.load()
+    p = ctypes.c_void_p(256)
+    rc = h.sgh_picard_iter_fused_norms_noalloc(
+        _lib.F64, 256, 256, 300, p, p, p, p, p, None, None, None, 0.1, None, 257, None,
+        ctypes.c_void_p(512), ctypes.cast(1024, ctypes.POINTER(ctypes.c_double)), None, 0.0,
+        None)
+    assert rc == -1
+    assert b"fixed scratch" in h.sgh_last_error()
