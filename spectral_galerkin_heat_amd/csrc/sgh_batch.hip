@@ -23,104 +23,14 @@
 #include <cmath>
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
-#include "sgh_common.h"
-#include "sgh_mfma.h"
+#include "sgh_batch.h"
 
 static_assert(sizeof(sgh_cuboid_spec) == 176, "sgh_cuboid_spec layout (mirrored in _lib.py)");
 
 namespace sgh {
-
-constexpr int kBT = 256;        // threads per block
-constexpr int kRowsMin = 16;    // fewest surface rows per transform item (one MFMA row tile)
-constexpr int kBMaxN = 256;     // widest surface axis
-constexpr int kStatusStride = 4;  // n_it, converged, rel, AA of the current step
-constexpr int kMetricStride = 5;  // P_laser, T_max, n_it, converged, rel
-constexpr int kSurfArrays = 10;
-constexpr int kStageAhead = 4;   // global loads in flight per thread while staging into LDS
-constexpr int kSurfLds = 150 * 1024;  // default LDS bytes per surface workgroup
-constexpr int kLdsMax = 159 * 1024;   // dynamic LDS cap: gfx950's 160 KiB less the static part
-constexpr int kLoopLdsMax = 158 * 1024;  // the same for the persistent step kernel (four passes'
-                                         // static reduction buffers)
-constexpr int kXcds = 8;          // MI355X: workgroup b runs on XCD b mod 8
-constexpr int kCusPerXcd = 32;    // MI355X: 256 CUs
-
-struct CuboidDesc {
-  int nx, ny, nz, S;
-  long long a_off;
-  int s_off;
-  int bx_off, by_off;      // offsets of B̂ (n×n) and B̂ᵀ for x and y in the matrix table
-  int bxT_off, byT_off;
-  int t1_off;              // offset of the 1-D tables (ex, kx2 | ey, ky2 | ez, kz2)
-  int row_item0, nrowblk;  // first per-cuboid transform slot and block count
-  int col_item0, ncolblk;  // first per-cuboid volume slot and block count
-  int max_it;
-  int n_steps;             // laser-table length
-  long long laser_off;     // row offset into the laser table (6 doubles per row)
-  double dx, dy, inv_sqrt_LxLy, Lz;
-  double alpha, rho_cp, dt;
-  double w0, w1, inv_arc;  // √(1/Lz), √(2/Lz), 1/(α·ρc)
-  double omega, tol, n_elem;
-  double f1, f2, T_boil, T_liq;
-  double absorptivity, rx, ry, order, area_factor;
-  int cell_integrated, live;  // live: the slot holds a cuboid
-  int item_rows, ncb;         // surface rows per transform item (16 or 32); column blocks
-};
-
-// Wide surfaces split their transform items by output columns too (ncb blocks of cb_width
-// columns, a multiple of 16): each item still forms all of Y = L[rows,:]·X, but stages and
-// multiplies only its own columns of R, so a single large cuboid spreads over ncb times as
-// many workgroups. Item blk covers row block blk / ncb and column block blk % ncb.
-__host__ __device__ inline int cb_width(const CuboidDesc& d) {
-  const int nb = d.ncb > 1 ? d.ncb : 1;
-  return ((d.nx + nb - 1) / nb + 15) & ~15;
-}
-__host__ __device__ inline int n_items(const CuboidDesc& d) { return d.nrowblk * d.ncb; }
-
-struct Item {
-  int c, blk;
-};
-
-// A surface work item with a copy of its cuboid's descriptor, so that a surface
-// workgroup finds everything it needs in one load (no item -> descriptor chain).
-struct ItemDesc {
-  CuboidDesc d;
-  int c, blk;
-};
-
-template <typename F>
-struct BatchPtrs {
-  F* a;            // packed modal state
-  F* G;            // surface forcing estimate (modal)
-  F* ZE;           // Σ w·aE (modal)
-  F* U;            // Σ aE·KK·w (modal)
-  F* Ztt;          // Σ w²·KK
-  F* Wtt;          // Σ w²·KK²
-  F* q_las;        // laser flux (physical)
-  F* q_net;        // net top flux (physical)
-  F* q_evap;       // evaporation flux of the current iterate (physical)
-  F* q_evap_old;   // previous step's evaporation flux
-  F* Xinv;         // ZE + G∘Ztt: modal input of the inverse surface transform
-  const F* mats;   // B̂ and B̂ᵀ matrices
-  const double* t1;        // 1-D tables
-  const double* laser;     // laser rows
-  const CuboidDesc* desc;
-  const Item* row_items;
-  const ItemDesc* row_descs;  // row_items[i] with its cuboid's descriptor
-  const Item* col_items;
-  double* row_slots;       // per row item: rr, gu, g2w  (3 doubles)
-  double* col_slots;       // per col item: AA
-  double* aux_slots;       // per row item: P_laser, T_max (2 doubles)
-  int* done_at;            // per cuboid: iteration at which the step finished, −1 active
-  double* status;          // per cuboid: n_it, converged, rel
-  int* step_idx;           // per cuboid: step counter
-  double* metrics;         // [step][cuboid][5]
-  unsigned* bar;           // per cuboid: barrier arrivals of the persistent step kernel
-  int* loop_err;           // set when a barrier of the persistent step kernel timed out
-  int metrics_rows;        // capacity in steps
-  int n_cuboids;
-};
 
 __device__ __forceinline__ double dexp_(double x) { return exp(x); }
 
@@ -172,15 +82,21 @@ __global__ void __launch_bounds__(kBT) k_b_setup(BatchPtrs<F> P) {
 // ---------------------------------------------------------------------------------
 // volume pass: finalize step n (a ← aE + KK·w·G) and gather ZE, U, AA of step n+1
 // ---------------------------------------------------------------------------------
+// mode 0: gather only (priming a new cuboid); 1: finalize the step of a cuboid whose step
+// counter has not advanced yet (per-pass launches); 2: finalize the cuboids the resident
+// step kernel ran in this step (done_at < INT_MAX; it has advanced the counter itself).
 template <typename F>
-__global__ void __launch_bounds__(kBT) k_b_volume(BatchPtrs<F> P, int finalize) {
+__global__ void __launch_bounds__(kBT) k_b_volume(BatchPtrs<F> P, int mode) {
   constexpr int PC = 8;  // z-planes loaded ahead per column
   __shared__ double sh[kBT / kWave];
   const Item it = P.col_items[blockIdx.x];
   if (it.c < 0) return;
   const CuboidDesc d = P.desc[it.c];
   const int s = it.blk * kBT + threadIdx.x;
-  const bool active_cuboid = P.step_idx[it.c] < d.n_steps || !finalize;
+  const int finalize = mode != 0;
+  const bool active_cuboid = mode == 0 ? true
+                             : mode == 1 ? P.step_idx[it.c] < d.n_steps
+                                         : P.done_at[it.c] != 0x7fffffff;
   double aa = 0.0;
   if (s < d.S && active_cuboid) {
     const int n = s / d.nx, m = s - n * d.nx;
@@ -614,6 +530,17 @@ struct Plan {
   std::vector<int> size_refs;
   Arena arena_m;
   int max_nz = 0;    // deepest cuboid the 1-D table slots hold (t1_stride)
+  // split-bf16 tables of the resident step kernel (mode counts up to kResMaxN), one block
+  // of four pad32(n)² matrices per entry of `sizes` (−1: none)
+  std::vector<long long> size_off16;
+  Arena arena_m16;
+  void* mats16 = nullptr;
+  long long mats16_cap = 0;
+  bool resident = false;     // variant rows = −1 requested
+  bool res_on = false;       // ... and every live cuboid allows it (mixed precision, ≤ 128 wide)
+  std::vector<int> hres;     // resident work list: cuboid per workgroup (−1 idle)
+  int* res_list = nullptr;
+  long long res_cap = 0;
   int t1_stride = 0;
   size_t shm = 0;
   int bud = 0;
@@ -701,8 +628,12 @@ struct SurfVariant {
   int rows, threads;
 };
 // rows = 0: each cuboid's items hold 16 or 32 rows (assign_heights), on the 32-row kernels.
+// rows = −1: the resident step kernel (one workgroup of 1024 threads per cuboid), mixed
+// precision and surfaces up to 128 × 128 only.
+constexpr int kResidentRows = -1;
 constexpr SurfVariant kSurfVariants[] = {{16, 256}, {16, 512}, {16, 1024}, {32, 512},
-                                         {32, 1024}, {0, 512}, {0, 1024}};
+                                         {32, 1024}, {0, 512}, {0, 1024},
+                                         {kResidentRows, kResThreads}};
 constexpr int kNumSurfVariants = sizeof(kSurfVariants) / sizeof(kSurfVariants[0]);
 
 template <typename F>
@@ -790,8 +721,8 @@ static hipError_t allow_large_lds(size_t bytes) {
   hipError_t e = hipSuccess;
   for (const SurfVariant& v : kSurfVariants) {
     for (int cs = 0; cs < 2; ++cs) {
-      SurfFn<F> f[5];
-      surf_lookup<F>(v.rows, v.threads, cs == 1, f);
+      SurfFn<F> f[5] = {};
+      if (!surf_lookup<F>(v.rows, v.threads, cs == 1, f)) continue;
       f[4] = reinterpret_cast<SurfFn<F>>(loop_lookup<F>(v.rows, v.threads, cs == 1));
       for (int m = 0; m < 5; ++m) {
         if (!f[m]) continue;
@@ -818,6 +749,30 @@ int enqueue_step(Plan* p, hipStream_t st) {
   }
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
+  if constexpr (std::is_same<F, float>::value) {
+    if (p->res_on) {
+      // the whole surface part of the step in one launch, then the volume pass
+      if (tm) hipEventRecord(p->ev[0], st);
+      chk(launch_resident(P, p->mats16, p->res_list, (unsigned)p->hres.size(), st));
+      if (tm) hipEventRecord(p->ev[1], st);
+      if (tm) hipEventRecord(p->ev[2], st);
+      hipLaunchKernelGGL(k_b_volume<F>, dim3(ncol), dim3(kBT), 0, st, P, 2);
+      if (tm) hipEventRecord(p->ev[3], st);
+      SGH_CHECK_LAUNCH("sgh_batch resident step");
+      if (tm) {
+        e = hipEventSynchronize(p->ev[3]);
+        if (e != hipSuccess) return fail_hip(e, "sgh_batch timing");
+        float a = 0.f, b = 0.f;
+        hipEventElapsedTime(&a, p->ev[0], p->ev[1]);
+        hipEventElapsedTime(&b, p->ev[2], p->ev[3]);
+        p->t_surface_ms += a;
+        p->n_surface += 1;
+        p->t_volume_ms += b;
+        p->n_volume += 1;
+      }
+      return SGH_OK;
+    }
+  }
   hipLaunchKernelGGL(k_b_prep<F>, dim3(nrow), dim3(kBT), 0, st, P);
   if (p->loop_on) {
     // the whole step's surface work in one launch (timed as a whole)
@@ -928,7 +883,7 @@ static bool assign_heights(Plan* p) {
 static void build_xcd_items(Plan* p) {
   std::vector<double> load(kXcds, 0.0);
   std::vector<int> count(kXcds, 0);
-  std::vector<std::vector<Item>> rb(kXcds), cb(kXcds);
+  std::vector<std::vector<Item>> rb(kXcds), cb(kXcds), res(kXcds);
   std::vector<int> order;
   for (int c = 0; c < p->n; ++c)
     if (p->hdesc[c].live) order.push_back(c);
@@ -952,6 +907,7 @@ static void build_xcd_items(Plan* p) {
     const CuboidDesc& d = p->hdesc[c];
     for (int b = 0; b < n_items(d); ++b) rb[x].push_back({c, b});
     for (int b = 0; b < d.ncolblk; ++b) cb[x].push_back({c, b});
+    res[x].push_back({c, 0});
   }
   auto interleave = [](std::vector<std::vector<Item>>& bins, std::vector<Item>& out) {
     size_t m = 0;
@@ -962,6 +918,10 @@ static void build_xcd_items(Plan* p) {
   };
   interleave(rb, p->hrow);
   interleave(cb, p->hcol);
+  std::vector<Item> rl;
+  interleave(res, rl);
+  p->hres.clear();
+  for (const Item& it : rl) p->hres.push_back(it.c);
   p->max_it = 0;
   for (int c : order) p->max_it = std::max(p->max_it, p->hdesc[c].max_it);
 }
@@ -981,7 +941,7 @@ static void free_plan(Plan* p) {
   void* bufs[] = {p->a, p->surf, p->mats, p->t1, p->laser, p->desc, p->row_items, p->row_descs,
                   p->col_items,
                   p->prime_items, p->row_slots, p->col_slots, p->aux_slots, p->done_at,
-                  p->status, p->step_idx, p->metrics, p->bar};
+                  p->status, p->step_idx, p->metrics, p->bar, p->mats16, p->res_list};
   for (void* b : bufs)
     if (b) hipFree(b);
   delete p;
@@ -990,6 +950,20 @@ static void free_plan(Plan* p) {
 template <typename T>
 static hipError_t dmalloc(T** ptr, size_t bytes) {
   return hipMalloc((void**)ptr, bytes < 16 ? 16 : bytes);
+}
+
+// float -> bf16 bits, round to nearest even (finite values), and back.
+static inline uint16_t bf16_bits(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+static inline float bf16_value(uint16_t b) {
+  const uint32_t u = (uint32_t)b << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
 }
 
 // Offset of the (B̂, B̂ᵀ) pair for mode count m, uploading it if no live cuboid uses m yet;
@@ -1030,10 +1004,44 @@ static long long mats_offset(Plan* p, int m, int* rc) {
     *rc = fail_hip(e, "sgh_batch matrix upload");
     return -1;
   }
+  long long off16 = -1;
+  if (m <= kResMaxN) {
+    const int np = pad32(m);
+    const long long mm16 = (long long)np * np;
+    off16 = p->arena_m16.alloc(4 * mm16, 64);
+    if (off16 >= 0) {
+      std::vector<uint16_t> t((size_t)(4 * mm16), 0);
+      for (int r = 0; r < m; ++r)
+        for (int c = 0; c < m; ++c) {
+          const double v = B[(size_t)r * m + c];
+          const uint16_t hi = bf16_bits((float)v);
+          const uint16_t lo = bf16_bits((float)(v - (double)bf16_value(hi)));
+          t[(size_t)r * np + c] = hi;                  // B̂
+          t[(size_t)(mm16 + r * np + c)] = lo;
+          t[(size_t)(2 * mm16 + c * np + r)] = hi;     // B̂ᵀ
+          t[(size_t)(3 * mm16 + c * np + r)] = lo;
+        }
+      e = hipMemcpy((uint16_t*)p->mats16 + off16, t.data(), t.size() * 2, hipMemcpyHostToDevice);
+      if (e != hipSuccess) {
+        p->arena_m16.release(off16, 4 * mm16);
+        p->arena_m.release(off, need);
+        *rc = fail_hip(e, "sgh_batch bf16 matrix upload");
+        return -1;
+      }
+    }
+  }
   p->sizes.push_back(m);
   p->size_off.push_back(off);
+  p->size_off16.push_back(off16);
   p->size_refs.push_back(1);
   return off;
+}
+
+// Offset of the split-bf16 tables of mode count m (−1: none); m must be in the table.
+static long long mats16_offset(const Plan* p, int m) {
+  for (size_t i = 0; i < p->sizes.size(); ++i)
+    if (p->sizes[i] == m) return p->size_off16[i];
+  return -1;
 }
 
 // Drop one reference to mode count m's matrices; the last one frees their table room.
@@ -1042,8 +1050,11 @@ static void mats_release(Plan* p, int m) {
     if (p->sizes[i] == m) {
       if (--p->size_refs[i] > 0) return;
       p->arena_m.release(p->size_off[i], 2LL * rup64((long long)m * m));
+      if (p->size_off16[i] >= 0)
+        p->arena_m16.release(p->size_off16[i], 4LL * pad32(m) * pad32(m));
       p->sizes.erase(p->sizes.begin() + i);
       p->size_off.erase(p->size_off.begin() + i);
+      p->size_off16.erase(p->size_off16.begin() + i);
       p->size_refs.erase(p->size_refs.begin() + i);
       return;
     }
@@ -1099,6 +1110,8 @@ static int add_cuboid(Plan* p, const sgh_cuboid_spec& sp, int c, hipStream_t st)
   d.bxT_off = (int)(bx + rup64((long long)sp.nx * sp.nx));
   d.by_off = (int)by;
   d.byT_off = (int)(by + rup64((long long)sp.ny * sp.ny));
+  d.bx16_off = (int)mats16_offset(p, sp.nx);
+  d.by16_off = (int)mats16_offset(p, sp.ny);
   d.t1_off = c * p->t1_stride;
   const double alpha = sp.k / (sp.rho * sp.cp);
   std::vector<double> t1;
@@ -1202,6 +1215,15 @@ static int rebuild_lists(Plan* p, hipStream_t st) {
                             : loop_lookup<float>(p->rows, p->threads, p->colsplit) != nullptr;
   p->loop_on = p->loop_pref && has_loop && (long long)p->hrow.size() <= p->n_cu &&
                p->shm <= (size_t)kLoopLdsMax;
+  // the resident step kernel: mixed precision, every live surface within its tables
+  p->res_on = p->resident && p->dtype == SGH_F32;
+  for (int c = 0; c < p->n && p->res_on; ++c)
+    if (p->hdesc[c].live && (p->hdesc[c].bx16_off < 0 || p->hdesc[c].by16_off < 0))
+      p->res_on = false;
+  if ((long long)p->hres.size() > p->res_cap) {
+    set_error("sgh_batch: resident work list exceeds capacity");
+    return SGH_ERR_STATE;
+  }
   if ((long long)p->hrow.size() > p->row_cap || (long long)p->hcol.size() > p->col_cap) {
     set_error("sgh_batch: work list exceeds capacity");
     return SGH_ERR_STATE;
@@ -1221,6 +1243,9 @@ static int rebuild_lists(Plan* p, hipStream_t st) {
                        hipMemcpyHostToDevice, st);
   if (e == hipSuccess && !p->hcol.empty())
     e = hipMemcpyAsync(p->col_items, p->hcol.data(), p->hcol.size() * sizeof(Item),
+                       hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && !p->hres.empty())
+    e = hipMemcpyAsync(p->res_list, p->hres.data(), p->hres.size() * sizeof(int),
                        hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return fail_hip(e, "sgh_batch work lists");
@@ -1259,6 +1284,9 @@ int sgh_batch_pool_create(int dtype, int max_cuboids, long long state_capacity,
   p->t1_stride = 2 * (kBMaxN + kBMaxN + max_nz);
   p->mats_cap = 2LL * 16 * kBMaxN * kBMaxN;   // room for 16 distinct 256-wide pairs
   p->arena_m.init(p->mats_cap);
+  p->mats16_cap = 4LL * 16 * kResMaxN * kResMaxN;   // 16 distinct sizes up to 128
+  p->arena_m16.init(p->mats16_cap);
+  p->res_cap = (long long)kXcds * max_cuboids;
   p->bud = surface_budget(dtype, kernel_rows(p->rows), kSurfLds);
   p->shm = (size_t)kSurfLds;
   if (p->bud < 0) {
@@ -1290,6 +1318,10 @@ int sgh_batch_pool_create(int dtype, int max_cuboids, long long state_capacity,
   chk(dmalloc(&p->step_idx, (size_t)max_cuboids * sizeof(int)));
   chk(dmalloc(&p->metrics, (size_t)metrics_rows * max_cuboids * kMetricStride * sizeof(double)));
   chk(dmalloc(&p->bar, ((size_t)max_cuboids + 1) * sizeof(unsigned)));
+  chk(dmalloc(&p->mats16, (size_t)p->mats16_cap * 2));
+  chk(dmalloc(&p->res_list, (size_t)p->res_cap * sizeof(int)));
+  if (dtype == SGH_F32)
+    chk(resident_allow_lds());
   if (e == hipSuccess) e = hipMemset(p->bar, 0, ((size_t)max_cuboids + 1) * sizeof(unsigned));
   if (e == hipSuccess) {
     int dev = 0;
@@ -1620,6 +1652,15 @@ int sgh_batch_set_variant(void* plan, int rows, int threads, int lds_bytes, void
               threads);
   SGH_REQUIRE(lds_bytes >= 4096 && lds_bytes <= kLdsMax, SGH_ERR_ARG,
               "sgh_batch_set_variant: lds_bytes=%d outside [4096, %d]", lds_bytes, kLdsMax);
+  const bool res = rows == kResidentRows;
+  SGH_REQUIRE(!res || p->dtype == SGH_F32, SGH_ERR_ARG,
+              "sgh_batch_set_variant: the resident step kernel runs in mixed precision only");
+  // the resident variant keeps a per-pass launch shape for steps whose cuboids it cannot
+  // hold (a surface wider than 128): 16-row items, 1024 threads, the given LDS budget
+  if (res) {
+    rows = 16;
+    threads = 1024;
+  }
   const int bud = surface_budget(p->dtype, kernel_rows(rows), lds_bytes);
   SGH_REQUIRE(bud >= 0, SGH_ERR_ARG,
               "sgh_batch_set_variant: %d LDS bytes cannot stage a 256-wide operand", lds_bytes);
@@ -1630,6 +1671,7 @@ int sgh_batch_set_variant(void* plan, int rows, int threads, int lds_bytes, void
   p->threads = threads;
   p->bud = bud;
   p->shm = (size_t)lds_bytes;
+  p->resident = res;
   for (int c = 0; c < p->n; ++c) {
     CuboidDesc& d = p->hdesc[c];
     if (!d.live) continue;
@@ -1671,8 +1713,8 @@ int sgh_batch_rewind(void* plan, void* stream) {
 int sgh_batch_get_variant(void* plan, int* out3) {
   Plan* p = static_cast<Plan*>(plan);
   SGH_REQUIRE(p && out3, SGH_ERR_ARG, "sgh_batch_get_variant: bad args");
-  out3[0] = p->rows;
-  out3[1] = p->threads;
+  out3[0] = p->resident ? kResidentRows : p->rows;
+  out3[1] = p->resident ? kResThreads : p->threads;
   out3[2] = (int)p->shm;
   return SGH_OK;
 }

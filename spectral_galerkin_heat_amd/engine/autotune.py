@@ -1,7 +1,8 @@
 """Autotuner for the batched engine's surface transforms.
 
 The batched step is dominated by the surface-transform launches (two per Picard
-iteration). Their launch shape has three knobs: rows of the surface per work item
+iteration), or, in the resident variant (rows = −1), by one launch per step in which a
+workgroup runs a whole cuboid's Picard loop. Their launch shape has three knobs: rows of the surface per work item
 (the MFMA row tiles one workgroup owns: 16 or 32), threads per workgroup (waves per CU:
 256, 512 or 1024) and LDS bytes per workgroup (how much of each operand is staged at
 once; up to 159 KiB of gfx950's 160 KiB per workgroup, and a smaller budget lets two
@@ -103,7 +104,9 @@ def autotune(contexts, precision="mixed", steps=4, warmup=1, reps=2, lds_choices
     table = []
     try:
         for rows, threads in surface_variants():
-            for lds in lds_choices:
+            # the resident step kernel (rows < 0) stages nothing by the LDS budget; the budget
+            # only shapes its per-pass fallback, so one choice is timed
+            for lds in (lds_choices[:1] if rows < 0 else lds_choices):
                 try:
                     eng.set_variant(rows, threads, lds)
                 except _lib.SGHError:

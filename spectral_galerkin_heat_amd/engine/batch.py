@@ -176,7 +176,10 @@ class BatchEngine:
     # -- surface-transform variant -----------------------------------------------------
     def set_variant(self, rows, threads, lds_bytes=150 * 1024):
         """Rows per transform item, threads per workgroup and LDS bytes per workgroup of
-        the batched surface transforms (takes effect at the next ``run``)."""
+        the batched surface transforms (takes effect at the next ``run``). ``rows = -1``
+        selects the resident step kernel (one workgroup per cuboid runs its whole Picard
+        loop; mixed precision, surfaces up to 128 × 128, otherwise the per-pass launches of
+        16-row items with the given LDS budget run)."""
         call("sgh_batch_set_variant", self._plan, int(rows), int(threads), int(lds_bytes),
              self.stream.cuda_stream)
 
