@@ -96,6 +96,9 @@ struct BatchPtrs {
   double* metrics;         // [step][cuboid][5]
   unsigned* bar;           // per cuboid: barrier arrivals of the persistent step kernel
   int* loop_err;           // set when a barrier of the persistent step kernel timed out
+  long long surf_stride;   // elements from one surface array to the next (G, ZE, U, Ztt, …)
+  float* zwu;              // FP32 pools: (ZE, Ztt, Wtt, U) of each surface element, packed
+                           // for the resident step kernel; null otherwise
   int metrics_rows;        // capacity in steps
   int n_cuboids;
 };

@@ -77,6 +77,10 @@ __global__ void __launch_bounds__(kBT) k_b_setup(BatchPtrs<F> P) {
   }
   P.Ztt[d.s_off + s] = (F)z;
   P.Wtt[d.s_off + s] = (F)w2;
+  if (P.zwu) {
+    P.zwu[4 * (d.s_off + s) + 1] = (float)z;
+    P.zwu[4 * (d.s_off + s) + 2] = (float)w2;
+  }
 }
 
 // ---------------------------------------------------------------------------------
@@ -143,6 +147,10 @@ __global__ void __launch_bounds__(kBT) k_b_volume(BatchPtrs<F> P, int mode) {
     }
     P.ZE[d.s_off + s] = (F)ze;
     P.U[d.s_off + s] = (F)u;
+    if (P.zwu) {
+      P.zwu[4 * (d.s_off + s)] = (float)ze;
+      P.zwu[4 * (d.s_off + s) + 3] = (float)u;
+    }
     if (finalize) P.q_evap_old[d.s_off + s] = P.q_evap[d.s_off + s];
   }
   const double t = block_sum(aa, sh);
@@ -535,6 +543,7 @@ struct Plan {
   std::vector<long long> size_off16;
   Arena arena_m16;
   void* mats16 = nullptr;
+  void* zwu = nullptr;       // FP32: (ZE, Ztt, Wtt, U) per surface element, 16 bytes
   long long mats16_cap = 0;
   bool resident = false;     // variant rows = −1 requested
   bool res_on = false;       // ... and every live cuboid allows it (mixed precision, ≤ 128 wide)
@@ -601,6 +610,8 @@ BatchPtrs<F> ptrs(Plan* p) {
   P.q_evap = s + 7 * T;
   P.q_evap_old = s + 8 * T;
   P.Xinv = s + 9 * T;
+  P.surf_stride = T;
+  P.zwu = (float*)p->zwu;
   P.mats = (const F*)p->mats;
   P.t1 = p->t1;
   P.laser = p->laser;
@@ -941,7 +952,8 @@ static void free_plan(Plan* p) {
   void* bufs[] = {p->a, p->surf, p->mats, p->t1, p->laser, p->desc, p->row_items, p->row_descs,
                   p->col_items,
                   p->prime_items, p->row_slots, p->col_slots, p->aux_slots, p->done_at,
-                  p->status, p->step_idx, p->metrics, p->bar, p->mats16, p->res_list};
+                  p->status, p->step_idx, p->metrics, p->bar, p->mats16, p->res_list,
+                  p->zwu};
   for (void* b : bufs)
     if (b) hipFree(b);
   delete p;
@@ -1215,8 +1227,10 @@ static int rebuild_lists(Plan* p, hipStream_t st) {
                             : loop_lookup<float>(p->rows, p->threads, p->colsplit) != nullptr;
   p->loop_on = p->loop_pref && has_loop && (long long)p->hrow.size() <= p->n_cu &&
                p->shm <= (size_t)kLoopLdsMax;
-  // the resident step kernel: mixed precision, every live surface within its tables
-  p->res_on = p->resident && p->dtype == SGH_F32;
+  // the resident step kernel: mixed precision, every live surface within its tables, the
+  // surface arrays addressable with 31-bit byte offsets
+  p->res_on = p->resident && p->dtype == SGH_F32 &&
+              (long long)kSurfArrays * p->total_S * 4 < 0x7fffffffLL;
   for (int c = 0; c < p->n && p->res_on; ++c)
     if (p->hdesc[c].live && (p->hdesc[c].bx16_off < 0 || p->hdesc[c].by16_off < 0))
       p->res_on = false;
@@ -1320,8 +1334,10 @@ int sgh_batch_pool_create(int dtype, int max_cuboids, long long state_capacity,
   chk(dmalloc(&p->bar, ((size_t)max_cuboids + 1) * sizeof(unsigned)));
   chk(dmalloc(&p->mats16, (size_t)p->mats16_cap * 2));
   chk(dmalloc(&p->res_list, (size_t)p->res_cap * sizeof(int)));
-  if (dtype == SGH_F32)
+  if (dtype == SGH_F32) {
+    chk(dmalloc(&p->zwu, (size_t)p->total_S * 4 * sizeof(float)));
     chk(resident_allow_lds());
+  }
   if (e == hipSuccess) e = hipMemset(p->bar, 0, ((size_t)max_cuboids + 1) * sizeof(unsigned));
   if (e == hipSuccess) {
     int dev = 0;

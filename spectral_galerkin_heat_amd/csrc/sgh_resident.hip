@@ -15,21 +15,24 @@ namespace sgh {
 // step is this launch plus the volume pass instead of 2·cap + 4 launches, and a cuboid's
 // arithmetic depends on nothing but its own data. FP32 state and fields; the 2-D
 // transforms are four axis contractions per iteration, each on the matrix cores as three
-// bf16 products (hi·hi + hi·lo + lo·hi of x = hi + lo, FP32 accumulation), with the
-// cosine matrix (split bf16, read from global memory, L2-resident) as the A operand and
-// the field (split bf16, in LDS as [column][k]) as the B operand:
-//   inverse  P = B̂yᵀ·Xinv  (rows j, cols m)      Tᵀ = B̂xᵀ·Pᵀ (rows i, cols j)
-//   forward  Rᵀ = B̂x·qᵀ    (rows m, cols j)      S = B̂y·R    (rows n, cols m)
-// Every extent is padded to a multiple of 32 (the MFMA k step) with zero matrix entries and
-// zero field entries. Two LDS operand buffers alternate: a stage reads one and writes its
-// result (or the field its epilogue forms) into the other, one 16×16 tile at a time, so a
-// stage needs one barrier and a wave holds one tile's accumulator. The per-element fields
-// (ZE, Ztt, Wtt, U, q_las) are read from global memory (L2), the next tile's while the
-// current one multiplies; G lives in registers for the step. Surfaces up to 128 × 128;
-// FP64 and wider surfaces use the per-pass launches.
+// bf16 products (hi·hi + hi·lo + lo·hi of x = hi + lo, FP32 accumulation). Every stage
+// computes O[a][t] = Σ_k D[a][k]·W[k][t]: the field D (split bf16, in LDS as [a][k]) is the
+// A operand, the cosine matrix W (split bf16, read from global memory, L2-resident, held in
+// registers for the stage) the B operand, and the accumulator's four consecutive a of one t
+// per lane are stored as [t][a], the layout the next stage contracts over a in:
+//   inverse  V[m][j] = Σ_n X[m][n]·B̂y[n][j]     T[j][i] = Σ_m V[j][m]·B̂x[m][i]
+//   forward  R[i][n] = Σ_j q[i][j]·B̂y[n][j]     S[n][m] = Σ_i R[n][i]·B̂x[m][i]
+// (X stored [m][n] and q stored [i][j] in LDS.) The second stage of each transform has t
+// along x, so a wave's 16 consecutive lanes hold 16 consecutive x of the surface fields its
+// epilogue reads and writes in global memory ([y][x] layout). Every extent is padded
+// to a multiple of 32 (the MFMA k step) with zero matrix entries and zero field entries.
+// Two LDS operand buffers alternate: a stage reads one and writes its result (or the field
+// its epilogue forms) into the other, one 16×16 tile at a time, so a stage needs one
+// barrier. The per-element fields (ZE, Ztt, Wtt, U, G, q_las) are read from global memory
+// (L2) a tile ahead of their use. Surfaces up to 128 × 128; FP64 and wider surfaces use the
+// per-pass launches.
 // ---------------------------------------------------------------------------------
 constexpr int kResWaves = kResThreads / kWave;
-constexpr int kResTPW = 8;               // most 16×16 output tiles of one wave in a stage
 constexpr int kResBuf = kResMaxN * kResLd;   // bf16 elements of one hi or lo operand buffer
 
 typedef __bf16 bf4v __attribute__((ext_vector_type(4)));
@@ -54,20 +57,25 @@ __device__ __forceinline__ int opaque_lane(int v) {
   return v;
 }
 
-// Output tiles of one wave in a stage of RT × CT 16×16 tiles: the waves are grouped per row
-// tile (g waves each), so a wave's tiles share their A rows; wave w takes row tile w / g and
-// column tiles (w mod g) + g·q, q < nct. rt < 0: no tiles.
+// Tiles of one wave in a stage producing O[a][t] as NT × NA 16×16 tiles (t: the index the
+// stage's matrix introduces, a: the field's free index). The waves are grouped per t tile
+// (g waves each), so a wave's tiles share the matrix fragments it holds; wave w takes t tile
+// w / g and a tiles (w mod g) + g·q, q < nq. tt < 0: no tiles.
 struct ResTiles {
-  int rt, ct0, g, nct;
-  __device__ ResTiles(int RT, int CT, int wave) {
-    g = kResWaves / RT > 0 ? kResWaves / RT : 1;
-    rt = wave / g;
-    ct0 = wave % g;
-    nct = rt < RT && ct0 < CT ? (CT - ct0 + g - 1) / g : 0;
-    if (nct == 0) rt = -1;
+  int tt, a0, g, nq;
+  __device__ ResTiles(int NT, int NA, int wave) {
+    g = kResWaves / NT > 0 ? kResWaves / NT : 1;
+    tt = wave / g;
+    a0 = wave % g;
+    nq = tt < NT && a0 < NA ? (NA - a0 + g - 1) / g : 0;
+    if (nq == 0) tt = -1;
   }
-  __device__ int col(int q, int lane) const { return (ct0 + g * q) * 16 + (lane & 15); }
-  __device__ int row(int lane, int v) const { return rt * 16 + 4 * (lane >> 4) + v; }
+  // row a of the field whose k values lane `lane` reads for tile q
+  __device__ int arow(int q, int lane) const { return (a0 + g * q) * 16 + (lane & 15); }
+  // first of the four consecutive a the accumulator of tile q holds in lane `lane`
+  __device__ int aout(int q, int lane) const { return (a0 + g * q) * 16 + 4 * (lane >> 4); }
+  // t of lane `lane`'s outputs
+  __device__ int tcol(int lane) const { return tt * 16 + (lane & 15); }
 };
 
 // One operand buffer pair (hi, lo) in LDS: bf16 [column][k] with row stride kResLd.
@@ -76,10 +84,9 @@ struct ResBuf {
   __bf16* lo;
 };
 
-// Field values of one lane's 4 output rows (r..r+3 of column c) into an operand buffer,
-// split into bf16 hi / lo: transposed ([c][r], the next stage contracts over r) or
-// straight ([r][c], it contracts over c).
-__device__ __forceinline__ void res_store_t(const ResBuf& b, int r, int c, const float (&x)[4]) {
+// One lane's four outputs O[a..a+3][t] into an operand buffer as [t][a] (one 8-byte store
+// each for hi and lo), split into bf16 hi / lo.
+__device__ __forceinline__ void res_store(const ResBuf& b, int a, int t, const float (&x)[4]) {
   bf4v h, l;
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
@@ -87,83 +94,106 @@ __device__ __forceinline__ void res_store_t(const ResBuf& b, int r, int c, const
     h[v] = hv;
     l[v] = (__bf16)(x[v] - (float)hv);
   }
-  *reinterpret_cast<bf4v*>(b.hi + c * kResLd + r) = h;
-  *reinterpret_cast<bf4v*>(b.lo + c * kResLd + r) = l;
-}
-__device__ __forceinline__ void res_store_n(const ResBuf& b, int r, int c, const float (&x)[4]) {
-#pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const __bf16 hv = (__bf16)x[v];
-    b.hi[(r + v) * kResLd + c] = hv;
-    b.lo[(r + v) * kResLd + c] = (__bf16)(x[v] - (float)hv);
-  }
+  *reinterpret_cast<bf4v*>(b.hi + t * kResLd + a) = h;
+  *reinterpret_cast<bf4v*>(b.lo + t * kResLd + a) = l;
 }
 
-// A-operand fragments of one wave for a stage: its row tile's rows, every k step (hi, lo).
-struct ResA {
+// Matrix fragments of one wave for a stage: W[k][t] of its t tile, every k step (hi, lo).
+struct ResW {
   bf8v h[4], l[4];
 };
 
-// Load the A fragments of the wave's row tile in tiles t (no-op without tiles): hi / lo
-// row-major matrices at m16 + off_h / off_l in global memory, row stride lda, KS k steps of
+// Load the matrix fragments of the wave's t tile (no-op without tiles) from Wᵀ[t][k]: hi / lo
+// row-major matrices at m16 + off_h / off_l in global memory, row stride ld, KS k steps of
 // 32 (the offsets, not the pointers, are made opaque, so the loads stay global loads).
-__device__ __forceinline__ void res_load_a(ResA& a, const ResTiles& t, const __bf16* m16,
-                                           long long off_h, long long off_l, int lda, int KS,
+__device__ __forceinline__ void res_load_w(ResW& w, const ResTiles& t, const __bf16* m16,
+                                           long long off_h, long long off_l, int ld, int KS,
                                            int lane) {
-  if (t.rt < 0) return;
-  const __bf16* Ah = m16 + opaque(off_h);
-  const __bf16* Al = m16 + opaque(off_l);
+  if (t.tt < 0) return;
+  const __bf16* Wh = m16 + opaque(off_h);
+  const __bf16* Wl = m16 + opaque(off_l);
   lane = opaque_lane(lane);
-  const long long arow = (long long)(t.rt * 16 + (lane & 15)) * opaque(lda) + 8 * (lane >> 4);
+  const long long row = (long long)t.tcol(lane) * opaque(ld) + 8 * (lane >> 4);
 #pragma unroll
-  for (int ks = 0; ks < 4; ++ks)
-    if (ks < KS) {
-      a.h[ks] = *reinterpret_cast<const bf8v*>(Ah + arow + 32 * ks);
-      a.l[ks] = *reinterpret_cast<const bf8v*>(Al + arow + 32 * ks);
-      // pinned in registers for the stage: without this the register allocator reloads
-      // them from memory for every tile (and waits for each reload)
-      asm volatile("" : "+v"(a.h[ks]), "+v"(a.l[ks]));
-    }
+  for (int ks = 0; ks < 4; ++ks) {
+    // k steps beyond KS re-read the last one (unused; unconditional loads keep the wait
+    // counts exact)
+    const int kk = ks < KS ? ks : KS - 1;
+    w.h[ks] = *reinterpret_cast<const bf8v*>(Wh + row + 32 * kk);
+    w.l[ks] = *reinterpret_cast<const bf8v*>(Wl + row + 32 * kk);
+    // pinned in registers for the stage: without this the register allocator reloads
+    // them from memory for every tile (and waits for each reload)
+    asm volatile("" : "+v"(w.h[ks]), "+v"(w.l[ks]));
+  }
 }
 
-// One stage: for each of the wave's tiles, acc = Σ_k A[r][k]·B[k][c] over K = 32·KS (three
-// bf16 products, each in an accumulator of its own), then epi(q, acc, lane).
-// A: the fragments in `a`, loaded beforehand; after the last tile `next()` reloads them
-// for the following stage, so that their latency passes under the barrier. B: the source buffer. pre(q, lane) starts the loads tile q's epilogue needs;
-// it is called two tiles ahead. Ends with the barrier after which the epilogue's stores
-// are visible.
-template <typename Next, typename Pre, typename Epi>
-__device__ __forceinline__ void res_stage(const ResTiles& t, ResA& a, int KS, const ResBuf& src,
-                                          int lane, Next&& next, Pre&& pre, Epi&& epi) {
-  if (t.rt >= 0) {
-    lane = opaque_lane(lane);
-    pre(0, lane);
-    pre(1, lane);
+// One stage with KS k steps (K = 32·KS): for each of the wave's tiles q < nq,
+// acc = Σ_k D[a][k]·W[k][t] as three bf16 products into one accumulator, then
+// epi(q, slot, acc, lane). W: the fragments in `w`, loaded beforehand. D: the source buffer,
+// read AH k steps ahead of the products that use it through a ring of R fragment pairs (the
+// k steps of all tiles form one sequence; beyond the wave's last tile the reads repeat that
+// tile, so every read is unconditional and in bounds). The loop is rolled two tiles at a
+// time (the code stays small enough for the instruction cache; ring slots, the two
+// accumulators and the two epilogue slots are fixed per half). The epilogue of tile q runs
+// after tile q + 1's products are issued, so its arithmetic overlaps them; pre(q, slot, lane)
+// starts the loads that epilogue needs once the epilogue of tile q − 2 has used the slot.
+template <int KS, typename Pre, typename Epi>
+__device__ __forceinline__ void res_tiles(const ResTiles& t, const ResW& w, const ResBuf& src,
+                                          int lane, Pre&& pre, Epi&& epi) {
+  constexpr int R = KS == 1 ? 2 : (KS == 3 ? 3 : 4);
+  constexpr int AH = R - 1;
+  static_assert((2 * KS) % R == 0, "ring slots repeat every two tiles");
+  const int nq = t.nq;
+  bf8v dh[R], dl[R];
+  // (lane is laundered per half: addresses are formed where they are used instead of being
+  // hoisted out of the loop and kept live)
+  auto dload = [&](int q, int ks, int slot, int ln) __attribute__((always_inline)) {
+    const int qq = q < nq ? q : nq - 1;
+    const int off = t.arow(qq, ln) * kResLd + 8 * (ln >> 4) + 32 * ks;
+    dh[slot] = *reinterpret_cast<const bf8v*>(src.hi + off);
+    dl[slot] = *reinterpret_cast<const bf8v*>(src.lo + off);
+  };
+  f4v acc[2];
+  auto half = [&](int q, int par) __attribute__((always_inline)) {
+    const int ln = opaque_lane(lane);
+    if (q < nq) {
+      f4v& c = acc[par];
+      c = f4v{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int q = 0; q < kResTPW; ++q) {
-      if (q >= t.nct) break;
-      pre(q + 2, lane);
-      const int boff = t.col(q, lane) * kResLd + 8 * (lane >> 4);
-      // the three products accumulate separately, so consecutive MFMAs are independent
-      f4v ahh = f4v{0.f, 0.f, 0.f, 0.f}, ahl = ahh, alh = ahh;
-      // B fragments one k step ahead of the products that use them
-      bf8v bh = *reinterpret_cast<const bf8v*>(src.hi + boff);
-      bf8v bl = *reinterpret_cast<const bf8v*>(src.lo + boff);
-#pragma unroll
-      for (int ks = 0; ks < 4; ++ks) {
-        if (ks >= KS) break;
-        bf8v nh = bh, nl = bl;
-        if (ks + 1 < KS) {
-          nh = *reinterpret_cast<const bf8v*>(src.hi + boff + 32 * (ks + 1));
-          nl = *reinterpret_cast<const bf8v*>(src.lo + boff + 32 * (ks + 1));
-        }
-        ahh = mma_bf16(a.h[ks], bh, ahh);
-        ahl = mma_bf16(a.h[ks], bl, ahl);
-        alh = mma_bf16(a.l[ks], bh, alh);
-        bh = nh;
-        bl = nl;
+      for (int ks = 0; ks < KS; ++ks) {
+        const int s = par * KS + ks;   // k step within the two-tile period
+        dload(q + (ks + AH) / KS, (ks + AH) % KS, (s + AH) % R, ln);
+        c = mma_bf16(dh[s % R], w.h[ks], c);
+        c = mma_bf16(dh[s % R], w.l[ks], c);
+        c = mma_bf16(dl[s % R], w.h[ks], c);
       }
-      epi(q, ahh + (ahl + alh), lane);
+    }
+    if (q >= 1 && q - 1 < nq) epi(q - 1, par ^ 1, acc[par ^ 1], ln);
+    pre(q + 1, par ^ 1, ln);
+  };
+  pre(0, 0, lane);
+#pragma unroll
+  for (int s = 0; s < AH; ++s) dload(s / KS, s % KS, s % R, lane);
+#pragma unroll 1
+  for (int q0 = 0; q0 <= nq; q0 += 2) {
+    half(q0, 0);
+    half(q0 + 1, 1);
+  }
+}
+
+// One stage: the wave's tiles (res_tiles, specialised on the number of k steps), then
+// `next()`, which reloads the matrix fragments for the following stage so that their latency
+// passes under the barrier that ends the stage (after it the epilogue's stores are visible).
+template <typename Next, typename Pre, typename Epi>
+__device__ __forceinline__ void res_stage(const ResTiles& t, ResW& w, int KS, const ResBuf& src,
+                                          int lane, Next&& next, Pre&& pre, Epi&& epi) {
+  if (t.tt >= 0) {
+    lane = opaque_lane(lane);
+    switch (KS) {
+      case 1: res_tiles<1>(t, w, src, lane, pre, epi); break;
+      case 2: res_tiles<2>(t, w, src, lane, pre, epi); break;
+      case 3: res_tiles<3>(t, w, src, lane, pre, epi); break;
+      default: res_tiles<4>(t, w, src, lane, pre, epi); break;
     }
   }
   next();
@@ -186,24 +216,42 @@ __device__ __forceinline__ double res_sum(double v, double* sh) {
   return t;
 }
 
-// Surface offsets of one lane's 4 elements of tile q (rows of the tile along x when
-// rows_are_x, else along y); −1 outside the surface or beyond the wave's tiles.
-__device__ __forceinline__ void res_offsets(const ResTiles& t, int q, int lane, bool rows_are_x,
-                                            int nx, int ny, int (&off)[4]) {
-  const int c = t.col(q, lane);
+// The cuboid's surface arrays through buffer resources, with 32-bit byte offsets instead of
+// 64-bit addresses: r covers the separate arrays (array k — G 0, q_las 5, q_evap 7 — element
+// e at byte 4·(k·T + e) past the cuboid's first element), rz the packed (ZE, Ztt, Wtt, U)
+// records (element e at byte 16·e). An element outside the surface gets the offset kResOut,
+// beyond either range, where the hardware's range check returns 0 for a load and drops a
+// store.
+constexpr unsigned kResOut = 0x80000000u;
+struct ResFields {
+  __amdgpu_buffer_rsrc_t r, rz;
+  int fs;   // 4·T
+  __device__ float load(int k, unsigned vo) const {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, k * fs, 0));
+  }
+  __device__ void store(int k, unsigned vo, float v) const {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)vo, k * fs, 0);
+  }
+  __device__ f4v record(unsigned vo16) const {
+    return __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(rz, (int)vo16, 0, 0));
+  }
+};
+
+// Surface element indices y·nx + x of one lane's four outputs of tile q in a stage whose t
+// index runs along x and whose a index runs along y (the two stages with epilogues): byte
+// offsets into the separate arrays (vo) and into the packed records (vo16); kResOut outside
+// the surface or beyond the wave's tiles.
+__device__ __forceinline__ void res_voffs(const ResTiles& t, int q, int lane, int nx, int ny,
+                                          unsigned (&vo)[4], unsigned (&vo16)[4]) {
+  const int x = t.tcol(lane), y0 = t.aout(q, lane);
+  const bool in = q < t.nq && x < nx;
+  const unsigned e = (unsigned)(y0 * nx + x);
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
-    const int r = t.row(lane, v);
-    const int x = rows_are_x ? r : c, y = rows_are_x ? c : r;
-    off[v] = (q < t.nct && x < nx && y < ny) ? y * nx + x : -1;
+    const bool ok = in && y0 + v < ny;
+    vo[v] = ok ? 4u * (e + v * nx) : kResOut;
+    vo16[v] = ok ? 16u * (e + v * nx) : kResOut;
   }
-}
-
-// Four values of `a` at the offsets; element 0 stands in where off < 0 (the loads are
-// unconditional, so the wait counts stay exact; the caller ignores those values).
-__device__ __forceinline__ void res_load4(const float* a, const int (&off)[4], float (&o)[4]) {
-#pragma unroll
-  for (int v = 0; v < 4; ++v) o[v] = a[off[v] >= 0 ? off[v] : 0];
 }
 
 __global__ void __launch_bounds__(kResThreads)
@@ -228,20 +276,22 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
   // split-bf16 tables: B̂ hi, B̂ lo, B̂ᵀ hi, B̂ᵀ lo, each np × np
   const long long X16 = d.bx16_off, Y16 = d.by16_off;   // element offsets into m16
   const long long mx = (long long)nxp * nxp, my = (long long)nyp * nyp;
-  // tp: rows i (x), cols j (y): the second inverse stage and the first forward stage;
-  // tm: rows n / j (y), cols m (x modes): the second forward and the first inverse stage
-  const ResTiles tp(nxp / 16, nyp / 16, wave);
-  const ResTiles tm(nyp / 16, nxp / 16, wave);
+  // tA: t along y (n or j), a along x (i or m): the first stage of each transform;
+  // tB: t along x (m or i), a along y (n or j): the second stage, with the epilogues
+  const ResTiles tA(nyp / 16, nxp / 16, wave);
+  const ResTiles tB(nxp / 16, nyp / 16, wave);
   const int so = d.s_off;
+  ResFields fld;
+  fld.r = __builtin_amdgcn_make_buffer_rsrc(P.G + so, (short)0,
+                                            (int)(4 * (kSurfArrays * P.surf_stride - so)),
+                                            0x00020000);   // raw 32-bit words, range-checked
+  fld.fs = (int)(4 * P.surf_stride);
+  fld.rz = __builtin_amdgcn_make_buffer_rsrc(P.zwu + 4LL * so, (short)0,
+                                             (int)(16 * (P.surf_stride - so)), 0x00020000);
   F* const q_las = P.q_las + so;
-  F* const q_evap = P.q_evap + so;
-  const F* const ZE = P.ZE + so;
-  const F* const Ztt = P.Ztt + so;
-  const F* const Wtt = P.Wtt + so;
-  const F* const U = P.U + so;
-  auto no_pre = [](int, int) __attribute__((always_inline)) {};
+  auto no_pre = [](int, int, int) __attribute__((always_inline)) {};
 
-  // ---- step preparation: AA, laser flux, warm-start flux (physical layout, buffer 0) ----
+  // ---- step preparation: AA, laser flux, warm-start flux (buffer 0, as [i][j]) ----
   double aa = 0.0;
   for (int b = tid; b < d.ncolblk; b += kResThreads) aa += P.col_slots[d.col_item0 + b];
   const double AA = res_sum(aa, sh[0]);
@@ -311,8 +361,8 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
           psum += (double)qlf;
         }
         const __bf16 hv = (__bf16)qn;
-        b0.hi[j * kResLd + i] = hv;               // forward operand [j][i]
-        b0.lo[j * kResLd + i] = (__bf16)(qn - (float)hv);
+        b0.hi[i * kResLd + j] = hv;               // forward operand [i][j]
+        b0.lo[i * kResLd + j] = (__bf16)(qn - (float)hv);
       }
     }
   }
@@ -321,68 +371,67 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
   const double fwd_scale = d.dx * d.dy * d.inv_sqrt_LxLy;
   const double inv_scale = d.inv_sqrt_LxLy;
   const F fwd_scale_f = (F)fwd_scale, inv_scale_f = (F)inv_scale;
-  F g_reg[kResTPW][4];
 
-  // A fragments of the four stage kinds; each stage loads the next one's after its last
-  // product
-  ResA A;
-  auto a_fwd1 = [&]() __attribute__((always_inline)) {
-    res_load_a(A, tp, m16, X16, X16 + mx, nxp, KX, lane);
+  // matrix fragments of the four stage kinds; each stage loads the next one's after its
+  // last product
+  ResW W;
+  auto w_fwd1 = [&]() __attribute__((always_inline)) {     // W[j][n] = B̂y[n][j]
+    res_load_w(W, tA, m16, Y16, Y16 + my, nyp, KY, lane);
   };
-  auto a_fwd2 = [&]() __attribute__((always_inline)) {
-    res_load_a(A, tm, m16, Y16, Y16 + my, nyp, KY, lane);
+  auto w_fwd2 = [&]() __attribute__((always_inline)) {     // W[i][m] = B̂x[m][i]
+    res_load_w(W, tB, m16, X16, X16 + mx, nxp, KX, lane);
   };
-  auto a_inv1 = [&]() __attribute__((always_inline)) {
-    res_load_a(A, tm, m16, Y16 + 2 * my, Y16 + 3 * my, nyp, KY, lane);
+  auto w_inv1 = [&]() __attribute__((always_inline)) {     // W[n][j] = B̂y[n][j]
+    res_load_w(W, tA, m16, Y16 + 2 * my, Y16 + 3 * my, nyp, KY, lane);
   };
-  auto a_inv2 = [&]() __attribute__((always_inline)) {
-    res_load_a(A, tp, m16, X16 + 2 * mx, X16 + 3 * mx, nxp, KX, lane);
+  auto w_inv2 = [&]() __attribute__((always_inline)) {     // W[m][i] = B̂x[m][i]
+    res_load_w(W, tB, m16, X16 + 2 * mx, X16 + 3 * mx, nxp, KX, lane);
   };
-  auto a_none = []() __attribute__((always_inline)) {};
+  auto w_none = []() __attribute__((always_inline)) {};
 
   // first stage of each transform: the result goes to the other buffer as the second
   // stage's operand
-  auto stage_fwd1 = [&]() __attribute__((always_inline)) {    // Rᵀ = B̂x·qᵀ (rows m, cols j)
-    res_stage(tp, A, KX, buf0, lane, a_fwd2, no_pre,
-              [&](int q, const f4v& a, int lane) __attribute__((always_inline)) {
+  auto stage_fwd1 = [&]() __attribute__((always_inline)) {    // R[i][n], stored [n][i]
+    res_stage(tA, W, KY, buf0, lane, w_fwd2, no_pre,
+              [&](int q, int, const f4v& a, int lane) __attribute__((always_inline)) {
                 const float o[4] = {a[0], a[1], a[2], a[3]};
-                res_store_n(buf1, tp.row(lane, 0), tp.col(q, lane), o);   // [m][j]
+                res_store(buf1, tA.aout(q, lane), tA.tcol(lane), o);
               });
   };
-  auto stage_inv1 = [&]() __attribute__((always_inline)) {    // P = B̂yᵀ·Xinv (rows j, cols m)
-    res_stage(tm, A, KY, buf0, lane, a_inv2, no_pre,
-              [&](int q, const f4v& a, int lane) __attribute__((always_inline)) {
+  auto stage_inv1 = [&]() __attribute__((always_inline)) {    // V[m][j], stored [j][m]
+    res_stage(tA, W, KY, buf0, lane, w_inv2, no_pre,
+              [&](int q, int, const f4v& a, int lane) __attribute__((always_inline)) {
                 const float o[4] = {a[0], a[1], a[2], a[3]};
-                res_store_n(buf1, tm.row(lane, 0), tm.col(q, lane), o);   // [j][m]
+                res_store(buf1, tA.aout(q, lane), tA.tcol(lane), o);
               });
   };
 
   // ---- warm start: G = fwd·B̂y q_net B̂xᵀ; Xinv = ZE + G∘Ztt into buffer 0 ----
-  a_fwd1();
+  // (outside the surface every epilogue input is 0: the matrices' padding is zero, and
+  // loads beyond the surface return 0, so the operand padding stays 0 without selects)
+  w_fwd1();
   stage_fwd1();
   {
     const int nxk = opaque(nx), nyk = opaque(ny);
-    const F* ZEk = P.ZE + opaque(so);
-    const F* Zttk = P.Ztt + opaque(so);
-    float ze[3][4], zt[3][4];
-    res_stage(tm, A, KY, buf1, lane, a_inv1,
-              [&](int q, int lane) __attribute__((always_inline)) {
-                int off[4];
-                res_offsets(tm, q, lane, false, nxk, nyk, off);
-                res_load4(ZEk, off, ze[q % 3]);
-                res_load4(Zttk, off, zt[q % 3]);
+    f4v zw[2][4];   // (ZE, Ztt, Wtt, U) of the tile's elements
+    res_stage(tB, W, KX, buf1, lane, w_inv1,
+              [&](int q, int sl, int lane) __attribute__((always_inline)) {
+                unsigned vo[4], vo16[4];
+                res_voffs(tB, q, lane, nxk, nyk, vo, vo16);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) zw[sl][v] = fld.record(vo16[v]);
               },
-              [&](int q, const f4v& a, int lane) __attribute__((always_inline)) {
-                int off[4];
-                res_offsets(tm, q, lane, false, nxk, nyk, off);
+              [&](int q, int sl, const f4v& a, int lane) __attribute__((always_inline)) {
+                unsigned vo[4], vo16[4];
+                res_voffs(tB, q, lane, nxk, nyk, vo, vo16);
                 float xo[4];
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                  const F g = off[v] >= 0 ? a[v] * fwd_scale_f : 0.f;
-                  g_reg[q][v] = g;
-                  xo[v] = off[v] >= 0 ? ze[q % 3][v] + g * zt[q % 3][v] : 0.f;
+                  const F g = a[v] * fwd_scale_f;
+                  fld.store(0, vo[v], g);
+                  xo[v] = zw[sl][v][0] + g * zw[sl][v][1];
                 }
-                res_store_t(buf0, tm.row(lane, 0), tm.col(q, lane), xo);   // [m][n]
+                res_store(buf0, tB.aout(q, lane), tB.tcol(lane), xo);   // [m][n]
               });
   }
 
@@ -392,6 +441,11 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
   double rel = __builtin_nan("");
   const F omega = (F)d.omega;
   const F T_liq = (F)d.T_liq, f1 = (F)d.f1, f2 = (F)d.f2, T_boil = (F)d.T_boil;
+  // below T_cut the FP32 evaporation law is exactly 0: exp's argument, in powers of two,
+  // is under −160, past the smallest subnormal, so the transcendentals are skipped there
+  // (f2 > 0; otherwise there is no such temperature)
+  const F T_cut = d.f2 > 0.0 ? (F)(d.T_boil / (1.0 + 160.0 / (d.f2 * M_LOG2E))) : -INFINITY;
+  const F T_hot = T_liq > T_cut ? T_liq : T_cut;
   const int max_it = d.max_it;
   const double tol = d.tol, n_elem = d.n_elem;
 #pragma unroll 1
@@ -400,79 +454,84 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
     stage_inv1();
     {
       const int nxk = opaque(nx), nyk = opaque(ny);
-      const F* qlk = P.q_las + opaque(so);
-      F* qek = P.q_evap + opaque(so);
-      float ql[3][4];
-      res_stage(tp, A, KX, buf1, lane, a_fwd1,
-                [&](int q, int lane) __attribute__((always_inline)) {
-                  int off[4];
-                  res_offsets(tp, q, lane, true, nxk, nyk, off);
-                  res_load4(qlk, off, ql[q % 3]);
+      float ql[2][4];
+      res_stage(tB, W, KX, buf1, lane, w_fwd1,
+                [&](int q, int sl, int lane) __attribute__((always_inline)) {
+                  unsigned vo[4], vo16[4];
+                  res_voffs(tB, q, lane, nxk, nyk, vo, vo16);
+#pragma unroll
+                  for (int v = 0; v < 4; ++v) ql[sl][v] = fld.load(5, vo[v]);
                 },
-                [&](int q, const f4v& a, int lane) __attribute__((always_inline)) {
-                  int off[4];
-                  res_offsets(tp, q, lane, true, nxk, nyk, off);
+                [&](int q, int sl, const f4v& a, int lane) __attribute__((always_inline)) {
+                  unsigned vo[4], vo16[4];
+                  res_voffs(tB, q, lane, nxk, nyk, vo, vo16);
+                  F T[4], qe[4] = {F(0), F(0), F(0), F(0)};
+                  bool ev[4], hot = false;
+#pragma unroll
+                  for (int v = 0; v < 4; ++v) {
+                    T[v] = a[v] * inv_scale_f;
+                    // (the surface's padding, T = 0, never evaporates, even when T_liq ≤ 0)
+                    ev[v] = vo[v] != kResOut && T[v] >= T_hot;
+                    hot = hot || ev[v];
+                  }
+                  // hardware reciprocal, reciprocal square root and exponential (about 1 ulp
+                  // each), only where a lane holds an evaporating element
+                  if (hot) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                      const F rT = __builtin_amdgcn_rcpf(T[v]);
+                      const F e = f1 * (__builtin_amdgcn_rsqf(T[v]) *
+                                        __expf(f2 * (F(1) - T_boil * rT)));
+                      qe[v] = ev[v] ? e : F(0);
+                    }
+                  }
                   float qo[4];
 #pragma unroll
                   for (int v = 0; v < 4; ++v) {
-                    qo[v] = 0.f;
-                    if (off[v] >= 0) {
-                      const F T = a[v] * inv_scale_f;
-                      F qe = F(0);
-                      // hardware reciprocal, reciprocal square root and exponential (about
-                      // 1 ulp each): the evaporation law is the FP32 hot spot of the loop
-                      if (T >= T_liq) {
-                        const F rT = __builtin_amdgcn_rcpf(T);
-                        qe = f1 * (__builtin_amdgcn_rsqf(T) * __expf(f2 * (F(1) - T_boil * rT)));
-                      }
-                      qek[off[v]] = qe;
-                      qo[v] = ql[q % 3][v] - qe;
-                    }
+                    fld.store(7, vo[v], qe[v]);
+                    qo[v] = ql[sl][v] - qe[v];
                   }
-                  res_store_t(buf0, tp.row(lane, 0), tp.col(q, lane), qo);   // [j][i]
+                  res_store(buf0, tB.aout(q, lane), tB.tcol(lane), qo);   // [i][j]
                 });
     }
-    // forward + relaxation + norm partials; Xinv into buffer 0
+    // forward + relaxation + norm partials (FP32 per tile, FP64 over tiles); Xinv into
+    // buffer 0
     stage_fwd1();
     double v0 = 0.0, v1 = 0.0, v2 = 0.0;
     {
       const int nxk = opaque(nx), nyk = opaque(ny);
-      const F* ZEk = P.ZE + opaque(so);
-      const F* Zttk = P.Ztt + opaque(so);
-      const F* Wttk = P.Wtt + opaque(so);
-      const F* Uk = P.U + opaque(so);
-      float ze[3][4], zt[3][4], wt[3][4], uu[3][4];
-      res_stage(tm, A, KY, buf1, lane, a_inv1,
-                [&](int q, int lane) __attribute__((always_inline)) {
-                  int off[4];
-                  res_offsets(tm, q, lane, false, nxk, nyk, off);
-                  res_load4(ZEk, off, ze[q % 3]);
-                  res_load4(Zttk, off, zt[q % 3]);
-                  res_load4(Wttk, off, wt[q % 3]);
-                  res_load4(Uk, off, uu[q % 3]);
-                },
-                [&](int q, const f4v& a, int lane) __attribute__((always_inline)) {
-                  int off[4];
-                  res_offsets(tm, q, lane, false, nxk, nyk, off);
-                  float xo[4];
+      f4v zw[2][4];   // (ZE, Ztt, Wtt, U) of the tile's elements
+      float gg[2][4];
+      res_stage(tB, W, KX, buf1, lane, w_inv1,
+                [&](int q, int sl, int lane) __attribute__((always_inline)) {
+                  unsigned vo[4], vo16[4];
+                  res_voffs(tB, q, lane, nxk, nyk, vo, vo16);
 #pragma unroll
                   for (int v = 0; v < 4; ++v) {
-                    xo[v] = 0.f;
-                    if (off[v] >= 0) {
-                      const F Sr = a[v] * fwd_scale_f;
-                      const F g = g_reg[q][v];
-                      const F dd = Sr - g;
-                      const F w = wt[q % 3][v];
-                      // FP32 products, FP64 sums
-                      v0 += (double)(dd * dd * w);
-                      v1 += (double)(g * uu[q % 3][v]);
-                      v2 += (double)(g * g * w);
-                      const F gn = g + omega * dd;
-                      g_reg[q][v] = gn;
-                      xo[v] = ze[q % 3][v] + gn * zt[q % 3][v];
-                    }
+                    zw[sl][v] = fld.record(vo16[v]);
+                    gg[sl][v] = fld.load(0, vo[v]);
                   }
-                  res_store_t(buf0, tm.row(lane, 0), tm.col(q, lane), xo);   // [m][n]
+                },
+                [&](int q, int sl, const f4v& a, int lane) __attribute__((always_inline)) {
+                  unsigned vo[4], vo16[4];
+                  res_voffs(tB, q, lane, nxk, nyk, vo, vo16);
+                  float xo[4], p0 = 0.f, p1 = 0.f, p2 = 0.f;
+#pragma unroll
+                  for (int v = 0; v < 4; ++v) {
+                    const F g = gg[sl][v];
+                    const F dd = a[v] * fwd_scale_f - g;
+                    const F w = zw[sl][v][2];
+                    p0 += dd * dd * w;
+                    p1 += g * zw[sl][v][3];
+                    p2 += g * g * w;
+                    const F gn = g + omega * dd;
+                    fld.store(0, vo[v], gn);
+                    xo[v] = zw[sl][v][0] + gn * zw[sl][v][1];
+                  }
+                  v0 += (double)p0;
+                  v1 += (double)p1;
+                  v2 += (double)p2;
+                  res_store(buf0, tB.aout(q, lane), tB.tcol(lane), xo);   // [m][n]
                 });
     }
     // the three sums in one exchange, in a fixed order
@@ -498,28 +557,21 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
     if (conv) break;
   }
 
-  // ---- step end: T_s of the final iterate, its maximum; G for the volume pass ----
+  // ---- step end: T_s of the final iterate and its maximum (G is in place for the volume
+  // pass) ----
   stage_inv1();
   double tmax = -1.0 / 0.0;
-  res_stage(tp, A, KX, buf1, lane, a_none, no_pre,
-            [&](int q, const f4v& a, int lane) __attribute__((always_inline)) {
-              int off[4];
-              res_offsets(tp, q, lane, true, nx, ny, off);
+  res_stage(tB, W, KX, buf1, lane, w_none, no_pre,
+            [&](int q, int, const f4v& a, int lane) __attribute__((always_inline)) {
+              unsigned vo[4], vo16[4];
+              res_voffs(tB, q, lane, nx, ny, vo, vo16);
 #pragma unroll
               for (int v = 0; v < 4; ++v)
-                if (off[v] >= 0) {
+                if (vo[v] != kResOut) {
                   const double T = (double)(a[v] * inv_scale_f);
                   tmax = T > tmax ? T : tmax;
                 }
             });
-#pragma unroll
-  for (int q = 0; q < kResTPW; ++q) {
-    int off[4];
-    res_offsets(tm, q, lane, false, nx, ny, off);
-#pragma unroll
-    for (int v = 0; v < 4; ++v)
-      if (off[v] >= 0) P.G[so + off[v]] = g_reg[q][v];
-  }
   tmax = wave_max(tmax);
   if (lane == 0) sh[0][wave] = tmax;
   __syncthreads();

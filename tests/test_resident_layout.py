@@ -1,23 +1,28 @@
 """CPU emulation of the resident step kernel's data movement (csrc/sgh_resident.hip).
 
-The kernel chains four MFMA contractions per Picard iteration through two LDS operand
-buffers, with every extent padded to a multiple of 32. This test replays, in NumPy and
-for every lane of every wave, the same tile assignment, A-fragment addresses (cosine
-tables in global memory), B-fragment addresses (LDS, [column][k] with row stride 136)
-and epilogue stores, and checks for each surface shape that
+Every stage of the kernel computes O[a][t] = Σ_k D[a][k]·W[k][t] on the matrix cores: the
+field D is read from an LDS operand buffer ([a][k], row stride 136 bf16), the cosine matrix
+W from its table in global memory (Wᵀ[t][k], zero-padded to pad32(n)²), and each lane's
+four consecutive a of one t are stored as [t][a] into the other buffer. This test replays,
+in NumPy and for every lane of every wave, the kernel's tile assignment, its ring of
+operand reads (the k steps of all tiles as one sequence, read ahead and clamped to the
+wave's last tile), the matrix fragment reads and the stores, and checks for each surface
+shape that
 
 * every address stays inside its buffer (LDS: 128 × 136 bf16 per buffer; tables:
-  pad32(n)² per matrix; surface fields: nx·ny elements);
-* every LDS operand entry a stage reads was written by the stage before it, and the
-  entries outside the surface hold zero;
-* the chained stages compute the 2-D transforms of the solver: T = B_yᵀ X B_x (inverse)
-  and S = B_y q B_xᵀ (forward), here in float64 with the kernel's layouts.
+  pad32(n)² per matrix);
+* every LDS operand entry a stage reads was written by the stage before it, including the
+  zero padding;
+* the chained stages compute the 2-D transforms of the solver, T = B̂yᵀ X B̂x (inverse) and
+  S = B̂y q B̂xᵀ (forward), here in float64 with the kernel's layouts;
+* the epilogue offsets y·nx + x stay inside the surface.
 """
 
 import numpy as np
 import pytest
 
 WAVE = 64
+WAVES = 8         # kResThreads / 64
 LD = 136          # kResLd
 MAXN = 128        # kResMaxN
 
@@ -26,22 +31,32 @@ def pad32(n):
     return (n + 31) // 32 * 32
 
 
+def ring(KS):
+    """Ring size and read-ahead distance of res_tiles<KS>."""
+    R = {1: 2, 2: 4, 3: 3, 4: 4}[KS]
+    assert (2 * KS) % R == 0
+    return R, R - 1
+
+
 class Tiles:
-    """ResTiles(RT, CT, wave) of the kernel."""
+    """ResTiles(NT, NA, wave) of the kernel."""
 
-    def __init__(self, RT, CT, wave, nwaves):
-        self.g = max(1, nwaves // RT)
-        self.rt = wave // self.g
-        self.ct0 = wave % self.g
-        self.nct = (CT - self.ct0 + self.g - 1) // self.g if (self.rt < RT and self.ct0 < CT) else 0
-        if self.nct == 0:
-            self.rt = -1
+    def __init__(self, NT, NA, wave):
+        self.g = max(1, WAVES // NT)
+        self.tt = wave // self.g
+        self.a0 = wave % self.g
+        self.nq = (NA - self.a0 + self.g - 1) // self.g if (self.tt < NT and self.a0 < NA) else 0
+        if self.nq == 0:
+            self.tt = -1
 
-    def col(self, q, lane):
-        return (self.ct0 + self.g * q) * 16 + (lane & 15)
+    def arow(self, q, lane):
+        return (self.a0 + self.g * q) * 16 + (lane & 15)
 
-    def row(self, lane, v):
-        return self.rt * 16 + 4 * (lane >> 4) + v
+    def aout(self, q, lane):
+        return (self.a0 + self.g * q) * 16 + 4 * (lane >> 4)
+
+    def tcol(self, lane):
+        return self.tt * 16 + (lane & 15)
 
 
 def basis(n):
@@ -50,43 +65,50 @@ def basis(n):
     return np.where(k == 0, 1.0, np.sqrt(2.0)) * np.cos(np.pi * k * (2 * i + 1) / (2 * n))
 
 
-def table(n):
-    """The kernel's per-size table: B̂ and B̂ᵀ, zero-padded to pad32(n) (one plane each)."""
-    p = pad32(n)
-    B = np.zeros((p, p))
-    B[:n, :n] = basis(n)
-    return B, B.T.copy()
+def padded(M):
+    p = pad32(M.shape[0])
+    out = np.zeros((p, p))
+    out[:M.shape[0], :M.shape[1]] = M
+    return out
 
 
-def run_stage(A, src, written, RT, CT, KS, nwaves, epi):
-    """One res_stage: every wave, tile and k step; A-fragment rows and k ranges, B reads
-    from `src` at [c][k] checked against `written`; epi(r, c, value) per output."""
-    lda = A.shape[0]
-    for wave in range(nwaves):
-        t = Tiles(RT, CT, wave, nwaves)
-        if t.rt < 0:
+def run_stage(WT, src, written, NT, NA, KS, dst=None, dst_w=None):
+    """One res_stage. WT: the table Wᵀ[t][k] (pad32 square). Returns O[a][t] for the
+    padded tiles and, with dst, stores it as [t][a] the way res_store does."""
+    R, AH = ring(KS)
+    p = WT.shape[0]
+    out = {}
+    for wave in range(WAVES):
+        t = Tiles(NT, NA, wave)
+        if t.tt < 0:
             continue
-        for q in range(t.nct):
-            acc = np.zeros((WAVE, 4))
-            for ks in range(KS):
-                for lane in range(WAVE):
-                    arow = t.rt * 16 + (lane & 15)
-                    k0 = 32 * ks + 8 * (lane >> 4)
-                    assert arow < lda and k0 + 8 <= lda
-                    c = t.col(q, lane)
-                    boff = c * LD + k0
-                    assert 0 <= boff and boff + 8 <= MAXN * LD
-                    assert written[c, k0:k0 + 8].all(), (c, k0)
-                # the 16×16 tile product, lane-wise as the accumulator layout stores it
-                for lane in range(WAVE):
-                    c = t.col(q, lane)
-                    for v in range(4):
-                        r = t.row(lane, v)
-                        ks_range = slice(32 * ks, 32 * ks + 32)
-                        acc[lane, v] += A[r, ks_range] @ src[c, ks_range]
+        # matrix fragments, k steps beyond KS clamped to the last one
+        for ks in range(4):
+            kk = min(ks, KS - 1)
             for lane in range(WAVE):
+                row, k0 = t.tcol(lane), 32 * kk + 8 * (lane >> 4)
+                assert row < p and k0 + 8 <= p
+        # operand reads: the sequence s = q·KS + ks for q < nq, then AH steps beyond
+        for s in range(t.nq * KS + AH):
+            q, ks = divmod(s, KS)
+            qq = min(q, t.nq - 1)
+            for lane in range(WAVE):
+                a, k0 = t.arow(qq, lane), 32 * ks + 8 * (lane >> 4)
+                off = a * LD + k0
+                assert 0 <= off and off + 8 <= MAXN * LD and k0 + 8 <= LD
+                assert written[a, k0:k0 + 8].all(), (a, k0)
+        for q in range(t.nq):
+            for lane in range(WAVE):
+                tc = t.tcol(lane)
                 for v in range(4):
-                    epi(t.row(lane, v), t.col(q, lane), acc[lane, v])
+                    a = t.aout(q, lane) + v
+                    val = src[a, :32 * KS] @ WT[tc, :32 * KS]
+                    out[a, tc] = val
+                    if dst is not None:
+                        assert tc < MAXN and a < LD - 8
+                        dst[tc, a] = val
+                        dst_w[tc, a] = True
+    return out
 
 
 def new_buffer():
@@ -95,95 +117,69 @@ def new_buffer():
 
 @pytest.mark.parametrize("nx,ny", [(128, 128), (96, 128), (128, 96), (48, 32), (32, 48),
                                    (64, 96), (2, 1), (17, 33), (100, 7), (128, 2)])
-@pytest.mark.parametrize("nwaves", [8])
-def test_resident_stage_chain_matches_the_transforms(nx, ny, nwaves):
+def test_resident_stage_chain_matches_the_transforms(nx, ny):
     rng = np.random.default_rng(nx * 1000 + ny)
     nxp, nyp = pad32(nx), pad32(ny)
-    Bx, BxT = table(nx)
-    By, ByT = table(ny)
+    Bx, By = padded(basis(nx)), padded(basis(ny))
+    KX, KY = nxp // 32, nyp // 32
 
-    # ---- forward transform of a physical field q[j][i] (prep writes [j][i], padded) ----
+    # ---- forward: q stored [i][j] (prep writes the padded surface) ----
     q = rng.standard_normal((ny, nx))
     buf, wr = new_buffer()
-    buf[:nyp, :nxp] = 0.0
-    buf[:ny, :nx] = q
-    wr[:nyp, :nxp] = True
-    # stage 3: Rᵀ = B̂x·qᵀ (rows m: x, cols j: y), tiles tp(nxp/16, nyp/16), stored [m][j]
-    nb, nw = new_buffer()
-
-    def store_n(dst, w, r, c, val):
-        assert r < MAXN and c < LD - 8
-        dst[r, c] = val
-        w[r, c] = True
-    run_stage(Bx, buf, wr, nxp // 16, nyp // 16, nxp // 32, nwaves,
-              lambda r, c, val: store_n(nb, nw, r, c, val))
-    # stage 4: S = B̂y·R (rows n, cols m), tiles tm(nyp/16, nxp/16)
+    buf[:nxp, :nyp] = 0.0
+    buf[:nx, :ny] = q.T
+    wr[:nxp, :nyp] = True
+    # fwd1: R[i][n] = Σ_j q[i][j] B̂y[n][j]; Wᵀ[n][j] = B̂y; stored [n][i]
+    b1, w1 = new_buffer()
+    run_stage(By, buf, wr, nyp // 16, nxp // 16, KY, b1, w1)
+    # fwd2: S[n][m] = Σ_i R[n][i] B̂x[m][i]; Wᵀ[m][i] = B̂x
+    out = run_stage(Bx, b1, w1, nxp // 16, nyp // 16, KX)
     S = np.zeros((nyp, nxp))
-
-    def keep(r, c, val):
-        S[r, c] = val
-    run_stage(By, nb, nw, nyp // 16, nxp // 16, nyp // 32, nwaves, keep)
-    ref = basis(ny) @ q @ basis(nx).T
+    for (a, t), v in out.items():
+        S[a, t] = v
+    ref = basis(ny) @ q @ basis(nx).T                       # [n][m]
     np.testing.assert_allclose(S[:ny, :nx], ref, atol=1e-10 * max(1.0, np.abs(ref).max()))
     assert np.abs(S[ny:, :]).max(initial=0) == 0 and np.abs(S[:, nx:]).max(initial=0) == 0
+    assert len(out) == nxp * nyp                             # every padded output, once
 
-    # ---- inverse transform of a modal field X[n][m], stored [m][n] by the epilogue ----
+    # ---- inverse: X as the forward epilogue stores it ([m][n], zero padding) ----
     X = rng.standard_normal((ny, nx))
     buf, wr = new_buffer()
-    Xs = np.zeros((nyp, nxp))
-    Xs[:ny, :nx] = X
+    buf[:nxp, :nyp] = 0.0
+    buf[:nx, :ny] = X.T
+    wr[:nxp, :nyp] = True
+    # inv1: V[m][j] = Σ_n X[m][n] B̂y[n][j]; Wᵀ[j][n] = B̂yᵀ; stored [j][m]
+    b1, w1 = new_buffer()
+    run_stage(By.T.copy(), buf, wr, nyp // 16, nxp // 16, KY, b1, w1)
+    # inv2: T[j][i] = Σ_m V[j][m] B̂x[m][i]; Wᵀ[i][m] = B̂xᵀ; stored [i][j]
+    b0, w0 = new_buffer()
+    run_stage(Bx.T.copy(), b1, w1, nxp // 16, nyp // 16, KX, b0, w0)
+    ref = basis(ny).T @ X @ basis(nx)                        # [j][i]
+    np.testing.assert_allclose(b0[:nx, :ny].T, ref, atol=1e-10 * max(1.0, np.abs(ref).max()))
+    assert w0[:nxp, :nyp].all()      # the next forward stage reads all of it
 
-    def store_t(dst, w, r, c, val):        # res_store_t: element (r, c) at [c][r]
-        assert c < MAXN and r < LD - 8
-        dst[c, r] = val
-        w[c, r] = True
-    # the forward epilogue writes every (n, m) of the padded modal tiles
-    for wave in range(nwaves):
-        t = Tiles(nyp // 16, nxp // 16, wave, nwaves)
-        if t.rt < 0:
+    # ---- epilogue offsets (second-stage tiles: t along x, a along y) ----
+    for wave in range(WAVES):
+        t = Tiles(nxp // 16, nyp // 16, wave)
+        if t.tt < 0:
             continue
-        for qq in range(t.nct):
+        for qq in range(t.nq):
             for lane in range(WAVE):
+                x = t.tcol(lane)
                 for v in range(4):
-                    r, c = t.row(lane, v), t.col(qq, lane)
-                    store_t(buf, wr, r, c, Xs[r, c])
-    # stage 1: P = B̂yᵀ·X (rows j, cols m), tiles tm; stored straight [j][m]
-    pb, pw = new_buffer()
-    run_stage(ByT, buf, wr, nyp // 16, nxp // 16, nyp // 32, nwaves,
-              lambda r, c, val: store_n(pb, pw, r, c, val))
-    # stage 2: Tᵀ = B̂xᵀ·Pᵀ (rows i, cols j), tiles tp
-    TT = np.zeros((nxp, nyp))
-
-    def keep_t(r, c, val):
-        TT[r, c] = val
-    run_stage(BxT, pb, pw, nxp // 16, nyp // 16, nxp // 32, nwaves, keep_t)
-    ref = basis(ny).T @ X @ basis(nx)
-    np.testing.assert_allclose(TT[:nx, :ny].T, ref, atol=1e-10 * max(1.0, np.abs(ref).max()))
-
-    # ---- epilogue offsets stay inside the surface ----
-    for wave in range(nwaves):
-        for t, rows_are_x in ((Tiles(nxp // 16, nyp // 16, wave, nwaves), True),
-                              (Tiles(nyp // 16, nxp // 16, wave, nwaves), False)):
-            if t.rt < 0:
-                continue
-            for qq in range(t.nct):
-                for lane in range(WAVE):
-                    for v in range(4):
-                        r, c = t.row(lane, v), t.col(qq, lane)
-                        x, y = (r, c) if rows_are_x else (c, r)
-                        if x < nx and y < ny:
-                            assert 0 <= y * nx + x < nx * ny
+                    y = t.aout(qq, lane) + v
+                    if x < nx and y < ny:
+                        assert 0 <= y * nx + x < nx * ny
 
 
 def test_every_padded_tile_has_a_wave():
-    # all RT × CT tiles of every stage shape are assigned, each once
-    for nwaves in (8, 16):
-        for RT in (2, 4, 6, 8):
-            for CT in (2, 4, 6, 8):
-                seen = []
-                for wave in range(nwaves):
-                    t = Tiles(RT, CT, wave, nwaves)
-                    if t.rt >= 0:
-                        seen += [(t.rt, t.ct0 + t.g * q) for q in range(t.nct)]
-                        assert t.nct <= (8 if nwaves == 8 else 4)
-                assert sorted(seen) == [(r, c) for r in range(RT) for c in range(CT)]
+    # all NT × NA tiles of every stage shape are assigned, each once
+    for NT in (1, 2, 4, 6, 8):
+        for NA in (1, 2, 4, 6, 8):
+            seen = []
+            for wave in range(WAVES):
+                t = Tiles(NT, NA, wave)
+                if t.tt >= 0:
+                    seen += [(t.tt, t.a0 + t.g * q) for q in range(t.nq)]
+                    assert t.nq <= 8
+            assert sorted(seen) == [(r, c) for r in range(NT) for c in range(NA)]
