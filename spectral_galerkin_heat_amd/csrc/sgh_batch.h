@@ -99,6 +99,8 @@ struct BatchPtrs {
   long long surf_stride;   // elements from one surface array to the next (G, ZE, U, Ztt, …)
   float* zwu;              // FP32 pools: (ZE, Ztt, Wtt, U) of each surface element, packed
                            // for the resident step kernel; null otherwise
+  float* res_work;         // FP32 pools: per cuboid slot, kResWork floats of the resident
+                           // step kernel's tiled G, q_evap and q_las; null otherwise
   int metrics_rows;        // capacity in steps
   int n_cuboids;
 };
@@ -109,6 +111,7 @@ constexpr int kResMaxN = 128;            // widest surface axis
 constexpr int kResLd = kResMaxN + 8;     // bf16 per LDS operand row: 16-byte rows, staggered banks
 constexpr int kResThreads = 512;
 constexpr int kResLdsBytes = 4 * kResMaxN * kResLd * 2;   // two hi / lo operand buffer pairs
+constexpr int kResWork = 3 * kResMaxN * kResMaxN;          // work floats per cuboid slot
 
 __host__ __device__ inline int pad32(int n) { return (n + 31) & ~31; }
 

@@ -544,6 +544,7 @@ struct Plan {
   Arena arena_m16;
   void* mats16 = nullptr;
   void* zwu = nullptr;       // FP32: (ZE, Ztt, Wtt, U) per surface element, 16 bytes
+  void* res_work = nullptr;  // FP32: kResWork floats per cuboid slot (resident kernel)
   long long mats16_cap = 0;
   bool resident = false;     // variant rows = −1 requested
   bool res_on = false;       // ... and every live cuboid allows it (mixed precision, ≤ 128 wide)
@@ -612,6 +613,7 @@ BatchPtrs<F> ptrs(Plan* p) {
   P.Xinv = s + 9 * T;
   P.surf_stride = T;
   P.zwu = (float*)p->zwu;
+  P.res_work = (float*)p->res_work;
   P.mats = (const F*)p->mats;
   P.t1 = p->t1;
   P.laser = p->laser;
@@ -953,7 +955,7 @@ static void free_plan(Plan* p) {
                   p->col_items,
                   p->prime_items, p->row_slots, p->col_slots, p->aux_slots, p->done_at,
                   p->status, p->step_idx, p->metrics, p->bar, p->mats16, p->res_list,
-                  p->zwu};
+                  p->zwu, p->res_work};
   for (void* b : bufs)
     if (b) hipFree(b);
   delete p;
@@ -1336,6 +1338,7 @@ int sgh_batch_pool_create(int dtype, int max_cuboids, long long state_capacity,
   chk(dmalloc(&p->res_list, (size_t)p->res_cap * sizeof(int)));
   if (dtype == SGH_F32) {
     chk(dmalloc(&p->zwu, (size_t)p->total_S * 4 * sizeof(float)));
+    chk(dmalloc(&p->res_work, (size_t)max_cuboids * kResWork * sizeof(float)));
     chk(resident_allow_lds());
   }
   if (e == hipSuccess) e = hipMemset(p->bar, 0, ((size_t)max_cuboids + 1) * sizeof(unsigned));

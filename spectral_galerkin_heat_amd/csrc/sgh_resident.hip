@@ -280,6 +280,12 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
   // tB: t along x (m or i), a along y (n or j): the second stage, with the epilogues
   const ResTiles tA(nyp / 16, nxp / 16, wave);
   const ResTiles tB(nxp / 16, nyp / 16, wave);
+  // work-slot index (in 16-byte units) of this lane's outputs of tB tile q (clamped to the
+  // wave's last tile)
+  auto tidx = [&](int q, int ln) __attribute__((always_inline)) {
+    const int qq = q < tB.nq ? q : tB.nq - 1;
+    return ((tB.a0 + tB.g * qq) * (nxp / 16) + tB.tt) * 64 + ln;
+  };
   const int so = d.s_off;
   ResFields fld;
   fld.r = __builtin_amdgcn_make_buffer_rsrc(P.G + so, (short)0,
@@ -289,6 +295,15 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
   fld.rz = __builtin_amdgcn_make_buffer_rsrc(P.zwu + 4LL * so, (short)0,
                                              (int)(16 * (P.surf_stride - so)), 0x00020000);
   F* const q_las = P.q_las + so;
+  // the cuboid's work slot: G, q_evap and q_las of the step in the second stages' tile order
+  // (tile (a, t) of tB at ((a·NTB + t)·64 + lane)·4 + v), so that a lane's four outputs of a
+  // tile are one 16-byte access; written back to the surface arrays at the step end
+  float* const wk = P.res_work + (long long)c * kResWork;
+  f4v* const Gt = reinterpret_cast<f4v*>(wk);
+  f4v* const QEt = reinterpret_cast<f4v*>(wk + kResMaxN * kResMaxN);
+  float* const QL = wk + 2 * kResMaxN * kResMaxN;
+  const f4v* const QLt = reinterpret_cast<const f4v*>(QL);
+  const int NTB = nxp / 16;
   auto no_pre = [](int, int, int) __attribute__((always_inline)) {};
 
   // ---- step preparation: AA, laser flux, warm-start flux (buffer 0, as [i][j]) ----
@@ -336,7 +351,7 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
       for (int u = 0; u < 4; ++u) {
         const int i = ij[u][0], j = ij[u][1];
         if (j >= nyp) break;
-        float qn = 0.f;
+        float qn = 0.f, qlw = 0.f;
         if (in_s[u]) {
           const double x = (i + 0.5) * d.dx, y = (j + 0.5) * d.dy;
           double ql;
@@ -357,12 +372,14 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
                                           ty * ((1.0 - tx) * (double)nb[u][2] + tx * (double)nb[u][3])
                                     : 0.0;
           q_las[j * nx + i] = qlf;
+          qlw = qlf;
           qn = (F)((double)qlf - (double)(F)sv);
           psum += (double)qlf;
         }
         const __bf16 hv = (__bf16)qn;
         b0.hi[i * kResLd + j] = hv;               // forward operand [i][j]
         b0.lo[i * kResLd + j] = (__bf16)(qn - (float)hv);
+        QL[(((j >> 4) * NTB + (i >> 4)) * 64 + (i & 15) + 16 * ((j & 15) >> 2)) * 4 + (j & 3)] = qlw;
       }
     }
   }
@@ -425,12 +442,14 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
                 unsigned vo[4], vo16[4];
                 res_voffs(tB, q, lane, nxk, nyk, vo, vo16);
                 float xo[4];
+                f4v g4;
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                   const F g = a[v] * fwd_scale_f;
-                  fld.store(0, vo[v], g);
+                  g4[v] = g;
                   xo[v] = zw[sl][v][0] + g * zw[sl][v][1];
                 }
+                Gt[tidx(q, lane)] = g4;
                 res_store(buf0, tB.aout(q, lane), tB.tcol(lane), xo);   // [m][n]
               });
   }
@@ -454,13 +473,12 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
     stage_inv1();
     {
       const int nxk = opaque(nx), nyk = opaque(ny);
-      float ql[2][4];
+      f4v ql[2];
       res_stage(tB, W, KX, buf1, lane, w_fwd1,
                 [&](int q, int sl, int lane) __attribute__((always_inline)) {
                   unsigned vo[4], vo16[4];
                   res_voffs(tB, q, lane, nxk, nyk, vo, vo16);
-#pragma unroll
-                  for (int v = 0; v < 4; ++v) ql[sl][v] = fld.load(5, vo[v]);
+                  ql[sl] = QLt[tidx(q, lane)];
                 },
                 [&](int q, int sl, const f4v& a, int lane) __attribute__((always_inline)) {
                   unsigned vo[4], vo16[4];
@@ -487,10 +505,8 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
                   }
                   float qo[4];
 #pragma unroll
-                  for (int v = 0; v < 4; ++v) {
-                    fld.store(7, vo[v], qe[v]);
-                    qo[v] = ql[sl][v] - qe[v];
-                  }
+                  for (int v = 0; v < 4; ++v) qo[v] = ql[sl][v] - qe[v];
+                  QEt[tidx(q, lane)] = f4v{qe[0], qe[1], qe[2], qe[3]};
                   res_store(buf0, tB.aout(q, lane), tB.tcol(lane), qo);   // [i][j]
                 });
     }
@@ -501,21 +517,20 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
     {
       const int nxk = opaque(nx), nyk = opaque(ny);
       f4v zw[2][4];   // (ZE, Ztt, Wtt, U) of the tile's elements
-      float gg[2][4];
+      f4v gg[2];
       res_stage(tB, W, KX, buf1, lane, w_inv1,
                 [&](int q, int sl, int lane) __attribute__((always_inline)) {
                   unsigned vo[4], vo16[4];
                   res_voffs(tB, q, lane, nxk, nyk, vo, vo16);
 #pragma unroll
-                  for (int v = 0; v < 4; ++v) {
-                    zw[sl][v] = fld.record(vo16[v]);
-                    gg[sl][v] = fld.load(0, vo[v]);
-                  }
+                  for (int v = 0; v < 4; ++v) zw[sl][v] = fld.record(vo16[v]);
+                  gg[sl] = Gt[tidx(q, lane)];
                 },
                 [&](int q, int sl, const f4v& a, int lane) __attribute__((always_inline)) {
                   unsigned vo[4], vo16[4];
                   res_voffs(tB, q, lane, nxk, nyk, vo, vo16);
                   float xo[4], p0 = 0.f, p1 = 0.f, p2 = 0.f;
+                  f4v g4;
 #pragma unroll
                   for (int v = 0; v < 4; ++v) {
                     const F g = gg[sl][v];
@@ -525,9 +540,10 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
                     p1 += g * zw[sl][v][3];
                     p2 += g * g * w;
                     const F gn = g + omega * dd;
-                    fld.store(0, vo[v], gn);
+                    g4[v] = gn;
                     xo[v] = zw[sl][v][0] + gn * zw[sl][v][1];
                   }
+                  Gt[tidx(q, lane)] = g4;
                   v0 += (double)p0;
                   v1 += (double)p1;
                   v2 += (double)p2;
@@ -557,14 +573,20 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
     if (conv) break;
   }
 
-  // ---- step end: T_s of the final iterate and its maximum (G is in place for the volume
-  // pass) ----
+  // ---- step end: T_s of the final iterate and its maximum; G and the last iteration's
+  // q_evap from the work slot into the surface arrays for the volume pass ----
   stage_inv1();
   double tmax = -1.0 / 0.0;
   res_stage(tB, W, KX, buf1, lane, w_none, no_pre,
             [&](int q, int, const f4v& a, int lane) __attribute__((always_inline)) {
               unsigned vo[4], vo16[4];
               res_voffs(tB, q, lane, nx, ny, vo, vo16);
+              const f4v g4 = Gt[tidx(q, lane)], qe4 = QEt[tidx(q, lane)];
+#pragma unroll
+              for (int v = 0; v < 4; ++v) {
+                fld.store(0, vo[v], g4[v]);
+                fld.store(7, vo[v], qe4[v]);
+              }
 #pragma unroll
               for (int v = 0; v < 4; ++v)
                 if (vo[v] != kResOut) {
