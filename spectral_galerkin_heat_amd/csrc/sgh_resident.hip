@@ -103,12 +103,13 @@ struct ResW {
   bf8v h[4], l[4];
 };
 
-// Load the matrix fragments of the wave's t tile (no-op without tiles) from Wᵀ[t][k]: hi / lo
-// row-major matrices at m16 + off_h / off_l in global memory, row stride ld, KS k steps of
-// 32 (the offsets, not the pointers, are made opaque, so the loads stay global loads).
+// Load slots [k0, k1) of the matrix fragments of the wave's t tile (no-op without tiles) from
+// Wᵀ[t][k]: hi / lo row-major matrices at m16 + off_h / off_l in global memory, row stride
+// ld, KS k steps of 32 (the offsets, not the pointers, are made opaque, so the loads stay
+// global loads).
 __device__ __forceinline__ void res_load_w(ResW& w, const ResTiles& t, const __bf16* m16,
                                            long long off_h, long long off_l, int ld, int KS,
-                                           int lane) {
+                                           int lane, int k0, int k1) {
   if (t.tt < 0) return;
   const __bf16* Wh = m16 + opaque(off_h);
   const __bf16* Wl = m16 + opaque(off_l);
@@ -116,14 +117,12 @@ __device__ __forceinline__ void res_load_w(ResW& w, const ResTiles& t, const __b
   const long long row = (long long)t.tcol(lane) * opaque(ld) + 8 * (lane >> 4);
 #pragma unroll
   for (int ks = 0; ks < 4; ++ks) {
+    if (ks < k0 || ks >= k1) continue;
     // k steps beyond KS re-read the last one (unused; unconditional loads keep the wait
     // counts exact)
     const int kk = ks < KS ? ks : KS - 1;
     w.h[ks] = *reinterpret_cast<const bf8v*>(Wh + row + 32 * kk);
     w.l[ks] = *reinterpret_cast<const bf8v*>(Wl + row + 32 * kk);
-    // pinned in registers for the stage: without this the register allocator reloads
-    // them from memory for every tile (and waits for each reload)
-    asm volatile("" : "+v"(w.h[ks]), "+v"(w.l[ks]));
   }
 }
 
@@ -132,20 +131,24 @@ __device__ __forceinline__ void res_load_w(ResW& w, const ResTiles& t, const __b
 // epi(q, slot, acc, lane). W: the fragments in `w`, loaded beforehand. D: the source buffer,
 // read AH k steps ahead of the products that use it through a ring of R fragment pairs (the
 // k steps of all tiles form one sequence; beyond the wave's last tile the reads repeat that
-// tile, so every read is unconditional and in bounds). The loop is rolled two tiles at a
-// time (the code stays small enough for the instruction cache; ring slots, the two
-// accumulators and the two epilogue slots are fixed per half). The epilogue of tile q runs
-// after tile q + 1's products are issued, so its arithmetic overlaps them; pre(q, slot, lane)
-// starts the loads that epilogue needs once the epilogue of tile q − 2 has used the slot.
-template <int KS, typename Pre, typename Epi>
-__device__ __forceinline__ void res_tiles(const ResTiles& t, const ResW& w, const ResBuf& src,
-                                          int lane, Pre&& pre, Epi&& epi) {
+// tile, so every read is unconditional and in bounds). Tile q uses ring slots, accumulator
+// and epilogue slot q mod 2: the loop runs two tiles per trip with no branch inside (the
+// code stays small enough for the instruction cache, and the wait counts for the ring stay
+// exact), the first tile and the last one or two are peeled. The epilogue of tile q runs
+// after tile q + 1's products are issued, so its arithmetic overlaps them;
+// pre(q, slot, lane) starts the loads that epilogue needs once the epilogue of tile q − 2
+// has used the slot. next(k0, k1) loads slots [k0, k1) of the following stage's matrix
+// fragments: slots KS..3 at the start (this stage does not use them), the others after the
+// last tile's products, so that the loads overlap the last epilogues and the barrier.
+template <int KS, typename Next, typename Pre, typename Epi>
+__device__ __forceinline__ void res_tiles(const ResTiles& t, ResW& w, const ResBuf& src,
+                                          int lane, Next&& next, Pre&& pre, Epi&& epi) {
   constexpr int R = KS == 1 ? 2 : (KS == 3 ? 3 : 4);
   constexpr int AH = R - 1;
   static_assert((2 * KS) % R == 0, "ring slots repeat every two tiles");
   const int nq = t.nq;
   bf8v dh[R], dl[R];
-  // (lane is laundered per half: addresses are formed where they are used instead of being
+  // (lane is laundered per tile: addresses are formed where they are used instead of being
   // hoisted out of the loop and kept live)
   auto dload = [&](int q, int ks, int slot, int ln) __attribute__((always_inline)) {
     const int qq = q < nq ? q : nq - 1;
@@ -154,49 +157,70 @@ __device__ __forceinline__ void res_tiles(const ResTiles& t, const ResW& w, cons
     dl[slot] = *reinterpret_cast<const bf8v*>(src.lo + off);
   };
   f4v acc[2];
-  auto half = [&](int q, int par) __attribute__((always_inline)) {
-    const int ln = opaque_lane(lane);
-    if (q < nq) {
-      f4v& c = acc[par];
-      c = f4v{0.f, 0.f, 0.f, 0.f};
+  auto mfma = [&](int q, int par, int ln) __attribute__((always_inline)) {
+    f4v& c = acc[par];
+    c = f4v{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int ks = 0; ks < KS; ++ks) {
-        const int s = par * KS + ks;   // k step within the two-tile period
-        dload(q + (ks + AH) / KS, (ks + AH) % KS, (s + AH) % R, ln);
-        c = mma_bf16(dh[s % R], w.h[ks], c);
-        c = mma_bf16(dh[s % R], w.l[ks], c);
-        c = mma_bf16(dl[s % R], w.h[ks], c);
-      }
+    for (int ks = 0; ks < KS; ++ks) {
+      const int s = par * KS + ks;   // k step within the two-tile period
+      dload(q + (ks + AH) / KS, (ks + AH) % KS, (s + AH) % R, ln);
+      c = mma_bf16(dh[s % R], w.h[ks], c);
+      c = mma_bf16(dh[s % R], w.l[ks], c);
+      c = mma_bf16(dl[s % R], w.h[ks], c);
     }
-    if (q >= 1 && q - 1 < nq) epi(q - 1, par ^ 1, acc[par ^ 1], ln);
-    pre(q + 1, par ^ 1, ln);
   };
+  next(KS, 4);
   pre(0, 0, lane);
 #pragma unroll
   for (int s = 0; s < AH; ++s) dload(s / KS, s % KS, s % R, lane);
+  mfma(0, 0, lane);
+  pre(1, 1, lane);
+  int q = 1;
 #pragma unroll 1
-  for (int q0 = 0; q0 <= nq; q0 += 2) {
-    half(q0, 0);
-    half(q0 + 1, 1);
+  for (; q + 1 < nq; q += 2) {
+    int ln = opaque_lane(lane);
+    mfma(q, 1, ln);
+    epi(q - 1, 0, acc[0], ln);
+    pre(q + 1, 0, ln);
+    ln = opaque_lane(lane);
+    mfma(q + 1, 0, ln);
+    epi(q, 1, acc[1], ln);
+    pre(q + 2, 1, ln);
+  }
+  const int ln = opaque_lane(lane);
+  if (q < nq) {   // nq even: tile q = nq − 1 remains
+    mfma(q, 1, ln);
+    next(0, KS);
+    epi(q - 1, 0, acc[0], ln);
+    epi(q, 1, acc[1], ln);
+  } else {        // nq odd: tile q − 1 = nq − 1 awaits its epilogue
+    next(0, KS);
+    epi(q - 1, 0, acc[0], ln);
   }
 }
 
-// One stage: the wave's tiles (res_tiles, specialised on the number of k steps), then
-// `next()`, which reloads the matrix fragments for the following stage so that their latency
-// passes under the barrier that ends the stage (after it the epilogue's stores are visible).
+// One stage: the wave's tiles (res_tiles, specialised on the number of k steps; it reloads the
+// matrix fragments for the following stage through next), then the barrier after which the
+// epilogue's stores are visible. A wave without tiles in this stage reloads them at once.
 template <typename Next, typename Pre, typename Epi>
 __device__ __forceinline__ void res_stage(const ResTiles& t, ResW& w, int KS, const ResBuf& src,
                                           int lane, Next&& next, Pre&& pre, Epi&& epi) {
   if (t.tt >= 0) {
     lane = opaque_lane(lane);
+    // the fragments are pinned in registers for the stage here, after the barrier that
+    // ended the previous stage (their loads were issued before it): without this the
+    // register allocator reloads them from memory for every tile
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) asm volatile("" : "+v"(w.h[ks]), "+v"(w.l[ks]));
     switch (KS) {
-      case 1: res_tiles<1>(t, w, src, lane, pre, epi); break;
-      case 2: res_tiles<2>(t, w, src, lane, pre, epi); break;
-      case 3: res_tiles<3>(t, w, src, lane, pre, epi); break;
-      default: res_tiles<4>(t, w, src, lane, pre, epi); break;
+      case 1: res_tiles<1>(t, w, src, lane, next, pre, epi); break;
+      case 2: res_tiles<2>(t, w, src, lane, next, pre, epi); break;
+      case 3: res_tiles<3>(t, w, src, lane, next, pre, epi); break;
+      default: res_tiles<4>(t, w, src, lane, next, pre, epi); break;
     }
+  } else {
+    next(0, 4);
   }
-  next();
   __syncthreads();
 }
 
@@ -317,6 +341,29 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
     const double I0 = on != 0.0 ? d.absorptivity * power / (d.area_factor * d.rx * d.ry) : 0.0;
     const double shy = vy * d.dt / d.dy, shx = vx * d.dt / d.dx;
     const F* qo = P.q_evap_old + so;
+    // the laser flux is separable (cell-integrated or Gaussian profile): its x and y factors
+    // are formed once, in buffer 1 (unused until the first transform)
+    double* const fac = reinterpret_cast<double*>(buf1.hi);   // x: [0, nx), y: [128, 128 + ny)
+    const bool sep = d.cell_integrated || d.order == 2.0;
+    if (sep) {
+      for (int e = tid; e < nx + ny; e += kResThreads) {
+        const bool isx = e < nx;
+        const int k = isx ? e : e - nx;
+        const double h = isx ? d.dx : d.dy, r = isx ? d.rx : d.ry, c0 = isx ? x0 : y0;
+        const double pos = (k + 0.5) * h;
+        double f;
+        if (d.cell_integrated) {
+          const double s2 = sqrt(2.0), cc = sqrt(M_PI / 8.0);
+          const double lo = (pos - c0 - h / 2.0) * s2 / r, hi = (pos - c0 + h / 2.0) * s2 / r;
+          f = r * cc * (erf(hi) - erf(lo)) / h;
+        } else {
+          const double u = (pos - c0) / r;
+          f = exp(-2.0 * (u * u));
+        }
+        fac[isx ? k : kResMaxN + k] = f;
+      }
+      __syncthreads();
+    }
     // four elements per pass over the padded surface (zero operand entries outside it):
     // the previous flux's neighbours of all four are loaded before any is used
     const ResBuf& b0 = buf0;
@@ -355,7 +402,9 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
         if (in_s[u]) {
           const double x = (i + 0.5) * d.dx, y = (j + 0.5) * d.dy;
           double ql;
-          if (d.cell_integrated) {
+          if (sep) {
+            ql = I0 * (fac[kResMaxN + j] * fac[i]);
+          } else if (d.cell_integrated) {
             const double s2 = sqrt(2.0), cc = sqrt(M_PI / 8.0);
             const double ax = (x - x0 - d.dx / 2.0) * s2 / d.rx, bx = (x - x0 + d.dx / 2.0) * s2 / d.rx;
             const double ay = (y - y0 - d.dy / 2.0) * s2 / d.ry, by = (y - y0 + d.dy / 2.0) * s2 / d.ry;
@@ -392,19 +441,19 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
   // matrix fragments of the four stage kinds; each stage loads the next one's after its
   // last product
   ResW W;
-  auto w_fwd1 = [&]() __attribute__((always_inline)) {     // W[j][n] = B̂y[n][j]
-    res_load_w(W, tA, m16, Y16, Y16 + my, nyp, KY, lane);
+  auto w_fwd1 = [&](int k0, int k1) __attribute__((always_inline)) {     // W[j][n] = B̂y[n][j]
+    res_load_w(W, tA, m16, Y16, Y16 + my, nyp, KY, lane, k0, k1);
   };
-  auto w_fwd2 = [&]() __attribute__((always_inline)) {     // W[i][m] = B̂x[m][i]
-    res_load_w(W, tB, m16, X16, X16 + mx, nxp, KX, lane);
+  auto w_fwd2 = [&](int k0, int k1) __attribute__((always_inline)) {     // W[i][m] = B̂x[m][i]
+    res_load_w(W, tB, m16, X16, X16 + mx, nxp, KX, lane, k0, k1);
   };
-  auto w_inv1 = [&]() __attribute__((always_inline)) {     // W[n][j] = B̂y[n][j]
-    res_load_w(W, tA, m16, Y16 + 2 * my, Y16 + 3 * my, nyp, KY, lane);
+  auto w_inv1 = [&](int k0, int k1) __attribute__((always_inline)) {     // W[n][j] = B̂y[n][j]
+    res_load_w(W, tA, m16, Y16 + 2 * my, Y16 + 3 * my, nyp, KY, lane, k0, k1);
   };
-  auto w_inv2 = [&]() __attribute__((always_inline)) {     // W[m][i] = B̂x[m][i]
-    res_load_w(W, tB, m16, X16 + 2 * mx, X16 + 3 * mx, nxp, KX, lane);
+  auto w_inv2 = [&](int k0, int k1) __attribute__((always_inline)) {     // W[m][i] = B̂x[m][i]
+    res_load_w(W, tB, m16, X16 + 2 * mx, X16 + 3 * mx, nxp, KX, lane, k0, k1);
   };
-  auto w_none = []() __attribute__((always_inline)) {};
+  auto w_none = [](int, int) __attribute__((always_inline)) {};
 
   // first stage of each transform: the result goes to the other buffer as the second
   // stage's operand
@@ -426,7 +475,7 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
   // ---- warm start: G = fwd·B̂y q_net B̂xᵀ; Xinv = ZE + G∘Ztt into buffer 0 ----
   // (outside the surface every epilogue input is 0: the matrices' padding is zero, and
   // loads beyond the surface return 0, so the operand padding stays 0 without selects)
-  w_fwd1();
+  w_fwd1(0, 4);
   stage_fwd1();
   {
     const int nxk = opaque(nx), nyk = opaque(ny);
