@@ -224,18 +224,51 @@ __device__ __forceinline__ void res_stage(const ResTiles& t, ResW& w, int KS, co
   __syncthreads();
 }
 
-// Sum of a double over the workgroup in a fixed order (wave sums, then the 16 wave
-// partials summed by one butterfly in every wave); every thread receives it. sh holds
-// kResWaves doubles; ends with a barrier.
-__device__ __forceinline__ double res_gather(const double* sh) {
-  const int lane = threadIdx.x & (kWave - 1);
-  return wave_sum(lane < kResWaves ? sh[lane] : 0.0);
+// v plus the value of the lane the DPP control selects (0 where it selects none, and in the
+// rows the row mask leaves out), both halves of the double moved by the same control.
+template <int Ctrl, int RowMask>
+__device__ __forceinline__ double res_dpp_add(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, Ctrl, RowMask, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), Ctrl, RowMask, 0xf, true);
+  return v + __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
 }
+__device__ __forceinline__ double res_lane(double v, int l) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+// Sum over the wave in a fixed order, with DPP lane moves instead of LDS shuffles: within
+// each row of 16 by row shifts 1, 2, 4, 8 (lane 15 then holds the row's sum), then across
+// rows by the row broadcasts of lane 15 (rows 1, 3) and lane 31 (rows 2, 3); the total is in
+// lane 63 and returned to every lane.
+__device__ __forceinline__ double res_wave_sum(double v) {
+  v = res_dpp_add<0x111, 0xf>(v);
+  v = res_dpp_add<0x112, 0xf>(v);
+  v = res_dpp_add<0x114, 0xf>(v);
+  v = res_dpp_add<0x118, 0xf>(v);
+  v = res_dpp_add<0x142, 0xa>(v);
+  v = res_dpp_add<0x143, 0xc>(v);
+  return res_lane(v, 63);
+}
+// Sums of up to three sets of kResWaves wave partials at once: set r in lanes 16r..16r+7 of
+// a row, summed by row shifts 1, 2, 4 (lane 16r + 7 then holds it).
+static_assert(kResWaves <= 8, "one row of 16 lanes sums the wave partials by shifts 1, 2, 4");
+__device__ __forceinline__ double res_sum8(double v) {
+  v = res_dpp_add<0x111, 0xf>(v);
+  v = res_dpp_add<0x112, 0xf>(v);
+  v = res_dpp_add<0x114, 0xf>(v);
+  return v;
+}
+
+// Sum of a double over the workgroup in a fixed order (wave sums, then the kResWaves wave
+// partials); every thread receives it. sh holds kResWaves doubles; ends with a barrier.
 __device__ __forceinline__ double res_sum(double v, double* sh) {
-  v = wave_sum(v);
+  v = res_wave_sum(v);
   if ((threadIdx.x & (kWave - 1)) == 0) sh[threadIdx.x / kWave] = v;
   __syncthreads();
-  const double t = res_gather(sh);
+  const int lane = threadIdx.x & (kWave - 1);
+  const double t = res_lane(res_sum8(lane < kResWaves ? sh[lane] : 0.0), kResWaves - 1);
   __syncthreads();
   return t;
 }
@@ -600,18 +633,20 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
                 });
     }
     // the three sums in one exchange, in a fixed order
-    v0 = wave_sum(v0);
-    v1 = wave_sum(v1);
-    v2 = wave_sum(v2);
+    v0 = res_wave_sum(v0);
+    v1 = res_wave_sum(v1);
+    v2 = res_wave_sum(v2);
     if (lane == 0) {
       sh[0][wave] = v0;
       sh[1][wave] = v1;
       sh[2][wave] = v2;
     }
     __syncthreads();
-    const double rr = res_gather(sh[0]);
-    const double gu = res_gather(sh[1]);
-    const double g2w = res_gather(sh[2]);
+    const int r = lane >> 4, l16 = lane & 15;
+    const double part = res_sum8(r < 3 && l16 < kResWaves ? sh[r < 3 ? r : 0][l16 < kResWaves ? l16 : 0] : 0.0);
+    const double rr = res_lane(part, kResWaves - 1);
+    const double gu = res_lane(part, 16 + kResWaves - 1);
+    const double g2w = res_lane(part, 32 + kResWaves - 1);
     __syncthreads();
     const double a2 = AA + 2.0 * gu + g2w;
     const double rms_diff = sqrt(rr / n_elem);
