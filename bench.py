@@ -36,6 +36,8 @@ METRIC = "mode-steps/s vs plain FP64 GPU at ≤1e-4 rel L2; p50/p99 solve latenc
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
+BF16_PEAK_TFLOPS = 2500.0
+N_CU = 256
 
 
 def log(msg):
@@ -47,12 +49,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--plain-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=25)
     ap.add_argument("--cpu-cuboids", type=int, default=64)
-    ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--serve-rate", type=float, default=20.0)
-    ap.add_argument("--serve-sweep", default="10,40,160",
+    ap.add_argument("--serve-sweep", default="40,160,640,2560",
                     help="comma-separated arrival rates (Hz) for the latency/throughput sweep; "
                          "empty to skip")
     ap.add_argument("--serve-tight-mib", type=int, default=48,
@@ -71,6 +73,9 @@ def parse():
                          "kernel that calibrates HBM byte counters under a profiler)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default surface-transform variant instead of autotuning")
+    ap.add_argument("--variant", default="",
+                    help="ROWS,THREADS,LDS_BYTES: run the mixed engine with this variant instead "
+                         "of autotuning it (rows -1: the resident step kernel; for profiling)")
     return ap.parse_args()
 
 
@@ -80,6 +85,17 @@ def surface_flops_per_launch(ctxs):
     for c in ctxs:
         nx, ny = int(c.geom.n.x), int(c.geom.n.y)
         tot += 2 * nx * ny * (nx + ny)
+    return tot
+
+
+def resident_flops_per_launch(ctxs, iters):
+    """bf16 MFMA flops of one resident step launch: per cuboid 2 + 2·n_it 2-D transforms
+    (warm start forward, inverse + forward per Picard iteration, step-end inverse), each
+    2·S·(nx + ny) FP32-equivalent flops done as three bf16 products (hi·hi + hi·lo + lo·hi)."""
+    tot = 0.0
+    for c, it in zip(ctxs, iters):
+        nx, ny = int(c.geom.n.x), int(c.geom.n.y)
+        tot += 3 * 2 * nx * ny * (nx + ny) * (2 + 2 * float(it))
     return tot
 
 
@@ -242,16 +258,21 @@ def main():
     tuning = {}
     if not args.no_tune:
         from spectral_galerkin_heat_amd.engine.autotune import autotune
-        for prec in ("mixed", "fp64"):
+        for prec in (("fp64",) if args.variant else ("mixed", "fp64")):
             tuning[prec] = autotune(ctxs, prec)
             log(f"autotune {prec}: {tuning[prec].summary()}")
             for row in sorted(tuning[prec].table, key=lambda r: r["ms_per_step"]):
                 log(f"    rows {row['rows']:2d} threads {row['threads']:4d} lds {row['lds_bytes'] >> 10:3d} KiB"
                     f"{' persistent' if row.get('loop') else ''}: {row['ms_per_step']:.4f} ms/step")
 
+    forced = tuple(int(v) for v in args.variant.split(",")) if args.variant else None
+
     def timed_engine(precision):
         tv = tuning.get(precision)
-        eng = BatchEngine(ctxs, precision=precision, variant=tv.summary() if tv else None)
+        variant = tv.summary() if tv else None
+        if forced is not None and precision == "mixed":
+            variant = forced
+        eng = BatchEngine(ctxs, precision=precision, variant=variant)
         eng.run(W)
         eng.synchronize()
         torch.cuda.synchronize()
@@ -329,7 +350,25 @@ def main():
     flops = surface_flops_per_launch(ctxs)
     sbytes = surface_bytes_per_launch(ctxs, 4)
     vbytes = volume_bytes_per_launch(ctxs, 4)
-    if share_surf >= share_vol:
+    resident = tm.get("surface_launches", 0) == instr and eng_mx.variant["rows"] < 0
+    if resident and share_surf >= share_vol:
+        # one launch per step, one workgroup (one CU) per cuboid: the bound is the matrix
+        # cores of the CUs it occupies; iterations per cuboid from the same segment's metrics
+        iters = eng_mx.metrics()[W + K:W + K + instr, :, 2].mean(axis=0)
+        rflops = resident_flops_per_launch(ctxs, iters)
+        tfs = rflops / (surf_ms * 1e-3) / 1e12
+        active = min(len(ctxs), N_CU) / N_CU
+        roofline = {"kernel": "k_b_resident (one workgroup per cuboid: laser flux, warm start, "
+                              "every Picard iteration and the step end in one launch; "
+                              "split-bf16 MFMA transforms)",
+                    "bound": "mfma", "achieved": round(tfs, 3), "peak": BF16_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(tfs / BF16_PEAK_TFLOPS, 5),
+                    "traffic": None, "avg_launch_us": round(surf_ms * 1e3, 2),
+                    "bf16_flops_per_launch": rflops,
+                    "picard_iterations_per_cuboid_mean": round(float(iters.mean()), 2),
+                    "active_cu_fraction": round(active, 4),
+                    "frac_of_active_cus": round(tfs / (BF16_PEAK_TFLOPS * active), 5)}
+    elif share_surf >= share_vol:
         # the binding roofline of the surface launches is the larger of the two lower
         # bounds on their time: algorithmic bytes at HBM peak, algorithmic flops at MFMA peak
         t_hbm = sbytes / (HBM_PEAK_GBS * 1e9)
@@ -458,7 +497,7 @@ def main():
                 ctx = make_context(cfg, lp)
                 reqs.append(Request(rid=i, ctx=ctx, n_steps=int(ctx.num.n_steps),
                                     arrival=float(arr[i])))
-            srv = ScanServer(precision="mixed", max_cuboids=128, hbm_budget_bytes=budget,
+            srv = ScanServer(precision="mixed", max_cuboids=256, hbm_budget_bytes=budget,
                              max_steps=2048, steps_per_tick=8, check_every=20)
             rep = srv.serve(reqs)
             fp64 = srv.fp64_checks(rep) if checks else []
@@ -502,7 +541,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (state, transforms, forcing) + f64 (E/KK tables, Picard norms, reductions)",
+        "dtype": ("f32 (state, fields) + split-bf16 MFMA transforms with f32 accumulation + f64 "
+                  "(E/KK tables, Picard norms, reductions)" if roofline["kernel"].startswith("k_b_resident")
+                  else "f32 (state, transforms, forcing) + f64 (E/KK tables, Picard norms, reductions)"),
         "data": "synthetic",
         "config": {
             "workload": "64 uneven cuboids in one batch (seed 0), constant-property single "

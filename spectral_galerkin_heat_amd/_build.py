@@ -38,15 +38,20 @@ def sources() -> list[str]:
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
-def build(verbose: bool = False, jobs: int = 4) -> str:
-    """Compile every ``csrc/*.hip`` and link ``libsgh_mi355x.so``; return its path."""
+def build(verbose: bool = False, jobs: int = 4, defines=(), libdir: str | None = None) -> str:
+    """Compile every ``csrc/*.hip`` and link ``libsgh_mi355x.so``; return its path.
+
+    ``defines`` (e.g. ``("SGH_RESIDENT_TIMESTAMPS",)``) and ``libdir`` build a diagnostic
+    library into a directory of its own, leaving the product library alone."""
     hipcc = _hipcc()
-    objdir = os.path.join(HERE, "_build")
+    out_lib = LIB_PATH if libdir is None else os.path.join(libdir, LIB_NAME)
+    objdir = os.path.join(HERE, "_build") if libdir is None else os.path.join(libdir, "_build")
     os.makedirs(objdir, exist_ok=True)
+    dflags = [f"-D{d}" for d in defines]
 
     def compile_one(src):
         obj = os.path.join(objdir, os.path.basename(src) + ".o")
-        cmd = [hipcc, *HIP_FLAGS, "-c", src, "-o", obj]
+        cmd = [hipcc, *HIP_FLAGS, *dflags, "-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         r = subprocess.run(cmd, capture_output=True, text=True)
@@ -56,15 +61,15 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
 
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(compile_one, sources()))
-    tmp = LIB_PATH + ".tmp"
+    tmp = out_lib + ".tmp"
     cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
-    os.replace(tmp, LIB_PATH)
-    return LIB_PATH
+    os.replace(tmp, out_lib)
+    return out_lib
 
 
 if __name__ == "__main__":

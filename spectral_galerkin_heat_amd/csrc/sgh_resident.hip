@@ -33,6 +33,23 @@ namespace sgh {
 // per-pass launches.
 // ---------------------------------------------------------------------------------
 constexpr int kResWaves = kResThreads / kWave;
+
+#ifdef SGH_RESIDENT_TIMESTAMPS
+// Diagnostic build only: per-workgroup cycle counts (s_memtime) of the last launch. Slot 0
+// entry, 1 step preparation done, 2 warm start done, 3 + 5k + {0, 1, 2, 3, 4} for Picard
+// iteration k < 20: first inverse stage, second inverse stage + evaporation, first forward
+// stage, second forward stage + relaxation, norm exchange done; 103 step end done; 104, 105
+// the cuboid's nx, ny.
+constexpr int kResTsSlots = 106, kResTsBlocks = 1024;
+__device__ unsigned long long g_res_ts[kResTsBlocks * kResTsSlots];
+#define RES_TS(slot)                                                                        \
+  do {                                                                                      \
+    if (threadIdx.x == 0 && blockIdx.x < kResTsBlocks)                                      \
+      g_res_ts[blockIdx.x * kResTsSlots + (slot)] = __builtin_amdgcn_s_memtime();          \
+  } while (0)
+#else
+#define RES_TS(slot) do {} while (0)
+#endif
 constexpr int kResBuf = kResMaxN * kResLd;   // bf16 elements of one hi or lo operand buffer
 
 typedef __bf16 bf4v __attribute__((ext_vector_type(4)));
@@ -324,6 +341,7 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
     if (threadIdx.x == 0) P.done_at[c] = 0x7fffffff;   // the volume pass leaves it alone
     return;
   }
+  RES_TS(0);
   __bf16* lds = reinterpret_cast<__bf16*>(smem_raw);
   const ResBuf buf0{lds, lds + kResBuf}, buf1{lds + 2 * kResBuf, lds + 3 * kResBuf};
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
@@ -466,6 +484,7 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
     }
   }
   const double P_sum = res_sum(psum, sh[1]);   // its barriers also complete the operand stores
+  RES_TS(1);
 
   const double fwd_scale = d.dx * d.dy * d.inv_sqrt_LxLy;
   const double inv_scale = d.inv_sqrt_LxLy;
@@ -536,6 +555,7 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
               });
   }
 
+  RES_TS(2);
   // ---- Picard loop ----
   int n_it = 0;
   bool conv = false;
@@ -553,6 +573,7 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
   for (int k = 0; k < max_it; ++k) {
     // inverse + evaporation: q = q_las − q_evap(T_s) into buffer 0
     stage_inv1();
+    if (k < 20) RES_TS(3 + 5 * k);
     {
       const int nxk = opaque(nx), nyk = opaque(ny);
       f4v ql[2];
@@ -592,9 +613,11 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
                   res_store(buf0, tB.aout(q, lane), tB.tcol(lane), qo);   // [i][j]
                 });
     }
+    if (k < 20) RES_TS(4 + 5 * k);
     // forward + relaxation + norm partials (FP32 per tile, FP64 over tiles); Xinv into
     // buffer 0
     stage_fwd1();
+    if (k < 20) RES_TS(5 + 5 * k);
     double v0 = 0.0, v1 = 0.0, v2 = 0.0;
     {
       const int nxk = opaque(nx), nyk = opaque(ny);
@@ -632,6 +655,7 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
                   res_store(buf0, tB.aout(q, lane), tB.tcol(lane), xo);   // [m][n]
                 });
     }
+    if (k < 20) RES_TS(6 + 5 * k);
     // the three sums in one exchange, in a fixed order
     v0 = res_wave_sum(v0);
     v1 = res_wave_sum(v1);
@@ -654,6 +678,7 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
     rel = rms_diff / (rms_old > 1e-9 ? rms_old : 1e-9);
     conv = rel < tol;
     n_it = k + 1;
+    if (k < 20) RES_TS(7 + 5 * k);
     if (conv) break;
   }
 
@@ -681,6 +706,13 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
   tmax = wave_max(tmax);
   if (lane == 0) sh[0][wave] = tmax;
   __syncthreads();
+  RES_TS(103);
+#ifdef SGH_RESIDENT_TIMESTAMPS
+  if (tid == 0 && blockIdx.x < kResTsBlocks) {
+    g_res_ts[blockIdx.x * kResTsSlots + 104] = (unsigned long long)nx;
+    g_res_ts[blockIdx.x * kResTsSlots + 105] = (unsigned long long)ny;
+  }
+#endif
   if (tid == 0) {
     double tm_all = -1.0 / 0.0;
     for (int w = 0; w < kResWaves; ++w) tm_all = sh[0][w] > tm_all ? sh[0][w] : tm_all;
@@ -715,3 +747,14 @@ hipError_t resident_allow_lds() {
 }
 
 }  // namespace sgh
+
+#ifdef SGH_RESIDENT_TIMESTAMPS
+// Copies the stamps of the last resident launch (kResTsBlocks × kResTsSlots values).
+extern "C" int sgh_debug_resident_timestamps(unsigned long long* host_out) {
+  return hipMemcpyFromSymbol(host_out, HIP_SYMBOL(sgh::g_res_ts),
+                             sizeof(unsigned long long) * sgh::kResTsBlocks * sgh::kResTsSlots) ==
+                 hipSuccess
+             ? 0
+             : -2;
+}
+#endif
