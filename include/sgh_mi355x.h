@@ -349,7 +349,11 @@ int sgh_batch_info(void* plan, long long* out6);
  * and returns how many exist.  sgh_batch_set_variant synchronises `stream`; the work
  * lists and the captured step graph are rebuilt by the next sgh_batch_run.  Results
  * are deterministic for a given variant; different variants sum per-item partials in
- * different groupings (last-bit differences). */
+ * different groupings (last-bit differences).  rows = -1 (threads 512) is the resident
+ * step kernel: FP32 plans only (refused for FP64), one workgroup per cuboid runs the
+ * whole surface part of a step in one launch with split-bf16 MFMA transforms; steps in
+ * which a live cuboid is wider than 128 use 16-row per-pass items with the given LDS
+ * budget instead.  sgh_batch_get_variant reports rows -1 while it is selected. */
 int sgh_batch_variants(int* out_pairs, int capacity);
 int sgh_batch_set_variant(void* plan, int rows, int threads, int lds_bytes, void* stream);
 /* out3 = {rows, threads, LDS bytes}. */
