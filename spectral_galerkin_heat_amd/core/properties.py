@@ -70,7 +70,9 @@ class TempProperty:
 
 # Layout of the packed device program (must match SghPropProgram in sgh_props.h):
 #   int32 len[6]; int32 ops[6][MAX_OPS]; double consts[6][MAX_CONSTS];
-#   double t_sol; double inv_band; int32 degenerate; int32 pad
+#   double t_sol; double inv_band; int32 degenerate; int32 same
+#   (bit b of `same`: the solid and liquid programs of property b — k, rho, c — are
+#   identical, so the device evaluates one and blends it with itself)
 # Branch order: k_solid, k_liquid, rho_solid, rho_liquid, c_solid, c_liquid.
 PROGRAM_BYTES = 6 * 4 + 6 * MAX_OPS * 4 + 6 * MAX_CONSTS * 8 + 8 + 8 + 4 + 4
 
@@ -86,12 +88,19 @@ def pack_property_program(exprs, t_sol: float, t_liq: float) -> bytes:
         consts_all.append(consts + [0.0] * (MAX_CONSTS - len(consts)))
     degenerate = t_liq <= t_sol
     inv_band = 0.0 if degenerate else 1.0 / (t_liq - t_sol)
+    same = 0
+    for b in range(3):
+        s_, l_ = 2 * b, 2 * b + 1
+        if (lens[s_] == lens[l_] and ops_all[s_] == ops_all[l_]
+                and struct.pack(f"<{MAX_CONSTS}d", *consts_all[s_]) ==
+                struct.pack(f"<{MAX_CONSTS}d", *consts_all[l_])):
+            same |= 1 << b
     blob = struct.pack("<6i", *lens)
     for ops in ops_all:
         blob += struct.pack(f"<{MAX_OPS}i", *ops)
     for consts in consts_all:
         blob += struct.pack(f"<{MAX_CONSTS}d", *consts)
-    blob += struct.pack("<ddii", float(t_sol), float(inv_band), int(degenerate), 0)
+    blob += struct.pack("<ddii", float(t_sol), float(inv_band), int(degenerate), same)
     assert len(blob) == PROGRAM_BYTES
     return blob
 

@@ -26,7 +26,7 @@ struct PropProgram {
   double t_sol;
   double inv_band;
   int32_t degenerate;
-  int32_t pad;
+  int32_t same;   // bit b: solid and liquid programs of property b (k, rho, c) identical
 };
 static_assert(sizeof(PropProgram) == 2736, "property program layout");
 
@@ -87,37 +87,55 @@ __device__ __forceinline__ F mfn(int f, F x) {
 constexpr int kProgThreads = 256;
 
 // Evaluate branch `w` of the program at temperature T (in precision F). Kernels that call
-// it run kProgThreads threads per block.
+// it run kProgThreads threads per block. The stack depth after each op depends on the
+// program only (it is uniform), so the top four entries live in registers (t0 the top) and
+// only deeper entries in LDS: an expression of depth ≤ 4 runs without memory traffic, each
+// op a few register moves, where a stack in LDS put a dependent LDS round trip in every op.
+// Every op computes exactly what the LDS stack computed (same operands, same order).
 template <typename F>
 __device__ F prog_eval(const PropProgram* __restrict__ P, int w, F T) {
   __shared__ F stk[kProgMaxStack * kProgThreads];
   F* st = stk + threadIdx.x;
+  F t0 = F(0), t1 = F(0), t2 = F(0), t3 = F(0);
   int sp = 0;
   const int n = P->len[w];
   const int32_t* ops = P->ops[w];
-#define SGH_ST(k) st[(k) * kProgThreads]
+  auto push = [&](F v) __attribute__((always_inline)) {
+    if (sp >= 4) st[(sp - 4) * kProgThreads] = t3;
+    t3 = t2;
+    t2 = t1;
+    t1 = t0;
+    t0 = v;
+    ++sp;
+  };
+  // the top two entries replaced by r
+  auto pop2 = [&](F r) __attribute__((always_inline)) {
+    t0 = r;
+    t1 = t2;
+    t2 = t3;
+    --sp;
+    if (sp >= 4) t3 = st[(sp - 4) * kProgThreads];
+  };
   for (int i = 0; i < n; ++i) {
     const int op = ops[i];
     switch (op) {
-      case OP_CONST: SGH_ST(sp) = (F)P->consts[w][ops[++i]]; ++sp; break;
-      case OP_T: SGH_ST(sp) = T; ++sp; break;
-      case OP_ADD: --sp; SGH_ST(sp - 1) = SGH_ST(sp - 1) + SGH_ST(sp); break;
-      case OP_SUB: --sp; SGH_ST(sp - 1) = SGH_ST(sp - 1) - SGH_ST(sp); break;
-      case OP_MUL: --sp; SGH_ST(sp - 1) = SGH_ST(sp - 1) * SGH_ST(sp); break;
-      case OP_DIV: --sp; SGH_ST(sp - 1) = SGH_ST(sp - 1) / SGH_ST(sp); break;
-      case OP_POW: --sp; SGH_ST(sp - 1) = mpow<F>(SGH_ST(sp - 1), SGH_ST(sp)); break;
-      case OP_NEG: SGH_ST(sp - 1) = -SGH_ST(sp - 1); break;
-      case OP_FN: { const int f = ops[++i]; SGH_ST(sp - 1) = mfn<F>(f, SGH_ST(sp - 1)); } break;
-      case OP_ATAN2: --sp; SGH_ST(sp - 1) = matan2<F>(SGH_ST(sp - 1), SGH_ST(sp)); break;
-      case OP_SQR: { const F v = SGH_ST(sp - 1); SGH_ST(sp - 1) = v * v; } break;
-      case OP_CUBE: { const F v = SGH_ST(sp - 1); SGH_ST(sp - 1) = v * v * v; } break;
-      case OP_RECIP: SGH_ST(sp - 1) = F(1) / SGH_ST(sp - 1); break;
+      case OP_CONST: push((F)P->consts[w][ops[++i]]); break;
+      case OP_T: push(T); break;
+      case OP_ADD: pop2(t1 + t0); break;
+      case OP_SUB: pop2(t1 - t0); break;
+      case OP_MUL: pop2(t1 * t0); break;
+      case OP_DIV: pop2(t1 / t0); break;
+      case OP_POW: pop2(mpow<F>(t1, t0)); break;
+      case OP_NEG: t0 = -t0; break;
+      case OP_FN: { const int f = ops[++i]; t0 = mfn<F>(f, t0); } break;
+      case OP_ATAN2: pop2(matan2<F>(t1, t0)); break;
+      case OP_SQR: t0 = t0 * t0; break;
+      case OP_CUBE: t0 = t0 * t0 * t0; break;
+      case OP_RECIP: t0 = F(1) / t0; break;
       default: break;
     }
   }
-  const F r = sp > 0 ? SGH_ST(0) : F(0);
-#undef SGH_ST
-  return r;
+  return sp > 0 ? t0 : F(0);
 }
 
 // Liquid fraction clamped to [0, 1]; a step at t_sol when the band is degenerate.
@@ -129,8 +147,14 @@ __device__ __forceinline__ F prog_fl(const PropProgram* __restrict__ P, F T) {
 }
 
 // Blended property (1 - f_l)·solid + f_l·liquid for branch pair `solid_idx`.
+// With identical solid and liquid programs the one value is blended with itself: the same
+// arithmetic as evaluating both, with half the interpretation.
 template <typename F>
 __device__ __forceinline__ F prog_blend(const PropProgram* __restrict__ P, int solid_idx, F T, F fl) {
+  if ((P->same >> (solid_idx >> 1)) & 1) {
+    const F v = prog_eval<F>(P, solid_idx, T);
+    return (F(1) - fl) * v + fl * v;
+  }
   return (F(1) - fl) * prog_eval<F>(P, solid_idx, T) + fl * prog_eval<F>(P, solid_idx + 1, T);
 }
 
