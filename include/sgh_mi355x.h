@@ -374,6 +374,20 @@ int sgh_batch_set_loop(void* plan, int enable, void* stream);
  * any wait time out at once, which tests use to exercise the error path). */
 int sgh_batch_set_loop_timeout(void* plan, double seconds, void* stream);
 int sgh_batch_loop_state(void* plan, int* out2);
+/* Resident teams (no reference counterpart; the reference has no batched engine).  With
+ * the resident step kernel, a cuboid's step runs on a team of 1 to team_max workgroups
+ * (1..8, default 8; at most one per 16 rows of the surface) that split every stage by rows
+ * and exchange the transformed fields through device memory twice per Picard iteration.
+ * The sizes are chosen per work list so that the longest team step is as short as the CUs
+ * allow (teams only while the batch fits one workgroup per CU).  Every sum is formed per
+ * tile and combined in a fixed order, so results are bit-identical for every team size.
+ * A team barrier's wait is bounded by the sgh_batch_set_loop_timeout bound and reported
+ * like the persistent kernel's.  sgh_batch_set_team_max synchronises `stream`; the work
+ * lists and step graph are rebuilt by the next run.  sgh_batch_team_sizes writes the team
+ * size of each slot in the last work lists (0: free slot, or the resident kernel not in
+ * use) to out[0 .. capacity) and returns the slot count. */
+int sgh_batch_set_team_max(void* plan, int team_max, void* stream);
+int sgh_batch_team_sizes(void* plan, int* out, int capacity);
 /* Step counters, Picard status and surface fields of every live cuboid back to zero
  * (synchronises `stream`); the modal states are kept: set them with
  * sgh_batch_set_state before the next run to restart from an initial condition. */

@@ -84,6 +84,8 @@ SIGNATURES = {
     "sgh_batch_set_loop": [vp, i32, vp],
     "sgh_batch_set_loop_timeout": [vp, f64, vp],
     "sgh_batch_loop_state": [vp, vp],
+    "sgh_batch_set_team_max": [vp, i32, vp],
+    "sgh_batch_team_sizes": [vp, vp, i32],
 }
 
 

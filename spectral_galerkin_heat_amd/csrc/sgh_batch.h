@@ -101,24 +101,69 @@ struct BatchPtrs {
                            // for the resident step kernel; null otherwise
   float* res_work;         // FP32 pools: per cuboid slot, kResWork floats of the resident
                            // step kernel's tiled G, q_evap and q_las; null otherwise
+  unsigned char* res_xch;  // FP32 pools: per cuboid slot, kResXchBytes through which the
+                           // workgroups of a resident team exchange fields and sums
   int metrics_rows;        // capacity in steps
   int n_cuboids;
 };
 
-// Resident step kernel (sgh_resident.hip): one 1024-thread workgroup per cuboid, mixed
-// precision, surfaces up to kResMaxN × kResMaxN.
+// Resident step kernel (sgh_resident.hip): a team of 1 to kResTeamMax 512-thread
+// workgroups per cuboid, mixed precision, surfaces up to kResMaxN × kResMaxN.
 constexpr int kResMaxN = 128;            // widest surface axis
 constexpr int kResLd = kResMaxN + 8;     // bf16 per LDS operand row: 16-byte rows, staggered banks
 constexpr int kResThreads = 512;
 constexpr int kResLdsBytes = 4 * kResMaxN * kResLd * 2;   // two hi / lo operand buffer pairs
 constexpr int kResWork = 3 * kResMaxN * kResMaxN;          // work floats per cuboid slot
+constexpr int kResTeamMax = 8;           // workgroups per cuboid (one per 16-row band at most)
+constexpr int kResTiles = 64;            // 16×16 tiles of a kResMaxN² surface
+constexpr int kResNormSlots = 4 * kResTiles;   // norm sums: one per 4-row group of a tile
+// Exchange area of a cuboid slot: the X and q operand fields (hi, lo; LDS layout), the
+// norm sums (3 × kResNormSlots floats) and one T_max per member (doubles), 256-byte aligned.
+constexpr long long kResXchField = 2LL * kResMaxN * kResLd;   // bf16 elements of a hi+lo field
+constexpr long long kResXchNorm = 2 * kResXchField * 2;        // byte offset of the norm sums
+constexpr long long kResXchBytes =
+    (kResXchNorm + 4LL * 3 * kResNormSlots + 8LL * kResTeamMax + 255) / 256 * 256;
 
 __host__ __device__ inline int pad32(int n) { return (n + 31) & ~31; }
 
-// Enqueue the resident step kernel over `nblocks` entries of `list` (cuboid per workgroup,
-// −1 idle); m16 holds the split-bf16 cosine tables the descriptors point into.
+// Work-list entry of member p of a team of tp workgroups for cuboid c (−1: idle).
+__host__ __device__ inline int res_entry(int c, int tp, int p) { return c * 64 + (tp - 1) * 8 + p; }
+
+// Barrier of the workgroups of one cuboid (the items of the persistent step kernel, the
+// members of a resident team). What they exchange is written and read device-coherently
+// (sc1 stores and loads, LDS DMA with sc1), so no cache flush is needed: every wave waits
+// until its stores have completed, the workgroup meets, and one thread adds its arrival to
+// the cuboid's counter (a device-scope atomic) and, unless `wait` is false, polls it with
+// device-scope loads until `target` arrivals. (Device-scope release/acquire fences
+// instead write back and invalidate the XCD's L2 at every barrier: measured 2.5× slower.)
+// The wait is bounded: past `ticks` of the 100 MHz wall clock (or once another barrier has
+// timed out) it records the failure in *err and goes on, so the kernel always ends; the
+// host reports it.
+constexpr unsigned long long kBarrierTicks = 2000000ULL;   // default: 20 ms
+__device__ __forceinline__ void cuboid_barrier(unsigned* cnt, unsigned target, int* err,
+                                               unsigned long long ticks, bool wait = true) {
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long t0 = wall_clock64();
+    while (wait && __hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+      if (wall_clock64() - t0 > ticks) {
+        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+  }
+  __syncthreads();
+}
+
+// Enqueue the resident step kernel over `nblocks` entries of `list` (res_entry per
+// workgroup, −1 idle); m16 holds the split-bf16 cosine tables the descriptors point into;
+// `ticks` bounds a team barrier's wait.
 hipError_t launch_resident(const BatchPtrs<float>& P, const void* m16, const int* list,
-                           unsigned nblocks, hipStream_t st);
+                           unsigned nblocks, unsigned long long ticks, hipStream_t st);
 // Allow the resident kernel its dynamic LDS (once per process is enough).
 hipError_t resident_allow_lds();
 

@@ -154,7 +154,11 @@ __global__ void __launch_bounds__(kBT) k_b_volume(BatchPtrs<F> P, int mode) {
     if (finalize) P.q_evap_old[d.s_off + s] = P.q_evap[d.s_off + s];
   }
   const double t = block_sum(aa, sh);
-  if (threadIdx.x == 0) P.col_slots[d.col_item0 + it.blk] = t;
+  if (threadIdx.x == 0) {
+    P.col_slots[d.col_item0 + it.blk] = t;
+    // after a resident step: the cuboid's team barrier counter back to zero for the next
+    if (mode == 2 && it.blk == 0) P.bar[it.c] = 0u;
+  }
 }
 
 // ---------------------------------------------------------------------------------
@@ -337,35 +341,6 @@ __global__ void __launch_bounds__(NT) k_b_surface_hts(BatchPtrs<F> P, int k, int
   }
 }
 
-// Barrier of the items of one cuboid inside the persistent step kernel. What the items
-// exchange (X fields, norm slots) is written and read device-coherently (sc1 stores and
-// loads, LDS DMA with sc1), so no cache flush is needed: every wave waits until its
-// stores have completed, the workgroup meets, and one thread adds its arrival to the
-// cuboid's counter (a device-scope atomic) and polls it with device-scope loads until
-// `target` arrivals. (Device-scope release/acquire fences instead write back and
-// invalidate the XCD's L2 at every barrier: measured 2.5× slower.) The wait is bounded:
-// past `ticks` of the 100 MHz wall clock (or once another barrier has timed out) it
-// records the failure in *err and goes on, so the kernel always ends; the host reports it.
-constexpr unsigned long long kBarrierTicks = 2000000ULL;   // default: 20 ms
-__device__ __forceinline__ void cuboid_barrier(unsigned* cnt, unsigned target, int* err,
-                                               unsigned long long ticks) {
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned long long t0 = wall_clock64();
-    while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-      if (wall_clock64() - t0 > ticks) {
-        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        break;
-      }
-      __builtin_amdgcn_s_sleep(2);
-    }
-  }
-  __syncthreads();
-}
-
 // Persistent form of one step's surface work (one launch instead of 2·cap + 2): each
 // workgroup runs its item through the warm start, every Picard iteration (inverse pass,
 // cuboid barrier, forward pass, cuboid barrier) and the step-end pass, taking the Picard
@@ -545,10 +520,13 @@ struct Plan {
   void* mats16 = nullptr;
   void* zwu = nullptr;       // FP32: (ZE, Ztt, Wtt, U) per surface element, 16 bytes
   void* res_work = nullptr;  // FP32: kResWork floats per cuboid slot (resident kernel)
+  void* res_xch = nullptr;   // FP32: kResXchBytes per cuboid slot (resident teams)
   long long mats16_cap = 0;
   bool resident = false;     // variant rows = −1 requested
   bool res_on = false;       // ... and every live cuboid allows it (mixed precision, ≤ 128 wide)
-  std::vector<int> hres;     // resident work list: cuboid per workgroup (−1 idle)
+  std::vector<int> hres;     // resident work list: res_entry per workgroup (−1 idle)
+  std::vector<int> hteam;    // resident team size per slot (0: none) of the last lists
+  int team_max = kResTeamMax;  // largest resident team
   int* res_list = nullptr;
   long long res_cap = 0;
   int t1_stride = 0;
@@ -614,6 +592,7 @@ BatchPtrs<F> ptrs(Plan* p) {
   P.surf_stride = T;
   P.zwu = (float*)p->zwu;
   P.res_work = (float*)p->res_work;
+  P.res_xch = (unsigned char*)p->res_xch;
   P.mats = (const F*)p->mats;
   P.t1 = p->t1;
   P.laser = p->laser;
@@ -766,7 +745,8 @@ int enqueue_step(Plan* p, hipStream_t st) {
     if (p->res_on) {
       // the whole surface part of the step in one launch, then the volume pass
       if (tm) hipEventRecord(p->ev[0], st);
-      chk(launch_resident(P, p->mats16, p->res_list, (unsigned)p->hres.size(), st));
+      chk(launch_resident(P, p->mats16, p->res_list, (unsigned)p->hres.size(),
+                          p->barrier_ticks, st));
       if (tm) hipEventRecord(p->ev[1], st);
       if (tm) hipEventRecord(p->ev[2], st);
       hipLaunchKernelGGL(k_b_volume<F>, dim3(ncol), dim3(kBT), 0, st, P, 2);
@@ -934,9 +914,87 @@ static void build_xcd_items(Plan* p) {
   std::vector<Item> rl;
   interleave(res, rl);
   p->hres.clear();
-  for (const Item& it : rl) p->hres.push_back(it.c);
+  for (const Item& it : rl) p->hres.push_back(it.c < 0 ? -1 : res_entry(it.c, 1, 0));
+  p->hteam.assign(p->n, 0);
+  for (int c : order) p->hteam[c] = 1;
   p->max_it = 0;
   for (int c : order) p->max_it = std::max(p->max_it, p->hdesc[c].max_it);
+}
+
+// Resident teams (sgh_resident.hip): how many workgroups each cuboid's step runs on. The
+// cost of a member's step is modelled per Picard iteration as a fixed part plus a part
+// per 16×16 tile of each stage the member owns, plus the two field exchanges of a team
+// (cycles, from per-workgroup phase stamps of the kernel on MI355X). Starting from one
+// workgroup per cuboid, the cuboid whose step is the longest gets one more member while
+// that shortens it, the batch still fits one workgroup per CU (members wait for each other)
+// and the teams still fit the XCDs (a team's members run on one XCD). The work list then
+// places each team on one XCD (list positions ≡ xcd mod 8), largest teams first.
+static double team_cost(const CuboidDesc& d, int tp) {
+  const int ntb = pad32(d.nx) / 16, nay = pad32(d.ny) / 16;
+  const int rows = (nay + tp - 1) / tp;
+  return 8.0 + 0.7 * ntb * rows + (tp > 1 ? 6.0 : 0.0);
+}
+
+static bool place_teams(Plan* p, const std::vector<int>& order) {
+  std::vector<int> by(order);
+  std::stable_sort(by.begin(), by.end(), [&](int a, int b) {
+    const int ta = p->hteam[a], tb = p->hteam[b];
+    if (ta != tb) return ta > tb;
+    return team_cost(p->hdesc[a], ta) > team_cost(p->hdesc[b], tb);
+  });
+  std::vector<int> count(kXcds, 0);
+  std::vector<double> load(kXcds, 0.0);
+  std::vector<std::vector<int>> bins(kXcds);
+  for (int c : by) {
+    const int tp = p->hteam[c];
+    int x = -1;
+    for (int y = 0; y < kXcds; ++y)
+      if (count[y] + tp <= kCusPerXcd && (x < 0 || load[y] < load[x])) x = y;
+    if (x < 0) return false;
+    count[x] += tp;
+    load[x] += tp * team_cost(p->hdesc[c], tp);
+    for (int m = 0; m < tp; ++m) bins[x].push_back(res_entry(c, tp, m));
+  }
+  size_t rows = 0;
+  for (auto& b : bins) rows = std::max(rows, b.size());
+  p->hres.clear();
+  for (size_t r = 0; r < rows; ++r)
+    for (int x = 0; x < kXcds; ++x) p->hres.push_back(r < bins[x].size() ? bins[x][r] : -1);
+  return true;
+}
+
+static void assign_teams(Plan* p) {
+  std::vector<int> order;
+  for (int c = 0; c < p->n; ++c)
+    if (p->hdesc[c].live) order.push_back(c);
+  const long long slots = std::min<long long>(p->n_cu, (long long)kXcds * kCusPerXcd);
+  if (p->team_max <= 1 || order.empty() || (long long)order.size() > slots) return;
+  long long total = (long long)order.size();
+  for (;;) {
+    int worst = -1;
+    double wc = -1.0;
+    for (int c : order) {
+      const double v = team_cost(p->hdesc[c], p->hteam[c]);
+      if (v > wc) { wc = v; worst = c; }
+    }
+    // the next team size that gives its largest member fewer rows
+    const CuboidDesc& d = p->hdesc[worst];
+    const int tp = p->hteam[worst], nay = pad32(d.ny) / 16, lim = std::min(p->team_max, nay);
+    int nt = tp + 1;
+    while (nt <= lim && (nay + nt - 1) / nt == (nay + tp - 1) / tp) ++nt;
+    if (nt > lim || total + (nt - tp) > slots || team_cost(d, nt) >= wc) break;
+    p->hteam[worst] = nt;
+    total += nt - tp;
+  }
+  // a team assignment that does not pack into the XCDs gives up members, largest teams
+  // first
+  while (!place_teams(p, order)) {
+    int big = -1;
+    for (int c : order)
+      if (big < 0 || p->hteam[c] > p->hteam[big]) big = c;
+    if (p->hteam[big] <= 1) break;
+    --p->hteam[big];
+  }
 }
 
 static void drop_graph(Plan* p) {
@@ -955,7 +1013,7 @@ static void free_plan(Plan* p) {
                   p->col_items,
                   p->prime_items, p->row_slots, p->col_slots, p->aux_slots, p->done_at,
                   p->status, p->step_idx, p->metrics, p->bar, p->mats16, p->res_list,
-                  p->zwu, p->res_work};
+                  p->zwu, p->res_work, p->res_xch};
   for (void* b : bufs)
     if (b) hipFree(b);
   delete p;
@@ -1236,6 +1294,8 @@ static int rebuild_lists(Plan* p, hipStream_t st) {
   for (int c = 0; c < p->n && p->res_on; ++c)
     if (p->hdesc[c].live && (p->hdesc[c].bx16_off < 0 || p->hdesc[c].by16_off < 0))
       p->res_on = false;
+  if (p->res_on) assign_teams(p);
+  else p->hteam.assign(p->n, 0);
   if ((long long)p->hres.size() > p->res_cap) {
     set_error("sgh_batch: resident work list exceeds capacity");
     return SGH_ERR_STATE;
@@ -1302,7 +1362,7 @@ int sgh_batch_pool_create(int dtype, int max_cuboids, long long state_capacity,
   p->arena_m.init(p->mats_cap);
   p->mats16_cap = 4LL * 16 * kResMaxN * kResMaxN;   // 16 distinct sizes up to 128
   p->arena_m16.init(p->mats16_cap);
-  p->res_cap = (long long)kXcds * max_cuboids;
+  p->res_cap = (long long)kXcds * std::max(max_cuboids, kCusPerXcd);
   p->bud = surface_budget(dtype, kernel_rows(p->rows), kSurfLds);
   p->shm = (size_t)kSurfLds;
   if (p->bud < 0) {
@@ -1339,6 +1399,7 @@ int sgh_batch_pool_create(int dtype, int max_cuboids, long long state_capacity,
   if (dtype == SGH_F32) {
     chk(dmalloc(&p->zwu, (size_t)p->total_S * 4 * sizeof(float)));
     chk(dmalloc(&p->res_work, (size_t)max_cuboids * kResWork * sizeof(float)));
+    chk(dmalloc(&p->res_xch, (size_t)max_cuboids * kResXchBytes));
     chk(resident_allow_lds());
   }
   if (e == hipSuccess) e = hipMemset(p->bar, 0, ((size_t)max_cuboids + 1) * sizeof(unsigned));
@@ -1552,9 +1613,10 @@ static int check_loop_error(Plan* p, const char* where) {
   if (err) {
     const int zero = 0;
     hipMemcpy(p->bar + p->n, &zero, sizeof(int), hipMemcpyHostToDevice);
-    set_error("%s: a cuboid barrier of the persistent step kernel timed out (not every "
-              "workgroup was resident); the steps since the last check are invalid. Turn the "
-              "persistent kernel off with sgh_batch_set_loop(plan, 0, stream)", where);
+    set_error("%s: a cuboid barrier of the persistent step kernel or of a resident team "
+              "timed out (not every workgroup was resident); the steps since the last check "
+              "are invalid. Turn the persistent kernel off with sgh_batch_set_loop(plan, 0, "
+              "stream), teams with sgh_batch_set_team_max(plan, 1, stream)", where);
     return SGH_ERR_STATE;
   }
   return SGH_OK;
@@ -1612,6 +1674,28 @@ int sgh_batch_set_loop_timeout(void* plan, double seconds, void* stream) {
   p->barrier_ticks = (unsigned long long)(seconds * 1e8);
   p->dirty = true;  // the captured step graph holds the old bound
   return SGH_OK;
+}
+
+int sgh_batch_set_team_max(void* plan, int team_max, void* stream) {
+  Plan* p = static_cast<Plan*>(plan);
+  SGH_REQUIRE(p && team_max >= 1 && team_max <= kResTeamMax, SGH_ERR_ARG,
+              "sgh_batch_set_team_max: team_max must be in [1, %d]", kResTeamMax);
+  hipError_t e = hipStreamSynchronize(reinterpret_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return fail_hip(e, "sgh_batch_set_team_max");
+  if (p->team_max != team_max) {
+    p->team_max = team_max;
+    p->dirty = true;  // work lists and the step graph are rebuilt by the next run
+  }
+  return SGH_OK;
+}
+
+int sgh_batch_team_sizes(void* plan, int* out, int capacity) {
+  Plan* p = static_cast<Plan*>(plan);
+  SGH_REQUIRE(p && capacity >= 0 && (out || capacity == 0), SGH_ERR_ARG,
+              "sgh_batch_team_sizes: bad args");
+  for (int c = 0; c < capacity && c < p->n; ++c)
+    out[c] = c < (int)p->hteam.size() ? p->hteam[c] : 0;
+  return p->n;
 }
 
 int sgh_batch_loop_state(void* plan, int* out2) {

@@ -36,11 +36,12 @@ constexpr int kResWaves = kResThreads / kWave;
 
 #ifdef SGH_RESIDENT_TIMESTAMPS
 // Diagnostic build only: per-workgroup cycle counts (s_memtime) of the last launch. Slot 0
-// entry, 1 step preparation done, 2 warm start done, 3 + 5k + {0, 1, 2, 3, 4} for Picard
-// iteration k < 20: first inverse stage, second inverse stage + evaporation, first forward
-// stage, second forward stage + relaxation, norm exchange done; 103 step end done; 104, 105
-// the cuboid's nx, ny.
-constexpr int kResTsSlots = 106, kResTsBlocks = 1024;
+// entry, 1 step preparation done, 2 warm start done, 3 + 7k + {0, ..., 6} for Picard
+// iteration k < 20: first inverse stage, second inverse stage + evaporation, q exchange,
+// first forward stage, second forward stage + relaxation, X exchange, norm sums done (the
+// exchanges of a single-member team take no time); 143 step end done; 144, 145 the cuboid's
+// nx and ny, with the team size and the member index in bits 16 and up.
+constexpr int kResTsSlots = 146, kResTsBlocks = 1024;
 __device__ unsigned long long g_res_ts[kResTsBlocks * kResTsSlots];
 #define RES_TS(slot)                                                                        \
   do {                                                                                      \
@@ -53,6 +54,7 @@ __device__ unsigned long long g_res_ts[kResTsBlocks * kResTsSlots];
 constexpr int kResBuf = kResMaxN * kResLd;   // bf16 elements of one hi or lo operand buffer
 
 typedef __bf16 bf4v __attribute__((ext_vector_type(4)));
+typedef unsigned u4v __attribute__((ext_vector_type(4)));
 
 // A uniform value unchanged, but opaque to the optimizer: addresses derived from it are
 // formed where they are used instead of being hoisted out of the Picard loop and kept live
@@ -75,22 +77,26 @@ __device__ __forceinline__ int opaque_lane(int v) {
 }
 
 // Tiles of one wave in a stage producing O[a][t] as NT × NA 16×16 tiles (t: the index the
-// stage's matrix introduces, a: the field's free index). The waves are grouped per t tile
-// (g waves each), so a wave's tiles share the matrix fragments it holds; wave w takes t tile
-// w / g and a tiles (w mod g) + g·q, q < nq. tt < 0: no tiles.
+// stage's matrix introduces, a: the field's free index), the t tiles numbered from toff and
+// the a tiles from aoff (a team member's share of a stage). The waves are grouped per t
+// tile (g waves each), so a wave's tiles share the matrix fragments it holds; wave w takes
+// t tile toff + w / g and a tiles aoff + (w mod g) + g·q, q < nq. tt < 0: no tiles.
 struct ResTiles {
   int tt, a0, g, nq;
-  __device__ ResTiles(int NT, int NA, int wave) {
+  __device__ ResTiles(int NT, int NA, int wave, int toff, int aoff) {
     g = kResWaves / NT > 0 ? kResWaves / NT : 1;
     tt = wave / g;
     a0 = wave % g;
     nq = tt < NT && a0 < NA ? (NA - a0 + g - 1) / g : 0;
-    if (nq == 0) tt = -1;
+    tt = nq == 0 ? -1 : tt + toff;
+    a0 += aoff;
   }
+  // a tile of the wave's tile q
+  __device__ int atile(int q) const { return a0 + g * q; }
   // row a of the field whose k values lane `lane` reads for tile q
-  __device__ int arow(int q, int lane) const { return (a0 + g * q) * 16 + (lane & 15); }
+  __device__ int arow(int q, int lane) const { return atile(q) * 16 + (lane & 15); }
   // first of the four consecutive a the accumulator of tile q holds in lane `lane`
-  __device__ int aout(int q, int lane) const { return (a0 + g * q) * 16 + 4 * (lane >> 4); }
+  __device__ int aout(int q, int lane) const { return atile(q) * 16 + 4 * (lane >> 4); }
   // t of lane `lane`'s outputs
   __device__ int tcol(int lane) const { return tt * 16 + (lane & 15); }
 };
@@ -157,7 +163,8 @@ __device__ __forceinline__ void res_load_w(ResW& w, const ResTiles& t, const __b
 // has used the slot. next(k0, k1) loads slots [k0, k1) of the following stage's matrix
 // fragments: slots KS..3 at the start (this stage does not use them), the others after the
 // last tile's products, so that the loads overlap the last epilogues and the barrier.
-template <int KS, typename Next, typename Pre, typename Epi>
+// EARLY: the caller has started the loads of the first EARLY tiles (0, 1 or 2) earlier.
+template <int KS, int EARLY, typename Next, typename Pre, typename Epi>
 __device__ __forceinline__ void res_tiles(const ResTiles& t, ResW& w, const ResBuf& src,
                                           int lane, Next&& next, Pre&& pre, Epi&& epi) {
   constexpr int R = KS == 1 ? 2 : (KS == 3 ? 3 : 4);
@@ -187,11 +194,11 @@ __device__ __forceinline__ void res_tiles(const ResTiles& t, ResW& w, const ResB
     }
   };
   next(KS, 4);
-  pre(0, 0, lane);
+  if (EARLY < 1) pre(0, 0, lane);
 #pragma unroll
   for (int s = 0; s < AH; ++s) dload(s / KS, s % KS, s % R, lane);
   mfma(0, 0, lane);
-  pre(1, 1, lane);
+  if (EARLY < 2) pre(1, 1, lane);
   int q = 1;
 #pragma unroll 1
   for (; q + 1 < nq; q += 2) {
@@ -219,7 +226,7 @@ __device__ __forceinline__ void res_tiles(const ResTiles& t, ResW& w, const ResB
 // One stage: the wave's tiles (res_tiles, specialised on the number of k steps; it reloads the
 // matrix fragments for the following stage through next), then the barrier after which the
 // epilogue's stores are visible. A wave without tiles in this stage reloads them at once.
-template <typename Next, typename Pre, typename Epi>
+template <int EARLY = 0, typename Next, typename Pre, typename Epi>
 __device__ __forceinline__ void res_stage(const ResTiles& t, ResW& w, int KS, const ResBuf& src,
                                           int lane, Next&& next, Pre&& pre, Epi&& epi) {
   if (t.tt >= 0) {
@@ -230,10 +237,10 @@ __device__ __forceinline__ void res_stage(const ResTiles& t, ResW& w, int KS, co
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) asm volatile("" : "+v"(w.h[ks]), "+v"(w.l[ks]));
     switch (KS) {
-      case 1: res_tiles<1>(t, w, src, lane, next, pre, epi); break;
-      case 2: res_tiles<2>(t, w, src, lane, next, pre, epi); break;
-      case 3: res_tiles<3>(t, w, src, lane, next, pre, epi); break;
-      default: res_tiles<4>(t, w, src, lane, next, pre, epi); break;
+      case 1: res_tiles<1, EARLY>(t, w, src, lane, next, pre, epi); break;
+      case 2: res_tiles<2, EARLY>(t, w, src, lane, next, pre, epi); break;
+      case 3: res_tiles<3, EARLY>(t, w, src, lane, next, pre, epi); break;
+      default: res_tiles<4, EARLY>(t, w, src, lane, next, pre, epi); break;
     }
   } else {
     next(0, 4);
@@ -276,6 +283,20 @@ __device__ __forceinline__ double res_sum8(double v) {
   v = res_dpp_add<0x112, 0xf>(v);
   v = res_dpp_add<0x114, 0xf>(v);
   return v;
+}
+
+// Sum of a float over each row of 16 lanes by row shifts 1, 2, 4, 8 (lane 16r + 15 then
+// holds row r's sum), in FP32.
+template <int Ctrl>
+__device__ __forceinline__ float res_dpp_addf(float v) {
+  return v + __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
+                                           0, __builtin_bit_cast(int, v), Ctrl, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float res_row_sum(float v) {
+  v = res_dpp_addf<0x111>(v);
+  v = res_dpp_addf<0x112>(v);
+  v = res_dpp_addf<0x114>(v);
+  return res_dpp_addf<0x118>(v);
 }
 
 // Sum of a double over the workgroup in a fixed order (wave sums, then the kResWaves wave
@@ -328,17 +349,37 @@ __device__ __forceinline__ void res_voffs(const ResTiles& t, int q, int lane, in
   }
 }
 
+// Teams. A cuboid runs on a team of TP workgroups (members mp = 0 .. TP − 1; TP = 1 for most
+// cuboids, more for the largest surfaces so that the batch fills the CUs), each owning the
+// 16-row tile bands [a_lo, a_hi) along y, in point space (j) and in mode space (n). Of the
+// four stages of an iteration, the two whose matrix introduces y (the first stages, t
+// along y) compute the member's t tiles and read the whole field; the two with t along x
+// (the second stages, with the epilogues) compute the member's a tiles, whose operand rows
+// the member's own first stage wrote. So a member needs the others' data twice per
+// iteration, the q field before the forward transform and the X field before the inverse
+// one: after each of those epilogues every member stores its column band of the field to
+// the cuboid's exchange area, the team meets at a barrier (all members are resident: the
+// host gives teams only to batches that fit one workgroup per CU, members of a team on one
+// XCD), and every member copies the whole field into LDS. The per-tile norm sums go
+// through the exchange area as well. Every sum of the step is formed per 16×16 tile and the
+// tiles are combined in a fixed order (the laser power, AA: by every member over the whole
+// surface), so all members take the same Picard decision and a cuboid's arithmetic is the
+// same for every team size.
 __global__ void __launch_bounds__(kResThreads)
-k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __restrict__ list) {
+k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __restrict__ list,
+             unsigned long long ticks) {
   using F = float;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double sh[3][kResWaves];
-  const int c = list[blockIdx.x];
-  if (c < 0) return;
+  __shared__ __align__(16) float nsl[3][kResNormSlots];   // norm sums (single-member teams)
+  const int ent = list[blockIdx.x];
+  if (ent < 0) return;
+  const int c = ent >> 6, TP = ((ent >> 3) & 7) + 1, mp = ent & 7;
   const CuboidDesc& d = P.desc[c];
   const int step = P.step_idx[c];
   if (!d.live || step >= d.n_steps) {
-    if (threadIdx.x == 0) P.done_at[c] = 0x7fffffff;   // the volume pass leaves it alone
+    // the volume pass leaves it alone
+    if (threadIdx.x == 0 && mp == 0) P.done_at[c] = 0x7fffffff;
     return;
   }
   RES_TS(0);
@@ -348,18 +389,63 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
   const int nx = d.nx, ny = d.ny;
   const int nxp = pad32(nx), nyp = pad32(ny);
   const int KX = nxp / 32, KY = nyp / 32;
+  const int NTB = nxp / 16, NAY = nyp / 16;
+  // the member's y tile bands (the host keeps TP ≤ NAY: at least one each)
+  const int a_lo = mp * NAY / TP, a_hi = (mp + 1) * NAY / TP;
   // split-bf16 tables: B̂ hi, B̂ lo, B̂ᵀ hi, B̂ᵀ lo, each np × np
   const long long X16 = d.bx16_off, Y16 = d.by16_off;   // element offsets into m16
   const long long mx = (long long)nxp * nxp, my = (long long)nyp * nyp;
-  // tA: t along y (n or j), a along x (i or m): the first stage of each transform;
-  // tB: t along x (m or i), a along y (n or j): the second stage, with the epilogues
-  const ResTiles tA(nyp / 16, nxp / 16, wave);
-  const ResTiles tB(nxp / 16, nyp / 16, wave);
+  // tA: t along y (n or j), a along x (i or m): the first stage of each transform, the
+  // member's t tiles; tB: t along x (m or i), a along y (n or j): the second stage, with the
+  // epilogues, the member's a tiles
+  const ResTiles tA(a_hi - a_lo, NTB, wave, a_lo, 0);
+  const ResTiles tB(NTB, a_hi - a_lo, wave, 0, a_lo);
   // work-slot index (in 16-byte units) of this lane's outputs of tB tile q (clamped to the
   // wave's last tile)
   auto tidx = [&](int q, int ln) __attribute__((always_inline)) {
     const int qq = q < tB.nq ? q : tB.nq - 1;
-    return ((tB.a0 + tB.g * qq) * (nxp / 16) + tB.tt) * 64 + ln;
+    return (tB.atile(qq) * NTB + tB.tt) * 64 + ln;
+  };
+  // the cuboid's exchange area (teams): X and q fields in the LDS layout (hi, then lo), the
+  // per-tile norm sums [3][kResTiles], one T_max per member
+  unsigned char* const xch = P.res_xch + (long long)c * kResXchBytes;
+  const __amdgpu_buffer_rsrc_t xr =
+      __builtin_amdgcn_make_buffer_rsrc(xch, (short)0, (int)kResXchBytes, 0x00020000);
+  __bf16* const xX = reinterpret_cast<__bf16*>(xch);
+  __bf16* const xQ = xX + kResXchField;
+  float* const xN = reinterpret_cast<float*>(xch + kResXchNorm);
+  double* const xT = reinterpret_cast<double*>(xN + 3 * kResNormSlots);
+  unsigned arrivals = 0;
+  // (teams) the member's column band [16·a_lo, 16·a_hi) of every row < nxp of operand
+  // buffer b → exchange field g (16-byte device-coherent stores); team barrier; rows < nxp
+  // of g (the whole field, LDS layout) → b by device-coherent LDS DMA
+  auto exchange = [&](const ResBuf& b, const __bf16* g) __attribute__((always_inline)) {
+    const int ur = 2 * (a_hi - a_lo);   // 16-byte units of the band in one row
+    const int per = nxp * ur;           // ... in the hi (or the lo) field
+    const long long gb = (long long)(g - xX) * 2;   // byte offset of g in the exchange area
+    for (int u = tid; u < 2 * per; u += kResThreads) {
+      const int h = u >= per ? 1 : 0;
+      const int v = u - h * per;
+      const int r = v / ur;
+      const int off = r * kResLd + 16 * a_lo + 8 * (v - r * ur);
+      const bf8v x = *reinterpret_cast<const bf8v*>((h ? b.lo : b.hi) + off);
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4v, x), xr,
+                                             (int)(gb + 2LL * (h * kResBuf + off)), 0,
+                                             kCoherentAux);
+    }
+    arrivals += TP;
+    cuboid_barrier(P.bar + c, arrivals, P.loop_err, ticks);
+    const int units = nxp * (kResLd / 8);   // 16-byte units of rows < nxp
+    for (int h = 0; h < 2; ++h) {
+      const __bf16* src = g + h * kResBuf;
+      __bf16* const dst = h ? b.lo : b.hi;
+      for (int u0 = wave * kWave; u0 < units; u0 += kResThreads) {
+        const int u = u0 + lane;
+        if (u < units) dma16<kCoherentAux>(src + 8 * u, dst + 8 * u0);
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
   };
   const int so = d.s_off;
   ResFields fld;
@@ -378,7 +464,6 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
   f4v* const QEt = reinterpret_cast<f4v*>(wk + kResMaxN * kResMaxN);
   float* const QL = wk + 2 * kResMaxN * kResMaxN;
   const f4v* const QLt = reinterpret_cast<const f4v*>(QL);
-  const int NTB = nxp / 16;
   auto no_pre = [](int, int, int) __attribute__((always_inline)) {};
 
   // ---- step preparation: AA, laser flux, warm-start flux (buffer 0, as [i][j]) ----
@@ -415,12 +500,27 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
       }
       __syncthreads();
     }
-    // four elements per pass over the padded surface (zero operand entries outside it):
-    // the previous flux's neighbours of all four are loaded before any is used
+    // the laser flux at surface element (i, j)
+    auto laser_at = [&](int i, int j) __attribute__((always_inline)) -> double {
+      if (sep) return I0 * (fac[kResMaxN + j] * fac[i]);
+      const double x = (i + 0.5) * d.dx, y = (j + 0.5) * d.dy;
+      if (d.cell_integrated) {
+        const double s2 = sqrt(2.0), cc = sqrt(M_PI / 8.0);
+        const double ax = (x - x0 - d.dx / 2.0) * s2 / d.rx, bx = (x - x0 + d.dx / 2.0) * s2 / d.rx;
+        const double ay = (y - y0 - d.dy / 2.0) * s2 / d.ry, by = (y - y0 + d.dy / 2.0) * s2 / d.ry;
+        return I0 * ((d.ry * cc * (erf(by) - erf(ay)) / d.dy) * (d.rx * cc * (erf(bx) - erf(ax)) / d.dx));
+      }
+      const double ux = (x - x0) / d.rx, uy = (y - y0) / d.ry;
+      const double r2 = ux * ux + uy * uy;
+      return I0 * (d.order == 2.0 ? exp(-2.0 * r2) : exp(-2.0 * pow(r2, d.order / 2.0)));
+    };
+    // four elements per pass over the member's rows of the padded surface (zero operand
+    // entries outside it): the previous flux's neighbours of all four are loaded before any
+    // is used
     const ResBuf& b0 = buf0;
-    const int total = nyp * nxp;
+    const int e_lo = 16 * a_lo * nxp, e_hi = 16 * a_hi * nxp;
 #pragma unroll 1
-    for (int e0 = tid; e0 < total; e0 += 4 * kResThreads) {
+    for (int e0 = e_lo + tid; e0 < e_hi; e0 += 4 * kResThreads) {
       float nb[4][4];
       int ij[4][2];
       double tw[4][2];
@@ -430,8 +530,8 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
         const int e = e0 + u * kResThreads;
         const int j = e / nxp, i = e - j * nxp;
         ij[u][0] = i;
-        ij[u][1] = j;
-        in_s[u] = e < total && i < nx && j < ny;
+        ij[u][1] = e < e_hi ? j : nyp;
+        in_s[u] = e < e_hi && i < nx && j < ny;
         const double py = j - shy, px = i - shx;
         const double fy = floor(py), fx = floor(px);
         tw[u][0] = py - fy;
@@ -451,21 +551,7 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
         if (j >= nyp) break;
         float qn = 0.f, qlw = 0.f;
         if (in_s[u]) {
-          const double x = (i + 0.5) * d.dx, y = (j + 0.5) * d.dy;
-          double ql;
-          if (sep) {
-            ql = I0 * (fac[kResMaxN + j] * fac[i]);
-          } else if (d.cell_integrated) {
-            const double s2 = sqrt(2.0), cc = sqrt(M_PI / 8.0);
-            const double ax = (x - x0 - d.dx / 2.0) * s2 / d.rx, bx = (x - x0 + d.dx / 2.0) * s2 / d.rx;
-            const double ay = (y - y0 - d.dy / 2.0) * s2 / d.ry, by = (y - y0 + d.dy / 2.0) * s2 / d.ry;
-            ql = I0 * ((d.ry * cc * (erf(by) - erf(ay)) / d.dy) * (d.rx * cc * (erf(bx) - erf(ax)) / d.dx));
-          } else {
-            const double ux = (x - x0) / d.rx, uy = (y - y0) / d.ry;
-            const double r2 = ux * ux + uy * uy;
-            ql = I0 * (d.order == 2.0 ? exp(-2.0 * r2) : exp(-2.0 * pow(r2, d.order / 2.0)));
-          }
-          const F qlf = (F)ql;
+          const F qlf = (F)laser_at(i, j);
           // bilinear shift of the previous evaporation flux, scipy order-1 'constant'
           const double ty = tw[u][0], tx = tw[u][1];
           const double sv = in_r[u] ? (1.0 - ty) * ((1.0 - tx) * (double)nb[u][0] + tx * (double)nb[u][1]) +
@@ -474,7 +560,6 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
           q_las[j * nx + i] = qlf;
           qlw = qlf;
           qn = (F)((double)qlf - (double)(F)sv);
-          psum += (double)qlf;
         }
         const __bf16 hv = (__bf16)qn;
         b0.hi[i * kResLd + j] = hv;               // forward operand [i][j]
@@ -482,8 +567,15 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
         QL[(((j >> 4) * NTB + (i >> 4)) * 64 + (i & 15) + 16 * ((j & 15) >> 2)) * 4 + (j & 3)] = qlw;
       }
     }
+    // the laser power over the whole surface, in the same order for every team size
+#pragma unroll 1
+    for (int e = tid; e < nx * ny; e += kResThreads) {
+      const int j = e / nx;
+      psum += (double)(F)laser_at(e - j * nx, j);
+    }
   }
   const double P_sum = res_sum(psum, sh[1]);   // its barriers also complete the operand stores
+  if (TP > 1) exchange(buf0, xQ);
   RES_TS(1);
 
   const double fwd_scale = d.dx * d.dy * d.inv_sqrt_LxLy;
@@ -554,6 +646,7 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
                 res_store(buf0, tB.aout(q, lane), tB.tcol(lane), xo);   // [m][n]
               });
   }
+  if (TP > 1) exchange(buf0, xX);
 
   RES_TS(2);
   // ---- Picard loop ----
@@ -569,20 +662,36 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
   const F T_hot = T_liq > T_cut ? T_liq : T_cut;
   const int max_it = d.max_it;
   const double tol = d.tol, n_elem = d.n_elem;
+  // epilogue inputs of the two stages with epilogues (tB tiles, slot = tile parity): q_las
+  // of the evaporation stage; (ZE, Ztt, Wtt, U) and G of the relaxation stage. The loads of
+  // the first tiles (two of the evaporation stage, one of the relaxation stage, as registers
+  // allow) start before the stage that precedes each of them, so that a wave with one or two
+  // tiles (the members of a team) does not wait for them.
+  f4v ql[2];
+  f4v zw[2][4];
+  f4v gg[2];
+  auto pre_evap = [&](int q, int sl, int lane) __attribute__((always_inline)) {
+    ql[sl] = QLt[tidx(q, lane)];
+  };
+  auto pre_relax = [&](int q, int sl, int lane) __attribute__((always_inline)) {
+    unsigned vo[4], vo16[4];
+    res_voffs(tB, q, lane, opaque(nx), opaque(ny), vo, vo16);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) zw[sl][v] = fld.record(vo16[v]);
+    gg[sl] = Gt[tidx(q, lane)];
+  };
 #pragma unroll 1
   for (int k = 0; k < max_it; ++k) {
     // inverse + evaporation: q = q_las − q_evap(T_s) into buffer 0
+    if (tB.tt >= 0) {
+      pre_evap(0, 0, lane);
+      pre_evap(1, 1, lane);
+    }
     stage_inv1();
-    if (k < 20) RES_TS(3 + 5 * k);
+    if (k < 20) RES_TS(3 + 7 * k);
     {
       const int nxk = opaque(nx), nyk = opaque(ny);
-      f4v ql[2];
-      res_stage(tB, W, KX, buf1, lane, w_fwd1,
-                [&](int q, int sl, int lane) __attribute__((always_inline)) {
-                  unsigned vo[4], vo16[4];
-                  res_voffs(tB, q, lane, nxk, nyk, vo, vo16);
-                  ql[sl] = QLt[tidx(q, lane)];
-                },
+      res_stage<2>(tB, W, KX, buf1, lane, w_fwd1, pre_evap,
                 [&](int q, int sl, const f4v& a, int lane) __attribute__((always_inline)) {
                   unsigned vo[4], vo16[4];
                   res_voffs(tB, q, lane, nxk, nyk, vo, vo16);
@@ -613,27 +722,18 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
                   res_store(buf0, tB.aout(q, lane), tB.tcol(lane), qo);   // [i][j]
                 });
     }
-    if (k < 20) RES_TS(4 + 5 * k);
-    // forward + relaxation + norm partials (FP32 per tile, FP64 over tiles); Xinv into
-    // buffer 0
+    if (k < 20) RES_TS(4 + 7 * k);
+    if (TP > 1) exchange(buf0, xQ);
+    if (k < 20) RES_TS(5 + 7 * k);
+    // forward + relaxation + norm sums: per lane in FP32 over its four elements, per row
+    // group of 16 lanes (4 rows × 16 columns of the tile) by DPP row shifts in FP32, to slot
+    // (a·8 + t)·4 + group of tile (a, t); Xinv into buffer 0
+    if (tB.tt >= 0) pre_relax(0, 0, lane);
     stage_fwd1();
-    if (k < 20) RES_TS(5 + 5 * k);
-    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+    if (k < 20) RES_TS(6 + 7 * k);
     {
-      const int nxk = opaque(nx), nyk = opaque(ny);
-      f4v zw[2][4];   // (ZE, Ztt, Wtt, U) of the tile's elements
-      f4v gg[2];
-      res_stage(tB, W, KX, buf1, lane, w_inv1,
-                [&](int q, int sl, int lane) __attribute__((always_inline)) {
-                  unsigned vo[4], vo16[4];
-                  res_voffs(tB, q, lane, nxk, nyk, vo, vo16);
-#pragma unroll
-                  for (int v = 0; v < 4; ++v) zw[sl][v] = fld.record(vo16[v]);
-                  gg[sl] = Gt[tidx(q, lane)];
-                },
+      res_stage<1>(tB, W, KX, buf1, lane, w_inv1, pre_relax,
                 [&](int q, int sl, const f4v& a, int lane) __attribute__((always_inline)) {
-                  unsigned vo[4], vo16[4];
-                  res_voffs(tB, q, lane, nxk, nyk, vo, vo16);
                   float xo[4], p0 = 0.f, p1 = 0.f, p2 = 0.f;
                   f4v g4;
 #pragma unroll
@@ -649,36 +749,62 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
                     xo[v] = zw[sl][v][0] + gn * zw[sl][v][1];
                   }
                   Gt[tidx(q, lane)] = g4;
-                  v0 += (double)p0;
-                  v1 += (double)p1;
-                  v2 += (double)p2;
                   res_store(buf0, tB.aout(q, lane), tB.tcol(lane), xo);   // [m][n]
+                  p0 = res_row_sum(p0);
+                  p1 = res_row_sum(p1);
+                  p2 = res_row_sum(p2);
+                  if ((lane & 15) == 15) {
+                    const int slot = (tB.atile(q) * 8 + tB.tt) * 4 + (lane >> 4);
+                    if (TP == 1) {
+                      nsl[0][slot] = p0;
+                      nsl[1][slot] = p1;
+                      nsl[2][slot] = p2;
+                    } else {
+                      st_x<float, true>(xN + slot, p0);
+                      st_x<float, true>(xN + kResNormSlots + slot, p1);
+                      st_x<float, true>(xN + 2 * kResNormSlots + slot, p2);
+                    }
+                  }
                 });
     }
-    if (k < 20) RES_TS(6 + 5 * k);
-    // the three sums in one exchange, in a fixed order
-    v0 = res_wave_sum(v0);
-    v1 = res_wave_sum(v1);
-    v2 = res_wave_sum(v2);
-    if (lane == 0) {
-      sh[0][wave] = v0;
-      sh[1][wave] = v1;
-      sh[2][wave] = v2;
+    if (k < 20) RES_TS(7 + 7 * k);
+    if (TP > 1) exchange(buf0, xX);
+    if (k < 20) RES_TS(8 + 7 * k);
+    // the three sums over all tiles of the surface: lane a·8 + t sums the four row groups of
+    // tile (a, t) in FP64, then the lanes in a fixed order (every wave, every member)
+    const bool tin = (lane >> 3) < NAY && (lane & 7) < NTB;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    if (tin) {
+      f4v r0, r1, r2;
+      if (TP == 1) {
+        r0 = *reinterpret_cast<const f4v*>(&nsl[0][4 * lane]);
+        r1 = *reinterpret_cast<const f4v*>(&nsl[1][4 * lane]);
+        r2 = *reinterpret_cast<const f4v*>(&nsl[2][4 * lane]);
+      } else {
+        const int nb = (int)kResXchNorm + 16 * lane;
+        r0 = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(xr, nb, 0, kCoherentAux));
+        r1 = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(
+                                         xr, nb + 4 * kResNormSlots, 0, kCoherentAux));
+        r2 = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(
+                                         xr, nb + 8 * kResNormSlots, 0, kCoherentAux));
+      }
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        s0 += (double)r0[g];
+        s1 += (double)r1[g];
+        s2 += (double)r2[g];
+      }
     }
-    __syncthreads();
-    const int r = lane >> 4, l16 = lane & 15;
-    const double part = res_sum8(r < 3 && l16 < kResWaves ? sh[r < 3 ? r : 0][l16 < kResWaves ? l16 : 0] : 0.0);
-    const double rr = res_lane(part, kResWaves - 1);
-    const double gu = res_lane(part, 16 + kResWaves - 1);
-    const double g2w = res_lane(part, 32 + kResWaves - 1);
-    __syncthreads();
+    const double rr = res_wave_sum(s0);
+    const double gu = res_wave_sum(s1);
+    const double g2w = res_wave_sum(s2);
     const double a2 = AA + 2.0 * gu + g2w;
     const double rms_diff = sqrt(rr / n_elem);
     const double rms_old = sqrt(a2 / n_elem);
     rel = rms_diff / (rms_old > 1e-9 ? rms_old : 1e-9);
     conv = rel < tol;
     n_it = k + 1;
-    if (k < 20) RES_TS(7 + 5 * k);
+    if (k < 20) RES_TS(9 + 7 * k);
     if (conv) break;
   }
 
@@ -706,16 +832,29 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
   tmax = wave_max(tmax);
   if (lane == 0) sh[0][wave] = tmax;
   __syncthreads();
-  RES_TS(103);
+  RES_TS(143);
 #ifdef SGH_RESIDENT_TIMESTAMPS
   if (tid == 0 && blockIdx.x < kResTsBlocks) {
-    g_res_ts[blockIdx.x * kResTsSlots + 104] = (unsigned long long)nx;
-    g_res_ts[blockIdx.x * kResTsSlots + 105] = (unsigned long long)ny;
+    g_res_ts[blockIdx.x * kResTsSlots + 144] = (unsigned long long)(nx | (TP << 16));
+    g_res_ts[blockIdx.x * kResTsSlots + 145] = (unsigned long long)(ny | (mp << 16));
   }
 #endif
-  if (tid == 0) {
-    double tm_all = -1.0 / 0.0;
+  double tm_all = -1.0 / 0.0;
+  if (tid == 0)
     for (int w = 0; w < kResWaves; ++w) tm_all = sh[0][w] > tm_all ? sh[0][w] : tm_all;
+  if (TP > 1) {
+    // every member's maximum to its slot; member 0 waits for all and finishes the step
+    if (tid == 0) st_x<double, true>(xT + mp, tm_all);
+    arrivals += TP;
+    cuboid_barrier(P.bar + c, arrivals, P.loop_err, ticks, mp == 0);
+    if (mp != 0) return;
+    if (tid == 0)
+      for (int m = 0; m < TP; ++m) {
+        const double t = ld_x<double, true>(xT + m);
+        tm_all = t > tm_all ? t : tm_all;
+      }
+  }
+  if (tid == 0) {
     double* st = P.status + kStatusStride * c;
     st[0] = n_it;
     st[1] = conv ? 1.0 : 0.0;
@@ -735,9 +874,9 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
 }
 
 hipError_t launch_resident(const BatchPtrs<float>& P, const void* m16, const int* list,
-                           unsigned nblocks, hipStream_t st) {
+                           unsigned nblocks, unsigned long long ticks, hipStream_t st) {
   hipLaunchKernelGGL(k_b_resident, dim3(nblocks), dim3(kResThreads), kResLdsBytes, st, P,
-                     (const __bf16*)m16, list);
+                     (const __bf16*)m16, list, ticks);
   return hipGetLastError();
 }
 

@@ -208,6 +208,21 @@ class BatchEngine:
         call("sgh_batch_loop_state", self._plan, C.cast(out, C.c_void_p))
         return {"requested": bool(out[0]), "active": bool(out[1])}
 
+    def set_team_max(self, team_max: int):
+        """Largest team of workgroups a cuboid's step may run on in the resident step kernel
+        (1 to 8; default 8). Team sizes are chosen per work list; results are bit-identical
+        for every team size."""
+        call("sgh_batch_set_team_max", self._plan, int(team_max), self.stream.cuda_stream)
+
+    @property
+    def team_sizes(self):
+        """Team size of each slot in the last work lists (0: free, or the resident kernel
+        not in use)."""
+        n = int(call("sgh_batch_team_sizes", self._plan, None, 0))
+        out = (C.c_int * max(n, 1))()
+        call("sgh_batch_team_sizes", self._plan, C.cast(out, C.c_void_p), n)
+        return [int(out[i]) for i in range(n)]
+
     def rewind(self):
         """Restart every cuboid from its initial condition at step 0."""
         call("sgh_batch_rewind", self._plan, self.stream.cuda_stream)

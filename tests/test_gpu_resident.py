@@ -144,3 +144,50 @@ def test_resident_c4_1000_steps_against_fp64_engine():
     print(f"C4 1000 steps resident vs FP64: worst rel-L2 {worst:.2e}, worst (T-T0) {worst_exc:.2e}")
     res.close()
     ref.close()
+
+
+def test_team_sizes_give_identical_results():
+    # the 64-cuboid batch: the largest surfaces run on teams of several workgroups; with the
+    # teams capped at 1 or 3 the states and metrics must be the same bit for bit
+    cases = batch_uneven_cuboids(n=64, seed=0, n_steps=30)
+    runs = {}
+    for cap in (8, 1, 3):
+        eng = _engine(cases)
+        eng.set_team_max(cap)
+        eng.run(30)
+        eng.synchronize()
+        sizes = eng.team_sizes
+        runs[cap] = ([eng.state(i).clone() for i in range(eng.n)], eng.metrics(), sizes)
+        print(f"team_max {cap}: team sizes {sorted(sizes, reverse=True)[:12]}..., "
+              f"{sum(sizes)} workgroups")
+        eng.close()
+    assert max(runs[8][2]) > 1 and max(runs[1][2]) == 1 and max(runs[3][2]) <= 3
+    assert sum(runs[8][2]) <= 256
+    for cap in (1, 3):
+        for i, (a, b) in enumerate(zip(runs[8][0], runs[cap][0])):
+            assert torch.equal(a, b), (cap, i, float((a - b).abs().max()))
+        np.testing.assert_array_equal(runs[8][1], runs[cap][1])
+
+
+def test_large_cuboid_alone_on_a_team_matches_oracle():
+    cfg, lp = raster_multitrack(mesh=(128, 128, 16), n_steps=30, size=(2.0e-3, 2.0e-3, 0.25e-3))
+    eng = _engine([(cfg, lp)])
+    eng.run(30)
+    eng.synchronize()
+    assert eng.team_sizes[0] == 8, eng.team_sizes
+    ref, its = _oracle(cfg, lp, 30)
+    err = modal_rel_l2(eng.state(0).double().cpu().numpy(), ref.a)
+    met = eng.metrics()
+    print(f"128x128x16 on a team of 8, 30 steps: rel-L2 vs oracle {err:.2e}, iterations "
+          f"{met[:, 0, 2].astype(int).tolist()[-5:]} vs {its[-5:]}")
+    assert err < 1e-4
+    assert abs(met[:, 0, 2].mean() - np.mean(its)) <= 1.0
+    single = _engine([(cfg, lp)])
+    single.set_team_max(1)
+    single.run(30)
+    single.synchronize()
+    assert single.team_sizes[0] == 1
+    assert torch.equal(single.state(0), eng.state(0))
+    np.testing.assert_array_equal(single.metrics(), met)
+    single.close()
+    eng.close()
