@@ -115,6 +115,7 @@ constexpr int kResThreads = 512;
 constexpr int kResLdsBytes = 4 * kResMaxN * kResLd * 2;   // two hi / lo operand buffer pairs
 constexpr int kResWork = 3 * kResMaxN * kResMaxN;          // work floats per cuboid slot
 constexpr int kResTeamMax = 8;           // workgroups per cuboid (one per 16-row band at most)
+constexpr int kWavesPerRes = kResThreads / 64;   // waves of a resident workgroup
 constexpr int kResTiles = 64;            // 16×16 tiles of a kResMaxN² surface
 constexpr int kResNormSlots = 4 * kResTiles;   // norm sums: one per 4-row group of a tile
 // Exchange area of a cuboid slot: the X and q operand fields (hi, lo; LDS layout), the

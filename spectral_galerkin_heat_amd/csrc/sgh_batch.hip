@@ -922,17 +922,27 @@ static void build_xcd_items(Plan* p) {
 }
 
 // Resident teams (sgh_resident.hip): how many workgroups each cuboid's step runs on. The
-// cost of a member's step is modelled per Picard iteration as a fixed part plus a part
-// per 16×16 tile of each stage the member owns, plus the two field exchanges of a team
-// (cycles, from per-workgroup phase stamps of the kernel on MI355X). Starting from one
-// workgroup per cuboid, the cuboid whose step is the longest gets one more member while
-// that shortens it, the batch still fits one workgroup per CU (members wait for each other)
-// and the teams still fit the XCDs (a team's members run on one XCD). The work list then
-// places each team on one XCD (list positions ≡ xcd mod 8), largest teams first.
+// cost of a member's Picard iteration is modelled in thousands of cycles, fitted to
+// per-workgroup phase stamps of the kernel on MI355X: alone, 6 + 3·(KX + KY) +
+// 0.25·(TA·KY + TB·KX), with KX, KY the k steps of the x and y contractions and TA, TB the
+// tiles per wave of the two stage kinds; a member of a team of tp saves 0.88·K per tile
+// per wave it no longer holds (K the mean k steps) and pays 13 for its two field exchanges
+// (each a device-coherent store, barrier and copy of about 2.6 µs). Starting from one
+// workgroup per cuboid, the cuboid whose step is the longest moves to the smallest larger
+// team that shortens it, while the batch still fits one workgroup per CU (members wait for
+// each other) and the teams still fit the XCDs (a team's members run on one XCD). The work
+// list then places each team on one XCD (list positions ≡ xcd mod 8), largest teams first.
 static double team_cost(const CuboidDesc& d, int tp) {
   const int ntb = pad32(d.nx) / 16, nay = pad32(d.ny) / 16;
+  const int kx = pad32(d.nx) / 32, ky = pad32(d.ny) / 32;
+  const int ga = std::max(1, kWavesPerRes / nay), gb = std::max(1, kWavesPerRes / ntb);
+  const int ta = (ntb + ga - 1) / ga, tb = (nay + gb - 1) / gb;
+  const double alone = 6.0 + 3.0 * (kx + ky) + 0.25 * (ta * ky + tb * kx);
+  if (tp <= 1) return alone;
   const int rows = (nay + tp - 1) / tp;
-  return 8.0 + 0.7 * ntb * rows + (tp > 1 ? 6.0 : 0.0);
+  const int t1 = (ntb * nay + kWavesPerRes - 1) / kWavesPerRes;
+  const int tt = (ntb * rows + kWavesPerRes - 1) / kWavesPerRes;
+  return alone - 0.88 * 0.5 * (kx + ky) * (t1 - tt) + 13.0;
 }
 
 static bool place_teams(Plan* p, const std::vector<int>& order) {
@@ -977,12 +987,12 @@ static void assign_teams(Plan* p) {
       const double v = team_cost(p->hdesc[c], p->hteam[c]);
       if (v > wc) { wc = v; worst = c; }
     }
-    // the next team size that gives its largest member fewer rows
+    // the smallest larger team that shortens its step
     const CuboidDesc& d = p->hdesc[worst];
     const int tp = p->hteam[worst], nay = pad32(d.ny) / 16, lim = std::min(p->team_max, nay);
     int nt = tp + 1;
-    while (nt <= lim && (nay + nt - 1) / nt == (nay + tp - 1) / tp) ++nt;
-    if (nt > lim || total + (nt - tp) > slots || team_cost(d, nt) >= wc) break;
+    while (nt <= lim && team_cost(d, nt) >= wc) ++nt;
+    if (nt > lim || total + (nt - tp) > slots) break;
     p->hteam[worst] = nt;
     total += nt - tp;
   }

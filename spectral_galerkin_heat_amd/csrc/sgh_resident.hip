@@ -83,7 +83,7 @@ __device__ __forceinline__ int opaque_lane(int v) {
 // t tile toff + w / g and a tiles aoff + (w mod g) + g·q, q < nq. tt < 0: no tiles.
 struct ResTiles {
   int tt, a0, g, nq;
-  __device__ ResTiles(int NT, int NA, int wave, int toff, int aoff) {
+  __device__ __forceinline__ ResTiles(int NT, int NA, int wave, int toff, int aoff) {
     g = kResWaves / NT > 0 ? kResWaves / NT : 1;
     tt = wave / g;
     a0 = wave % g;
