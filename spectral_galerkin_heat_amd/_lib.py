@@ -138,6 +138,25 @@ class SGHError(RuntimeError):
     """A libsgh_mi355x call returned an error code."""
 
 
+class DeviceOutOfMemory(SGHError):
+    """A libsgh_mi355x call ran out of device memory (SGH_ERR_NOMEM from an allocation)."""
+
+
+NOMEM = -4   # SGH_ERR_NOMEM
+
+
+def is_device_oom(exc: BaseException) -> bool:
+    """Whether ``exc`` (or an exception in its cause chain) is a device out-of-memory error:
+    this library's DeviceOutOfMemory or torch's allocator error."""
+    seen = set()
+    while exc is not None and id(exc) not in seen:
+        seen.add(id(exc))
+        if isinstance(exc, DeviceOutOfMemory) or type(exc).__name__ == "OutOfMemoryError":
+            return True
+        exc = exc.__cause__ or exc.__context__
+    return False
+
+
 class _Lib:
     def __init__(self):
         self._h = None
@@ -182,7 +201,8 @@ def last_error() -> str:
 def call(name: str, *args) -> int:
     rc = getattr(lib, name)(*args)
     if rc < 0:
-        raise SGHError(f"{name} failed ({rc}): {last_error()}")
+        msg = f"{name} failed ({rc}): {last_error()}"
+        raise DeviceOutOfMemory(msg) if rc == NOMEM else SGHError(msg)
     return rc
 
 

@@ -54,11 +54,44 @@ def make_xp(device):
     def vdot(a, b):
         return torch.dot(a.reshape(-1).to(torch.float64), b.reshape(-1).to(torch.float64))
 
+    def _t(x):
+        return x if isinstance(x, torch.Tensor) else torch.as_tensor(x, device=device)
+
+    def arange(n, dtype=None):
+        out = torch.arange(n, device=device)
+        return out.to(_torch_dtype(dtype)) if dtype is not None else out
+
+    def where(cond, x, y):
+        c = _t(cond)
+        return torch.where(c, _t(x).to(device), _t(y).to(device))
+
+    def power(x, y):
+        return torch.pow(_t(x), y)
+
+    def gradient(f, *spacing, axis=None):
+        # numpy.gradient with uniform spacings and edge_order 1 (the reference's use)
+        axes = list(range(f.dim())) if axis is None else [axis]
+        sp = list(spacing) if spacing else [1.0] * len(axes)
+        if len(sp) == 1 and len(axes) > 1:
+            sp = sp * len(axes)
+        out = [torch.gradient(f, spacing=float(h), dim=ax, edge_order=1)[0]
+               for ax, h in zip(axes, sp)]
+        return out[0] if axis is not None or len(out) == 1 else out
+
+    def tensordot(a, b, axes=2):
+        return torch.tensordot(a, b, dims=axes)
+
+    def einsum(spec, *ops, optimize=None):
+        return torch.einsum(spec, *ops)
+
     ns = types.SimpleNamespace(
         zeros=zeros, empty=empty, full=full, asarray=asarray, copyto=copyto, vdot=vdot,
         zeros_like=torch.zeros_like, empty_like=torch.empty_like,
         sum=torch.sum, max=torch.max, abs=torch.abs, sqrt=torch.sqrt,
         maximum=torch.maximum, multiply=torch.mul, subtract=torch.sub,
+        exp=lambda x: torch.exp(_t(x)), cos=lambda x: torch.cos(_t(x)),
+        arange=arange, where=where, power=power, gradient=gradient, tensordot=tensordot,
+        einsum=einsum, pi=np.pi,
         float32=np.float32, float64=np.float64, __name__="mi355x_xp",
     )
     return ns

@@ -16,9 +16,11 @@ namespace sgh {
 // Per-thread last-error string returned by sgh_last_error().
 void set_error(const char* fmt, ...);
 
+// A HIP error as a return code: device memory exhaustion is SGH_ERR_NOMEM (callers can
+// back off), anything else SGH_ERR_HIP.
 inline int fail_hip(hipError_t e, const char* where) {
   set_error("%s: %s", where, hipGetErrorString(e));
-  return SGH_ERR_HIP;
+  return e == hipErrorOutOfMemory ? SGH_ERR_NOMEM : SGH_ERR_HIP;
 }
 
 #define SGH_CHECK_LAUNCH(where)                                        \

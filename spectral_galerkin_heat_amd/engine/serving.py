@@ -25,6 +25,7 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
+from .._lib import DeviceOutOfMemory
 from .batch import BatchEngine, BatchPool, PoolFull, footprint_bytes
 
 __all__ = ["Request", "ScanServer", "poisson_arrivals"]
@@ -94,17 +95,29 @@ class ServeReport:
 
 
 class ScanServer:
+    MIN_BUDGET = 64 << 20   # smallest pool budget tried after device out-of-memory errors
+
     def __init__(self, precision="mixed", max_cuboids=128, hbm_budget_bytes=8 << 30,
                  max_nz=64, max_steps=2048, steps_per_tick=8, check_every=20,
                  keep_results=True):
         word = 8 if precision == "fp64" else 4
-        # split the budget between the packed state and the surface fields
-        state_cap = int(hbm_budget_bytes * 0.85) // word
-        surf_cap = int(hbm_budget_bytes * 0.15) // (10 * word)
-        self.pool = BatchPool(precision=precision, max_cuboids=max_cuboids,
-                              state_capacity=state_cap, surface_capacity=surf_cap,
-                              laser_capacity=max_cuboids * max_steps, max_nz=max_nz,
-                              metrics_rows=max_steps)
+        # split the budget between the packed state and the surface fields; a pool the
+        # device cannot hold is retried at half the budget (down to MIN_BUDGET)
+        budget = int(hbm_budget_bytes)
+        while True:
+            state_cap = int(budget * 0.85) // word
+            surf_cap = int(budget * 0.15) // (10 * word)
+            try:
+                self.pool = BatchPool(precision=precision, max_cuboids=max_cuboids,
+                                      state_capacity=state_cap, surface_capacity=surf_cap,
+                                      laser_capacity=max_cuboids * max_steps, max_nz=max_nz,
+                                      metrics_rows=max_steps)
+                break
+            except DeviceOutOfMemory:
+                if budget // 2 < self.MIN_BUDGET:
+                    raise
+                budget //= 2
+        self.hbm_budget_bytes = budget
         self.precision = precision
         self.max_steps = max_steps
         self.steps_per_tick = int(steps_per_tick)

@@ -710,6 +710,12 @@ class _PlainIter:
                 pass
             self._host_ptr = None
 
+    # polling of the pinned norm words: a tight spin for the first SPIN_S seconds (an
+    # iteration takes tens to hundreds of µs and the next one is already enqueued, so
+    # noticing a little late does not idle the GPU), then yields of the host thread, then
+    # short sleeps
+    SPIN_S, YIELD_S, NAP_S = 2e-4, 2e-3, 5e-5
+
     def wait(self, k, timeout_s=60.0):
         """The norms of iteration k once its fused pass has written them."""
         import time as _time
@@ -719,11 +725,16 @@ class _PlainIter:
         if h[4 * p + 3] != want:
             t0 = _time.perf_counter()
             while h[4 * p + 3] != want:
-                if _time.perf_counter() - t0 > timeout_s:
+                el = _time.perf_counter() - t0
+                if el > timeout_s:
                     torch.cuda.synchronize()   # raises a device error if one occurred
                     if h[4 * p + 3] != want:
                         raise _lib.SGHError(f"Picard iteration {k}: no residual after "
                                             f"{timeout_s} s")
+                elif el > self.YIELD_S:
+                    _time.sleep(self.NAP_S)
+                elif el > self.SPIN_S:
+                    _time.sleep(0)
         return float(h[4 * p]), float(h[4 * p + 1]), float(h[4 * p + 2])
 
     def refresh(self, solver, S, prog, kbar, have_bot):

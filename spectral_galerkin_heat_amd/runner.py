@@ -12,7 +12,12 @@ temperature ``temperature_nodes.npy`` (``reconstruct_temperature_DCT``), per-ste
 metrics ``metrics.csv`` and full nonlinear checkpoints ``checkpoint_<step>.npz`` that
 ``--resume`` continues from bit for bit.
 
-There is no CPU fallback: a device error ends the run with the error.
+There is no CPU fallback: a device error ends the run with the error. Where the
+reference's CLI retries on its CPU backend after a GPU out-of-memory error
+(simulations/main.py:77-131), ``main`` removes the failed run's output directory, as the
+reference does before its retry, and exits with status 2 and a message naming the state's
+device footprint, so a caller can retry on the reference's NumPy backend or with a smaller
+mesh.
 """
 
 from __future__ import annotations
@@ -22,6 +27,8 @@ import copy
 import csv
 import logging
 import os
+import shutil
+import sys
 import time
 
 import numpy as np
@@ -124,10 +131,29 @@ def main(argv=None):
     args = ap.parse_args(argv)
     logging.basicConfig(level=logging.INFO, format="%(asctime)s %(levelname)s %(message)s")
     ctx = load_context(args.config, backend=args.backend, n_steps=args.steps)
-    summary = HeatRunner(ctx, out_dir=args.out, checkpoint_every=args.checkpoint_every).run(
-        resume=args.resume)
+    created = args.out is not None and not os.path.exists(args.out)
+    try:
+        summary = HeatRunner(ctx, out_dir=args.out, checkpoint_every=args.checkpoint_every).run(
+            resume=args.resume)
+    except Exception as exc:
+        from ._lib import is_device_oom
+        if not is_device_oom(exc):
+            raise
+        if created and os.path.isdir(args.out):
+            shutil.rmtree(args.out, ignore_errors=True)
+        logger.error("device out of memory: %s. State footprint of this configuration: %s. "
+                     "The mi355x backend has no CPU fallback; rerun on the reference's numpy "
+                     "backend or with a smaller mesh.", exc, _footprint(ctx))
+        sys.exit(2)
     logger.info("done: %s", summary)
     return summary
+
+
+def _footprint(ctx) -> str:
+    """Rough device bytes of the plain solver's state (modal arrays and buffers)."""
+    n = int(np.prod(tuple(int(v) for v in ctx.geom.n)))
+    word = 8 if str(getattr(ctx.num, "dtype", "float64")).endswith("64") else 4
+    return f"about {12 * n * word / 2**30:.2f} GiB for {n} modes ({word}-byte words)"
 
 
 if __name__ == "__main__":

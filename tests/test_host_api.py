@@ -8,6 +8,7 @@ tests/unit_tests/solvers/test_build_solver.py); no GPU is touched.
 
 import copy
 import math
+import os
 
 import numpy as np
 import pytest
@@ -264,3 +265,109 @@ def test_runner_reads_a_reference_style_yaml_with_gcode(tmp_path):
     assert ctx.backend == "mi355x" and ctx.num.n_steps == 7 and ctx.fine.refinement == 2
     ls = ctx.laser_path.get_state(1e-4, 4e-6)
     assert ls.is_on and abs(ls.x - 0.215e-3) < 1e-6 and abs(ls.v[0] - 0.15) < 1e-3
+
+
+def test_xp_namespace_matches_numpy_on_the_reference_calls():
+    # the array calls the reference's solver, laser and property code make through
+    # backend.xp (spectral.py, laser.py:52, properties.py:71, spectral_ops.py:158-166,
+    # spectral_state.py:119), on torch tensors, against NumPy
+    import torch
+    from spectral_galerkin_heat_amd.backends.mi355x_backend import make_xp
+    xp = make_xp("cpu")
+    rng = np.random.default_rng(5)
+    f = rng.standard_normal((4, 5, 6))
+    t = torch.tensor(f)
+    for a, b in zip(xp.gradient(t, 0.1, 0.2, 0.3), np.gradient(f, 0.1, 0.2, 0.3)):
+        np.testing.assert_allclose(a.numpy(), b, rtol=1e-14, atol=1e-14)
+    np.testing.assert_allclose(xp.gradient(t, 0.2, axis=2).numpy(), np.gradient(f, 0.2, axis=2),
+                               rtol=1e-14, atol=1e-14)
+    np.testing.assert_array_equal(xp.power(-1.0, xp.arange(5, dtype=np.float64)).numpy(),
+                                  np.power(-1.0, np.arange(5.0)))
+    np.testing.assert_array_equal(xp.where(t >= 0.1, 1.0, 0.0).numpy(),
+                                  np.where(f >= 0.1, 1.0, 0.0))
+    np.testing.assert_allclose(xp.exp(-2.0 * t * t).numpy(), np.exp(-2.0 * f * f), rtol=1e-15)
+    np.testing.assert_allclose(xp.cos(xp.pi * t).numpy(), np.cos(np.pi * f), rtol=1e-14,
+                               atol=1e-15)
+    B = rng.standard_normal((6, 6))
+    np.testing.assert_allclose(xp.tensordot(t, torch.tensor(B), axes=([2], [1])).numpy(),
+                               np.tensordot(f, B, axes=([2], [1])), rtol=1e-13)
+    np.testing.assert_allclose(xp.einsum("zyx,Xx->zyX", t, torch.tensor(B)).numpy(),
+                               np.einsum("zyx,Xx->zyX", f, B), rtol=1e-13)
+    assert float(xp.vdot(t, t)) == pytest.approx(float(np.vdot(f, f)), rel=1e-14)
+
+
+def test_device_out_of_memory_is_its_own_error(monkeypatch):
+    # the library's SGH_ERR_NOMEM surfaces as DeviceOutOfMemory; torch's allocator error
+    # and chained causes are recognised the same way
+    from spectral_galerkin_heat_amd import _lib
+
+    class Stub:
+        def sgh_fake(self):
+            return -4
+
+        def sgh_fail(self):
+            return -2
+
+        def sgh_last_error(self):
+            return b"hipMalloc: out of memory"
+
+    monkeypatch.setattr(_lib, "lib", Stub())
+    with pytest.raises(_lib.DeviceOutOfMemory, match="out of memory"):
+        _lib.call("sgh_fake")
+    with pytest.raises(_lib.SGHError) as ei:
+        _lib.call("sgh_fail")
+    assert not isinstance(ei.value, _lib.DeviceOutOfMemory)
+    assert _lib.is_device_oom(_lib.DeviceOutOfMemory("x"))
+    OOM = type("OutOfMemoryError", (RuntimeError,), {})
+    try:
+        try:
+            raise OOM("torch")
+        except OOM as inner:
+            raise ValueError("wrapped") from inner
+    except ValueError as outer:
+        assert _lib.is_device_oom(outer)
+    assert not _lib.is_device_oom(ValueError("other"))
+
+
+def test_serving_pool_backs_off_after_device_out_of_memory(monkeypatch):
+    from spectral_galerkin_heat_amd import _lib
+    from spectral_galerkin_heat_amd.engine import serving
+    tried = []
+
+    class Pool:
+        def __init__(self, precision, max_cuboids, state_capacity, surface_capacity,
+                     laser_capacity, max_nz, metrics_rows):
+            tried.append(state_capacity * 4 / 0.85)
+            if state_capacity * 4 > (1 << 30):
+                raise _lib.DeviceOutOfMemory("sgh_batch_pool_create failed (-4)")
+
+    monkeypatch.setattr(serving, "BatchPool", Pool)
+    srv = serving.ScanServer(hbm_budget_bytes=8 << 30)
+    assert srv.hbm_budget_bytes == 1 << 30 and len(tried) == 4
+    # below the smallest budget the error reaches the caller
+    monkeypatch.setattr(serving.ScanServer, "MIN_BUDGET", 4 << 30)
+    with pytest.raises(_lib.DeviceOutOfMemory):
+        serving.ScanServer(hbm_budget_bytes=8 << 30)
+
+
+def test_runner_removes_its_output_after_device_out_of_memory(monkeypatch, tmp_path):
+    import yaml
+    from spectral_galerkin_heat_amd import _lib, runner
+    from spectral_galerkin_heat_amd.workloads import single_track_constant
+    cfg, _ = single_track_constant(mesh=(16, 8, 4), n_steps=2)
+    path = tmp_path / "cfg.yaml"
+    path.write_text(yaml.safe_dump(cfg))
+    out = tmp_path / "run"
+
+    class Runner:
+        def __init__(self, ctx, out_dir=None, checkpoint_every=0):
+            self.out = out_dir
+
+        def run(self, resume=None):
+            os.makedirs(self.out, exist_ok=True)
+            raise _lib.DeviceOutOfMemory("sgh_batch_create failed (-4): out of memory")
+
+    monkeypatch.setattr(runner, "HeatRunner", Runner)
+    with pytest.raises(SystemExit) as ei:
+        runner.main([str(path), "--out", str(out)])
+    assert ei.value.code == 2 and not out.exists()
