@@ -352,21 +352,24 @@ def main():
     vbytes = volume_bytes_per_launch(ctxs, 4)
     resident = tm.get("surface_launches", 0) == instr and eng_mx.variant["rows"] < 0
     if resident and share_surf >= share_vol:
-        # one launch per step, one workgroup (one CU) per cuboid: the bound is the matrix
-        # cores of the CUs it occupies; iterations per cuboid from the same segment's metrics
+        # one launch per step, a team of workgroups (one CU each) per cuboid: the bound is
+        # the matrix cores of the CUs it occupies; iterations per cuboid from the same
+        # segment's metrics
         iters = eng_mx.metrics()[W + K:W + K + instr, :, 2].mean(axis=0)
         rflops = resident_flops_per_launch(ctxs, iters)
         tfs = rflops / (surf_ms * 1e-3) / 1e12
-        active = min(len(ctxs), N_CU) / N_CU
-        roofline = {"kernel": "k_b_resident (one workgroup per cuboid: laser flux, warm start, "
-                              "every Picard iteration and the step end in one launch; "
-                              "split-bf16 MFMA transforms)",
+        teams = eng_mx.team_sizes
+        active = min(max(sum(teams), len(ctxs)), N_CU) / N_CU
+        roofline = {"kernel": "k_b_resident (a team of 1-8 workgroups per cuboid: laser flux, "
+                              "warm start, every Picard iteration and the step end in one "
+                              "launch; split-bf16 MFMA transforms)",
                     "bound": "mfma", "achieved": round(tfs, 3), "peak": BF16_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(tfs / BF16_PEAK_TFLOPS, 5),
                     "traffic": None, "avg_launch_us": round(surf_ms * 1e3, 2),
                     "bf16_flops_per_launch": rflops,
                     "picard_iterations_per_cuboid_mean": round(float(iters.mean()), 2),
                     "active_cu_fraction": round(active, 4),
+                    "team_sizes": sorted((t for t in teams if t > 0), reverse=True),
                     "frac_of_active_cus": round(tfs / (BF16_PEAK_TFLOPS * active), 5)}
     elif share_surf >= share_vol:
         # the binding roofline of the surface launches is the larger of the two lower
