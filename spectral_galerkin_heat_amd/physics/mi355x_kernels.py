@@ -716,6 +716,16 @@ class _PlainIter:
     # short sleeps
     SPIN_S, YIELD_S, NAP_S = 2e-4, 2e-3, 5e-5
 
+    def ready(self, k):
+        """Whether iteration k's fused pass has written its norms (no wait)."""
+        p = k & 1
+        return self.norms_host[4 * p + 3] == float(self.seq_of[p])
+
+    def read(self, k):
+        """The norms of iteration k (after ``ready(k)``)."""
+        h, p = self.norms_host, k & 1
+        return float(h[4 * p]), float(h[4 * p + 1]), float(h[4 * p + 2])
+
     def wait(self, k, timeout_s=60.0):
         """The norms of iteration k once its fused pass has written them."""
         import time as _time
@@ -767,7 +777,24 @@ def picard_step(solver, t, dt):
     iteration: surface temperature → evaporation/convection flux → projection →
     [latent and property-correction volume sources] → fused ETD1 + relaxation + FP64
     norms + next-iterate surface contraction, one host read of the residual ratio.
+    The host waits for each residual (``picard_step_steps`` is the same step as a
+    generator that yields instead, so that many solvers can share the host thread).
     """
+    gen = picard_step_steps(solver, t, dt, wait=True)
+    while True:
+        try:
+            next(gen)
+        except StopIteration as done:
+            return done.value
+
+
+def picard_step_steps(solver, t, dt, wait=False, timeout_s=60.0):
+    """``picard_step`` as a generator: enqueues the step's kernels on the current stream
+    and, whenever the next Picard decision needs a residual the device has not written
+    yet, yields (``wait=False``) or waits for it (``wait=True``); its return value
+    (``StopIteration.value``) is ``(state, metrics)``. A caller that resumes it must make
+    the stream it started on current again."""
+    import time as _time
     S = solver.state
     g, b = S.grid, S.buffers
     ctx = solver.context
@@ -850,7 +877,20 @@ def picard_step(solver, t, dt):
     for iter_k in range(maxit):
         if iter_k + 1 < maxit:
             enqueue(iter_k + 1)
-        rms_diff, _, rel = pi.wait(iter_k)
+        if wait:
+            rms_diff, _, rel = pi.wait(iter_k, timeout_s)
+        else:
+            t_wait = None
+            while not pi.ready(iter_k):
+                if t_wait is None:
+                    t_wait = _time.perf_counter()
+                elif _time.perf_counter() - t_wait > timeout_s:
+                    torch.cuda.synchronize()   # raises a device error if one occurred
+                    if not pi.ready(iter_k):
+                        raise _lib.SGHError(f"Picard iteration {iter_k}: no residual after "
+                                            f"{timeout_s} s")
+                yield
+            rms_diff, _, rel = pi.read(iter_k)
         if solver.track_picard_history:
             rho_k = None
             if iter_k > 0 and solver.picard_history:
