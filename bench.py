@@ -61,6 +61,9 @@ def parse():
                     help="pool size of the last sweep point, small enough that admission "
                          "by memory footprint has to queue requests")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-plain", type=int, default=10,
+                    help="plain-lane requests (latent heat, convection, T-dependent "
+                         "properties, box mode) mixed into a 100-request serving point")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--box-steps", type=int, default=40,
@@ -493,7 +496,7 @@ def main():
         shapes = [serving_request(rng) for _ in range(args.serve_requests)]
         big = sum(1 for cfg, _ in shapes if np.prod(cfg["domain"]["mesh"]) >= 256 * 128 * 64)
 
-        def serve(rate, budget, checks):
+        def serve(rate, budget, checks, shapes=shapes):
             arr = poisson_arrivals(rate, len(shapes), np.random.default_rng(2))
             reqs = []
             for i, (cfg, lp) in enumerate(shapes):
@@ -516,6 +519,11 @@ def main():
                    "peak_active": sm["peak_active"],
                    "peak_pool_mib": round(sm["peak_pool_bytes"] / 2**20, 1),
                    "wall_s": round(sm["wall_s"], 3)}
+            if sm["plain_requests"]:
+                pl = [r.latency for r in rep.requests if r.plain]
+                out["plain_requests"] = sm["plain_requests"]
+                out["plain_p50_latency_s"] = round(float(np.percentile(pl, 50)), 4)
+                out["plain_max_latency_s"] = round(float(np.max(pl)), 4)
             if checks:
                 out["fp64_recheck_rel_l2"] = [float(f"{c:.3e}") for c in fp64]
             return out
@@ -532,6 +540,24 @@ def main():
             # the same load with a small pool: admission by footprint saturates and queues
             sweep.append(serve(rates[-1], args.serve_tight_mib << 20, False))
             log(f"sweep (tight pool): {sweep[-1]}")
+        if args.serve_plain > 0:
+            # production physics beside the batched requests: every tenth of the first 100
+            # requests replaced by a plain-lane case (T-dependent track with latent heat and
+            # convection, FP32; every fifth a box-mode track, FP64)
+            from spectral_galerkin_heat_amd.workloads import single_track_box, single_track_tdep
+            prng = np.random.default_rng(3)
+            mixed = list(shapes[:100])
+            for j in range(min(args.serve_plain, 10)):
+                n = int(prng.integers(40, 101))
+                if j % 5 == 4:
+                    case = single_track_box(mesh=(40, 24, 60), n_steps=n,
+                                            box_size=(0.3e-3, 0.3e-3, 0.05e-3))
+                else:
+                    case = single_track_tdep(mesh=(64, 32, 16), n_steps=n, dtype="float32")
+                mixed[10 * j] = case
+            serving_mixed = serve(args.serve_rate * 2, 8 << 30, True, shapes=mixed)
+            log(f"serving (mixed physics): {serving_mixed}")
+            serving["mixed_physics"] = serving_mixed
 
     out = {
         "metric": METRIC,

@@ -24,7 +24,7 @@ from ..backends import get_backend
 from ..physics.mi355x_kernels import picard_step_steps
 from ..solvers.spectral import SpectralSolver
 
-__all__ = ["PlainBatch"]
+__all__ = ["PlainBatch", "PlainLane"]
 
 
 class PlainBatch:
@@ -106,3 +106,70 @@ class PlainBatch:
                  float(bool(m["picard_converged"])), float(m["picard_rel_err"]))
                 for m in self.metrics[i]]
         return np.array(rows, dtype=np.float64).reshape(-1, 5)
+
+
+class PlainLane:
+    """The plain-path side of the request loop: cuboids join and leave between steps.
+
+    Each admitted request gets a solver on a stream of its own; ``advance`` moves every
+    member forward by whole steps, round-robining the host over their Picard loops as
+    ``PlainBatch.run`` does, and reports the requests whose last step has been enqueued.
+    """
+
+    def __init__(self, max_active=16):
+        self.max_active = int(max_active)
+        self._members = {}      # key -> dict(solver, ctx, stream, n_steps, done, metrics, gen)
+        self._backend = None
+
+    def __len__(self):
+        return len(self._members)
+
+    def has_room(self):
+        return len(self._members) < self.max_active
+
+    def add(self, key, ctx, n_steps):
+        if self._backend is None:
+            self._backend = get_backend("mi355x")
+        s = SpectralSolver(self._backend, ctx)
+        s.initialize()
+        st = torch.cuda.Stream()
+        st.wait_stream(torch.cuda.current_stream())
+        m = {"solver": s, "ctx": ctx, "stream": st, "n_steps": int(n_steps), "done": 0,
+             "metrics": [], "gen": None}
+        self._members[key] = m
+
+    def _one_step(self, m):
+        dt = m["ctx"].num.dt
+        _, met = yield from picard_step_steps(m["solver"], m["done"] * dt, dt)
+        m["metrics"].append(met)
+        m["done"] += 1
+
+    def advance(self, steps=1):
+        """Every member runs up to ``steps`` more steps (fewer if its count ends first);
+        returns the keys of members whose last step is enqueued."""
+        todo = {k: min(int(steps), m["n_steps"] - m["done"]) for k, m in self._members.items()}
+        todo = {k: v for k, v in todo.items() if v > 0}
+        while todo:
+            for k in list(todo):
+                m = self._members[k]
+                with torch.cuda.stream(m["stream"]):
+                    if m["gen"] is None:
+                        m["gen"] = self._one_step(m)
+                    try:
+                        next(m["gen"])
+                    except StopIteration:
+                        m["gen"] = None
+                        todo[k] -= 1
+                        if todo[k] == 0:
+                            del todo[k]
+        return [k for k, m in self._members.items() if m["done"] >= m["n_steps"]]
+
+    def take(self, key):
+        """Final modal state (device) and per-step metrics [n_steps, 5] of a finished
+        member, which leaves the lane."""
+        m = self._members.pop(key)
+        m["stream"].synchronize()
+        rows = [(float(x["P_laser"]), float(x["T_surface_max"]), float(x["n_evap_iter"]),
+                 float(bool(x["picard_converged"])), float(x["picard_rel_err"]))
+                for x in m["metrics"]]
+        return m["solver"].state.a, np.array(rows, dtype=np.float64).reshape(-1, 5)

@@ -10,7 +10,10 @@ time. The server keeps every admitted simulation's modal state resident in HBM i
   its packed arrays (admission by memory footprint; a request that does not fit waits),
 
 so requests that arrive while others are on the GPU join the running batch and their
-work overlaps on the device. A simulation's arithmetic depends only on its own data
+work overlaps on the device. Requests whose physics the surface-collapsed pool does not
+cover (latent heat, convection, temperature-dependent properties, a fine box) go to a
+plain lane (``PlainLane``: an unchanged ``SpectralSolver`` per request on a stream of its
+own, ``plain_slots`` at a time), which advances one step per tick beside the pool. A simulation's arithmetic depends only on its own data
 (fixed per-cuboid work decomposition and reduction order), so its result is the same
 alone and in any mix. Every ``check_every``-th request is re-solved alone in FP64 and
 its relative L2 against the served (mixed-precision) result is recorded.
@@ -26,7 +29,8 @@ import numpy as np
 import torch
 
 from .._lib import DeviceOutOfMemory
-from .batch import BatchEngine, BatchPool, PoolFull, footprint_bytes
+from .batch import BatchEngine, BatchPool, PoolFull, check_batchable, footprint_bytes
+from .plain_batch import PlainLane
 
 __all__ = ["Request", "ScanServer", "poisson_arrivals"]
 
@@ -45,6 +49,7 @@ class Request:
     metrics: object = None         # [n_steps, 5] numpy
     fp64_rel_l2: float = float("nan")
     footprint: int = 0
+    plain: bool = False            # served by the plain lane
 
     @property
     def latency(self) -> float:
@@ -84,6 +89,10 @@ class ServeReport:
             "peak_active": self.peak_active,
             "peak_pool_bytes": self.peak_bytes,
             "requests": len(self.requests),
+            "plain_requests": sum(1 for r in self.requests if r.plain),
+            "plain_p50_latency_s": (float(np.percentile([r.latency for r in self.requests
+                                                         if r.plain], 50))
+                                    if any(r.plain for r in self.requests) else None),
             "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,
             "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None,
             "mean_latency_s": float(lat.mean()) if len(lat) else None,
@@ -99,7 +108,7 @@ class ScanServer:
 
     def __init__(self, precision="mixed", max_cuboids=128, hbm_budget_bytes=8 << 30,
                  max_nz=64, max_steps=2048, steps_per_tick=8, check_every=20,
-                 keep_results=True):
+                 keep_results=True, plain_slots=16):
         word = 8 if precision == "fp64" else 4
         # split the budget between the packed state and the surface fields; a pool the
         # device cannot hold is retried at half the budget (down to MIN_BUDGET)
@@ -125,6 +134,7 @@ class ScanServer:
         self.keep_results = keep_results
         self.device_steps = 0
         self._last_refusal = ""
+        self.plain = PlainLane(plain_slots) if plain_slots > 0 else None
 
     def _admit(self, req: Request, now: float) -> bool:
         try:
@@ -146,20 +156,36 @@ class ScanServer:
         active: dict = {}             # slot -> request
         done: list = []
         for r in pending:
+            try:
+                check_batchable(r.ctx)
+            except ValueError:
+                if self.plain is None:
+                    raise
+                r.plain = True
+                continue
             if r.n_steps > self.max_steps:
                 raise ValueError(f"request {r.rid}: {r.n_steps} steps > max_steps {self.max_steps}")
             try:
                 self.pool.check_admissible(r.ctx, r.n_steps)
             except ValueError as exc:
                 raise ValueError(f"request {r.rid}: {exc}") from None
+        pwaiting: list = []
+        pactive: dict = {}            # rid -> request (plain lane)
         t0 = clock()
         finish_heap: list = []        # (device step at which it completes, rid)
         by_rid = {r.rid: r for r in queue}
         peak_active = peak_bytes = 0
-        while pending or waiting or active:
+        while pending or waiting or active or pwaiting or pactive:
             now = clock() - t0
             while pending and pending[0].arrival <= now:
-                waiting.append(pending.pop(0))
+                r = pending.pop(0)
+                (pwaiting if r.plain else waiting).append(r)
+            while pwaiting and self.plain.has_room():
+                r = pwaiting.pop(0)
+                self.plain.add(r.rid, r.ctx, r.n_steps)
+                r.admitted_at = now
+                r.footprint = 12 * r.modes * np.dtype(r.ctx.num.dtype).itemsize
+                pactive[r.rid] = r
             admitted = []
             for r in list(waiting):
                 if not self._admit(r, now):
@@ -168,23 +194,37 @@ class ScanServer:
                 active[r.slot] = r
                 heapq.heappush(finish_heap, (r.start_step + r.n_steps, r.rid))
                 admitted.append(r)
-            peak_active = max(peak_active, len(active))
+            peak_active = max(peak_active, len(active) + len(pactive))
             peak_bytes = max(peak_bytes, sum(r.footprint for r in active.values()))
             if not active and waiting:
                 raise RuntimeError(f"request {waiting[0].rid} does not fit an empty pool "
                                    f"(raise hbm_budget_bytes): {self._last_refusal}")
-            if not active:
+            if not active and not pactive:
                 if pending:
                     sleep(max(0.0, pending[0].arrival - (clock() - t0)))
                 continue
-            # run until the next completion, at most steps_per_tick steps
-            next_finish = finish_heap[0][0]
-            k = max(1, min(self.steps_per_tick, next_finish - self.device_steps))
-            self.pool.run(k)
-            self.device_steps += k
+            next_finish = None
+            if active:
+                # run until the next completion, at most steps_per_tick steps (enqueued; the
+                # device runs them while the host drives the plain lane)
+                next_finish = finish_heap[0][0]
+                k = max(1, min(self.steps_per_tick, next_finish - self.device_steps))
+                self.pool.run(k)
+                self.device_steps += k
+            if pactive:
+                # one step of every plain request; its kernels run beside the pool's
+                for rid in self.plain.advance(1):
+                    r = pactive.pop(rid)
+                    a, met = self.plain.take(rid)
+                    r.completed_at = clock() - t0
+                    if self.keep_results:
+                        r.result = a.clone()
+                    r.metrics = met
+                    done.append(r)
+            if not active:
+                continue
             if self.device_steps >= next_finish or pending:
                 self.pool.synchronize()
-            now = clock() - t0
             while finish_heap and finish_heap[0][0] <= self.device_steps:
                 _, rid = heapq.heappop(finish_heap)
                 r = by_rid[rid]
@@ -209,6 +249,9 @@ class ScanServer:
         for r in report.requests:
             if r.rid % self.check_every != 0 or r.result is None:
                 continue
+            if r.plain:
+                r.fp64_rel_l2 = _plain_fp64_rel_l2(r)
+                continue
             ref = BatchEngine([r.ctx], precision="fp64", n_steps=r.n_steps)
             ref.run(r.n_steps)
             ref.synchronize()
@@ -220,3 +263,25 @@ class ScanServer:
 
     def close(self):
         self.pool.close()
+
+
+def _plain_fp64_rel_l2(r):
+    """Relative L2 of a plain-lane result against the same request re-solved alone with
+    the FP64 plain solver (pinned to the FP64 oracle at 1e-9)."""
+    import copy
+
+    from ..backends import get_backend
+    from ..solvers.spectral import SpectralSolver
+    import numpy as _np
+    ctx = copy.deepcopy(r.ctx)
+    ctx.num.dtype = _np.float64          # the served run's constants, FP64 arithmetic
+    if getattr(ctx.num, "transform", "mfma") == "bf16x3":
+        ctx.num.transform = "mfma"
+    s = SpectralSolver(get_backend("mi355x"), ctx)
+    s.initialize()
+    dt = ctx.num.dt
+    for k in range(r.n_steps):
+        s.step(k * dt, dt)
+    b = s.state.a.double()
+    a = r.result.double()
+    return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))

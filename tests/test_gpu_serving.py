@@ -161,3 +161,45 @@ def test_c5_poisson_stream_with_rasters():
     checks = srv.fp64_checks(rep)
     assert len(checks) == 3 and max(checks) < 1e-4, checks
     srv.close()
+
+
+def test_serving_mixes_plain_lane_requests_and_each_equals_its_solo_run():
+    # latent-heat, convection and T-dependent requests go to the plain lane beside the
+    # batched pool; served results equal the solo runs bit for bit, and the FP64 re-check
+    # of a plain request holds 1e-4
+    from spectral_galerkin_heat_amd.backends import get_backend
+    from spectral_galerkin_heat_amd.engine.serving import Request, ScanServer
+    from spectral_galerkin_heat_amd.solvers.spectral import SpectralSolver
+    from spectral_galerkin_heat_amd.workloads import single_track_box, single_track_tdep
+    reqs = _requests(6, seed=21, steps_lo=6, steps_hi=14, gap=0.003, burst=2)
+    plain_cases = [single_track_tdep(mesh=(32, 16, 8), n_steps=8, dtype="float32"),
+                   single_track_box(mesh=(40, 24, 60), n_steps=5,
+                                    box_size=(0.3e-3, 0.3e-3, 0.05e-3)),
+                   single_track_tdep(mesh=(48, 32, 16), n_steps=7)]
+    for j, (cfg, lp) in enumerate(plain_cases):
+        sim = cfg["simulation"]
+        steps = int(round(sim["duration"] / sim["dt"]))
+        reqs.append(Request(rid=100 + 20 * j, ctx=make_context(cfg, lp), n_steps=steps,
+                            arrival=0.002 * j))
+    srv = ScanServer(precision="mixed", max_cuboids=8, hbm_budget_bytes=256 << 20,
+                     max_steps=64, check_every=20, plain_slots=2)
+    rep = srv.serve(reqs)
+    srv.fp64_checks(rep)
+    summ = rep.summary()
+    assert summ["requests"] == len(reqs) and summ["plain_requests"] == 3
+    for r in rep.requests:
+        if not r.plain:
+            continue
+        s = SpectralSolver(get_backend("mi355x"), r.ctx)
+        s.initialize()
+        dt = r.ctx.num.dt
+        for k in range(r.n_steps):
+            s.step(k * dt, dt)
+        torch.cuda.synchronize()
+        assert torch.equal(s.state.a, r.result), r.rid
+        assert r.metrics.shape == (r.n_steps, 5)
+        print(f"plain request {r.rid} {tuple(r.result.shape)}: identical to its solo run, "
+              f"latency {r.latency:.3f} s, FP64 re-check {r.fp64_rel_l2:.2e}")
+        if not np.isnan(r.fp64_rel_l2):
+            assert r.fp64_rel_l2 < 1e-4
+    srv.close()

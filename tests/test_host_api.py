@@ -192,11 +192,13 @@ def test_scan_server_rejects_a_request_that_never_fits():
         def add(self, ctx, n_steps):
             raise PoolFull("no room")
 
+    from spectral_galerkin_heat_amd.workloads import make_context, single_track_constant
     srv = ScanServer.__new__(ScanServer)
     srv.pool, srv.precision, srv.max_steps = NoRoomPool(), "mixed", 64
     srv.steps_per_tick, srv.check_every, srv.keep_results, srv.device_steps = 8, 20, False, 0
-    srv._last_refusal = ""
-    req = Request(rid=7, ctx=None, n_steps=4, arrival=0.0)
+    srv._last_refusal, srv.plain = "", None
+    cfg, lp = single_track_constant(mesh=(16, 8, 4), n_steps=4)
+    req = Request(rid=7, ctx=make_context(cfg, lp), n_steps=4, arrival=0.0)
     with pytest.raises(RuntimeError, match="request 7 does not fit.*no room"):
         srv.serve([req], clock=lambda: 0.0, sleep=lambda s: None)
 
@@ -227,9 +229,13 @@ def test_pool_admission_refuses_what_no_free_room_can_hold():
             BatchPool.check_admissible(pool, ctx, n_steps)
 
     srv = ScanServer.__new__(ScanServer)
-    srv.pool, srv.max_steps = Pool(), 64
+    srv.pool, srv.max_steps, srv.plain = Pool(), 64, None
     req = Request(rid=3, ctx=make_context(deep, lp), n_steps=4, arrival=0.0)
     with pytest.raises(ValueError, match="request 3: nz = 24 exceeds"):
+        srv.serve([req], clock=lambda: 0.0, sleep=lambda s: None)
+    # physics outside the batched engine: refused without a plain lane
+    req = Request(rid=4, ctx=make_context(tdep, lp2), n_steps=2, arrival=0.0)
+    with pytest.raises(ValueError, match="latent heat"):
         srv.serve([req], clock=lambda: 0.0, sleep=lambda s: None)
 
 
