@@ -40,8 +40,10 @@ constexpr int kResWaves = kResThreads / kWave;
 // iteration k < 20: first inverse stage, second inverse stage + evaporation, q exchange,
 // first forward stage, second forward stage + relaxation, X exchange, norm sums done (the
 // exchanges of a single-member team take no time); 143 step end done; 144, 145 the cuboid's
-// nx and ny, with the team size and the member index in bits 16 and up.
-constexpr int kResTsSlots = 146, kResTsBlocks = 1024;
+// nx and ny, with the team size and the member index in bits 16 and up; 146 + 4s + {0..3}
+// inside stage s (0: first inverse, 1: second forward) of iteration 5: stage entry, matrix
+// fragments in registers, tiles done, barrier passed (wave 0's view).
+constexpr int kResTsSlots = 154, kResTsBlocks = 1024;
 __device__ unsigned long long g_res_ts[kResTsBlocks * kResTsSlots];
 #define RES_TS(slot)                                                                        \
   do {                                                                                      \
@@ -228,7 +230,12 @@ __device__ __forceinline__ void res_tiles(const ResTiles& t, ResW& w, const ResB
 // epilogue's stores are visible. A wave without tiles in this stage reloads them at once.
 template <int EARLY = 0, typename Next, typename Pre, typename Epi>
 __device__ __forceinline__ void res_stage(const ResTiles& t, ResW& w, int KS, const ResBuf& src,
-                                          int lane, Next&& next, Pre&& pre, Epi&& epi) {
+                                          int lane, Next&& next, Pre&& pre, Epi&& epi,
+                                          int ts = -1) {
+  (void)ts;   // diagnostic builds: stamps ts .. ts + 3 inside the stage
+#ifdef SGH_RESIDENT_TIMESTAMPS
+  if (ts >= 0) RES_TS(ts);
+#endif
   if (t.tt >= 0) {
     lane = opaque_lane(lane);
     // the fragments are pinned in registers for the stage here, after the barrier that
@@ -236,16 +243,25 @@ __device__ __forceinline__ void res_stage(const ResTiles& t, ResW& w, int KS, co
     // register allocator reloads them from memory for every tile
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) asm volatile("" : "+v"(w.h[ks]), "+v"(w.l[ks]));
+#ifdef SGH_RESIDENT_TIMESTAMPS
+    if (ts >= 0) RES_TS(ts + 1);
+#endif
     switch (KS) {
       case 1: res_tiles<1, EARLY>(t, w, src, lane, next, pre, epi); break;
       case 2: res_tiles<2, EARLY>(t, w, src, lane, next, pre, epi); break;
       case 3: res_tiles<3, EARLY>(t, w, src, lane, next, pre, epi); break;
       default: res_tiles<4, EARLY>(t, w, src, lane, next, pre, epi); break;
     }
+#ifdef SGH_RESIDENT_TIMESTAMPS
+    if (ts >= 0) RES_TS(ts + 2);
+#endif
   } else {
     next(0, 4);
   }
   __syncthreads();
+#ifdef SGH_RESIDENT_TIMESTAMPS
+  if (ts >= 0) RES_TS(ts + 3);
+#endif
 }
 
 // v plus the value of the lane the DPP control selects (0 where it selects none, and in the
@@ -608,12 +624,12 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
                 res_store(buf1, tA.aout(q, lane), tA.tcol(lane), o);
               });
   };
-  auto stage_inv1 = [&]() __attribute__((always_inline)) {    // V[m][j], stored [j][m]
+  auto stage_inv1 = [&](int ts = -1) __attribute__((always_inline)) {   // V[m][j], as [j][m]
     res_stage(tA, W, KY, buf0, lane, w_inv2, no_pre,
               [&](int q, int, const f4v& a, int lane) __attribute__((always_inline)) {
                 const float o[4] = {a[0], a[1], a[2], a[3]};
                 res_store(buf1, tA.aout(q, lane), tA.tcol(lane), o);
-              });
+              }, ts);
   };
 
   // ---- warm start: G = fwd·B̂y q_net B̂xᵀ; Xinv = ZE + G∘Ztt into buffer 0 ----
@@ -687,7 +703,7 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
       pre_evap(0, 0, lane);
       pre_evap(1, 1, lane);
     }
-    stage_inv1();
+    stage_inv1(k == 5 ? 146 : -1);
     if (k < 20) RES_TS(3 + 7 * k);
     {
       const int nxk = opaque(nx), nyk = opaque(ny);
@@ -765,7 +781,7 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
                       st_x<float, true>(xN + 2 * kResNormSlots + slot, p2);
                     }
                   }
-                });
+                }, k == 5 ? 150 : -1);
     }
     if (k < 20) RES_TS(7 + 7 * k);
     if (TP > 1) exchange(buf0, xX);

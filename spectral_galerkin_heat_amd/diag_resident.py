@@ -21,7 +21,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DIAG_DIR = os.path.join(HERE, "_diag")
-SLOTS, BLOCKS = 146, 1024
+SLOTS, BLOCKS = 154, 1024
 STAGES = ("inverse 1", "inverse 2 + evaporation", "q exchange", "forward 1",
           "forward 2 + relaxation", "X exchange", "norm sums")
 PER = len(STAGES)
@@ -95,6 +95,14 @@ def run(steps, team_max=8):
         shapes.setdefault(key, []).append(tot[b])
     for (sx, sy, st), v in sorted(shapes.items(), key=lambda kv: -np.mean(kv[1])):
         lines.append(f"| {sx} × {sy} | {st} | {len(v)} | {np.mean(v):.0f} |")
+    # inside two stages of iteration 5 (wave 0's stamps): entry, fragments in registers,
+    # tiles done, barrier passed
+    lines += ["", "Inside the stages of iteration 6 (wave 0, slowest workgroup): "
+              "fragments wait / tiles / barrier wait, cycles"]
+    for name, base in (("inverse 1", 146), ("forward 2 + relaxation", 150)):
+        e = [int(t[base + j]) for j in range(4)]
+        if min(e) > 0:
+            lines.append(f"* {name}: {e[1] - e[0]} / {e[2] - e[1]} / {e[3] - e[2]}")
     return "\n".join(lines)
 
 
