@@ -434,11 +434,23 @@ typedef struct SghPlainIter {
   double* partials; double* norms; double* norms_host;
   unsigned int* counter;   /* device word at 0: the fused pass's last-workgroup count */
   double seq;              /* written after the norms; the host waits for it */
+  void* side_stream;       /* NULL, or from sgh_plain_iter_fork_init */
+  void* ev_fork; void* ev_join;
 } SghPlainIter;
 
 /* sizeof(SghPlainIter), so a binding can check its mirror of the layout. */
 size_t sgh_plain_iter_size(void);
 int sgh_plain_iterate(const SghPlainIter* d, int parity, void* stream);
+/* A side stream (default priority) and two events for sgh_plain_iterate (no reference
+ * counterpart): with them, an iteration that has a latent source enqueues its surface
+ * chain (z-partial sum, 2-D inverse with the flux forcing, 2-D forward) on the side stream,
+ * forked from `stream` at the iteration's start, while the latent source's reconstruction,
+ * source and projection go on `stream`; `stream` waits for the side chain before the
+ * ETD1 update that needs both.  The kernels and their inputs are those of the serial
+ * order, so results are bit-identical.  sgh_plain_iter_fork_release destroys them (after
+ * the caller has synchronised) and sets the fields to NULL. */
+int sgh_plain_iter_fork_init(SghPlainIter* d);
+int sgh_plain_iter_fork_release(SghPlainIter* d);
 
 #ifdef __cplusplus
 }

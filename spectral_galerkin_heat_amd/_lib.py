@@ -50,6 +50,8 @@ SIGNATURES = {
     "sgh_shift3d_xy": [i32, i32, i32, i32, vp, f64, f64, i32, vp, vp],
     "sgh_box_face_peaks": [i32, i32, i32, i32, vp, vp, vp],
     "sgh_plain_iter_size": [],
+    "sgh_plain_iter_fork_init": [vp],
+    "sgh_plain_iter_fork_release": [vp],
     "sgh_plain_iterate": [vp, i32, vp],
     "sgh_box_latent_source": [i32, i32, i32, i32, i32, vp, i32, i32, i32, vp, i64, vp, i64, vp,
                               vp, f64, f64, f64, f64, f64, vp, vp, vp, vp],
@@ -131,7 +133,7 @@ class SghPlainIter(C.Structure):
            ("fBxT", vp), ("fByT", vp), ("fBzT", vp), ("fBx", vp), ("fBy", vp), ("fBz", vp),
            ("T_prev_box", vp), ("box_work", vp),
            ("partials", vp), ("norms", vp), ("norms_host", vp), ("counter", vp),
-           ("seq", C.c_double)])
+           ("seq", C.c_double), ("side_stream", vp), ("ev_fork", vp), ("ev_join", vp)])
 
 
 class SGHError(RuntimeError):

@@ -31,6 +31,38 @@ extern "C" {
 
 size_t sgh_plain_iter_size(void) { return sizeof(SghPlainIter); }
 
+int sgh_plain_iter_fork_init(SghPlainIter* d) {
+  SGH_REQUIRE(d && !d->side_stream && !d->ev_fork && !d->ev_join, SGH_ERR_ARG,
+              "sgh_plain_iter_fork_init: null descriptor or already initialised");
+  hipStream_t s = nullptr;
+  hipEvent_t a = nullptr, b = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&a, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    if (b) hipEventDestroy(b);
+    if (a) hipEventDestroy(a);
+    if (s) hipStreamDestroy(s);
+    return sgh::fail_hip(e, "sgh_plain_iter_fork_init");
+  }
+  d->side_stream = s;
+  d->ev_fork = a;
+  d->ev_join = b;
+  return SGH_OK;
+}
+
+int sgh_plain_iter_fork_release(SghPlainIter* d) {
+  SGH_REQUIRE(d, SGH_ERR_ARG, "sgh_plain_iter_fork_release: null descriptor");
+  hipError_t e = hipSuccess;
+  if (d->side_stream) e = hipStreamSynchronize(reinterpret_cast<hipStream_t>(d->side_stream));
+  if (d->ev_join) hipEventDestroy(reinterpret_cast<hipEvent_t>(d->ev_join));
+  if (d->ev_fork) hipEventDestroy(reinterpret_cast<hipEvent_t>(d->ev_fork));
+  if (d->side_stream) hipStreamDestroy(reinterpret_cast<hipStream_t>(d->side_stream));
+  d->side_stream = d->ev_fork = d->ev_join = nullptr;
+  if (e != hipSuccess) return sgh::fail_hip(e, "sgh_plain_iter_fork_release");
+  return SGH_OK;
+}
+
 int sgh_plain_iterate(const SghPlainIter* d, int parity, void* stream) {
   SGH_REQUIRE(d && (parity == 0 || parity == 1), SGH_ERR_ARG, "sgh_plain_iterate: bad args");
   SGH_REQUIRE(d->latent_mode >= 0 && d->latent_mode <= 2, SGH_ERR_ARG,
@@ -48,14 +80,32 @@ int sgh_plain_iterate(const SghPlainIter* d, int parity, void* stream) {
     rc = (expr);                 \
     if (rc != SGH_OK) return rc; \
   } while (0)
+  // The surface chain and the latent source depend only on a_old (and the z-partials the
+  // previous fused pass left), so with a side stream they run concurrently: the side
+  // stream starts after everything already on `stream` (the previous iteration, whose
+  // consumers of S_top and T_s are then done) and `stream` waits for it before the ETD1
+  // update.
+  const bool fork = d->side_stream && d->latent_mode != 0;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  hipStream_t side = reinterpret_cast<hipStream_t>(d->side_stream);
+  void* sst = fork ? d->side_stream : stream;
+  if (fork) {
+    hipError_t e = hipEventRecord(reinterpret_cast<hipEvent_t>(d->ev_fork), st);
+    if (e == hipSuccess) e = hipStreamWaitEvent(side, reinterpret_cast<hipEvent_t>(d->ev_fork), 0);
+    if (e != hipSuccess) return sgh::fail_hip(e, "sgh_plain_iterate fork");
+  }
   // top-surface temperature of a_old: the z-partials left by the previous fused pass
-  SGH_TRY(sgh_sum_parts(d->dtype, S, d->nchunks, d->zc, stream));
+  SGH_TRY(sgh_sum_parts(d->dtype, S, d->nchunks, d->zc, sst));
   SGH_TRY(sgh_sep2d_forcing(d->dtype, ny, nx, d->sByT, d->sBx, d->zc, 1.0, d->q_las, d->Pa,
                             d->T_boil, d->dH, d->R_v, d->T_liq, d->h_top, d->T0,
                             d->rescale ? d->prog : nullptr, d->kbar, d->T_s,
-                            d->q_evap[parity], d->q_tmp, stream));
+                            d->q_evap[parity], d->q_tmp, sst));
   SGH_TRY(sgh_sep2d(d->dtype, 1, ny, nx, d->sBy, d->sBxT, d->q_tmp, 1, d->dxdy, d->S_top,
-                    stream));
+                    sst));
+  if (fork) {
+    hipError_t e = hipEventRecord(reinterpret_cast<hipEvent_t>(d->ev_join), side);
+    if (e != hipSuccess) return sgh::fail_hip(e, "sgh_plain_iterate join");
+  }
   const void* Q_modes = nullptr;
   const void* lprog = d->rescale ? d->prog : nullptr;
   if (d->latent_mode == 2) {
@@ -83,6 +133,10 @@ int sgh_plain_iterate(const SghPlainIter* d, int parity, void* stream) {
     SGH_TRY(volume_transform(d, d->vBx, d->vBy, d->vBz, d->Q_latent[parity], d->dV,
                              d->Q_modes[parity], stream));
     Q_modes = d->Q_modes[parity];
+  }
+  if (fork) {
+    hipError_t e = hipStreamWaitEvent(st, reinterpret_cast<hipEvent_t>(d->ev_join), 0);
+    if (e != hipSuccess) return sgh::fail_hip(e, "sgh_plain_iterate join");
   }
   const void* wb = d->have_bottom ? d->w_bot : nullptr;
   const void* Sb = d->have_bottom ? d->S_bot_raw : nullptr;

@@ -698,6 +698,15 @@ class _PlainIter:
         d.norms_host = hp.value
         self.seq = 0
         self.seq_of = [0, 0]
+        # a latent source's chain runs beside the surface chain on a side stream
+        self.forked = False
+        if d.latent_mode != 0 and _PlainIter.FORK:
+            call("sgh_plain_iter_fork_init", self.ref)
+            self.forked = True
+
+    # whether new descriptors get a side stream for the surface chain (results are
+    # bit-identical either way)
+    FORK = True
 
     def __del__(self):
         ptr_ = getattr(self, "_host_ptr", None)
@@ -705,6 +714,9 @@ class _PlainIter:
             try:
                 # a speculative iteration may still be writing these words
                 torch.cuda.synchronize()
+                if getattr(self, "forked", False):
+                    _lib.lib.sgh_plain_iter_fork_release(self.ref)
+                    self.forked = False
                 _lib.lib.sgh_host_free(ptr_)
             except Exception:  # interpreter shutdown
                 pass

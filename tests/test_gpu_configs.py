@@ -125,3 +125,33 @@ def test_c3_raster_mixed_graph_within_1e4_and_melt_pool():
     assert checked >= 2   # the laser was on, and melting, at two or more checkpoints
     for e in engines.values():
         e.close()
+
+
+@pytest.mark.parametrize("case", ["c2_grid_latent", "box_latent"])
+def test_plain_side_stream_is_bit_identical(case):
+    # the surface chain of each Picard iteration on a side stream beside the latent source
+    # (the default) against everything on one stream: same states and metrics bit for bit
+    from spectral_galerkin_heat_amd.physics import mi355x_kernels as K
+    from spectral_galerkin_heat_amd.workloads import single_track_box
+    if case == "c2_grid_latent":
+        cfg, lp = single_track_tdep(n_steps=4)
+    else:
+        cfg, lp = single_track_box(mesh=(40, 24, 60), n_steps=4)
+    runs = {}
+    try:
+        for fork in (True, False):
+            K._PlainIter.FORK = fork
+            s, ctx = _plain(cfg, lp)
+            dt = ctx.num.dt
+            mets = []
+            for n in range(4):
+                _, m = s.step(n * dt, dt)
+                mets.append((float(m["P_laser"]), float(m["T_surface_max"]),
+                             int(m["n_evap_iter"]), float(m["picard_rel_err"])))
+            torch.cuda.synchronize()
+            runs[fork] = (s.state.a.clone(), mets, s.state._plain_iter.forked)
+    finally:
+        K._PlainIter.FORK = True
+    assert runs[True][2] is True and runs[False][2] is False
+    assert torch.equal(runs[True][0], runs[False][0])
+    assert runs[True][1] == runs[False][1]
