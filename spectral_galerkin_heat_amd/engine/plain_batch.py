@@ -40,6 +40,7 @@ class PlainBatch:
         self.solvers = []
         for ctx in self.contexts:
             s = SpectralSolver(backend, ctx)
+            s._plain_fork = False   # concurrent solvers: no side stream per solver
             s.initialize()
             self.solvers.append(s)
         # default priority (the only one used anywhere in this package)
@@ -131,6 +132,7 @@ class PlainLane:
         if self._backend is None:
             self._backend = get_backend("mi355x")
         s = SpectralSolver(self._backend, ctx)
+        s._plain_fork = False   # concurrent solvers: no side stream per solver
         s.initialize()
         st = torch.cuda.Stream()
         st.wait_stream(torch.cuda.current_stream())

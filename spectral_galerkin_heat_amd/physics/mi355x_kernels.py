@@ -698,15 +698,21 @@ class _PlainIter:
         d.norms_host = hp.value
         self.seq = 0
         self.seq_of = [0, 0]
-        # a latent source's chain runs beside the surface chain on a side stream
+        # a latent source's chain runs beside the surface chain on a side stream: for a
+        # solver that has the GPU to itself and enough modes that the overlap outweighs the
+        # two cross-stream waits per iteration (solvers stepped concurrently, as in
+        # PlainBatch, turn it off: there the waits stall the hardware queues they share)
         self.forked = False
-        if d.latent_mode != 0 and _PlainIter.FORK:
+        if (d.latent_mode != 0 and _PlainIter.FORK and getattr(solver, "_plain_fork", True)
+                and nx * ny * nz >= _PlainIter.FORK_MIN_MODES):
             call("sgh_plain_iter_fork_init", self.ref)
             self.forked = True
 
     # whether new descriptors get a side stream for the surface chain (results are
-    # bit-identical either way)
+    # bit-identical either way), and from how many modes (64×32×16 tracks measured slower
+    # with it, 128×64×32 and 80×48×120 faster)
     FORK = True
+    FORK_MIN_MODES = 100_000
 
     def __del__(self):
         ptr_ = getattr(self, "_host_ptr", None)

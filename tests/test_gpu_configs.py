@@ -138,6 +138,7 @@ def test_plain_side_stream_is_bit_identical(case):
     else:
         cfg, lp = single_track_box(mesh=(40, 24, 60), n_steps=4)
     runs = {}
+    K._PlainIter.FORK_MIN_MODES = 0   # the side stream at any size
     try:
         for fork in (True, False):
             K._PlainIter.FORK = fork
@@ -152,6 +153,7 @@ def test_plain_side_stream_is_bit_identical(case):
             runs[fork] = (s.state.a.clone(), mets, s.state._plain_iter.forked)
     finally:
         K._PlainIter.FORK = True
+        K._PlainIter.FORK_MIN_MODES = 100_000
     assert runs[True][2] is True and runs[False][2] is False
     assert torch.equal(runs[True][0], runs[False][0])
     assert runs[True][1] == runs[False][1]
