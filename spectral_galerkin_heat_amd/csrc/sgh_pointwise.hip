@@ -53,15 +53,16 @@ k_etd_tables(F* __restrict__ E, F* __restrict__ KK, int nx, int ny, int nz, doub
 // ---------------------------------------------------------------------------------
 // Modal pointwise updates
 // ---------------------------------------------------------------------------------
-template <typename F>
+// Volume kernels index in I: unsigned (32-bit division) when the volume has fewer than
+// 2^31 elements, long long otherwise (64-bit division costs several times as much).
+template <typename F, typename I>
 __global__ void __launch_bounds__(kThreads)
 k_etd1_update(const F* __restrict__ aK, const F* __restrict__ KK, const F* __restrict__ wt,
               const F* __restrict__ St, const F* __restrict__ wb, const F* __restrict__ Sb,
-              const F* __restrict__ Q, F* __restrict__ out, int S, long long N) {
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < N;
-       i += (long long)gridDim.x * blockDim.x) {
-    const int p = (int)(i / S);
-    const int s = (int)(i - (long long)p * S);
+              const F* __restrict__ Q, F* __restrict__ out, int S, I N) {
+  for (I i = blockIdx.x * (I)blockDim.x + threadIdx.x; i < N; i += (I)gridDim.x * blockDim.x) {
+    const int p = (int)(i / (I)S);
+    const int s = (int)(i - (I)p * S);
     F f = wt[p] * St[s];
     if (wb) f += wb[p] * Sb[s];
     if (Q) f += Q[i];
@@ -77,14 +78,13 @@ k_add_volume(F* __restrict__ a, const F* __restrict__ KK, const F* __restrict__ 
     a[i] += KK[i] * Q[i];
 }
 
-template <typename F>
+template <typename F, typename I>
 __global__ void __launch_bounds__(kThreads)
 k_add_surface(F* __restrict__ a, const F* __restrict__ KK, const F* __restrict__ w,
-              const F* __restrict__ B, int S, long long N) {
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < N;
-       i += (long long)gridDim.x * blockDim.x) {
-    const int p = (int)(i / S);
-    a[i] += KK[i] * w[p] * B[i - (long long)p * S];
+              const F* __restrict__ B, int S, I N) {
+  for (I i = blockIdx.x * (I)blockDim.x + threadIdx.x; i < N; i += (I)gridDim.x * blockDim.x) {
+    const int p = (int)(i / (I)S);
+    a[i] += KK[i] * w[p] * B[i - (I)p * S];
   }
 }
 
@@ -438,47 +438,56 @@ k_latent(const F* __restrict__ T, const F* __restrict__ Tp, long long N, LatentP
 }
 
 // numpy.gradient (edge_order=1) along one axis of stride `st` and length `n` at index c.
-template <typename F>
-__device__ __forceinline__ F grad1(const F* __restrict__ f, long long i, int c, int n,
-                                   long long st, F inv_d) {
+template <typename F, typename I>
+__device__ __forceinline__ F grad1(const F* __restrict__ f, I i, int c, int n, I st, F inv_d) {
   if (n < 2) return F(0);
   if (c == 0) return (f[i + st] - f[i]) * inv_d;
   if (c == n - 1) return (f[i] - f[i - st]) * inv_d;
   return (f[i + st] - f[i - st]) * (F(0.5) * inv_d);
 }
 
-template <typename F>
+// (x, y, z) of flat index i of an nx × ny × nz volume
+template <typename I>
+__device__ __forceinline__ void xyz_of(I i, int nx, int ny, int& x, int& y, int& z) {
+  const I r = i / (I)nx;
+  const I q = r / (I)ny;
+  x = (int)(i - r * (I)nx);
+  y = (int)(r - q * (I)ny);
+  z = (int)q;
+}
+
+template <typename F, typename I>
 __global__ void __launch_bounds__(kThreads)
 k_corr_grad(const F* __restrict__ T, const PropProgram* __restrict__ prog, F kbar, int nx,
             int ny, int nz, F idx, F idy, F idz, F* __restrict__ gx, F* __restrict__ gy,
             F* __restrict__ gz) {
-  const long long S = (long long)nx * ny, N = S * nz;
-  for (long long i = blockIdx.x * (long long)kThreads + threadIdx.x; i < N;
-       i += (long long)gridDim.x * kThreads) {
-    const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / S);
+  const I S = (I)nx * ny, N = S * nz;
+  for (I i = blockIdx.x * (I)kThreads + threadIdx.x; i < N; i += (I)gridDim.x * kThreads) {
+    int x, y, z;
+    xyz_of<I>(i, nx, ny, x, y, z);
     const F t = T[i];
     const F kp = prog_blend<F>(prog, kKSolid, t, prog_fl<F>(prog, t)) - kbar;
-    gx[i] = kp * grad1(T, i, x, nx, 1, idx);
-    gy[i] = kp * grad1(T, i, y, ny, nx, idy);
-    gz[i] = kp * grad1(T, i, z, nz, S, idz);
+    gx[i] = kp * grad1<F, I>(T, i, x, nx, 1, idx);
+    gy[i] = kp * grad1<F, I>(T, i, y, ny, (I)nx, idy);
+    gz[i] = kp * grad1<F, I>(T, i, z, nz, S, idz);
   }
 }
 
-template <typename F>
+template <typename F, typename I>
 __global__ void __launch_bounds__(kThreads)
 k_corr_div(const F* __restrict__ gx, const F* __restrict__ gy, const F* __restrict__ gz,
            const F* __restrict__ T, const F* __restrict__ Tp,
            const PropProgram* __restrict__ prog, F abar, int nx, int ny, int nz, F idx, F idy,
            F idz, F idt, F* __restrict__ out) {
-  const long long S = (long long)nx * ny, N = S * nz;
-  for (long long i = blockIdx.x * (long long)kThreads + threadIdx.x; i < N;
-       i += (long long)gridDim.x * kThreads) {
-    const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / S);
+  const I S = (I)nx * ny, N = S * nz;
+  for (I i = blockIdx.x * (I)kThreads + threadIdx.x; i < N; i += (I)gridDim.x * kThreads) {
+    int x, y, z;
+    xyz_of<I>(i, nx, ny, x, y, z);
     const F t = T[i];
     const F fl = prog_fl<F>(prog, t);
     const F ap = prog_blend<F>(prog, kRhoSolid, t, fl) * prog_blend<F>(prog, kCSolid, t, fl) - abar;
-    const F div = grad1(gx, i, x, nx, 1, idx) + grad1(gy, i, y, ny, nx, idy) +
-                  grad1(gz, i, z, nz, S, idz);
+    const F div = grad1<F, I>(gx, i, x, nx, 1, idx) + grad1<F, I>(gy, i, y, ny, (I)nx, idy) +
+                  grad1<F, I>(gz, i, z, nz, S, idz);
     out[i] = div - ap * ((t - Tp[i]) * idt);
   }
 }
@@ -543,9 +552,15 @@ int sgh_etd1_update(int dtype, int nx, int ny, int nz, const void* aK, const voi
   const int S = nx * ny;
   const long long N = (long long)S * nz;
   SGH_DISPATCH(dtype, {
-    hipLaunchKernelGGL(k_etd1_update<F>, dim3(grid_for(N)), dim3(kThreads), 0, STREAM(stream),
-                       (const F*)aK, (const F*)KK, (const F*)wt, (const F*)St, (const F*)wb,
-                       (const F*)Sb, (const F*)Q, (F*)out, S, N);
+    if (N < (1LL << 31))
+      hipLaunchKernelGGL((k_etd1_update<F, unsigned>), dim3(grid_for(N)), dim3(kThreads), 0,
+                         STREAM(stream), (const F*)aK, (const F*)KK, (const F*)wt,
+                         (const F*)St, (const F*)wb, (const F*)Sb, (const F*)Q, (F*)out, S,
+                         (unsigned)N);
+    else
+      hipLaunchKernelGGL((k_etd1_update<F, long long>), dim3(grid_for(N)), dim3(kThreads), 0,
+                         STREAM(stream), (const F*)aK, (const F*)KK, (const F*)wt,
+                         (const F*)St, (const F*)wb, (const F*)Sb, (const F*)Q, (F*)out, S, N);
   });
   SGH_CHECK_LAUNCH("sgh_etd1_update");
   return SGH_OK;
@@ -569,8 +584,13 @@ int sgh_add_surface_source(int dtype, int nx, int ny, int nz, void* a, const voi
   const int S = nx * ny;
   const long long N = (long long)S * nz;
   SGH_DISPATCH(dtype, {
-    hipLaunchKernelGGL(k_add_surface<F>, dim3(grid_for(N)), dim3(kThreads), 0, STREAM(stream),
-                       (F*)a, (const F*)KK, (const F*)w, (const F*)B, S, N);
+    if (N < (1LL << 31))
+      hipLaunchKernelGGL((k_add_surface<F, unsigned>), dim3(grid_for(N)), dim3(kThreads), 0,
+                         STREAM(stream), (F*)a, (const F*)KK, (const F*)w, (const F*)B, S,
+                         (unsigned)N);
+    else
+      hipLaunchKernelGGL((k_add_surface<F, long long>), dim3(grid_for(N)), dim3(kThreads), 0,
+                         STREAM(stream), (F*)a, (const F*)KK, (const F*)w, (const F*)B, S, N);
   });
   SGH_CHECK_LAUNCH("sgh_add_surface_source");
   return SGH_OK;
@@ -879,12 +899,23 @@ int sgh_property_correction_source(int dtype, int nx, int ny, int nz, const void
     F* gy = gx + N;
     F* gz = gy + N;
     const F idx = (F)(1.0 / dx), idy = (F)(1.0 / dy), idz = (F)(1.0 / dz), idt = (F)(1.0 / dt);
-    hipLaunchKernelGGL(k_corr_grad<F>, dim3(grid_for(N)), dim3(kThreads), 0, STREAM(stream),
-                       (const F*)T, (const PropProgram*)prog, (F)kbar, nx, ny, nz, idx, idy,
-                       idz, gx, gy, gz);
-    hipLaunchKernelGGL(k_corr_div<F>, dim3(grid_for(N)), dim3(kThreads), 0, STREAM(stream),
-                       gx, gy, gz, (const F*)T, (const F*)T_prev, (const PropProgram*)prog,
-                       (F)abar, nx, ny, nz, idx, idy, idz, idt, (F*)out);
+    if (N < (1LL << 31)) {
+      hipLaunchKernelGGL((k_corr_grad<F, unsigned>), dim3(grid_for(N)), dim3(kThreads), 0,
+                         STREAM(stream), (const F*)T, (const PropProgram*)prog, (F)kbar, nx,
+                         ny, nz, idx, idy, idz, gx, gy, gz);
+      hipLaunchKernelGGL((k_corr_div<F, unsigned>), dim3(grid_for(N)), dim3(kThreads), 0,
+                         STREAM(stream), gx, gy, gz, (const F*)T, (const F*)T_prev,
+                         (const PropProgram*)prog, (F)abar, nx, ny, nz, idx, idy, idz, idt,
+                         (F*)out);
+    } else {
+      hipLaunchKernelGGL((k_corr_grad<F, long long>), dim3(grid_for(N)), dim3(kThreads), 0,
+                         STREAM(stream), (const F*)T, (const PropProgram*)prog, (F)kbar, nx,
+                         ny, nz, idx, idy, idz, gx, gy, gz);
+      hipLaunchKernelGGL((k_corr_div<F, long long>), dim3(grid_for(N)), dim3(kThreads), 0,
+                         STREAM(stream), gx, gy, gz, (const F*)T, (const F*)T_prev,
+                         (const PropProgram*)prog, (F)abar, nx, ny, nz, idx, idy, idz, idt,
+                         (F*)out);
+    }
   });
   SGH_CHECK_LAUNCH("sgh_property_correction_source");
   return SGH_OK;
