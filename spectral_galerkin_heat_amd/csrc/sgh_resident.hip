@@ -435,7 +435,8 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
   // (teams) the member's column band [16·a_lo, 16·a_hi) of every row < nxp of operand
   // buffer b → exchange field g (16-byte device-coherent stores); team barrier; rows < nxp
   // of g (the whole field, LDS layout) → b by device-coherent LDS DMA
-  auto exchange = [&](const ResBuf& b, const __bf16* g) __attribute__((always_inline)) {
+  auto exchange = [&](const ResBuf& b, const __bf16* g, auto&& after_barrier)
+                      __attribute__((always_inline)) {
     const int ur = 2 * (a_hi - a_lo);   // 16-byte units of the band in one row
     const int per = nxp * ur;           // ... in the hi (or the lo) field
     const long long gb = (long long)(g - xX) * 2;   // byte offset of g in the exchange area
@@ -451,6 +452,7 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
     }
     arrivals += TP;
     cuboid_barrier(P.bar + c, arrivals, P.loop_err, ticks);
+    after_barrier();   // loads that complete with the copy's wait
     const int units = nxp * (kResLd / 8);   // 16-byte units of rows < nxp
     for (int h = 0; h < 2; ++h) {
       const __bf16* src = g + h * kResBuf;
@@ -591,7 +593,7 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
     }
   }
   const double P_sum = res_sum(psum, sh[1]);   // its barriers also complete the operand stores
-  if (TP > 1) exchange(buf0, xQ);
+  if (TP > 1) exchange(buf0, xQ, [] {});
   RES_TS(1);
 
   const double fwd_scale = d.dx * d.dy * d.inv_sqrt_LxLy;
@@ -662,7 +664,7 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
                 res_store(buf0, tB.aout(q, lane), tB.tcol(lane), xo);   // [m][n]
               });
   }
-  if (TP > 1) exchange(buf0, xX);
+  if (TP > 1) exchange(buf0, xX, [] {});
 
   RES_TS(2);
   // ---- Picard loop ----
@@ -739,7 +741,7 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
                 });
     }
     if (k < 20) RES_TS(4 + 7 * k);
-    if (TP > 1) exchange(buf0, xQ);
+    if (TP > 1) exchange(buf0, xQ, [] {});
     if (k < 20) RES_TS(5 + 7 * k);
     // forward + relaxation + norm sums: per lane in FP32 over its four elements, per row
     // group of 16 lanes (4 rows × 16 columns of the tile) by DPP row shifts in FP32, to slot
@@ -784,25 +786,30 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
                 }, k == 5 ? 150 : -1);
     }
     if (k < 20) RES_TS(7 + 7 * k);
-    if (TP > 1) exchange(buf0, xX);
-    if (k < 20) RES_TS(8 + 7 * k);
     // the three sums over all tiles of the surface: lane a·8 + t sums the four row groups of
-    // tile (a, t) in FP64, then the lanes in a fixed order (every wave, every member)
+    // tile (a, t) in FP64, then the lanes in a fixed order (every wave, every member); a
+    // team's per-tile sums load from the exchange area while the X field is copied
     const bool tin = (lane >> 3) < NAY && (lane & 7) < NTB;
+    f4v r0 = {}, r1 = {}, r2 = {};
+    if (TP > 1)
+      exchange(buf0, xX, [&] {
+        if (tin) {
+          const int nb = (int)kResXchNorm + 16 * lane;
+          r0 = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(xr, nb, 0,
+                                                                            kCoherentAux));
+          r1 = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(
+                                           xr, nb + 4 * kResNormSlots, 0, kCoherentAux));
+          r2 = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(
+                                           xr, nb + 8 * kResNormSlots, 0, kCoherentAux));
+        }
+      });
+    if (k < 20) RES_TS(8 + 7 * k);
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     if (tin) {
-      f4v r0, r1, r2;
       if (TP == 1) {
         r0 = *reinterpret_cast<const f4v*>(&nsl[0][4 * lane]);
         r1 = *reinterpret_cast<const f4v*>(&nsl[1][4 * lane]);
         r2 = *reinterpret_cast<const f4v*>(&nsl[2][4 * lane]);
-      } else {
-        const int nb = (int)kResXchNorm + 16 * lane;
-        r0 = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(xr, nb, 0, kCoherentAux));
-        r1 = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(
-                                         xr, nb + 4 * kResNormSlots, 0, kCoherentAux));
-        r2 = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(
-                                         xr, nb + 8 * kResNormSlots, 0, kCoherentAux));
       }
 #pragma unroll
       for (int g = 0; g < 4; ++g) {
