@@ -57,6 +57,7 @@ constexpr int kResBuf = kResMaxN * kResLd;   // bf16 elements of one hi or lo op
 
 typedef __bf16 bf4v __attribute__((ext_vector_type(4)));
 typedef unsigned u4v __attribute__((ext_vector_type(4)));
+typedef unsigned u2v __attribute__((ext_vector_type(2)));
 
 // A uniform value unchanged, but opaque to the optimizer: addresses derived from it are
 // formed where they are used instead of being hoisted out of the Picard loop and kept live
@@ -121,6 +122,31 @@ __device__ __forceinline__ void res_store(const ResBuf& b, int a, int t, const f
   }
   *reinterpret_cast<bf4v*>(b.hi + t * kResLd + a) = h;
   *reinterpret_cast<bf4v*>(b.lo + t * kResLd + a) = l;
+}
+
+// res_store, and (teams) the same two 8-byte hi / lo values to the exchange field at byte
+// offset gb of the exchange area (same layout), device-coherently, so that the band is in
+// device memory as soon as the tile is done instead of being copied out of LDS after the
+// stage.
+__device__ __forceinline__ void res_store_x(const ResBuf& b, __amdgpu_buffer_rsrc_t xr,
+                                            int gb, bool team, int a, int t,
+                                            const float (&x)[4]) {
+  bf4v h, l;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const __bf16 hv = (__bf16)x[v];
+    h[v] = hv;
+    l[v] = (__bf16)(x[v] - (float)hv);
+  }
+  const int e = t * kResLd + a;
+  *reinterpret_cast<bf4v*>(b.hi + e) = h;
+  *reinterpret_cast<bf4v*>(b.lo + e) = l;
+  if (team) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2v, h), xr, gb + 2 * e, 0,
+                                          kCoherentAux);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2v, l), xr,
+                                          gb + 2 * (kResBuf + e), 0, kCoherentAux);
+  }
 }
 
 // Matrix fragments of one wave for a stage: W[k][t] of its t tile, every k step (hi, lo).
@@ -431,16 +457,18 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
   __bf16* const xQ = xX + kResXchField;
   float* const xN = reinterpret_cast<float*>(xch + kResXchNorm);
   double* const xT = reinterpret_cast<double*>(xN + 3 * kResNormSlots);
+  const int gbX = 0, gbQ = (int)(2 * kResXchField);   // byte offsets of the X and q fields
   unsigned arrivals = 0;
   // (teams) the member's column band [16·a_lo, 16·a_hi) of every row < nxp of operand
   // buffer b → exchange field g (16-byte device-coherent stores); team barrier; rows < nxp
   // of g (the whole field, LDS layout) → b by device-coherent LDS DMA
-  auto exchange = [&](const ResBuf& b, const __bf16* g, auto&& after_barrier)
+  auto exchange = [&](const ResBuf& b, const __bf16* g, bool stored, auto&& after_barrier)
                       __attribute__((always_inline)) {
     const int ur = 2 * (a_hi - a_lo);   // 16-byte units of the band in one row
     const int per = nxp * ur;           // ... in the hi (or the lo) field
     const long long gb = (long long)(g - xX) * 2;   // byte offset of g in the exchange area
-    for (int u = tid; u < 2 * per; u += kResThreads) {
+    // (stored: the stage's epilogues wrote the band to g already)
+    for (int u = stored ? 2 * per : tid; u < 2 * per; u += kResThreads) {
       const int h = u >= per ? 1 : 0;
       const int v = u - h * per;
       const int r = v / ur;
@@ -593,7 +621,7 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
     }
   }
   const double P_sum = res_sum(psum, sh[1]);   // its barriers also complete the operand stores
-  if (TP > 1) exchange(buf0, xQ, [] {});
+  if (TP > 1) exchange(buf0, xQ, false, [] {});
   RES_TS(1);
 
   const double fwd_scale = d.dx * d.dy * d.inv_sqrt_LxLy;
@@ -664,7 +692,7 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
                 res_store(buf0, tB.aout(q, lane), tB.tcol(lane), xo);   // [m][n]
               });
   }
-  if (TP > 1) exchange(buf0, xX, [] {});
+  if (TP > 1) exchange(buf0, xX, false, [] {});
 
   RES_TS(2);
   // ---- Picard loop ----
@@ -737,11 +765,12 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
 #pragma unroll
                   for (int v = 0; v < 4; ++v) qo[v] = ql[sl][v] - qe[v];
                   QEt[tidx(q, lane)] = f4v{qe[0], qe[1], qe[2], qe[3]};
-                  res_store(buf0, tB.aout(q, lane), tB.tcol(lane), qo);   // [i][j]
+                  res_store_x(buf0, xr, gbQ, TP > 1, tB.aout(q, lane), tB.tcol(lane),
+                              qo);   // [i][j]
                 });
     }
     if (k < 20) RES_TS(4 + 7 * k);
-    if (TP > 1) exchange(buf0, xQ, [] {});
+    if (TP > 1) exchange(buf0, xQ, true, [] {});
     if (k < 20) RES_TS(5 + 7 * k);
     // forward + relaxation + norm sums: per lane in FP32 over its four elements, per row
     // group of 16 lanes (4 rows × 16 columns of the tile) by DPP row shifts in FP32, to slot
@@ -767,7 +796,8 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
                     xo[v] = zw[sl][v][0] + gn * zw[sl][v][1];
                   }
                   Gt[tidx(q, lane)] = g4;
-                  res_store(buf0, tB.aout(q, lane), tB.tcol(lane), xo);   // [m][n]
+                  res_store_x(buf0, xr, gbX, TP > 1, tB.aout(q, lane), tB.tcol(lane),
+                              xo);   // [m][n]
                   p0 = res_row_sum(p0);
                   p1 = res_row_sum(p1);
                   p2 = res_row_sum(p2);
@@ -792,7 +822,7 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
     const bool tin = (lane >> 3) < NAY && (lane & 7) < NTB;
     f4v r0 = {}, r1 = {}, r2 = {};
     if (TP > 1)
-      exchange(buf0, xX, [&] {
+      exchange(buf0, xX, true, [&] {
         if (tin) {
           const int nb = (int)kResXchNorm + 16 * lane;
           r0 = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(xr, nb, 0,

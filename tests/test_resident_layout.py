@@ -265,6 +265,17 @@ def exchange(members, field, nxp):
     return out
 
 
+def assert_epilogue_writes_exactly_the_band(lo, hi, w, nxp):
+    """Inside the Picard loop the epilogues store their tiles straight to the exchange
+    field (same layout) instead of the exchange copying the band out of LDS: that stores
+    the same units only if the epilogue writes every unit of the member's band (rows < nxp,
+    columns [16·lo, 16·hi)) and nothing outside it (another member's band)."""
+    band = np.zeros_like(w)
+    band[:nxp, 16 * lo:16 * hi] = True
+    assert (w == band).all(), (lo, hi)
+    assert 2 * (XCH_FIELD + KRES_BUF + (nxp - 1) * LD + 16 * hi) <= XCH_NORM
+
+
 @pytest.mark.parametrize("nx,ny", [(128, 128), (96, 128), (128, 96), (48, 64), (32, 33),
                                    (100, 70), (17, 128)])
 def test_resident_team_stages_match_the_transforms(nx, ny):
@@ -300,6 +311,7 @@ def test_resident_team_stages_match_the_transforms(nx, ny):
             for (a, t), v in out.items():
                 b0[t, a] = v
                 w0[t, a] = True
+            assert_epilogue_writes_exactly_the_band(lo, hi, w0, nxp)
             X_members.append((lo, hi, b0, w0))
         ref = basis(ny) @ q @ basis(nx).T
         np.testing.assert_allclose(S[:ny, :nx], ref, atol=1e-10 * max(1.0, np.abs(ref).max()))
@@ -323,6 +335,7 @@ def test_resident_team_stages_match_the_transforms(nx, ny):
             out = run_member_stage(Bx.T.copy(), b1, w1, NTB, hi - lo, KX, 0, lo, b0, w0)
             for (a, t), v in out.items():
                 T[a, t] = v
+            assert_epilogue_writes_exactly_the_band(lo, hi, w0, nxp)
             q_members.append((lo, hi, b0, w0))
         ref = basis(ny).T @ X @ basis(nx)
         np.testing.assert_allclose(T[:ny, :nx], ref, atol=1e-10 * max(1.0, np.abs(ref).max()))
