@@ -926,8 +926,8 @@ static void build_xcd_items(Plan* p) {
 // per-workgroup phase stamps of the kernel on MI355X: alone, 6 + 3·(KX + KY) +
 // 0.25·(TA·KY + TB·KX), with KX, KY the k steps of the x and y contractions and TA, TB the
 // tiles per wave of the two stage kinds; a member of a team of tp saves 0.88·K per tile
-// per wave it no longer holds (K the mean k steps) and pays 13 for its two field exchanges
-// (each a device-coherent store, barrier and copy of about 2.6 µs). Starting from one
+// per wave it no longer holds (K the mean k steps) and pays 10.5 for its two field exchanges
+// (each the epilogue's device-coherent stores, a barrier and a copy: about 2 µs). Starting from one
 // workgroup per cuboid, the cuboid whose step is the longest moves to the smallest larger
 // team that shortens it, while the batch still fits one workgroup per CU (members wait for
 // each other) and the teams still fit the XCDs (a team's members run on one XCD). The work
@@ -942,7 +942,7 @@ static double team_cost(const CuboidDesc& d, int tp) {
   const int rows = (nay + tp - 1) / tp;
   const int t1 = (ntb * nay + kWavesPerRes - 1) / kWavesPerRes;
   const int tt = (ntb * rows + kWavesPerRes - 1) / kWavesPerRes;
-  return alone - 0.88 * 0.5 * (kx + ky) * (t1 - tt) + 13.0;
+  return alone - 0.88 * 0.5 * (kx + ky) * (t1 - tt) + 10.5;
 }
 
 static bool place_teams(Plan* p, const std::vector<int>& order) {
