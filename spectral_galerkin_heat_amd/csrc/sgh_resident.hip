@@ -42,8 +42,9 @@ constexpr int kResWaves = kResThreads / kWave;
 // exchanges of a single-member team take no time); 143 step end done; 144, 145 the cuboid's
 // nx and ny, with the team size and the member index in bits 16 and up; 146 + 4s + {0..3}
 // inside stage s (0: first inverse, 1: second forward) of iteration 5: stage entry, matrix
-// fragments in registers, tiles done, barrier passed (wave 0's view).
-constexpr int kResTsSlots = 154, kResTsBlocks = 1024;
+// fragments in registers, tiles done, barrier passed (wave 0's view); 154..157 inside the
+// step preparation: AA summed, laser factors formed, flux pass done, laser power summed.
+constexpr int kResTsSlots = 158, kResTsBlocks = 1024;
 __device__ unsigned long long g_res_ts[kResTsBlocks * kResTsSlots];
 #define RES_TS(slot)                                                                        \
   do {                                                                                      \
@@ -516,6 +517,7 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
   double aa = 0.0;
   for (int b = tid; b < d.ncolblk; b += kResThreads) aa += P.col_slots[d.col_item0 + b];
   const double AA = res_sum(aa, sh[0]);
+  RES_TS(154);
   double psum = 0.0;
   {
     const double* L = P.laser + (d.laser_off + step) * 6;
@@ -546,6 +548,7 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
       }
       __syncthreads();
     }
+    RES_TS(155);
     // the laser flux at surface element (i, j)
     auto laser_at = [&](int i, int j) __attribute__((always_inline)) -> double {
       if (sep) return I0 * (fac[kResMaxN + j] * fac[i]);
@@ -613,14 +616,34 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
         QL[(((j >> 4) * NTB + (i >> 4)) * 64 + (i & 15) + 16 * ((j & 15) >> 2)) * 4 + (j & 3)] = qlw;
       }
     }
-    // the laser power over the whole surface, in the same order for every team size
-#pragma unroll 1
-    for (int e = tid; e < nx * ny; e += kResThreads) {
-      const int j = e / nx;
-      psum += (double)(F)laser_at(e - j * nx, j);
+#ifdef SGH_RESIDENT_TIMESTAMPS
+    // (diagnostic build) the flux pass's stores drained here, so that stamp 156 splits the
+    // pass from the laser power sum
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+#endif
+    RES_TS(156);
+    // the laser power over the whole surface, in the same order for every team size: thread
+    // tid adds elements tid, tid + kResThreads, ... (e = j·nx + i) to one accumulator; (i, j)
+    // advance incrementally, and the loop is unrolled so that several elements' factor
+    // loads are in flight together
+    {
+      const int di = kResThreads % nx, dj = kResThreads / nx;
+      int i = tid % nx, j = tid / nx;
+#pragma unroll 4
+      for (int e = tid; e < nx * ny; e += kResThreads) {
+        psum += (double)(F)laser_at(i, j);
+        i += di;
+        j += dj;
+        if (i >= nx) {
+          i -= nx;
+          ++j;
+        }
+      }
     }
   }
   const double P_sum = res_sum(psum, sh[1]);   // its barriers also complete the operand stores
+  RES_TS(157);
   if (TP > 1) exchange(buf0, xQ, false, [] {});
   RES_TS(1);
 

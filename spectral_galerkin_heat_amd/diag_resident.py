@@ -21,7 +21,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DIAG_DIR = os.path.join(HERE, "_diag")
-SLOTS, BLOCKS = 154, 1024
+SLOTS, BLOCKS = 158, 1024
 STAGES = ("inverse 1", "inverse 2 + evaporation", "q exchange", "forward 1",
           "forward 2 + relaxation", "X exchange", "norm sums")
 PER = len(STAGES)
@@ -103,6 +103,12 @@ def run(steps, team_max=8):
         e = [int(t[base + j]) for j in range(4)]
         if min(e) > 0:
             lines.append(f"* {name}: {e[1] - e[0]} / {e[2] - e[1]} / {e[3] - e[2]}")
+    # inside the step preparation: AA, laser factors, flux pass, laser power, q exchange
+    p = [int(t[0])] + [int(t[s]) for s in (154, 155, 156, 157)] + [int(t[1])]
+    if min(p) > 0:
+        lines += ["", "Inside the step preparation (slowest workgroup): AA sum / laser factors / "
+                  "flux pass / laser power / q exchange, cycles",
+                  "* " + " / ".join(str(p[k + 1] - p[k]) for k in range(5))]
     return "\n".join(lines)
 
 
