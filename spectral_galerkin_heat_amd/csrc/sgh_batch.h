@@ -119,11 +119,13 @@ constexpr int kWavesPerRes = kResThreads / 64;   // waves of a resident workgrou
 constexpr int kResTiles = 64;            // 16×16 tiles of a kResMaxN² surface
 constexpr int kResNormSlots = 4 * kResTiles;   // norm sums: one per 4-row group of a tile
 // Exchange area of a cuboid slot: the X and q operand fields (hi, lo; LDS layout), the
-// norm sums (3 × kResNormSlots floats) and one T_max per member (doubles), 256-byte aligned.
+// norm sums (3 × kResNormSlots floats), one T_max per member and the laser power of every
+// 16-row band (doubles), 256-byte aligned.
 constexpr long long kResXchField = 2LL * kResMaxN * kResLd;   // bf16 elements of a hi+lo field
 constexpr long long kResXchNorm = 2 * kResXchField * 2;        // byte offset of the norm sums
 constexpr long long kResXchBytes =
-    (kResXchNorm + 4LL * 3 * kResNormSlots + 8LL * kResTeamMax + 255) / 256 * 256;
+    (kResXchNorm + 4LL * 3 * kResNormSlots + 8LL * kResTeamMax + 8LL * (kResMaxN / 16) + 255) /
+    256 * 256;
 
 __host__ __device__ inline int pad32(int n) { return (n + 31) & ~31; }
 

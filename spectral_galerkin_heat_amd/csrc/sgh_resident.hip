@@ -405,9 +405,10 @@ __device__ __forceinline__ void res_voffs(const ResTiles& t, int q, int lane, in
 // host gives teams only to batches that fit one workgroup per CU, members of a team on one
 // XCD), and every member copies the whole field into LDS. The per-tile norm sums go
 // through the exchange area as well. Every sum of the step is formed per 16×16 tile and the
-// tiles are combined in a fixed order (the laser power, AA: by every member over the whole
-// surface), so all members take the same Picard decision and a cuboid's arithmetic is the
-// same for every team size.
+// tiles are combined in a fixed order (AA: by every member over the whole surface; the laser
+// power per 16-row band, by the band's member, and the bands in order by member 0), so all
+// members take the same Picard decision and a cuboid's arithmetic is the same for every team
+// size.
 __global__ void __launch_bounds__(kResThreads)
 k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __restrict__ list,
              unsigned long long ticks) {
@@ -415,6 +416,7 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double sh[3][kResWaves];
   __shared__ __align__(16) float nsl[3][kResNormSlots];   // norm sums (single-member teams)
+  __shared__ double psl[kResMaxN / 16][kResWaves];        // laser power: band × wave partials
   const int ent = list[blockIdx.x];
   if (ent < 0) return;
   const int c = ent >> 6, TP = ((ent >> 3) & 7) + 1, mp = ent & 7;
@@ -450,7 +452,7 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
     return (tB.atile(qq) * NTB + tB.tt) * 64 + ln;
   };
   // the cuboid's exchange area (teams): X and q fields in the LDS layout (hi, then lo), the
-  // per-tile norm sums [3][kResTiles], one T_max per member
+  // per-tile norm sums [3][kResTiles], one T_max per member, the laser power per band
   unsigned char* const xch = P.res_xch + (long long)c * kResXchBytes;
   const __amdgpu_buffer_rsrc_t xr =
       __builtin_amdgcn_make_buffer_rsrc(xch, (short)0, (int)kResXchBytes, 0x00020000);
@@ -458,6 +460,7 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
   __bf16* const xQ = xX + kResXchField;
   float* const xN = reinterpret_cast<float*>(xch + kResXchNorm);
   double* const xT = reinterpret_cast<double*>(xN + 3 * kResNormSlots);
+  double* const xP = xT + kResTeamMax;   // laser power per 16-row band
   const int gbX = 0, gbQ = (int)(2 * kResXchField);   // byte offsets of the X and q fields
   unsigned arrivals = 0;
   // (teams) the member's column band [16·a_lo, 16·a_hi) of every row < nxp of operand
@@ -518,7 +521,6 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
   for (int b = tid; b < d.ncolblk; b += kResThreads) aa += P.col_slots[d.col_item0 + b];
   const double AA = res_sum(aa, sh[0]);
   RES_TS(154);
-  double psum = 0.0;
   {
     const double* L = P.laser + (d.laser_off + step) * 6;
     const double x0 = L[0], y0 = L[1], power = L[2], on = L[3], vx = L[4], vy = L[5];
@@ -567,9 +569,11 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
     // entries outside it): the previous flux's neighbours of all four are loaded before any
     // is used
     const ResBuf& b0 = buf0;
-    const int e_lo = 16 * a_lo * nxp, e_hi = 16 * a_hi * nxp;
+    const int eb = 16 * nxp;   // elements of one 16-row band (at most 4 per thread)
 #pragma unroll 1
-    for (int e0 = e_lo + tid; e0 < e_hi; e0 += 4 * kResThreads) {
+    for (int b = a_lo; b < a_hi; ++b) {
+      const int e0 = b * eb + tid, e_hi = (b + 1) * eb;
+      double pb = 0.0;   // this thread's laser power of the band
       float nb[4][4];
       int ij[4][2];
       double tw[4][2];
@@ -609,12 +613,15 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
           q_las[j * nx + i] = qlf;
           qlw = qlf;
           qn = (F)((double)qlf - (double)(F)sv);
+          pb += (double)qlf;
         }
         const __bf16 hv = (__bf16)qn;
         b0.hi[i * kResLd + j] = hv;               // forward operand [i][j]
         b0.lo[i * kResLd + j] = (__bf16)(qn - (float)hv);
         QL[(((j >> 4) * NTB + (i >> 4)) * 64 + (i & 15) + 16 * ((j & 15) >> 2)) * 4 + (j & 3)] = qlw;
       }
+      pb = res_wave_sum(pb);
+      if (lane == 0) psl[b][wave] = pb;
     }
 #ifdef SGH_RESIDENT_TIMESTAMPS
     // (diagnostic build) the flux pass's stores drained here, so that stamp 156 splits the
@@ -623,26 +630,16 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
     __syncthreads();
 #endif
     RES_TS(156);
-    // the laser power over the whole surface, in the same order for every team size: thread
-    // tid adds elements tid, tid + kResThreads, ... (e = j·nx + i) to one accumulator; (i, j)
-    // advance incrementally, and the loop is unrolled so that several elements' factor
-    // loads are in flight together
-    {
-      const int di = kResThreads % nx, dj = kResThreads / nx;
-      int i = tid % nx, j = tid / nx;
-#pragma unroll 4
-      for (int e = tid; e < nx * ny; e += kResThreads) {
-        psum += (double)(F)laser_at(i, j);
-        i += di;
-        j += dj;
-        if (i >= nx) {
-          i -= nx;
-          ++j;
-        }
-      }
-    }
   }
-  const double P_sum = res_sum(psum, sh[1]);   // its barriers also complete the operand stores
+  // the band partials of the laser power, summed over the waves in order; a team's go to
+  // the exchange area for member 0 (the barrier also completes the operand stores)
+  __syncthreads();
+  if (tid >= a_lo && tid < a_hi) {
+    double v = 0.0;
+    for (int w = 0; w < kResWaves; ++w) v += psl[tid][w];
+    if (TP > 1) st_x<double, true>(xP + tid, v);
+    else psl[tid][0] = v;
+  }
   RES_TS(157);
   if (TP > 1) exchange(buf0, xQ, false, [] {});
   RES_TS(1);
@@ -931,6 +928,10 @@ k_b_resident(BatchPtrs<float> P, const __bf16* __restrict__ m16, const int* __re
       }
   }
   if (tid == 0) {
+    // the laser power: band partials in band order, the same for every team size (a single
+    // member's own partials are in psl, written before the preparation's barrier)
+    double P_sum = 0.0;
+    for (int b = 0; b < NAY; ++b) P_sum += TP > 1 ? ld_x<double, true>(xP + b) : psl[b][0];
     double* st = P.status + kStatusStride * c;
     st[0] = n_it;
     st[1] = conv ? 1.0 : 0.0;

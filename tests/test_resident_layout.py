@@ -191,7 +191,7 @@ XCH_FIELD = 2 * KRES_BUF                      # bf16 of a hi+lo exchange field
 XCH_NORM = 2 * XCH_FIELD * 2                  # byte offset of the per-tile norm sums
 TEAM_MAX, TILES = 8, 64
 NORM_SLOTS = 4 * TILES                        # one per 4-row group of a tile
-XCH_BYTES = (XCH_NORM + 4 * 3 * NORM_SLOTS + 8 * TEAM_MAX + 255) // 256 * 256
+XCH_BYTES = (XCH_NORM + 4 * 3 * NORM_SLOTS + 8 * TEAM_MAX + 8 * (MAXN // 16) + 255) // 256 * 256
 
 
 class TeamTiles(Tiles):
@@ -364,7 +364,10 @@ def test_resident_team_stages_match_the_transforms(nx, ny):
                       for g in range(4))
         assert read == sorted(slots)
         assert len(set(units)) == len(units) and max(units) < MAXN * MAXN // 4
-        assert XCH_NORM + 4 * 3 * NORM_SLOTS + 8 * TEAM_MAX <= XCH_BYTES
+        # T_max per member, then the laser power per 16-row band (member 0 sums them in
+        # band order)
+        assert XCH_NORM + 4 * 3 * NORM_SLOTS + 8 * TEAM_MAX + 8 * NAY <= XCH_BYTES
+        assert NAY <= MAXN // 16
 
 
 def test_team_work_list_entries_round_trip():
