@@ -922,30 +922,27 @@ static void build_xcd_items(Plan* p) {
 }
 
 // Resident teams (sgh_resident.hip): how many workgroups each cuboid's step runs on. The
-// cost of a member's Picard iteration is modelled in thousands of cycles, fitted to the
-// per-shape cycle counts of the kernel on MI355X (the 64-cuboid batch, every shape alone and
-// the team sizes it gets): 7.94 + 3.09·K + 0.341·K·T, with K = KX + KY the k steps of the x
-// and y contractions and T the tiles per wave a member holds in every stage
-// (ceil(rows·NTB / 8) for its rows of 16-row bands); a member with several tiles per wave
-// also pays ≈9.5 for its two field exchanges, while members with one tile per wave run their
-// stages below the linear model by about as much (measured: the 128 × 128, 128 × 96 and
-// 96 × 128 teams), so a one-tile team is priced by the stage model alone. Starting from one
+// cost of a member's Picard iteration is modelled in thousands of cycles, fitted to
+// per-workgroup phase stamps of the kernel on MI355X: alone, 6 + 3·(KX + KY) +
+// 0.25·(TA·KY + TB·KX), with KX, KY the k steps of the x and y contractions and TA, TB the
+// tiles per wave of the two stage kinds; a member of a team of tp saves 0.88·K per tile
+// per wave it no longer holds (K the mean k steps) and pays 10.5 for its two field exchanges
+// (each the epilogue's device-coherent stores, a barrier and a copy: about 2 µs). Starting from one
 // workgroup per cuboid, the cuboid whose step is the longest moves to the smallest larger
 // team that shortens it, while the batch still fits one workgroup per CU (members wait for
-// each other) and the teams still fit the XCDs (a team's members run on one XCD); a cuboid
-// that cannot improve is set aside and the loop goes on with cuboids within kTeamMargin of
-// it (the model's error), so that every step near the longest reaches the one-tile floor.
-// The work list then places each team on one XCD (list positions ≡ xcd mod 8), largest
-// teams first.
-constexpr double kTeamMargin = 0.05;
+// each other) and the teams still fit the XCDs (a team's members run on one XCD). The work
+// list then places each team on one XCD (list positions ≡ xcd mod 8), largest teams first.
 static double team_cost(const CuboidDesc& d, int tp) {
   const int ntb = pad32(d.nx) / 16, nay = pad32(d.ny) / 16;
-  const int K = pad32(d.nx) / 32 + pad32(d.ny) / 32;
+  const int kx = pad32(d.nx) / 32, ky = pad32(d.ny) / 32;
+  const int ga = std::max(1, kWavesPerRes / nay), gb = std::max(1, kWavesPerRes / ntb);
+  const int ta = (ntb + ga - 1) / ga, tb = (nay + gb - 1) / gb;
+  const double alone = 6.0 + 3.0 * (kx + ky) + 0.25 * (ta * ky + tb * kx);
+  if (tp <= 1) return alone;
   const int rows = (nay + tp - 1) / tp;
-  const int T = (rows * ntb + kWavesPerRes - 1) / kWavesPerRes;
-  double c = 7.94 + 3.09 * K + 0.341 * K * T;
-  if (tp > 1 && T > 1) c += 9.5;
-  return c;
+  const int t1 = (ntb * nay + kWavesPerRes - 1) / kWavesPerRes;
+  const int tt = (ntb * rows + kWavesPerRes - 1) / kWavesPerRes;
+  return alone - 0.88 * 0.5 * (kx + ky) * (t1 - tt) + 10.5;
 }
 
 static bool place_teams(Plan* p, const std::vector<int>& order) {
@@ -983,27 +980,19 @@ static void assign_teams(Plan* p) {
   const long long slots = std::min<long long>(p->n_cu, (long long)kXcds * kCusPerXcd);
   if (p->team_max <= 1 || order.empty() || (long long)order.size() > slots) return;
   long long total = (long long)order.size();
-  std::vector<char> stuck(p->n, 0);
-  double floor_cost = 0.0;   // the longest step that cannot improve
   for (;;) {
     int worst = -1;
     double wc = -1.0;
     for (int c : order) {
-      if (stuck[c]) continue;
       const double v = team_cost(p->hdesc[c], p->hteam[c]);
       if (v > wc) { wc = v; worst = c; }
     }
-    if (worst < 0 || wc < (1.0 - kTeamMargin) * floor_cost) break;
     // the smallest larger team that shortens its step
     const CuboidDesc& d = p->hdesc[worst];
     const int tp = p->hteam[worst], nay = pad32(d.ny) / 16, lim = std::min(p->team_max, nay);
     int nt = tp + 1;
     while (nt <= lim && team_cost(d, nt) >= wc) ++nt;
-    if (nt > lim || total + (nt - tp) > slots) {
-      stuck[worst] = 1;
-      floor_cost = std::max(floor_cost, wc);
-      continue;
-    }
+    if (nt > lim || total + (nt - tp) > slots) break;
     p->hteam[worst] = nt;
     total += nt - tp;
   }
