@@ -413,10 +413,12 @@ __global__ void __launch_bounds__(NT) k_b_loop(BatchPtrs<F> P, int bud, int lds_
 
 // Per-step metrics and step-counter advance (single block).
 template <typename F>
-__global__ void __launch_bounds__(kBT) k_b_metrics(BatchPtrs<F> P) {
+__global__ void __launch_bounds__(kBT) k_b_metrics(BatchPtrs<F> P, int skip_resident) {
   for (int c = threadIdx.x; c < P.n_cuboids; c += kBT) {
     const CuboidDesc d = P.desc[c];
     if (!d.live) continue;
+    // a cuboid the resident step kernel ran has written its row and advanced its counter
+    if (skip_resident && d.bx16_off >= 0 && d.by16_off >= 0) continue;
     const int step = P.step_idx[c];
     if (step >= d.n_steps) continue;
     double pl = 0.0, tm = -1.0 / 0.0;
@@ -523,7 +525,8 @@ struct Plan {
   void* res_xch = nullptr;   // FP32: kResXchBytes per cuboid slot (resident teams)
   long long mats16_cap = 0;
   bool resident = false;     // variant rows = −1 requested
-  bool res_on = false;       // ... and every live cuboid allows it (mixed precision, ≤ 128 wide)
+  bool res_ok = false;       // ... and it applies (mixed precision, 31-bit surface offsets)
+  bool res_on = false;       // ... and a live cuboid fits it (both surface sides ≤ 128)
   std::vector<int> hres;     // resident work list: res_entry per workgroup (−1 idle)
   std::vector<int> hteam;    // resident team size per slot (0: none) of the last lists
   int team_max = kResTeamMax;  // largest resident team
@@ -571,6 +574,10 @@ struct Plan {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double t_surface_ms = 0.0, t_volume_ms = 0.0;
   long long n_surface = 0, n_volume = 0;
+  // a step holding both resident and per-pass cuboids runs the resident launch on a side
+  // stream beside the per-pass launches (created with the first such work list)
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 template <typename F>
@@ -732,7 +739,8 @@ template <typename F>
 int enqueue_step(Plan* p, hipStream_t st) {
   BatchPtrs<F> P = ptrs<F>(p);
   const unsigned nrow = (unsigned)p->hrow.size(), ncol = (unsigned)p->hcol.size();
-  if (nrow == 0) return SGH_OK;
+  const bool res = p->res_on;
+  if (nrow == 0 && !res) return SGH_OK;
   const bool tm = p->timing;
   SurfFn<F> f[4];
   if (!surf_lookup<F>(p->rows, p->threads, p->colsplit, f)) {
@@ -741,58 +749,57 @@ int enqueue_step(Plan* p, hipStream_t st) {
   }
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
-  if constexpr (std::is_same<F, float>::value) {
-    if (p->res_on) {
-      // the whole surface part of the step in one launch, then the volume pass
-      if (tm) hipEventRecord(p->ev[0], st);
-      chk(launch_resident(P, p->mats16, p->res_list, (unsigned)p->hres.size(),
-                          p->barrier_ticks, st));
-      if (tm) hipEventRecord(p->ev[1], st);
-      if (tm) hipEventRecord(p->ev[2], st);
-      hipLaunchKernelGGL(k_b_volume<F>, dim3(ncol), dim3(kBT), 0, st, P, 2);
-      if (tm) hipEventRecord(p->ev[3], st);
-      SGH_CHECK_LAUNCH("sgh_batch resident step");
-      if (tm) {
-        e = hipEventSynchronize(p->ev[3]);
-        if (e != hipSuccess) return fail_hip(e, "sgh_batch timing");
-        float a = 0.f, b = 0.f;
-        hipEventElapsedTime(&a, p->ev[0], p->ev[1]);
-        hipEventElapsedTime(&b, p->ev[2], p->ev[3]);
-        p->t_surface_ms += a;
-        p->n_surface += 1;
-        p->t_volume_ms += b;
-        p->n_volume += 1;
-      }
-      return SGH_OK;
-    }
+  // timed surface work: with the resident kernel every surface launch of the step, else
+  // the Picard iterations' launches (or the persistent launch)
+  if (tm && res) hipEventRecord(p->ev[0], st);
+  // with per-pass cuboids in the step too (and not timing kernels), the resident launch
+  // runs on the side stream: its workgroups and the per-pass ones share the CUs, and each
+  // cuboid's kernels (so its results) are those of a step without the others
+  const bool fork = res && nrow > 0 && !tm && p->side;
+  if (fork) {
+    chk(hipEventRecord(p->ev_fork, st));
+    chk(hipStreamWaitEvent(p->side, p->ev_fork, 0));
   }
-  hipLaunchKernelGGL(k_b_prep<F>, dim3(nrow), dim3(kBT), 0, st, P);
-  if (p->loop_on) {
+  if constexpr (std::is_same<F, float>::value) {
+    // the whole surface part of the step of every cuboid within the resident kernel's
+    // tables in one launch (each advances its own step counter and writes its metrics row)
+    if (res)
+      chk(launch_resident(P, p->mats16, p->res_list, (unsigned)p->hres.size(),
+                          p->barrier_ticks, fork ? p->side : st));
+  }
+  if (fork) chk(hipEventRecord(p->ev_join, p->side));
+  if (nrow > 0) hipLaunchKernelGGL(k_b_prep<F>, dim3(nrow), dim3(kBT), 0, st, P);
+  if (nrow == 0) {
+    // resident cuboids only
+  } else if (p->loop_on) {
     // the whole step's surface work in one launch (timed as a whole)
     LoopFn<F> lf = loop_lookup<F>(p->rows, p->threads, p->colsplit);
-    if (tm) hipEventRecord(p->ev[0], st);
     int bud = p->bud, lds_elems = (int)(p->shm / p->elem), max_it = p->max_it;
     unsigned long long ticks = p->barrier_ticks;
     void* args[] = {&P, &bud, &lds_elems, &max_it, &ticks};
+    if (tm && !res) hipEventRecord(p->ev[0], st);
     chk(hipLaunchKernel(reinterpret_cast<const void*>(lf), dim3(nrow), dim3(p->threads), args,
                         p->shm, st));
-    if (tm) hipEventRecord(p->ev[1], st);
-    if (tm) hipEventRecord(p->ev[2], st);
+    if (tm && !res) hipEventRecord(p->ev[1], st);
   } else {
     chk(launch_surface<F>(f, 0, nrow, p, P, 0, st));
-    if (tm) hipEventRecord(p->ev[0], st);
+    if (tm && !res) hipEventRecord(p->ev[0], st);
     for (int k = 0; k < p->max_it; ++k) {
       chk(launch_surface<F>(f, 2, nrow, p, P, k, st));
       chk(launch_surface<F>(f, 1, nrow, p, P, k, st));
     }
-    if (tm) hipEventRecord(p->ev[1], st);
+    if (tm && !res) hipEventRecord(p->ev[1], st);
     chk(launch_surface<F>(f, 3, nrow, p, P, p->max_it, st));
-    if (tm) hipEventRecord(p->ev[2], st);
   }
+  if (fork) chk(hipStreamWaitEvent(st, p->ev_join, 0));
+  if (tm && res) hipEventRecord(p->ev[1], st);
   if (e != hipSuccess) return fail_hip(e, "sgh_batch surface launch");
-  hipLaunchKernelGGL(k_b_volume<F>, dim3(ncol), dim3(kBT), 0, st, P, 1);
+  // mode 2 finalizes every cuboid that stepped (done_at < INT_MAX): the per-pass ones, which
+  // k_b_prep marked, and the resident ones; mode 1 needs the step counters not yet advanced
+  if (tm) hipEventRecord(p->ev[2], st);
+  hipLaunchKernelGGL(k_b_volume<F>, dim3(ncol), dim3(kBT), 0, st, P, res ? 2 : 1);
   if (tm) hipEventRecord(p->ev[3], st);
-  hipLaunchKernelGGL(k_b_metrics<F>, dim3(1), dim3(kBT), 0, st, P);
+  if (nrow > 0) hipLaunchKernelGGL(k_b_metrics<F>, dim3(1), dim3(kBT), 0, st, P, res ? 1 : 0);
   SGH_CHECK_LAUNCH("sgh_batch step");
   if (tm) {
     hipError_t e = hipEventSynchronize(p->ev[3]);
@@ -801,7 +808,8 @@ int enqueue_step(Plan* p, hipStream_t st) {
     hipEventElapsedTime(&a, p->ev[0], p->ev[1]);
     hipEventElapsedTime(&b, p->ev[2], p->ev[3]);
     p->t_surface_ms += a;
-    p->n_surface += p->loop_on ? 1LL : 2LL * p->max_it;
+    p->n_surface += res ? 1LL + (nrow == 0 ? 0LL : 2LL * p->max_it + 2LL)
+                        : p->loop_on ? 1LL : 2LL * p->max_it;
     p->t_volume_ms += b;
     p->n_volume += 1;
   }
@@ -873,9 +881,19 @@ static bool assign_heights(Plan* p) {
   return changed;
 }
 
+// A live cuboid's step runs on the resident step kernel when the resident variant applies
+// and its split-bf16 tables exist (both surface sides at most kResMaxN); per-pass launches
+// run the others in the same step. k_b_metrics applies the same rule on the device.
+static bool on_resident(const Plan* p, const CuboidDesc& d) {
+  return p->res_ok && d.live && d.bx16_off >= 0 && d.by16_off >= 0;
+}
+
+// Work lists: per-pass row items of the cuboids the resident kernel does not run, volume
+// column items of every live cuboid, and (teams of one) the resident list; each cuboid's
+// items go to the least loaded XCD (list position ≡ XCD mod 8).
 static void build_xcd_items(Plan* p) {
-  std::vector<double> load(kXcds, 0.0);
-  std::vector<int> count(kXcds, 0);
+  std::vector<double> load(kXcds, 0.0), rload(kXcds, 0.0);
+  std::vector<int> count(kXcds, 0), rcount(kXcds, 0);
   std::vector<std::vector<Item>> rb(kXcds), cb(kXcds), res(kXcds);
   std::vector<int> order;
   for (int c = 0; c < p->n; ++c)
@@ -888,19 +906,30 @@ static void build_xcd_items(Plan* p) {
     const double ca = cost(a), cb_ = cost(b);
     return ca != cb_ ? ca > cb_ : a < b;
   });
+  p->hteam.assign(p->n, 0);
+  p->max_it = 0;
   for (int c : order) {
-    // least loaded XCD that still has a CU for each of the cuboid's items; else the one
-    // with the fewest items
+    // least loaded XCD that still has a CU for each of the cuboid's workgroups (its items,
+    // or its one resident workgroup); else the one with the fewest
+    const CuboidDesc& d = p->hdesc[c];
+    const bool r = on_resident(p, d);
+    std::vector<double>& ld = r ? rload : load;
+    std::vector<int>& ct = r ? rcount : count;
+    const int w = r ? 1 : n_items(d);
     int x = -1;
     for (int y = 0; y < kXcds; ++y)
-      if (count[y] + n_items(p->hdesc[c]) <= kCusPerXcd && (x < 0 || load[y] < load[x])) x = y;
-    if (x < 0) x = (int)(std::min_element(count.begin(), count.end()) - count.begin());
-    load[x] += cost(c);
-    count[x] += n_items(p->hdesc[c]);
-    const CuboidDesc& d = p->hdesc[c];
-    for (int b = 0; b < n_items(d); ++b) rb[x].push_back({c, b});
+      if (ct[y] + w <= kCusPerXcd && (x < 0 || ld[y] < ld[x])) x = y;
+    if (x < 0) x = (int)(std::min_element(ct.begin(), ct.end()) - ct.begin());
+    ld[x] += cost(c);
+    ct[x] += w;
     for (int b = 0; b < d.ncolblk; ++b) cb[x].push_back({c, b});
-    res[x].push_back({c, 0});
+    if (r) {
+      res[x].push_back({c, 0});
+      p->hteam[c] = 1;
+    } else {
+      for (int b = 0; b < n_items(d); ++b) rb[x].push_back({c, b});
+      p->max_it = std::max(p->max_it, d.max_it);
+    }
   }
   auto interleave = [](std::vector<std::vector<Item>>& bins, std::vector<Item>& out) {
     size_t m = 0;
@@ -915,10 +944,7 @@ static void build_xcd_items(Plan* p) {
   interleave(res, rl);
   p->hres.clear();
   for (const Item& it : rl) p->hres.push_back(it.c < 0 ? -1 : res_entry(it.c, 1, 0));
-  p->hteam.assign(p->n, 0);
-  for (int c : order) p->hteam[c] = 1;
-  p->max_it = 0;
-  for (int c : order) p->max_it = std::max(p->max_it, p->hdesc[c].max_it);
+  p->res_on = !p->hres.empty();
 }
 
 // Resident teams (sgh_resident.hip): how many workgroups each cuboid's step runs on. The
@@ -930,7 +956,8 @@ static void build_xcd_items(Plan* p) {
 // (each the epilogue's device-coherent stores, a barrier and a copy: about 2 µs). Starting from one
 // workgroup per cuboid, the cuboid whose step is the longest moves to the smallest larger
 // team that shortens it, while the batch still fits one workgroup per CU (members wait for
-// each other) and the teams still fit the XCDs (a team's members run on one XCD). The work
+// each other; CUs for the per-pass items of the step are kept free) and the teams still fit
+// the XCDs (a team's members run on one XCD). The work
 // list then places each team on one XCD (list positions ≡ xcd mod 8), largest teams first.
 static double team_cost(const CuboidDesc& d, int tp) {
   const int ntb = pad32(d.nx) / 16, nay = pad32(d.ny) / 16;
@@ -976,9 +1003,13 @@ static bool place_teams(Plan* p, const std::vector<int>& order) {
 static void assign_teams(Plan* p) {
   std::vector<int> order;
   for (int c = 0; c < p->n; ++c)
-    if (p->hdesc[c].live) order.push_back(c);
-  const long long slots = std::min<long long>(p->n_cu, (long long)kXcds * kCusPerXcd);
-  if (p->team_max <= 1 || order.empty() || (long long)order.size() > slots) return;
+    if (p->hteam[c] > 0) order.push_back(c);
+  const long long hw = std::min<long long>(p->n_cu, (long long)kXcds * kCusPerXcd);
+  if (p->team_max <= 1 || order.empty() || (long long)order.size() > hw) return;
+  // per-pass items of the same step (run beside the resident launch) keep a CU each
+  long long items = 0;
+  for (const Item& it : p->hrow) items += it.c >= 0 ? 1 : 0;
+  const long long slots = std::max<long long>((long long)order.size(), hw - items);
   long long total = (long long)order.size();
   for (;;) {
     int worst = -1;
@@ -1019,6 +1050,9 @@ static void free_plan(Plan* p) {
   drop_graph(p);
   for (auto& e : p->ev)
     if (e) hipEventDestroy(e);
+  if (p->ev_fork) hipEventDestroy(p->ev_fork);
+  if (p->ev_join) hipEventDestroy(p->ev_join);
+  if (p->side) hipStreamDestroy(p->side);
   void* bufs[] = {p->a, p->surf, p->mats, p->t1, p->laser, p->desc, p->row_items, p->row_descs,
                   p->col_items,
                   p->prime_items, p->row_slots, p->col_slots, p->aux_slots, p->done_at,
@@ -1286,6 +1320,10 @@ static int rebuild_lists(Plan* p, hipStream_t st) {
         e = hipMemcpy(p->desc + c, &p->hdesc[c], sizeof(CuboidDesc), hipMemcpyHostToDevice);
     if (e != hipSuccess) return fail_hip(e, "sgh_batch heights");
   }
+  // the resident step kernel: mixed precision, the surface arrays addressable with 31-bit
+  // byte offsets; it runs the live cuboids within its tables, per-pass launches the rest
+  p->res_ok = p->resident && p->dtype == SGH_F32 &&
+              (long long)kSurfArrays * p->total_S * 4 < 0x7fffffffLL;
   build_xcd_items(p);
   p->colsplit = false;
   for (int c = 0; c < p->n; ++c)
@@ -1295,17 +1333,15 @@ static int rebuild_lists(Plan* p, hipStream_t st) {
   const bool has_loop = p->dtype == SGH_F64
                             ? loop_lookup<double>(p->rows, p->threads, p->colsplit) != nullptr
                             : loop_lookup<float>(p->rows, p->threads, p->colsplit) != nullptr;
-  p->loop_on = p->loop_pref && has_loop && (long long)p->hrow.size() <= p->n_cu &&
-               p->shm <= (size_t)kLoopLdsMax;
-  // the resident step kernel: mixed precision, every live surface within its tables, the
-  // surface arrays addressable with 31-bit byte offsets
-  p->res_on = p->resident && p->dtype == SGH_F32 &&
-              (long long)kSurfArrays * p->total_S * 4 < 0x7fffffffLL;
-  for (int c = 0; c < p->n && p->res_on; ++c)
-    if (p->hdesc[c].live && (p->hdesc[c].bx16_off < 0 || p->hdesc[c].by16_off < 0))
-      p->res_on = false;
+  p->loop_on = p->loop_pref && has_loop && !p->res_on &&
+               (long long)p->hrow.size() <= p->n_cu && p->shm <= (size_t)kLoopLdsMax;
   if (p->res_on) assign_teams(p);
-  else p->hteam.assign(p->n, 0);
+  if (p->res_on && !p->hrow.empty() && !p->side) {
+    e = hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming);
+    if (e != hipSuccess) return fail_hip(e, "sgh_batch side stream");
+  }
   if ((long long)p->hres.size() > p->res_cap) {
     set_error("sgh_batch: resident work list exceeds capacity");
     return SGH_ERR_STATE;
@@ -1578,7 +1614,7 @@ int sgh_batch_run(void* plan, int n_steps, int use_graph, void* stream) {
     rc = rebuild_lists(p, st);
     if (rc) return rc;
   }
-  if (p->hrow.empty()) return SGH_OK;
+  if (p->hrow.empty() && p->hres.empty()) return SGH_OK;
   if (p->timing) use_graph = 0;
   if (use_graph) {
     if (p->graph_exec && p->graph_stream != st) drop_graph(p);

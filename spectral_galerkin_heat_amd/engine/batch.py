@@ -216,8 +216,8 @@ class BatchEngine:
 
     @property
     def team_sizes(self):
-        """Team size of each slot in the last work lists (0: free, or the resident kernel
-        not in use)."""
+        """Team size of each slot in the last work lists (0: free, or stepped by the
+        per-pass launches)."""
         n = int(call("sgh_batch_team_sizes", self._plan, None, 0))
         out = (C.c_int * max(n, 1))()
         call("sgh_batch_team_sizes", self._plan, C.cast(out, C.c_void_p), n)

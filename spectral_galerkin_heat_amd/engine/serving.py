@@ -104,11 +104,20 @@ class ServeReport:
 
 
 class ScanServer:
+    """Continuous-batching request loop over a ``BatchPool`` plus a plain lane.
+
+    ``variant``: the pool's surface-transform variant, ``(rows, threads, lds_bytes)``;
+    ``"resident"`` (the default in mixed precision) runs every cuboid whose surface sides are
+    at most 128 on the resident step kernel and the wider ones on 16-row per-pass launches in
+    the same step; None keeps the per-pass default. A request's result depends only on its
+    own cuboid and the variant, never on what else is in the pool.
+    """
     MIN_BUDGET = 64 << 20   # smallest pool budget tried after device out-of-memory errors
+    RESIDENT = (-1, 512, 150 * 1024)
 
     def __init__(self, precision="mixed", max_cuboids=128, hbm_budget_bytes=8 << 30,
                  max_nz=64, max_steps=2048, steps_per_tick=8, check_every=20,
-                 keep_results=True, plain_slots=16):
+                 keep_results=True, plain_slots=16, variant="resident"):
         word = 8 if precision == "fp64" else 4
         # split the budget between the packed state and the surface fields; a pool the
         # device cannot hold is retried at half the budget (down to MIN_BUDGET)
@@ -126,6 +135,11 @@ class ScanServer:
                 if budget // 2 < self.MIN_BUDGET:
                     raise
                 budget //= 2
+        if isinstance(variant, str) and variant == "resident":
+            variant = self.RESIDENT if precision == "mixed" else None
+        if variant is not None:
+            self.pool.set_variant(*variant)
+        self.variant = self.pool.variant
         self.hbm_budget_bytes = budget
         self.precision = precision
         self.max_steps = max_steps

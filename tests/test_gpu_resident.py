@@ -3,7 +3,7 @@
 One workgroup per cuboid runs the whole surface part of a step with split-bf16 MFMA
 transforms and FP32 fields. Checked: each cuboid against the FP64 oracle (≤ 1e-4
 relative L2), alone equals in a mix (bit for bit), graph replay equals eager launches,
-a batch holding a surface wider than 128 falls back to the per-pass launches, and the
+a surface wider than 128 runs on per-pass launches beside the resident ones, and the
 64-cuboid benchmark batch over 1000 steps against the FP64 engine (≤ 1e-4, melt-pool
 width and depth within one cell).
 """
@@ -103,15 +103,39 @@ def test_resident_alone_equals_in_mix_and_graph_equals_eager():
     mix.close()
 
 
-def test_wide_surface_falls_back_to_per_pass_launches():
+def test_wide_surface_runs_per_pass_beside_the_resident_cuboids():
+    # a surface wider than 128 runs on per-pass launches in the same step as the resident
+    # launch of the others: each cuboid matches the oracle, and each equals its run alone
+    # (the narrow ones on the resident kernel, the wide one on per-pass launches)
     cfg, lp = raster_multitrack(mesh=(160, 32, 16), n_steps=20, size=(2.0e-3, 0.5e-3, 0.25e-3))
     cases = batch_uneven_cuboids(n=2, seed=3, n_steps=20) + [(cfg, lp)]
     eng = _engine(cases)
-    eng.run(20)
+    eng.set_timing(True)
+    eng.run(2)
     eng.synchronize()
+    t = eng.timing()
+    eng.set_timing(False)
+    teams = eng.team_sizes
+    assert teams[0] >= 1 and teams[1] >= 1 and teams[2] == 0, teams
+    assert t["surface_launches"] > 2 and t["volume_launches"] == 2, t
+    eng.run(18)
+    eng.synchronize()
+    # the resident launch on its side stream, eager instead of graph-replayed
+    eager = _engine(cases)
+    eager.run(20, use_graph=False)
+    eager.synchronize()
+    for i in range(len(cases)):
+        assert torch.equal(eager.state(i), eng.state(i)), i
+    eager.close()
     for i, (c, l) in enumerate(cases):
         ref, _ = _oracle(c, l, 20)
         assert modal_rel_l2(eng.state(i).double().cpu().numpy(), ref.a) < 1e-4, i
+        alone = _engine([cases[i]])
+        alone.run(20)
+        alone.synchronize()
+        assert torch.equal(alone.state(0), eng.state(i)), i
+        np.testing.assert_array_equal(alone.metrics()[:, 0], eng.metrics()[:, i])
+        alone.close()
     eng.close()
 
 

@@ -106,7 +106,7 @@ def test_served_results_equal_solo_runs():
     # five arrive together for four slots: one joins only after the pool has stepped
     assert any(r.start_step > 0 for r in rep.requests)
     for r in rep.requests:
-        solo = BatchEngine([r.ctx], precision="mixed", n_steps=r.n_steps)
+        solo = BatchEngine([r.ctx], precision="mixed", n_steps=r.n_steps, variant=srv.variant)
         solo.run(r.n_steps)
         solo.synchronize()
         assert torch.equal(solo.state(0), r.result), r.rid
@@ -152,7 +152,7 @@ def test_c5_poisson_stream_with_rasters():
     by = {r.rid: r for r in rep.requests}
     for rid in (rasters[0], 7, 33):
         r = by[rid]
-        solo = BatchEngine([r.ctx], precision="mixed", n_steps=r.n_steps)
+        solo = BatchEngine([r.ctx], precision="mixed", n_steps=r.n_steps, variant=srv.variant)
         solo.run(r.n_steps)
         solo.synchronize()
         assert torch.equal(solo.state(0), r.result), rid

@@ -346,10 +346,15 @@ def test_serving_pool_backs_off_after_device_out_of_memory(monkeypatch):
             tried.append(state_capacity * 4 / 0.85)
             if state_capacity * 4 > (1 << 30):
                 raise _lib.DeviceOutOfMemory("sgh_batch_pool_create failed (-4)")
+            self.variant = None
+
+        def set_variant(self, rows, threads, lds_bytes):
+            self.variant = {"rows": rows, "threads": threads, "lds_bytes": lds_bytes}
 
     monkeypatch.setattr(serving, "BatchPool", Pool)
     srv = serving.ScanServer(hbm_budget_bytes=8 << 30)
     assert srv.hbm_budget_bytes == 1 << 30 and len(tried) == 4
+    assert srv.variant["rows"] == -1    # mixed precision serves on the resident kernel
     # below the smallest budget the error reaches the caller
     monkeypatch.setattr(serving.ScanServer, "MIN_BUDGET", 4 << 30)
     with pytest.raises(_lib.DeviceOutOfMemory):
