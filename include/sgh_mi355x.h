@@ -350,10 +350,12 @@ int sgh_batch_info(void* plan, long long* out6);
  * lists and the captured step graph are rebuilt by the next sgh_batch_run.  Results
  * are deterministic for a given variant; different variants sum per-item partials in
  * different groupings (last-bit differences).  rows = -1 (threads 512) is the resident
- * step kernel: FP32 plans only (refused for FP64), one workgroup per cuboid runs the
- * whole surface part of a step in one launch with split-bf16 MFMA transforms; steps in
- * which a live cuboid is wider than 128 use 16-row per-pass items with the given LDS
- * budget instead.  sgh_batch_get_variant reports rows -1 while it is selected. */
+ * step kernel: FP32 plans only (refused for FP64), a team of 1-8 workgroups per cuboid
+ * runs the whole surface part of a step in one launch with split-bf16 MFMA transforms;
+ * cuboids with a surface side wider than 128 run on 16-row per-pass items with the given
+ * LDS budget in the same step (beside the resident launch, on a second stream the plan
+ * owns), so every cuboid's results are those of its run alone.  sgh_batch_get_variant
+ * reports rows -1 while it is selected. */
 int sgh_batch_variants(int* out_pairs, int capacity);
 int sgh_batch_set_variant(void* plan, int rows, int threads, int lds_bytes, void* stream);
 /* out3 = {rows, threads, LDS bytes}. */

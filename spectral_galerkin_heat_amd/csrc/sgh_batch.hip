@@ -627,8 +627,8 @@ struct SurfVariant {
   int rows, threads;
 };
 // rows = 0: each cuboid's items hold 16 or 32 rows (assign_heights), on the 32-row kernels.
-// rows = −1: the resident step kernel (one workgroup of 1024 threads per cuboid), mixed
-// precision and surfaces up to 128 × 128 only.
+// rows = −1: the resident step kernel (a team of 512-thread workgroups per cuboid), mixed
+// precision and surfaces up to 128 × 128; wider ones on 16-row per-pass items beside it.
 constexpr int kResidentRows = -1;
 constexpr SurfVariant kSurfVariants[] = {{16, 256}, {16, 512}, {16, 1024}, {32, 512},
                                          {32, 1024}, {0, 512}, {0, 1024},
@@ -1804,8 +1804,8 @@ int sgh_batch_set_variant(void* plan, int rows, int threads, int lds_bytes, void
   const bool res = rows == kResidentRows;
   SGH_REQUIRE(!res || p->dtype == SGH_F32, SGH_ERR_ARG,
               "sgh_batch_set_variant: the resident step kernel runs in mixed precision only");
-  // the resident variant keeps a per-pass launch shape for steps whose cuboids it cannot
-  // hold (a surface wider than 128): 16-row items, 1024 threads, the given LDS budget
+  // the resident variant keeps a per-pass launch shape for the cuboids it cannot hold (a
+  // surface side wider than 128): 16-row items, 1024 threads, the given LDS budget
   if (res) {
     rows = 16;
     threads = 1024;
