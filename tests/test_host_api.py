@@ -355,6 +355,11 @@ def test_serving_pool_backs_off_after_device_out_of_memory(monkeypatch):
     srv = serving.ScanServer(hbm_budget_bytes=8 << 30)
     assert srv.hbm_budget_bytes == 1 << 30 and len(tried) == 4
     assert srv.variant["rows"] == -1    # mixed precision serves on the resident kernel
+    # FP64 has no resident kernel; variant=None keeps the pool's per-pass default
+    assert serving.ScanServer(precision="fp64", hbm_budget_bytes=1 << 30).variant is None
+    assert serving.ScanServer(hbm_budget_bytes=1 << 30, variant=None).variant is None
+    assert serving.ScanServer(hbm_budget_bytes=1 << 30,
+                              variant=(16, 1024, 150 * 1024)).variant["rows"] == 16
     # below the smallest budget the error reaches the caller
     monkeypatch.setattr(serving.ScanServer, "MIN_BUDGET", 4 << 30)
     with pytest.raises(_lib.DeviceOutOfMemory):
